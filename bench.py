@@ -1,0 +1,153 @@
+"""Benchmark: plain batched staggered step on one MI355X.
+
+Workload: 4096 seeded uneven random-tree cells (200-2000 CVs each, about
+4.6 M compartments), Na/KDR/leak everywhere, CaL + CalciumDetailed on the
+dendrites, FP32, dt = 0.025 ms.  A step is one staggered time step of every
+cell; the timed loop replays a hipGraph of persistent-kernel launches.
+
+Prints one JSON line (metric, value, roofline, cpu_baseline, accuracy).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_CA_CV = 96.0  # algorithmic HBM bytes per compartment-step, CV with CaL + Ca pool
+BYTES_PLAIN_CV = 76.0  # same without Ca
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(models, probe_cvs, n_steps, gpu_soma, dt):
+    """Oracle (FP64 NumPy restatement, one core) on a bounded sample of the
+    same workload; also returns the GPU-vs-oracle soma RMSE on that sample."""
+    from oracle.staggered import StaggeredOracle
+
+    cvs = 0
+    rm = []
+    t0 = time.perf_counter()
+    for i, m in enumerate(models):
+        o = StaggeredOracle(m)
+        o.reset_state(m.V0)
+        tr = o.run(dt, n_steps, probe_cvs=[int(probe_cvs[i])])
+        cvs += o.C
+        rm.append(float(np.sqrt(np.mean((tr[:, 0] - gpu_soma[:, i]) ** 2))))
+    el = time.perf_counter() - t0
+    return cvs * n_steps / el, el, rm
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cells", type=int, default=4096)
+    ap.add_argument("--chunk", type=int, default=200)
+    ap.add_argument("--threads", type=int, default=256)
+    ap.add_argument("--rows-per-pack", type=int, default=2600)
+    ap.add_argument("--cpu-sample-cells", type=int, default=2)
+    ap.add_argument("--cpu-sample-steps", type=int, default=800)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from braincell_amd import workloads as W
+    from braincell_amd.runtime import StaggerLayout, StaggerRuntime
+
+    dt = 0.025
+    t0 = time.perf_counter()
+    models = W.uneven_tree_batch(args.cells)
+    L = StaggerLayout(models, threads=args.threads, rows_per_pack=args.rows_per_pack)
+    log(f"layout {L.summary()} built in {time.perf_counter() - t0:.1f}s")
+    rt = StaggerRuntime(L, dt, precision=32)
+    chunk = args.chunk
+    K = (args.steps // chunk) * chunk
+    Wm = max(chunk, (args.warmup // chunk) * chunk)
+    trace = rt.new_trace(max(K, Wm))
+    # warmup (also used for the CPU-baseline accuracy sample)
+    rt.run(Wm, trace=trace[:Wm], chunk=chunk)
+    torch.cuda.synchronize()
+    rt.check_status()
+    warm_soma = trace[:Wm].double().cpu().numpy()
+    # timed: hipGraph of K / chunk persistent launches
+    rt.reset_state()
+    graph = rt.make_graph(K // chunk, chunk, trace)
+    stream = rt.stream_handle()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    graph.launch(stream)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    rt.check_status()
+    cu = float(L.n_cv_total) * K / el
+    log(f"timed {K} steps in {el:.3f}s -> {cu:.3e} compartment-updates/s")
+
+    # per-launch duration of the dominant kernel with HIP events on its stream
+    rt.reset_state()
+    n_ev = 5
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(n_ev + 1)]
+    torch.cuda.synchronize()
+    evs[0].record()
+    for i in range(n_ev):
+        rt.run(chunk, trace=trace[:chunk], chunk=chunk)
+        evs[i + 1].record()
+    torch.cuda.synchronize()
+    durs = [evs[i].elapsed_time(evs[i + 1]) * 1e-3 for i in range(n_ev)]
+    launch_s = float(np.mean(durs))
+    ca_cv = 0
+    for i, m in enumerate(models):
+        ca_cv += int(np.sum(np.asarray(m.ca["mask_branch"])[L.cells[i].geom.cv_branch])) if m.ca else 0
+    bytes_per_launch = (ca_cv * BYTES_CA_CV + (L.n_cv_total - ca_cv) * BYTES_PLAIN_CV) * chunk
+    achieved = bytes_per_launch / launch_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "stagger_kernel<float, NA|KDR|IL|CaL|pool, 8>", "launch_ms": round(launch_s * 1e3, 3),
+            "steps_per_launch": chunk}
+    tf = os.path.join(ROOT, "profiles", "stagger_c2_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            roof["traffic"] = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    acc = None
+    if not args.no_cpu:
+        ns = min(args.cpu_sample_steps, Wm)
+        sample = list(range(args.cpu_sample_cells))
+        cvals, cel, rms = cpu_baseline([models[i] for i in sample], [L.cells[i].probes[0] for i in sample], ns,
+                                       warm_soma[:ns, sample], dt)
+        cpu = {"value": float(f"{cvals:.4g}"), "unit": "compartment-updates/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 NumPy oracle, cells {sample} ({sum(L.cells[i].geom.n_cv for i in sample)} CVs) x "
+                         f"{ns} steps, {cel:.1f}s"}
+        acc = {"soma_rmse_mV_max": max(rms), "cells": len(sample), "steps": ns}
+    out = {
+        "metric": "compartment-updates/s", "value": float(f"{cu:.4g}"), "unit": "compartment-updates/s",
+        "n_gpus": 1, "steps": K, "warmup": Wm, "ms_per_step": round(el / K * 1e3, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"{args.cells} seeded uneven random-tree cells (200-2000 CVs), Na/KDR/leak + "
+                               f"dendritic CaL/CalciumDetailed, dt 0.025 ms, staggered plain step",
+                   "cells": args.cells, "compartments": int(L.n_cv_total), "dhs_rows": int(L.n_rows_total),
+                   "packs": L.n_packs, "threads": L.threads, "cv_per_thread": L.cv_per_thread,
+                   "steps_per_launch": chunk, "parallelism": "1 GPU, one workgroup per pack"},
+        "roofline": roof, "cpu_baseline": cpu, "accuracy": acc,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

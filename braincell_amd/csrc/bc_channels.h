@@ -1,0 +1,168 @@
+// Channel and ion-pool math shared by the staggered and single-compartment
+// kernels.  Each function restates one reference rate method:
+//
+//   Na_Ba2002   braincell/channel/sodium.py:158-170
+//   KDR_Ba2002  braincell/channel/potassium.py:206-214
+//   CaL_IS2008  braincell/channel/calcium.py:902-920
+//   CaT_HM1992  braincell/channel/calcium.py:345-373
+//   AHP_De1994  braincell/channel/potassium_calcium.py:179-185
+//   HCN_HM1992  braincell/channel/hyperpolarization_activated.py:124-132
+//   CalciumDetailed drive  braincell/ion/calcium.py:499-503
+//   Nernst E    braincell/ion/_base.py:462-469
+//   exp-Euler   braincell/quad/_exp_euler.py:462-465
+//
+// Units: mV, ms, mS/cm^2, uA/cm^2, mM.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define BC_CH_NA   (1u << 0)
+#define BC_CH_KDR  (1u << 1)
+#define BC_CH_IL   (1u << 2)
+#define BC_CH_KL   (1u << 3)
+#define BC_CH_CAL  (1u << 4)
+#define BC_CH_CAT  (1u << 5)
+#define BC_CH_AHP  (1u << 6)
+#define BC_CH_HCN  (1u << 7)
+#define BC_CH_POOL (1u << 8)
+
+// Mechanism-class table columns (per painted region of a cell).
+#define BC_CLS_DTCM   0   // dt / Cm  (ms cm^2 / uF)
+#define BC_CLS_G_NA   1   // mS/cm^2
+#define BC_CLS_G_KDR  2
+#define BC_CLS_G_IL   3
+#define BC_CLS_G_KL   4
+#define BC_CLS_G_CAL  5
+#define BC_CLS_G_CAT  6
+#define BC_CLS_G_AHP  7
+#define BC_CLS_G_HCN  8
+#define BC_CLS_CAMASK 9   // 1 where the Ca pool is painted
+#define BC_NCLS       12
+
+// Number of gate/pool states a channel set carries, and their slots.
+__host__ __device__ constexpr int bc_nstates(uint32_t ch) {
+  return ((ch & BC_CH_NA) ? 2 : 0) + ((ch & BC_CH_KDR) ? 1 : 0) + ((ch & BC_CH_CAL) ? 2 : 0) +
+         ((ch & BC_CH_CAT) ? 2 : 0) + ((ch & BC_CH_AHP) ? 1 : 0) + ((ch & BC_CH_HCN) ? 1 : 0) +
+         ((ch & BC_CH_POOL) ? 1 : 0);
+}
+__host__ __device__ constexpr int bc_slot_na(uint32_t) { return 0; }
+__host__ __device__ constexpr int bc_slot_kdr(uint32_t ch) { return ((ch & BC_CH_NA) ? 2 : 0); }
+__host__ __device__ constexpr int bc_slot_cal(uint32_t ch) { return bc_slot_kdr(ch) + ((ch & BC_CH_KDR) ? 1 : 0); }
+__host__ __device__ constexpr int bc_slot_cat(uint32_t ch) { return bc_slot_cal(ch) + ((ch & BC_CH_CAL) ? 2 : 0); }
+__host__ __device__ constexpr int bc_slot_ahp(uint32_t ch) { return bc_slot_cat(ch) + ((ch & BC_CH_CAT) ? 2 : 0); }
+__host__ __device__ constexpr int bc_slot_hcn(uint32_t ch) { return bc_slot_ahp(ch) + ((ch & BC_CH_AHP) ? 1 : 0); }
+__host__ __device__ constexpr int bc_slot_ca(uint32_t ch) { return bc_slot_hcn(ch) + ((ch & BC_CH_HCN) ? 1 : 0); }
+
+// Scalar constants of the channel set (same for every cell of a launch).
+// Temperature factors are folded into phi_* on the host.
+template <typename real>
+struct bc_chan_const {
+  real E_Na, E_K, E_IL;
+  real na_vsh, na_phi;
+  real kdr_vsh, kdr_phi;
+  real cal_vsh, cal_phi_p, cal_phi_q;
+  real cat_vsh, cat_phi_p, cat_phi_q;
+  real ahp_alpha, ahp_beta, ahp_n, ahp_phi;
+  real hcn_E, hcn_phi;
+  real ca_drive;   // 10 / (2 F d)  (mM/ms per uA/cm^2)
+  real ca_tau_inv; // 1 / tau (1/ms)
+  real ca_rest;    // C_rest (mM)
+  real ca_Co;      // Co (mM)
+  real ca_rtzf;    // 1e3 R T / (z F)  (mV)
+  real E_Ca_fixed; // used when no pool is present
+};
+
+template <typename real> struct bc_math;
+template <> struct bc_math<float> {
+  static __host__ __device__ __forceinline__ float ex(float x) { return expf(x); }
+  static __host__ __device__ __forceinline__ float em1(float x) { return expm1f(x); }
+  static __host__ __device__ __forceinline__ float lg(float x) { return logf(x); }
+  static __host__ __device__ __forceinline__ float pw(float x, float y) { return powf(x, y); }
+  static constexpr float exprel_eps = 1e-4f;
+};
+template <> struct bc_math<double> {
+  static __host__ __device__ __forceinline__ double ex(double x) { return exp(x); }
+  static __host__ __device__ __forceinline__ double em1(double x) { return expm1(x); }
+  static __host__ __device__ __forceinline__ double lg(double x) { return log(x); }
+  static __host__ __device__ __forceinline__ double pw(double x, double y) { return pow(x, y); }
+  static constexpr double exprel_eps = 1e-8;
+};
+
+// (e^x - 1) / x with its removable singularity filled by the series.
+template <typename real>
+__host__ __device__ __forceinline__ real bc_exprel(real x) {
+  if (fabs(x) < bc_math<real>::exprel_eps) return real(1) + x * (real(0.5) + x * real(1.0 / 6.0));
+  return bc_math<real>::em1(x) / x;
+}
+
+// y + dt * exprel(dt * lam) * f
+template <typename real>
+__host__ __device__ __forceinline__ real bc_expeuler(real y, real f, real lam, real dt) {
+  return y + dt * bc_exprel<real>(dt * lam) * f;
+}
+
+// Gate in alpha/beta form: f = phi (a (1 - x) - b x), lam = -phi (a + b).
+template <typename real>
+__host__ __device__ __forceinline__ real bc_gate_ab(real x, real a, real b, real phi, real dt) {
+  real f = phi * (a * (real(1) - x) - b * x);
+  real lam = -phi * (a + b);
+  return bc_expeuler<real>(x, f, lam, dt);
+}
+
+// Gate in inf/tau form: f = phi (inf - x) / tau, lam = -phi / tau.
+template <typename real>
+__host__ __device__ __forceinline__ real bc_gate_it(real x, real inf, real tau, real phi, real dt) {
+  real itau = real(1) / tau;
+  real f = phi * (inf - x) * itau;
+  real lam = -phi * itau;
+  return bc_expeuler<real>(x, f, lam, dt);
+}
+
+template <typename real>
+__host__ __device__ __forceinline__ void bc_na_rates(real V, real vsh, real& ap, real& bp, real& aq, real& bq) {
+  typedef bc_math<real> M;
+  real Vp = V - vsh;
+  ap = real(0.32 * 4.0) / bc_exprel<real>(-(Vp - real(13)) * real(0.25));
+  bp = real(0.28 * 5.0) / bc_exprel<real>((Vp - real(40)) * real(0.2));
+  aq = real(0.128) * M::ex(-(Vp - real(17)) / real(18));
+  bq = real(4) / (real(1) + M::ex(-(Vp - real(40)) * real(0.2)));
+}
+
+template <typename real>
+__host__ __device__ __forceinline__ void bc_kdr_rates(real V, real vsh, real& ap, real& bp) {
+  typedef bc_math<real> M;
+  real Vp = V - vsh;
+  ap = real(0.032 * 5.0) / bc_exprel<real>(-(Vp - real(15)) * real(0.2));
+  bp = real(0.5) * M::ex(-(Vp - real(10)) / real(40));
+}
+
+template <typename real>
+__host__ __device__ __forceinline__ void bc_cal_rates(real V, real vsh, real& pi, real& pt, real& qi, real& qt) {
+  typedef bc_math<real> M;
+  real Vp = V - vsh;
+  pi = real(1) / (real(1) + M::ex(-(Vp + real(10)) * real(0.25)));
+  real e1 = M::ex((Vp + real(5)) / real(15));
+  pt = real(0.4) + real(0.7) / (real(1) / e1 + e1);
+  qi = real(1) / (real(1) + M::ex((Vp + real(25)) * real(0.5)));
+  real e2 = M::ex((Vp + real(40)) / real(9.5));
+  qt = real(300) + real(100) / (e2 + real(1) / e2);
+}
+
+template <typename real>
+__host__ __device__ __forceinline__ void bc_cat_rates(real V, real vsh, real& pi, real& pt, real& qi, real& qt) {
+  typedef bc_math<real> M;
+  real Vp = V - vsh;
+  pi = real(1) / (real(1) + M::ex(-(Vp + real(59)) / real(6.2)));
+  pt = real(1) / (M::ex(-(Vp + real(132)) / real(16.7)) + M::ex((Vp + real(16.8)) / real(18.2))) + real(0.612);
+  qi = real(1) / (real(1) + M::ex((Vp + real(83)) * real(0.25)));
+  qt = (Vp >= real(-80)) ? (M::ex(-(Vp + real(22)) / real(10.5)) + real(28)) : M::ex((Vp + real(467)) / real(66.6));
+}
+
+template <typename real>
+__host__ __device__ __forceinline__ void bc_hcn_rates(real V, real& pi, real& pt) {
+  typedef bc_math<real> M;
+  pi = real(1) / (real(1) + M::ex((V + real(75)) / real(5.5)));
+  pt = real(1) / (M::ex(real(-0.086) * V - real(14.59)) + M::ex(real(0.0701) * V - real(1.87)));
+}

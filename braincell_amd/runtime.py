@@ -1,0 +1,581 @@
+"""Packed multi-cell runtime for the staggered kernel.
+
+``StaggerLayout`` lowers a list of :class:`~braincell_amd.model.CellModel`
+into packs (whole cells grouped per workgroup) and flat arrays;
+``StaggerRuntime`` uploads them as torch tensors on the GPU and drives the
+C ABI (``bc_stagger_run`` / ``bc_stagger_graph_create``).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import rates
+from ._native import BcPack, StaggerDesc, chan_const_type, check, lib
+from .channels import CA_POOL_BIT, CHANNELS, FARADAY, GAS_CONSTANT
+from .layout import build_chains, discretize_cell
+
+__all__ = ["StaggerLayout", "StaggerRuntime", "KERNEL_CHANNEL_SETS", "choose_channel_set", "first_step_at"]
+
+CH_ORDER = ("Na_Ba2002", "KDR_Ba2002", "IL", "K_Leak", "CaL_IS2008", "CaT_HM1992", "AHP_De1994", "HCN_HM1992")
+CLS_COL = {n: i + 1 for i, n in enumerate(CH_ORDER)}  # column in the class table
+NCLS = 12
+B = {n: CHANNELS[n].bit for n in CH_ORDER}
+KERNEL_CHANNEL_SETS = (
+    B["IL"],
+    B["Na_Ba2002"] | B["KDR_Ba2002"] | B["IL"],
+    B["Na_Ba2002"] | B["KDR_Ba2002"] | B["IL"] | B["CaL_IS2008"] | CA_POOL_BIT,
+    sum(B.values()) | CA_POOL_BIT,
+)
+INT32_MAX = 2 ** 31 - 1
+
+
+def choose_channel_set(mask: int) -> int:
+    for s in KERNEL_CHANNEL_SETS:
+        if mask & ~s == 0:
+            return s
+    raise ValueError(f"no kernel covers channel mask 0x{mask:x}")
+
+
+def state_slots(chset: int) -> list:
+    """(channel, gate index) per state slot, matching ``bc_slot_*``."""
+    out = []
+    if chset & B["Na_Ba2002"]:
+        out += [("Na_Ba2002", 0), ("Na_Ba2002", 1)]
+    if chset & B["KDR_Ba2002"]:
+        out += [("KDR_Ba2002", 0)]
+    if chset & B["CaL_IS2008"]:
+        out += [("CaL_IS2008", 0), ("CaL_IS2008", 1)]
+    if chset & B["CaT_HM1992"]:
+        out += [("CaT_HM1992", 0), ("CaT_HM1992", 1)]
+    if chset & B["AHP_De1994"]:
+        out += [("AHP_De1994", 0)]
+    if chset & B["HCN_HM1992"]:
+        out += [("HCN_HM1992", 0)]
+    if chset & CA_POOL_BIT:
+        out += [("Ca", 0)]
+    return out
+
+
+def first_step_at(t: float, dt: float) -> int:
+    """Smallest n >= 0 with n * dt >= t in float64 (t_n = n dt, ``run.py:64-131``)."""
+    if t <= 0.0:
+        return 0
+    if t / dt > INT32_MAX:
+        return INT32_MAX
+    n = max(0, int(np.ceil(t / dt)) - 1)
+    while n * dt < t:
+        n += 1
+    while n > 0 and (n - 1) * dt >= t:
+        n -= 1
+    return n
+
+
+def _phi(q10, temp, temp_ref):
+    return q10 ** ((temp - temp_ref) / 10.0)
+
+
+@dataclass
+class _CellHost:
+    geom: object
+    chains: object
+    cls_rows: np.ndarray  # [n_cls, NCLS] with column 0 = Cm (dt applied later)
+    cv_cls: np.ndarray  # [C]
+    clamps: list  # (kind, local index (cv or row), t0, t1, amp)
+    probes: np.ndarray  # CV indices
+
+
+class StaggerLayout:
+    """Host-side packed layout of a batch of cells."""
+
+    def __init__(self, models, *, threads: int = 256, cv_per_thread: int = None, rows_per_pack: int = None,
+                 probes="soma"):
+        if len(models) == 0:
+            raise ValueError("need at least one cell")
+        self.models = list(models)
+        mask = 0
+        for m in self.models:
+            m.validate()
+            for n in m.channels:
+                mask |= CHANNELS[n].bit
+            if m.ca is not None:
+                mask |= CA_POOL_BIT
+        self.channel_mask = mask
+        self.chset = choose_channel_set(mask)
+        self.slots = state_slots(self.chset)
+        self.cc = self._channel_constants()
+        self.cells = [self._lower_cell(m, probes) for m in self.models]
+        self.threads = int(threads)
+        ncv_max = max(c.geom.n_cv for c in self.cells)
+        if cv_per_thread is None:
+            cv_per_thread = 1
+            while cv_per_thread * self.threads < ncv_max:
+                cv_per_thread *= 2
+        self.cv_per_thread = int(cv_per_thread)
+        if self.cv_per_thread not in (1, 2, 4, 8, 16):
+            raise ValueError("cv_per_thread must be 1, 2, 4, 8 or 16")
+        cap_cv = self.threads * self.cv_per_thread
+        if ncv_max > cap_cv:
+            raise ValueError(f"a cell has {ncv_max} CVs > threads*cv_per_thread = {cap_cv}")
+        nrow_max = max(c.geom.n_point for c in self.cells)
+        self.rows_per_pack = int(max(nrow_max, rows_per_pack or nrow_max))
+        self.pack_cells = self._plan_packs(cap_cv)
+        self._assemble()
+
+    # ------------------------------------------------------------ constants
+    def _channel_constants(self):
+        ref = self.models[0]
+        params = {}
+        for m in self.models:
+            for n in m.channels:
+                p = m.channel_params(n)
+                p = {k: v for k, v in p.items() if k != "g_max"}
+                if n in params and params[n] != p:
+                    raise ValueError(f"{n}: channel parameters differ between cells of one batch")
+                params[n] = p
+            if (m.E_Na, m.E_K, m.E_Ca) != (ref.E_Na, ref.E_K, ref.E_Ca):
+                raise ValueError("fixed reversal potentials differ between cells of one batch")
+        pools = [m.ca for m in self.models if m.ca is not None]
+        for ca in pools:
+            if any(float(ca[k]) != float(pools[0][k]) for k in ("d", "tau", "C_rest", "Co", "temp", "Ci0")):
+                raise ValueError("Ca pool parameters differ between cells of one batch")
+        if pools:
+            for m in self.models:
+                if m.ca is None and any(CHANNELS[n].ion in ("Ca", "KCa") for n in m.channels):
+                    raise ValueError("a cell with Ca channels but no Ca pool cannot share a batch with pool cells")
+        cc = {k: 0.0 for k in ("E_Na", "E_K", "E_IL", "na_vsh", "na_phi", "kdr_vsh", "kdr_phi", "cal_vsh",
+                               "cal_phi_p", "cal_phi_q", "cat_vsh", "cat_phi_p", "cat_phi_q", "ahp_alpha",
+                               "ahp_beta", "ahp_n", "ahp_phi", "hcn_E", "hcn_phi", "ca_drive", "ca_tau_inv",
+                               "ca_rest", "ca_Co", "ca_rtzf", "E_Ca_fixed")}
+        cc["E_Na"], cc["E_K"], cc["E_Ca_fixed"] = ref.E_Na, ref.E_K, ref.E_Ca
+        dflt = {n: dict(CHANNELS[n].params) for n in CH_ORDER}
+        for n in CH_ORDER:
+            dflt[n].update(params.get(n, {}))
+        na, kd, cl, ct, ah, hc = (dflt[n] for n in ("Na_Ba2002", "KDR_Ba2002", "CaL_IS2008", "CaT_HM1992",
+                                                   "AHP_De1994", "HCN_HM1992"))
+        cc["E_IL"] = dflt["IL"]["E"]
+        cc["na_vsh"], cc["na_phi"] = na["V_sh"], _phi(na["q10"], na["temp"], na["temp_ref"])
+        cc["kdr_vsh"], cc["kdr_phi"] = kd["V_sh"], _phi(kd["q10"], kd["temp"], kd["temp_ref"])
+        cc["cal_vsh"] = cl["V_sh"]
+        cc["cal_phi_p"] = _phi(cl["q10_p"], cl["temp"], cl["temp_ref_p"])
+        cc["cal_phi_q"] = _phi(cl["q10_q"], cl["temp"], cl["temp_ref_q"])
+        cc["cat_vsh"] = ct["V_sh"]
+        cc["cat_phi_p"] = _phi(ct["q10_p"], ct["temp"], ct["temp_ref_p"])
+        cc["cat_phi_q"] = _phi(ct["q10_q"], ct["temp"], ct["temp_ref_q"])
+        cc["ahp_alpha"], cc["ahp_beta"], cc["ahp_n"], cc["ahp_phi"] = ah["alpha"], ah["beta"], ah["n"], ah["phi"]
+        cc["hcn_E"], cc["hcn_phi"] = hc["E"], _phi(hc["q10"], hc["temp"], hc["temp_ref"])
+        if pools:
+            ca = pools[0]
+            cc["ca_drive"] = 10.0 / (2.0 * FARADAY * float(ca["d"]))
+            cc["ca_tau_inv"] = 1.0 / float(ca["tau"])
+            cc["ca_rest"] = float(ca["C_rest"])
+            cc["ca_Co"] = float(ca["Co"])
+            cc["ca_rtzf"] = 1e3 * GAS_CONSTANT * float(ca["temp"]) / (2.0 * FARADAY)
+        return cc
+
+    # ---------------------------------------------------------------- cells
+    def _lower_cell(self, m, probes) -> _CellHost:
+        ncvb = m.n_cv_branch()
+        geom = discretize_cell(m.morph, ncvb, m.per_branch(m.cm), m.per_branch(m.ra))
+        chains = build_chains(geom.parent_rows)
+        nb = m.n_branches
+        table = np.zeros((nb, NCLS))
+        table[:, 0] = m.per_branch(m.cm)
+        for n, spec in m.channels.items():
+            table[:, CLS_COL[n]] = np.asarray(spec["g_branch"], dtype=np.float64) * 1e3  # mS/cm^2
+        if m.ca is not None:
+            table[:, 9] = np.asarray(m.ca["mask_branch"], dtype=np.float64)
+        uniq, inv = np.unique(table, axis=0, return_inverse=True)
+        cv_cls = inv.reshape(-1)[geom.cv_branch]
+        clamps = []
+        for (b, x, t0, dur, amp) in m.clamps:
+            kind, idx = geom.locate(int(b), float(x), ncvb)
+            clamps.append((kind, idx, float(t0), float(t0) + float(dur), float(amp)))
+        if isinstance(probes, str):
+            if probes == "soma":
+                kind, idx = geom.locate(0, 0.5, ncvb)
+                pr = np.asarray([idx if kind == "mid" else 0])
+            elif probes == "all":
+                pr = np.arange(geom.n_cv)
+            elif probes == "none":
+                pr = np.zeros(0, dtype=np.int64)
+            else:
+                raise ValueError(f"unknown probe spec {probes!r}")
+        else:
+            pr = np.asarray(probes, dtype=np.int64)
+        return _CellHost(geom, chains, uniq, cv_cls, clamps, pr)
+
+    def _plan_packs(self, cap_cv):
+        """First-fit decreasing by row count, under the row and CV budgets."""
+        order = np.argsort([-c.geom.n_point for c in self.cells], kind="stable")
+        packs, rows, cvs = [], [], []
+        for i in order.tolist():
+            r, c = self.cells[i].geom.n_point, self.cells[i].geom.n_cv
+            for p in range(len(packs)):
+                if rows[p] + r <= self.rows_per_pack and cvs[p] + c <= cap_cv:
+                    packs[p].append(i)
+                    rows[p] += r
+                    cvs[p] += c
+                    break
+            else:
+                packs.append([i])
+                rows.append(r)
+                cvs.append(c)
+        return packs
+
+    # ------------------------------------------------------------- assemble
+    def _assemble(self):
+        n_cells = len(self.cells)
+        packs = []
+        cv_info, bnd_info = [], []
+        chain_off, chain_top, kid_cnt, kids, level_off = [0], [], [], [], []
+        cr_row, cr_lo, cr_up, cr_dg = [], [], [], []
+        cls_tab, clamp_rows, clamp_t, clamp_amp_raw, clamp_kind, clamp_area = [], [], [], [], [], []
+        probe_row, probe_col = [], []
+        self.cell_cv0 = np.zeros(n_cells, dtype=np.int64)
+        self.cell_ncv = np.zeros(n_cells, dtype=np.int64)
+        self.cell_probe0 = np.zeros(n_cells + 1, dtype=np.int64)
+        probe_counts = np.asarray([len(c.probes) for c in self.cells])
+        self.cell_probe0[1:] = np.cumsum(probe_counts)
+        gcv = grow = gch = glv = gbnd = gcls = gclamp = gprobe = 0
+        for cells in self.pack_cells:
+            P = BcPack()
+            P.cv0, P.row0, P.ch0, P.lv0, P.bnd0, P.cls0, P.clamp0, P.probe0 = gcv, grow, gch, glv, gbnd, gcls, gclamp, gprobe
+            row_off = cv_off = cls_off = 0
+            nlev = max(self.cells[i].chains.n_levels for i in cells)
+            per_level = [[] for _ in range(nlev)]
+            cell_row_off = {}
+            for i in cells:
+                c = self.cells[i]
+                g = c.geom
+                self.cell_cv0[i] = gcv + cv_off
+                self.cell_ncv[i] = g.n_cv
+                cell_row_off[i] = row_off
+                rows = g.dynamic_rows + row_off
+                info = rows.astype(np.uint64) | ((c.cv_cls + cls_off).astype(np.uint64) << np.uint64(16))
+                if np.any(c.cv_cls + cls_off >= (1 << 14)):
+                    raise ValueError("too many mechanism classes in one pack")
+                clamp_cv = np.zeros(g.n_cv, dtype=bool)
+                clamp_bnd_rows = set()
+                for (kind, idx, t0, t1, amp) in c.clamps:
+                    if kind == "mid":
+                        clamp_cv[idx] = True
+                        clamp_rows.append(int(g.dynamic_rows[idx] + row_off))
+                        clamp_area.append(g.cv_area_um2[idx] * 1e-8)
+                    else:
+                        clamp_bnd_rows.add(int(idx))
+                        clamp_rows.append(int(idx + row_off))
+                        clamp_area.append(0.0)
+                    clamp_kind.append(kind)
+                    clamp_t.append((t0, t1))
+                    clamp_amp_raw.append(amp)
+                pflag = np.zeros(g.n_cv, dtype=bool)
+                pflag[c.probes] = True
+                info |= (pflag.astype(np.uint64) << np.uint64(30)) | (clamp_cv.astype(np.uint64) << np.uint64(31))
+                cv_info.append(info.astype(np.uint32))
+                for k, cvi in enumerate(c.probes.tolist()):
+                    probe_row.append(int(g.dynamic_rows[cvi] + row_off))
+                    probe_col.append(int(self.cell_probe0[i] + k))
+                brows = np.nonzero(~g.row_is_mid)[0]
+                bflag = np.asarray([r in clamp_bnd_rows for r in brows.tolist()], dtype=np.uint64)
+                bnd_info.append(((brows + row_off).astype(np.uint64) | (bflag << np.uint64(31))).astype(np.uint32))
+                for lv in range(c.chains.n_levels):
+                    for k in range(c.chains.level_off[lv], c.chains.level_off[lv + 1]):
+                        per_level[lv].append((i, k))
+                cls_tab.append(c.cls_rows)
+                row_off += g.n_point
+                cv_off += g.n_cv
+                cls_off += c.cls_rows.shape[0]
+            # merged chain numbering inside the pack
+            local = {}
+            lvl_bounds = [0]
+            for lv in range(nlev):
+                for key in per_level[lv]:
+                    local[key] = len(local)
+                lvl_bounds.append(len(local))
+            ordered = [None] * len(local)
+            for key, idx in local.items():
+                ordered[idx] = key
+            for (i, k) in ordered:
+                ch = self.cells[i].chains
+                g = self.cells[i].geom
+                ro = cell_row_off[i]
+                rws = ch.rows[ch.off[k]:ch.off[k + 1]]
+                cr_row.append(rws + ro)
+                cr_lo.append(g.lowers[rws])
+                cr_up.append(g.uppers[rws])
+                cr_dg.append(g.diag_axial[rws])
+                chain_off.append(chain_off[-1] + len(rws))
+                tp = ch.top_parent[k]
+                chain_top.append(tp + ro if tp >= 0 else -1)
+                kk = ch.kids[ch.kid_off[k]:ch.kid_off[k + 1]]
+                kid_cnt.append(len(kk))
+                kids.extend(local[(i, int(x))] for x in kk.tolist())
+            level_off.extend(lvl_bounds)
+            P.n_cv = cv_off
+            P.n_rows = row_off
+            P.n_ch = len(local)
+            P.n_lv = nlev
+            P.n_bnd = int(sum(len(b) for b in bnd_info)) - gbnd
+            P.n_cls = cls_off
+            P.n_clamp = len(clamp_rows) - gclamp
+            P.n_probe = len(probe_row) - gprobe
+            packs.append(P)
+            gcv += cv_off
+            grow += row_off
+            gch += len(local)
+            glv += nlev + 1
+            gbnd += P.n_bnd
+            gcls += cls_off
+            gclamp += P.n_clamp
+            gprobe += P.n_probe
+        self.packs = (BcPack * len(packs))(*packs)
+        self.pack_array = np.frombuffer(bytes(self.packs), dtype=np.int32).reshape(len(packs), 16).copy()
+        self.n_packs = len(packs)
+        self.n_cv_total = gcv
+        self.n_rows_total = grow
+        self.cv_info = np.concatenate(cv_info)
+        self.bnd_info = np.concatenate(bnd_info) if bnd_info else np.zeros(0, np.uint32)
+        self.chain_off = np.asarray(chain_off, dtype=np.int32)
+        self.chain_top = np.asarray(chain_top, dtype=np.int32)
+        self.kid_off = np.concatenate(([0], np.cumsum(kid_cnt))).astype(np.int32)
+        self.kids = np.asarray(kids, dtype=np.int32)
+        self.level_off = np.asarray(level_off, dtype=np.int32)
+        self.cr_row = np.concatenate(cr_row).astype(np.int32)
+        self.cr_lo = np.concatenate(cr_lo)
+        self.cr_up = np.concatenate(cr_up)
+        self.cr_dg = np.concatenate(cr_dg)
+        self.cls_tab = np.concatenate(cls_tab, axis=0)
+        self.clamp_row = np.asarray(clamp_rows, dtype=np.int32)
+        self.clamp_t = np.asarray(clamp_t, dtype=np.float64).reshape(-1, 2)
+        self.clamp_amp_raw = np.asarray(clamp_amp_raw, dtype=np.float64)
+        self.clamp_kind = clamp_kind
+        self.clamp_area = np.asarray(clamp_area, dtype=np.float64)
+        self.probe_row = np.asarray(probe_row, dtype=np.int32)
+        self.probe_col = np.asarray(probe_col, dtype=np.int32)
+        self.n_probes = int(self.cell_probe0[-1])
+        p = self.pack_array
+        self.max_rows = int(p[:, 3].max())
+        self.max_chains = int(p[:, 5].max())
+        self.max_cls = int(p[:, 11].max())
+
+    # ------------------------------------------------------------ dt-tables
+    def dt_tables(self, dt: float):
+        """Arrays that depend on dt: dt-scaled axial coefficients, dt/Cm in the
+        class table, clamp step windows and clamp right-hand-side terms."""
+        cls = self.cls_tab.copy()
+        cls[:, 0] = dt / cls[:, 0]
+        on = np.asarray([first_step_at(t0, dt) for t0, _ in self.clamp_t], dtype=np.int64)
+        off = np.asarray([first_step_at(t1, dt) for _, t1 in self.clamp_t], dtype=np.int64)
+        amp = np.empty(len(self.clamp_kind))
+        for e, kind in enumerate(self.clamp_kind):
+            if kind == "mid":
+                amp[e] = self.clamp_amp_raw[e] * 1e-3 / self.clamp_area[e]  # uA/cm^2
+            else:
+                amp[e] = dt * self.clamp_amp_raw[e] * 1e-3  # mV: dt * I[nA] / 1 uF
+        return {"cr_lo": dt * self.cr_lo, "cr_up": dt * self.cr_up, "cr_dg": dt * self.cr_dg, "cls_tab": cls,
+                "clamp_on": np.minimum(on, INT32_MAX).astype(np.int32),
+                "clamp_off": np.minimum(off, INT32_MAX).astype(np.int32), "clamp_amp": amp}
+
+    # ---------------------------------------------------------- init state
+    def initial_state(self, V0=None):
+        """reset_state: V at V0 (per cell model unless given), gates at their
+        steady state, Ci at its initializer.  Returns (V[n_cv], S[ns, n_cv])."""
+        V = np.empty(self.n_cv_total)
+        for i, m in enumerate(self.models):
+            v = m.V0 if V0 is None else (V0[i] if np.ndim(V0) else V0)
+            V[self.cell_cv0[i]:self.cell_cv0[i] + self.cell_ncv[i]] = v
+        S = np.zeros((len(self.slots), self.n_cv_total))
+        cc = self.cc
+        pools = [m.ca for m in self.models if m.ca is not None]
+        ci0 = np.full(self.n_cv_total, float(pools[0]["Ci0"]) if pools else 0.0)
+        for s, (name, k) in enumerate(self.slots):
+            if name == "Na_Ba2002":
+                S[s] = rates.na_steady(V, cc["na_vsh"])[k]
+            elif name == "KDR_Ba2002":
+                S[s] = rates.kdr_steady(V, cc["kdr_vsh"])
+            elif name == "CaL_IS2008":
+                S[s] = rates.cal_steady(V, cc["cal_vsh"])[k]
+            elif name == "CaT_HM1992":
+                S[s] = rates.cat_steady(V, cc["cat_vsh"])[k]
+            elif name == "AHP_De1994":
+                S[s] = rates.ahp_steady(ci0, cc["ahp_alpha"], cc["ahp_beta"], cc["ahp_n"])
+            elif name == "HCN_HM1992":
+                S[s] = rates.hcn_steady(V)
+            elif name == "Ca":
+                S[s] = ci0
+        return V, S
+
+    def cell_slice(self, i: int) -> slice:
+        return slice(int(self.cell_cv0[i]), int(self.cell_cv0[i] + self.cell_ncv[i]))
+
+    def summary(self) -> dict:
+        p = self.pack_array
+        return {"cells": len(self.cells), "packs": self.n_packs, "cv_total": int(self.n_cv_total),
+                "rows_total": int(self.n_rows_total), "max_rows": self.max_rows, "max_levels": int(p[:, 7].max()),
+                "threads": self.threads, "cv_per_thread": self.cv_per_thread, "channel_set": hex(self.chset)}
+
+
+class StaggerRuntime:
+    """Device buffers + C-ABI calls for a :class:`StaggerLayout`."""
+
+    def __init__(self, layout: StaggerLayout, dt: float, *, precision: int = 32, device="cuda"):
+        import torch
+
+        if precision not in (32, 64):
+            raise ValueError("precision must be 32 or 64")
+        self.torch = torch
+        self.L = layout
+        self.precision = precision
+        self.dtype = torch.float32 if precision == 32 else torch.float64
+        self.device = torch.device(device)
+        self.dt = float(dt)
+        lib()  # fail loudly when the native library is missing
+        if not lib().bc_stagger_supported(precision, layout.chset, layout.cv_per_thread):
+            raise RuntimeError(f"no kernel for channel set {hex(layout.chset)} / cv_per_thread "
+                               f"{layout.cv_per_thread}")
+        t = self._t
+        self.packs = t(layout.pack_array.reshape(-1), torch.int32)
+        self.cv_info = t(layout.cv_info.view(np.int32), torch.int32)
+        self.bnd_info = t(layout.bnd_info.view(np.int32), torch.int32)
+        self.chain_off = t(layout.chain_off, torch.int32)
+        self.chain_top = t(layout.chain_top, torch.int32)
+        self.kid_off = t(layout.kid_off, torch.int32)
+        self.kids = t(layout.kids if layout.kids.size else np.zeros(1, np.int32), torch.int32)
+        self.level_off = t(layout.level_off, torch.int32)
+        self.cr_row = t(layout.cr_row, torch.int32)
+        self.probe_row = t(layout.probe_row if layout.probe_row.size else np.zeros(1, np.int32), torch.int32)
+        self.probe_col = t(layout.probe_col if layout.probe_col.size else np.zeros(1, np.int32), torch.int32)
+        self.clamp_row = t(layout.clamp_row if layout.clamp_row.size else np.zeros(1, np.int32), torch.int32)
+        self.status = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self._set_dt_tables()
+        CC = chan_const_type(precision)
+        self._cc = CC(**{k: float(v) for k, v in layout.cc.items()})
+        V, S = layout.initial_state()
+        self.V = t(V, self.dtype)
+        self.S = t(S.reshape(-1) if S.size else np.zeros(1), self.dtype)
+        self.step_index = 0
+        self._graphs = {}
+
+    def _t(self, arr, dtype):
+        return self.torch.as_tensor(np.ascontiguousarray(arr)).to(device=self.device, dtype=dtype)
+
+    def _set_dt_tables(self):
+        tb = self.L.dt_tables(self.dt)
+        t = self._t
+        self.cr_lo = t(tb["cr_lo"], self.dtype)
+        self.cr_up = t(tb["cr_up"], self.dtype)
+        self.cr_dg = t(tb["cr_dg"], self.dtype)
+        self.cls_tab = t(tb["cls_tab"].reshape(-1), self.dtype)
+        n = max(1, tb["clamp_on"].size)
+        self.clamp_on = t(tb["clamp_on"] if tb["clamp_on"].size else np.zeros(n, np.int32), self.torch.int32)
+        self.clamp_off = t(tb["clamp_off"] if tb["clamp_off"].size else np.zeros(n, np.int32), self.torch.int32)
+        self.clamp_amp = t(tb["clamp_amp"] if tb["clamp_amp"].size else np.zeros(n), self.dtype)
+
+    def reset_state(self, V0=None):
+        V, S = self.L.initial_state(V0)
+        self.V.copy_(self._t(V, self.dtype))
+        if S.size:
+            self.S.copy_(self._t(S.reshape(-1), self.dtype))
+        self.step_index = 0
+        self.status.zero_()
+
+    def desc(self, n_steps: int, trace=None, step0: int = None) -> StaggerDesc:
+        L = self.L
+        d = StaggerDesc()
+        d.precision = self.precision
+        d.channels = L.chset
+        d.n_packs = L.n_packs
+        d.threads = L.threads
+        d.cv_per_thread = L.cv_per_thread
+        d.max_rows = L.max_rows
+        d.max_chains = L.max_chains
+        d.max_cls = L.max_cls
+        d.n_cv_total = L.n_cv_total
+        p = lambda x: ctypes.c_void_p(x.data_ptr())
+        d.packs, d.cv_info, d.bnd_info = p(self.packs), p(self.cv_info), p(self.bnd_info)
+        d.chain_off, d.chain_top, d.kid_off, d.kids = p(self.chain_off), p(self.chain_top), p(self.kid_off), p(self.kids)
+        d.level_off, d.cr_row = p(self.level_off), p(self.cr_row)
+        d.cr_lo, d.cr_up, d.cr_dg, d.cls_tab = p(self.cr_lo), p(self.cr_up), p(self.cr_dg), p(self.cls_tab)
+        d.clamp_row, d.clamp_on, d.clamp_off, d.clamp_amp = p(self.clamp_row), p(self.clamp_on), p(self.clamp_off), p(self.clamp_amp)
+        d.probe_row, d.probe_col = p(self.probe_row), p(self.probe_col)
+        d.chan_const = ctypes.cast(ctypes.pointer(self._cc), ctypes.c_void_p)
+        d.V, d.state = p(self.V), p(self.S)
+        if trace is not None:
+            if trace.dtype != self.dtype or trace.shape[-1] != L.n_probes or not trace.is_contiguous():
+                raise ValueError("trace must be a contiguous [n_steps, n_probes] tensor of the runtime dtype")
+            if trace.shape[0] < n_steps:
+                raise ValueError("trace has fewer rows than steps")
+            d.trace = p(trace)
+        elif L.n_probes > 0 and n_steps > 0:
+            raise ValueError("a trace buffer is required when probes exist")
+        d.trace_ld = L.n_probes
+        d.status = p(self.status)
+        d.step0 = self.step_index if step0 is None else int(step0)
+        d.n_steps = int(n_steps)
+        d.dt = self.dt
+        return d
+
+    def stream_handle(self):
+        return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
+
+    def new_trace(self, n_steps: int):
+        return self.torch.empty((n_steps, self.L.n_probes), dtype=self.dtype, device=self.device)
+
+    def run(self, n_steps: int, trace=None, chunk: int = None):
+        """Advance ``n_steps`` steps; returns the probe trace [n_steps, n_probes]."""
+        if trace is None:
+            trace = self.new_trace(n_steps)
+        chunk = n_steps if chunk is None else int(chunk)
+        done = 0
+        while done < n_steps:
+            k = min(chunk, n_steps - done)
+            d = self.desc(k, trace[done:] if self.L.n_probes else None)
+            self._keep = d
+            check(lib().bc_stagger_run(ctypes.byref(d), self.stream_handle()))
+            done += k
+            self.step_index += k
+        return trace
+
+    def make_graph(self, n_chunks: int, steps_per_chunk: int, trace):
+        """Capture ``n_chunks`` launches into a hipGraph that starts at the
+        current step index; replays always restart from that index."""
+        d = self.desc(steps_per_chunk, trace)
+        h = ctypes.c_void_p()
+        check(lib().bc_stagger_graph_create(ctypes.byref(d), n_chunks, steps_per_chunk, self.stream_handle(),
+                                            ctypes.byref(h)))
+        return _Graph(h, d)
+
+    def soma_trace_cols(self):
+        return self.L.cell_probe0[:-1]
+
+    def state_of_cell(self, i: int):
+        sl = self.L.cell_slice(i)
+        ncv = self.L.n_cv_total
+        V = self.V[sl].double().cpu().numpy()
+        S = self.S.view(len(self.L.slots), ncv)[:, sl].double().cpu().numpy() if self.L.slots else None
+        return V, S
+
+    def check_status(self):
+        if int(self.status.item()) != 0:
+            from ._native import NativeError
+            raise NativeError(4, "non-finite membrane voltage detected")
+
+
+class _Graph:
+    def __init__(self, handle, desc):
+        self.h = handle
+        self.desc = desc  # keep chan_const pointer alive
+
+    def launch(self, stream):
+        check(lib().bc_graph_launch(self.h, stream))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().bc_graph_destroy(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,145 @@
+/*
+ * braincell_amd C ABI: MI355X (gfx950) kernels for braincell's staggered
+ * multi-compartment step and independent exp-Euler point neurons.
+ *
+ * Conventions: every entry point returns an int status (BC_OK = 0) and
+ * leaves a thread-local message for bc_last_error().  Pointers are device
+ * pointers (allocated by the caller, e.g. torch) unless stated otherwise.
+ * No entry point allocates device memory except bc_graph_* (which owns a
+ * captured hipGraphExec).  A stream argument of NULL means the default
+ * stream.  Calls on one stream run in order; calls that share state buffers
+ * must not run concurrently.
+ */
+#ifndef BRAINCELL_AMD_H
+#define BRAINCELL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BC_OK = 0,
+  BC_INVALID_ARG = 1,
+  BC_INVALID_LAYOUT = 2,
+  BC_HIP_ERROR = 3,
+  BC_NONFINITE = 4,
+  BC_UNSUPPORTED = 5
+};
+
+/* Channel-set bits (which mechanisms a launch computes). */
+#define BC_CH_NA   (1u << 0) /* Na_Ba2002 */
+#define BC_CH_KDR  (1u << 1) /* KDR_Ba2002 */
+#define BC_CH_IL   (1u << 2) /* IL */
+#define BC_CH_KL   (1u << 3) /* K_Leak */
+#define BC_CH_CAL  (1u << 4) /* CaL_IS2008 */
+#define BC_CH_CAT  (1u << 5) /* CaT_HM1992 */
+#define BC_CH_AHP  (1u << 6) /* AHP_De1994 */
+#define BC_CH_HCN  (1u << 7) /* HCN_HM1992 */
+#define BC_CH_POOL (1u << 8) /* CalciumDetailed */
+
+/* Per-pack header: one workgroup integrates one pack (a forest of whole
+ * cells).  All *0 fields are global offsets into the corresponding arrays;
+ * rows, chains and mechanism classes inside a pack are numbered locally. */
+typedef struct bc_pack {
+  int32_t cv0, n_cv;       /* CV range: cv_info / state arrays          */
+  int32_t row0, n_rows;    /* DHS rows of the pack                      */
+  int32_t ch0, n_ch;       /* chain range: chain_off / chain_top / kids */
+  int32_t lv0, n_lv;       /* level_off[lv0 .. lv0 + n_lv] (pack chains)*/
+  int32_t bnd0, n_bnd;     /* boundary (algebraic) rows: bnd_info       */
+  int32_t cls0, n_cls;     /* mechanism classes: cls_tab rows           */
+  int32_t clamp0, n_clamp; /* clamp table entries                       */
+  int32_t probe0, n_probe; /* probe table entries                       */
+} bc_pack;
+
+/* Plain staggered run: n_steps steps of the staggered scheme for every pack.
+ *
+ * Arrays (global index ranges come from bc_pack):
+ *  cv_info[cv]    row | class << 16 (14 bits) | probe << 30 | clamp << 31
+ *  bnd_info[i]    row | clamp << 31
+ *  chain_off[k]   chain k owns chain rows chain_off[k] .. chain_off[k+1]-1
+ *                 (size n_chains_total + 1, global chain-row indices)
+ *  chain_top[k]   local row the chain's top hangs from, -1 for a root
+ *  kid_off[k]     kids of chain k: kids[kid_off[k] .. kid_off[k+1]-1]
+ *                 (local chain indices; size n_chains_total + 1)
+ *  level_off[.]   per pack n_lv + 1 local chain offsets
+ *  cr_row[j]      local row of chain row j (bottom to top)
+ *  cr_lo/up/dg[j] dt * lowers, dt * uppers, dt * diag_axial of that row
+ *  cls_tab[c*12+k] mechanism class table (dt/Cm, g per channel, Ca mask)
+ *  clamp_row/on/off/amp[i]  local row, active step range [on, off), and
+ *                 amplitude (uA/cm^2 on a CV row; mV rhs increment on a
+ *                 boundary row)
+ *  probe_row/col[i] local CV row and trace column
+ *  V[cv], state[s * n_cv_total + cv]  voltage and gate/pool states (in/out)
+ *  trace[step * trace_ld + col]  probe voltages after each step (out)
+ *  status         OR-ed with 1 when a non-finite voltage appears (out)
+ * real = float (precision 32) or double (precision 64). */
+typedef struct bc_stagger_desc {
+  int32_t precision;    /* 32 or 64 */
+  uint32_t channels;    /* BC_CH_* set compiled into the kernel */
+  int32_t n_packs;
+  int32_t threads;      /* workgroup size (64..1024, multiple of 64) */
+  int32_t cv_per_thread;/* register slots per thread: 1, 2, 4, 8 or 16 */
+  int32_t max_rows, max_chains, max_cls; /* LDS sizing (max over packs) */
+  int64_t n_cv_total;
+  const bc_pack* packs;
+  const uint32_t* cv_info;
+  const uint32_t* bnd_info;
+  const int32_t* chain_off;
+  const int32_t* chain_top;
+  const int32_t* kid_off;
+  const int32_t* kids;
+  const int32_t* level_off;
+  const int32_t* cr_row;
+  const void* cr_lo;
+  const void* cr_up;
+  const void* cr_dg;
+  const void* cls_tab;
+  const int32_t* clamp_row;
+  const int32_t* clamp_on;
+  const int32_t* clamp_off;
+  const void* clamp_amp;
+  const int32_t* probe_row;
+  const int32_t* probe_col;
+  const void* chan_const; /* host pointer to bc_chan_const<real> (passed by value) */
+  void* V;
+  void* state;
+  void* trace;
+  int64_t trace_ld;
+  uint32_t* status;
+  int32_t step0;   /* global index of the first step (clamp windows) */
+  int32_t n_steps;
+  double dt;       /* ms */
+} bc_stagger_desc;
+
+/* Library identity and diagnostics. */
+int bc_version(void);
+const char* bc_last_error(void);
+/* Channel set of a precision supported by bc_stagger_run; returns 1/0. */
+int bc_stagger_supported(int precision, uint32_t channels, int cv_per_thread);
+
+/* Runs n_steps staggered steps for every pack (one launch).
+ * Replaces, per step, braincell's staggered_step
+ * (braincell/quad/_staggered.py:47-154): cache_ion_total_currents
+ * (_multi_compartment/cell.py:2261-2280), dhs_voltage_step
+ * (_staggered.py:187-320: _build_dhs_numeric_state :449-513, comp_triang_raw
+ * :641-657, comp_backsub_raw :742-760) and _update_ion_channel_families
+ * (cell.py:2157-2217) with ind_exp_euler_step (quad/_exp_euler.py:330-478). */
+int bc_stagger_run(const bc_stagger_desc* desc, void* stream);
+
+/* hipGraph of a multi-chunk run: captures n_chunks launches of
+ * steps_per_chunk steps each (chunk c starts at desc->step0 + c *
+ * steps_per_chunk and writes trace rows from c * steps_per_chunk).
+ * Replaces the jitted scan of braincell's run loop
+ * (_multi_compartment/run.py:108-134). */
+int bc_stagger_graph_create(const bc_stagger_desc* desc, int32_t n_chunks, int32_t steps_per_chunk,
+                            void* stream, void** graph_out);
+int bc_graph_launch(void* graph, void* stream);
+int bc_graph_destroy(void* graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BRAINCELL_AMD_H */

@@ -1,0 +1,30 @@
+"""FP64 CPU oracle for braincell's staggered multi-compartment step.
+
+TEST INFRASTRUCTURE ONLY.  Nothing under ``braincell_amd/`` imports, links or
+executes anything here.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker or as
+the timed CPU baseline, never as a product path.
+
+The oracle restates the reference algorithm function by function; each
+function's docstring cites the reference ``file:line`` it follows.  The
+reference itself (JAX + brainstate + brainunit, Python >= 3.11) cannot run
+in this environment, so the restatement is pinned by the reference's own
+known answers, closed forms, dense linear algebra and finite differences
+(see ``tests/test_oracle_*.py``).
+
+Parity status per module:
+
+- ``discretize``: pinned (peel-level known answer, backsub-jump known
+  answers, scheduling invariants, frustum closed forms, Schur reduction).
+- ``dhs``: pinned (dense ``numpy.linalg.solve``, ``scipy.linalg.solve_banded``
+  on unbranched cables, the textbook sequential Hines sweep).
+- ``channels``: pinned (HH steady state, exp-Euler closed form
+  ``x_inf + (x - x_inf) exp(-dt (a+b) phi)``, finite-difference Jacobians,
+  Nernst formula).  The physical constants F and R are CODATA 2018 values;
+  brainunit's exact constants are parity unpinned.
+- ``staggered``: pinned on the passive cable (dense implicit-Euler solve) and
+  on the step-order invariants; the full active trajectory is restatement
+  pinned (no reference trajectory exists to compare against).
+- ``single``: pinned (leak RC closed form, dV/dt = I/C known answer,
+  exp-Euler on linear decay = exp(-0.01)).
+"""

@@ -1,0 +1,272 @@
+"""Oracle restatement (FP64) of braincell's staggered multi-compartment step.
+Test infrastructure only.
+
+One step (``braincell/quad/_staggered.py:126-154``), default ``"family"``
+update order:
+
+1. cache the Ca ion's total current at (V_n, gates_n, Ci_n)
+   (``_multi_compartment/cell.py:2261-2280``, ``_base_ion.py:325-364``);
+2. linearise the membrane current around V_n (point linearizer,
+   ``cell.py:1907-1963`` over ``currents.py:63-103``; clamp density
+   ``currents.py:166-193``, ``_compute/layouts.py:960-982``);
+   for ohmic channels ``linear = -sum(g_eff)/C`` and ``const = f - V linear``
+   (``_staggered.py:763-795``);
+3. DHS implicit-Euler solve (``_staggered.py:193-320``, see ``oracle.dhs``);
+4. Ca self-state exp-Euler with the cached current
+   (``cell.py:2093-2119, 2168-2174``, ``ion/_base.py:490-506``);
+5. every channel's gates by ind_exp_euler at V_{n+1} with ion infos that
+   already carry Ci_{n+1} (``cell.py:2189-2201``).
+
+The model description is a duck-typed object (see
+``braincell_amd.cell.CellModel``) carrying the morphology, CV policy, cable
+constants, per-branch channel densities, the Ca pool, fixed reversal
+potentials and current clamps.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import channels as ch
+from .discretize import EPS_PARAM, discretize, locate_branch_cv_by_x
+from .dhs import dhs_solve
+
+# Channels that read the Ca ion (their current feeds the Ca pool).
+CA_CHANNELS = ("CaL_IS2008", "CaT_HM1992")
+
+
+def _default(params, key, value):
+    return float(params.get(key, value))
+
+
+class StaggeredOracle:
+    """Single-cell FP64 staggered integrator over a ``CellModel``."""
+
+    def __init__(self, model):
+        self.model = model
+        cvs, bids, tree, sched, st = discretize(model.morph, model.cv_per_branch, model.cm, model.ra)
+        self.cvs, self.bids, self.tree, self.sched, self.st = cvs, bids, tree, sched, st
+        C = len(cvs)
+        self.C = C
+        self.area_cm2 = np.asarray([cv.area_um2 * 1e-8 for cv in cvs])
+        self.cm = np.asarray([cv.cm for cv in cvs])
+        branch_of = np.asarray([cv.branch_id for cv in cvs])
+        self.branch_of = branch_of
+        # channel densities per CV in mS/cm^2
+        self.g = {}
+        self.params = {}
+        for name, spec in model.channels.items():
+            gb = np.asarray(spec["g_branch"], dtype=np.float64)
+            self.g[name] = gb[branch_of] * 1e3
+            self.params[name] = dict(spec.get("params", {}))
+        self.E_Na = float(model.E_Na)
+        self.E_K = float(model.E_K)
+        self.ca = None
+        if model.ca is not None:
+            self.ca = dict(model.ca)
+            self.ca_mask = np.asarray(model.ca["mask_branch"], dtype=bool)[branch_of]
+        # clamps -> midpoint CV or boundary point (node_build.py:232-243)
+        self.mid_clamps = []  # (cv, delay, dur, amp)
+        self.edge_clamps = []  # (point_id, delay, dur, amp)
+        for (b, x, delay, dur, amp) in model.clamps:
+            kind, idx = self._locate(b, x)
+            (self.mid_clamps if kind == "mid" else self.edge_clamps).append((idx, delay, dur, amp))
+        self.V = None
+
+    # ----------------------------------------------------------------- layout
+    def _locate(self, b, x):
+        """mechanism.py:506-511 + node_build.py:232-243: the CV owning x; a
+        'prox'/'dist' role that is a branch endpoint maps to that endpoint
+        node, an internal CV boundary falls back to the owning midpoint."""
+        ids = self.bids[b]
+        if x <= 0.0 + EPS_PARAM:
+            return "edge", int(self.tree.branch_endpoint_node[b, 0])
+        if x >= 1.0 - EPS_PARAM:
+            return "edge", int(self.tree.branch_endpoint_node[b, 1])
+        cid = locate_branch_cv_by_x(ids, self.cvs, x)
+        return "mid", int(cid)
+
+    # ------------------------------------------------------------------ state
+    def reset_state(self, V0):
+        """cell.py:2529-2550: gates at steady state of V0, Ci at initializer."""
+        V0 = np.broadcast_to(np.asarray(V0, dtype=np.float64), (self.C,)).copy()
+        self.V = V0
+        self.gates = {}
+        for name in self.g:
+            self.gates[name] = self._steady(name, V0, None)
+        if self.ca is not None:
+            self.Ci = np.full(self.C, float(self.ca["Ci0"]))
+        else:
+            self.Ci = None
+        if "AHP_De1994" in self.g:
+            self.gates["AHP_De1994"] = self._steady("AHP_De1994", V0, self.Ci)
+
+    def _phi(self, name, which=""):
+        p = self.params[name]
+        if name in ("Na_Ba2002", "KDR_Ba2002"):
+            return ch.q10_phi(_default(p, "q10", 3.0), _default(p, "temp", ch.c2k(36.0)),
+                              _default(p, "temp_ref", ch.c2k(36.0)))
+        if name in ("CaL_IS2008", "CaT_HM1992"):
+            q = _default(p, "q10_" + which, 3.55 if which == "p" else 3.0)
+            return ch.q10_phi(q, _default(p, "temp", ch.c2k(36.0)), _default(p, "temp_ref_" + which, ch.c2k(24.0)))
+        if name == "HCN_HM1992":
+            return ch.q10_phi(_default(p, "q10", 1.0), _default(p, "temp", ch.c2k(36.0)),
+                              _default(p, "temp_ref", ch.c2k(36.0)))
+        if name == "AHP_De1994":
+            return _default(p, "phi", 1.0)
+        return 1.0
+
+    def _rates(self, name, V, Ci):
+        p = self.params[name]
+        if name == "Na_Ba2002":
+            return ch.na_ba2002(V, _default(p, "V_sh", -50.0))
+        if name == "KDR_Ba2002":
+            return ch.kdr_ba2002(V, _default(p, "V_sh", -50.0))
+        if name == "CaL_IS2008":
+            return ch.cal_is2008(V, _default(p, "V_sh", 0.0))
+        if name == "CaT_HM1992":
+            return ch.cat_hm1992(V, _default(p, "V_sh", -3.0))
+        if name == "AHP_De1994":
+            return ch.ahp_de1994(Ci, _default(p, "alpha", 48.0), _default(p, "beta", 0.09), _default(p, "n", 2.0))
+        if name == "HCN_HM1992":
+            return ch.hcn_hm1992(V)
+        return ()
+
+    def _steady(self, name, V, Ci):
+        r = self._rates(name, V, Ci)
+        if name == "Na_Ba2002":
+            ap, bp, aq, bq = r
+            return [ap / (ap + bp), aq / (aq + bq)]
+        if name == "KDR_Ba2002" or name == "AHP_De1994":
+            a, b = r
+            return [a / (a + b)]
+        if name in ("CaL_IS2008", "CaT_HM1992"):
+            return [r[0], r[2]]
+        if name == "HCN_HM1992":
+            return [r[0]]
+        return []
+
+    def _E(self, name, Ci):
+        p = self.params[name]
+        if name == "Na_Ba2002":
+            return self.E_Na
+        if name in ("KDR_Ba2002", "K_Leak", "AHP_De1994"):
+            return self.E_K
+        if name in CA_CHANNELS:
+            return self._E_ca(Ci)
+        if name == "IL":
+            return _default(p, "E", -70.0)
+        if name == "HCN_HM1992":
+            return _default(p, "E", 43.0)
+        raise KeyError(name)
+
+    def _E_ca(self, Ci):
+        if self.ca is None:
+            return float(self.model.E_Ca)
+        return ch.nernst(Ci, float(self.ca.get("Co", 2.0)), float(self.ca.get("temp", ch.c2k(36.0))), 2.0)
+
+    def _g_eff(self, name):
+        """conductance_factor (_base.py:550-558) times g_max, in mS/cm^2."""
+        gs = self.gates.get(name, [])
+        if name == "Na_Ba2002":
+            f = gs[0] ** 3 * gs[1]
+        elif name == "KDR_Ba2002":
+            f = gs[0] ** 4
+        elif name in ("CaL_IS2008", "CaT_HM1992"):
+            f = gs[0] ** 2 * gs[1]
+        elif name == "AHP_De1994":
+            f = gs[0] ** 2
+        elif name == "HCN_HM1992":
+            f = gs[0]
+        else:
+            f = 1.0
+        return self.g[name] * f
+
+    def membrane_current(self, V, Ci):
+        """Sum over channels of g_eff (E - V) (uA/cm^2) and of g_eff."""
+        I = np.zeros(self.C)
+        G = np.zeros(self.C)
+        for name in self.g:
+            ge = self._g_eff(name)
+            I = I + ge * (self._E(name, Ci) - V)
+            G = G + ge
+        return I, G
+
+    def ca_current(self, V, Ci):
+        """Ion.current (_base_ion.py:325-364) of the Ca pool: the Ca channels'
+        current masked to the pool's painted points."""
+        I = np.zeros(self.C)
+        for name in CA_CHANNELS:
+            if name in self.g:
+                I = I + self._g_eff(name) * (self._E(name, Ci) - V)
+        return np.where(self.ca_mask, I, 0.0)
+
+    def clamp_density(self, t):
+        """currents.py:166-193 with layouts.py:960-982: uA/cm^2 at midpoints."""
+        I = np.zeros(self.C)
+        for cv, delay, dur, amp in self.mid_clamps:
+            lt = t - delay
+            if lt >= 0.0 and lt < dur:
+                I[cv] = amp * 1e-3 / self.area_cm2[cv]  # nA -> uA, per cm^2
+        return I
+
+    def edge_current(self, t):
+        if not self.edge_clamps:
+            return None
+        cur = np.zeros((1, self.st.n_point))
+        for pid, delay, dur, amp in self.edge_clamps:
+            lt = t - delay
+            if lt >= 0.0 and lt < dur:
+                cur[0, pid] += amp
+        return cur
+
+    # ------------------------------------------------------------------- step
+    def step(self, t, dt):
+        V = self.V
+        Ci = self.Ci
+        I_ca = self.ca_current(V, Ci) if self.ca is not None else None
+        I, G = self.membrane_current(V, Ci)
+        I = I + self.clamp_density(t)
+        f = I / self.cm
+        lin = -G / self.cm
+        const = f - V * lin
+        Vn = dhs_solve(self.st, V[None, :], lin[None, :], const[None, :], dt, self.edge_current(t))[0]
+        if self.ca is not None:
+            fc = ch.ca_detailed_drive(I_ca, float(self.ca["d"])) + (float(self.ca["C_rest"]) - Ci) / float(self.ca["tau"])
+            Ci_n = ch.exp_euler(Ci, fc, -1.0 / float(self.ca["tau"]) * np.ones_like(Ci), dt)
+        else:
+            Ci_n = None
+        self._update_gates(Vn, Ci_n, dt)
+        self.V = Vn
+        self.Ci = Ci_n
+
+    def _update_gates(self, V, Ci, dt):
+        for name in list(self.gates.keys()):
+            gs = self.gates[name]
+            if not gs:
+                continue
+            r = self._rates(name, V, Ci)
+            if name == "Na_Ba2002":
+                ap, bp, aq, bq = r
+                phi = self._phi(name)
+                self.gates[name] = [ch.ab_update(gs[0], ap, bp, phi, dt), ch.ab_update(gs[1], aq, bq, phi, dt)]
+            elif name in ("KDR_Ba2002", "AHP_De1994"):
+                a, b = r
+                self.gates[name] = [ch.ab_update(gs[0], a, b, self._phi(name), dt)]
+            elif name in ("CaL_IS2008", "CaT_HM1992"):
+                pi, pt, qi, qt = r
+                self.gates[name] = [ch.it_update(gs[0], pi, pt, self._phi(name, "p"), dt),
+                                    ch.it_update(gs[1], qi, qt, self._phi(name, "q"), dt)]
+            elif name == "HCN_HM1992":
+                pi, pt = r
+                self.gates[name] = [ch.it_update(gs[0], pi, pt, self._phi(name), dt)]
+
+    def run(self, dt, n_steps, probe_cvs=None):
+        """run.py:108-134: t_n = n * dt; returns V traces [n_steps, len(probes)]
+        recorded after each step."""
+        probes = np.arange(self.C) if probe_cvs is None else np.asarray(probe_cvs)
+        out = np.empty((n_steps, probes.shape[0]))
+        for n in range(n_steps):
+            self.step(n * dt, dt)
+            out[n] = self.V[probes]
+        return out

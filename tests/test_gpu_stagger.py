@@ -1,0 +1,85 @@
+"""GPU parity of the staggered HIP kernel against the FP64 oracle."""
+
+import numpy as np
+import pytest
+
+from braincell_amd import workloads as W
+from tests.helpers import rmse, run_gpu, run_oracle, soma_cv, spike_times
+
+pytestmark = pytest.mark.gpu
+
+
+def test_single_tree_fp64_all_cvs_match_oracle(cuda):
+    m = W.single_tree_cell(0)
+    steps = 4000
+    ref, _ = run_oracle(m, steps)
+    got, _ = run_gpu([m], steps, precision=64, probes="all")
+    err = np.abs(got - ref).max()
+    print("fp64 max |dV| over 4000 steps x 256 CVs:", err)
+    assert err < 1e-8
+
+
+def test_single_tree_fp32_soma_rmse(cuda):
+    m = W.single_tree_cell(0)
+    steps = 4000
+    cv = soma_cv(m)
+    ref, _ = run_oracle(m, steps, probe_cvs=[cv])
+    got, _ = run_gpu([m], steps, precision=32, probes="soma")
+    e = rmse(got[:, 0], ref[:, 0])
+    print("fp32 soma RMSE (mV):", e)
+    assert e <= 0.05
+
+
+def test_uneven_batch_fp32_soma_rmse_and_spikes(cuda):
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=800) for i in range(4)]
+    steps, dt = 4000, 0.025
+    got, _ = run_gpu(models, steps, precision=32, probes="soma", rows_per_pack=2600)
+    n_spk = 0
+    for i, m in enumerate(models):
+        ref, _ = run_oracle(m, steps, probe_cvs=[soma_cv(m)])
+        e = rmse(got[:, i], ref[:, 0])
+        st_g, st_r = spike_times(got[:, i], dt), spike_times(ref[:, 0], dt)
+        print(f"cell {i}: RMSE {e:.2e} mV, spikes gpu {len(st_g)} oracle {len(st_r)}")
+        assert e <= 0.05
+        assert len(st_g) == len(st_r)
+        if len(st_r):
+            assert np.max(np.abs(st_g - st_r)) <= dt + 1e-9
+        n_spk += len(st_r)
+    assert n_spk > 0
+
+
+def test_uneven_batch_fp64_matches_oracle(cuda):
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=600) for i in range(3)]
+    steps = 1200
+    got, _ = run_gpu(models, steps, precision=64, probes="soma", rows_per_pack=2600)
+    for i, m in enumerate(models):
+        ref, _ = run_oracle(m, steps, probe_cvs=[soma_cv(m)])
+        assert np.abs(got[:, i] - ref[:, 0]).max() < 1e-8
+
+
+def test_cell_alone_equals_cell_in_pack_bitwise(cuda):
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=500) for i in range(3)]
+    together, _ = run_gpu(models, 400, precision=32, probes="soma", rows_per_pack=4000)
+    for i, m in enumerate(models):
+        alone, _ = run_gpu([m], 400, precision=32, probes="soma")
+        assert np.array_equal(alone[:, 0], together[:, i])
+
+
+def test_chunked_and_graph_runs_equal_one_launch(cuda):
+    import torch
+    from braincell_amd.runtime import StaggerLayout, StaggerRuntime
+
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=500) for i in range(5)]
+    one, _ = run_gpu(models, 600, precision=32, rows_per_pack=1500)
+    chunked, _ = run_gpu(models, 600, precision=32, rows_per_pack=1500, chunk=100)
+    assert np.array_equal(one, chunked)
+    L = StaggerLayout(models, rows_per_pack=1500)
+    rt = StaggerRuntime(L, 0.025, precision=32)
+    trace = rt.new_trace(600)
+    g = rt.make_graph(6, 100, trace)
+    for rep in range(3):
+        rt.reset_state()
+        torch.cuda.synchronize()
+        g.launch(rt.stream_handle())
+        torch.cuda.synchronize()
+        assert np.array_equal(trace.double().cpu().numpy(), one), f"replay {rep}"
