@@ -40,7 +40,7 @@ class StaggerDesc(ctypes.Structure):
         ("cr_lo", ctypes.c_void_p), ("cr_up", ctypes.c_void_p), ("cr_dg", ctypes.c_void_p),
         ("cls_tab", ctypes.c_void_p), ("clamp_row", ctypes.c_void_p), ("clamp_on", ctypes.c_void_p),
         ("clamp_off", ctypes.c_void_p), ("clamp_amp", ctypes.c_void_p), ("probe_row", ctypes.c_void_p),
-        ("probe_col", ctypes.c_void_p), ("chan_const", ctypes.c_void_p), ("V", ctypes.c_void_p),
+        ("probe_col", ctypes.c_void_p), ("chan_const", ctypes.c_void_p), ("V", ctypes.c_void_p), ("Vbnd", ctypes.c_void_p),
         ("state", ctypes.c_void_p), ("trace", ctypes.c_void_p), ("trace_ld", ctypes.c_int64),
         ("status", ctypes.c_void_p), ("step0", ctypes.c_int32), ("n_steps", ctypes.c_int32),
         ("dt", ctypes.c_double),

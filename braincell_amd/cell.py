@@ -1,0 +1,261 @@
+"""Front end: ``Cell`` (paint/place declarations) and ``CellBatch``.
+
+Mirrors the parts of ``braincell.Cell`` (``_multi_compartment/cell.py:163-2708``)
+that the staggered step touches: construction with a solver name resolved
+through the integrator registry (``cell.py:242, 2771-2776``; reassignment
+after ``init_state`` is rejected, ``cell.py:334-337``), ``paint`` / ``place``
+(``cell.py:383-400``), ``init_state`` / ``reset_state`` (``cell.py:490-552,
+2529-2550``), the update order switch (``ion_channel_update_order``) and
+``run(dt, duration)`` (``run.py:54-134``).
+
+A ``Cell`` lowers its declarations into a :class:`CellModel`; a ``CellBatch``
+runs many cells (uneven trees) in one packed launch.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .channels import IONS
+from .filter import resolve_region
+from .mech import CableProperty, Channel, CurrentClamp, Ion
+from .model import CellModel
+from .quad import get_integrator
+
+__all__ = ["Cell", "CellBatch", "CVPerBranch", "RunResult"]
+
+
+@dataclass(frozen=True)
+class CVPerBranch:
+    """Same number of CVs on every branch (``_discretization/policy.py:84-122``)."""
+
+    cv_per_branch: int = 1
+
+    def __post_init__(self):
+        if isinstance(self.cv_per_branch, bool) or not isinstance(self.cv_per_branch, int):
+            raise TypeError(f"cv_per_branch must be integer, got {self.cv_per_branch!r}.")
+        if self.cv_per_branch <= 0:
+            raise ValueError(f"cv_per_branch must be > 0, got {self.cv_per_branch!r}.")
+
+
+@dataclass
+class RunResult:
+    time: np.ndarray  # ms, step start times (run.py:64-71)
+    traces: dict
+
+
+class Cell:
+    """A multi-compartment cell integrated by a registry-selected solver."""
+
+    def __init__(self, morpho, *, cv_policy=None, solver="staggered_mi355x", precision: int = 32,
+                 V_th: float = 0.0, ion_channel_update_order: str = "family", device="cuda", name=None):
+        self.morpho = morpho
+        self.cv_policy = cv_policy or CVPerBranch(1)
+        self._solver = get_integrator(solver)
+        self.precision = int(precision)
+        self.V_th = float(V_th)
+        self.ion_channel_update_order = ion_channel_update_order
+        self.device = device
+        self.name = name
+        self._paint = []
+        self._place = []
+        self._runtime = None
+        self._initialized = False
+
+    # ---------------------------------------------------------- declarations
+    @property
+    def solver(self):
+        return self._solver
+
+    @solver.setter
+    def solver(self, value):
+        if self._initialized:
+            raise RuntimeError("Cell.solver cannot be reassigned after init_state().")
+        self._solver = get_integrator(value)
+
+    def paint(self, region, *mechanisms) -> "Cell":
+        for m in mechanisms:
+            if not isinstance(m, (CableProperty, Channel, Ion)):
+                raise TypeError(f"paint() takes CableProperty/Channel/Ion, got {type(m).__name__}")
+        self._paint.append((region, mechanisms))
+        self._runtime = None
+        return self
+
+    def place(self, locset, *mechanisms) -> "Cell":
+        for m in mechanisms:
+            if not isinstance(m, CurrentClamp):
+                raise TypeError(f"place() takes CurrentClamp, got {type(m).__name__}")
+        self._place.append((locset, mechanisms))
+        self._runtime = None
+        return self
+
+    def lower(self) -> CellModel:
+        """Lower paint/place rules to per-branch tables (later rules win)."""
+        mo = self.morpho
+        nb = mo.n_branches
+        cm = np.full(nb, 1.0)
+        ra = np.full(nb, 100.0)
+        v0 = np.full(nb, -65.0)
+        channels = {}
+        ca_mask = np.zeros(nb, dtype=bool)
+        ca_params = None
+        E = {"E_Na": IONS["SodiumFixed"]["E"], "E_K": IONS["PotassiumFixed"]["E"],
+             "E_Ca": IONS["CalciumFixed"]["E"]}
+        for region, mechs in self._paint:
+            mask = resolve_region(region, mo)
+            for m in mechs:
+                if isinstance(m, CableProperty):
+                    cm[mask] = m.membrane_capacitance
+                    ra[mask] = m.axial_resistivity
+                    v0[mask] = m.resting_potential
+                elif isinstance(m, Channel):
+                    p = m.resolved()
+                    spec = channels.setdefault(m.class_name, {"g_branch": np.zeros(nb), "params": {}})
+                    extra = {k: v for k, v in p.items() if k != "g_max"}
+                    if spec["params"] and spec["params"] != extra:
+                        raise ValueError(f"{m.class_name}: non-density parameters must be the same everywhere")
+                    spec["params"] = extra
+                    spec["g_branch"][mask] = float(p["g_max"])
+                elif isinstance(m, Ion):
+                    p = m.resolved()
+                    if m.class_name == "CalciumDetailed":
+                        if ca_params is not None and ca_params != p:
+                            raise ValueError("CalciumDetailed parameters must be the same everywhere")
+                        ca_params = p
+                        ca_mask |= mask
+                    elif m.class_name == "SodiumFixed":
+                        E["E_Na"] = float(p["E"])
+                    elif m.class_name == "PotassiumFixed":
+                        E["E_K"] = float(p["E"])
+                    elif m.class_name == "CalciumFixed":
+                        E["E_Ca"] = float(p["E"])
+        if len(set(v0.tolist())) > 1:
+            raise ValueError("resting_potential must be the same on all branches of one cell")
+        clamps = []
+        for locset, mechs in self._place:
+            for (b, x) in locset.locations(mo):
+                for m in mechs:
+                    for (t0, t1, amp) in m.segments():
+                        clamps.append((b, x, t0, t1 - t0, amp))
+        ca = None
+        if ca_params is not None:
+            ca = {"mask_branch": ca_mask, **{k: ca_params[k] for k in ("d", "tau", "C_rest", "Co", "Ci0", "temp")}}
+        return CellModel(morph=mo, cv_per_branch=self.cv_policy.cv_per_branch, cm=cm, ra=ra, V0=float(v0[0]),
+                         channels=channels, ca=ca, clamps=clamps, **E).validate()
+
+    # ----------------------------------------------------------------- state
+    def init_state(self, *, probes="soma", threads: int = 256):
+        """Build the discretisation, layout and device state (gates start at
+        the reference's ``init_state`` value only after ``reset_state``)."""
+        from .runtime import StaggerLayout
+
+        self._model = self.lower()
+        self._layout = StaggerLayout([self._model], threads=threads, probes=probes)
+        self._runtime = None
+        self._initialized = True
+        return self
+
+    def reset_state(self):
+        if not self._initialized:
+            self.init_state()
+        if self._runtime is not None:
+            self._runtime.reset_state()
+
+    def _runtime_for(self, dt: float):
+        from .runtime import StaggerRuntime
+
+        if not self._initialized:
+            self.init_state()
+        rt = self._runtime
+        if rt is None or rt.dt != float(dt):
+            old = rt
+            rt = StaggerRuntime(self._layout, dt, precision=self.precision, device=self.device)
+            if old is not None:  # keep the current state across a dt change
+                rt.V.copy_(old.V)
+                rt.S.copy_(old.S)
+                rt.Vbnd.copy_(old.Vbnd)
+                rt.step_index = old.step_index
+            self._runtime = rt
+        return rt
+
+    @property
+    def runtime(self):
+        return self._runtime
+
+    @property
+    def V(self):
+        """Membrane voltage per CV (mV), CV order of the discretisation."""
+        if self._runtime is None:
+            raise RuntimeError("no device state yet: run a step or call Cell.run first")
+        return self._runtime.V
+
+    # ------------------------------------------------------------------ run
+    def advance(self, t: float, dt: float, n_steps: int = 1):
+        """Native advance used by the registered step functions."""
+        rt = self._runtime_for(dt)
+        n0 = int(round(float(t) / float(dt)))
+        rt.step_index = n0
+        tr = rt.run(n_steps)
+        rt.step_index = n0 + n_steps
+        self.last_trace = tr
+        return tr
+
+    def update(self, *args):
+        """One step through the configured solver (``cell.py:2348-2387``)."""
+        return self.solver(self, *args)
+
+    def run(self, *, dt: float, duration: float, chunk: int = 200) -> RunResult:
+        """``run.py:54-134``: T = len(arange(0, duration, dt)) steps starting at
+        the cell's current time; returns the probe traces."""
+        import torch
+
+        rt = self._runtime_for(dt)
+        times = np.arange(0.0, float(duration), float(dt))
+        n = int(times.shape[0])
+        if n == 0:
+            raise ValueError("Cell.run(...) produced no timesteps; ensure duration > 0 and dt > 0.")
+        start = rt.step_index * float(dt)
+        tr = rt.run(n, chunk=chunk)
+        torch.cuda.synchronize()
+        rt.check_status()
+        return RunResult(time=start + times, traces={"V": tr.double().cpu().numpy()})
+
+
+class CellBatch:
+    """Many cells (uneven morphologies) integrated in one packed launch."""
+
+    def __init__(self, cells_or_models, *, precision: int = 32, threads: int = 256, rows_per_pack=None,
+                 probes="soma", device="cuda"):
+        from .runtime import StaggerLayout
+
+        models = [c.lower() if isinstance(c, Cell) else c for c in cells_or_models]
+        self.layout = StaggerLayout(models, threads=threads, rows_per_pack=rows_per_pack, probes=probes)
+        self.precision = precision
+        self.device = device
+        self._runtime = None
+
+    def runtime(self, dt: float):
+        from .runtime import StaggerRuntime
+
+        if self._runtime is None or self._runtime.dt != float(dt):
+            self._runtime = StaggerRuntime(self.layout, dt, precision=self.precision, device=self.device)
+        return self._runtime
+
+    def run(self, *, dt: float, duration: float, chunk: int = 200, use_graph: bool = True) -> RunResult:
+        import torch
+
+        rt = self.runtime(dt)
+        n = int(np.arange(0.0, float(duration), float(dt)).shape[0])
+        trace = rt.new_trace(n)
+        if use_graph and n % chunk == 0:
+            g = rt.make_graph(n // chunk, chunk, trace)
+            g.launch(rt.stream_handle())
+            rt.step_index += n
+        else:
+            rt.run(n, trace=trace, chunk=chunk)
+        torch.cuda.synchronize()
+        rt.check_status()
+        return RunResult(time=np.arange(n) * float(dt), traces={"V_soma": trace.double().cpu().numpy()})
+

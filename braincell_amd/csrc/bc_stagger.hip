@@ -5,18 +5,25 @@
 //
 //   A. (one thread per CV, gate/pool states in registers) linearise the
 //      membrane current at V_n: write diag = 1 + dt/Cm * sum(g_eff) and
-//      rhs = V_n + dt/Cm * (sum(g_eff E) + I_clamp) into LDS; remember the Ca
-//      pool current at (V_n, gates_n, Ci_n).  Boundary rows get diag 0 and
-//      rhs 0 (+ any clamp on them); the axial part is added from dt-scaled
-//      coefficients while the solver walks its chains.
-//   B. Hines solve on the row tree in LDS.  Rows are grouped into chains
-//      (runs whose parent has one child); a thread eliminates a whole chain
-//      bottom-to-top in registers, passing its top row's contribution to the
-//      parent junction through LDS.  Chains are processed level by level,
-//      leaves first; back-substitution runs the levels in reverse.
-//   C. (same CV threads) Ca pool by exp-Euler with the cached current, then
-//      every gate by exp-Euler at V_{n+1} with E_Ca and Ci at n+1; write
-//      probe voltages to the trace.
+//      rhs = dt/Cm * (sum(g_eff (E - V_n)) + I_clamp) into LDS; remember the
+//      Ca pool current at (V_n, gates_n, Ci_n).  Boundary rows get diag 0 and
+//      rhs 0 (+ any clamp on them).
+//   B. Hines solve on the row tree in LDS for dV = V_{n+1} - V_n.  The
+//      reference solves (I + dt A - dt L) V_{n+1} = V_n + dt (f - L V_n); the
+//      same system written for dV has the right-hand side dt (f - A V_n), so
+//      the axial defect dt A V_n is folded in edge by edge (from voltage
+//      differences) while a thread walks its chain.  In FP32 this keeps the
+//      solve's rounding relative to dV rather than V: the fat soma couples
+//      its CVs with dt g / C ~ 1e3, which costs three digits otherwise.
+//      Rows are grouped into chains (runs whose parent has one child); a
+//      thread eliminates a whole chain bottom-to-top in registers, passing
+//      its top row's contribution to the parent junction through LDS.
+//      Chains are processed level by level, leaves first; back-substitution
+//      runs the levels in reverse.  Boundary-row voltages persist between
+//      steps (the constraint rows are re-imposed by every solve).
+//   C. (same CV threads) V += dV; Ca pool by exp-Euler with the cached
+//      current, then every gate by exp-Euler at V_{n+1} with E_Ca and Ci at
+//      n+1; write probe voltages to the trace.
 //
 // Reference semantics: braincell/quad/_staggered.py:126-154 (step order),
 // :449-513 (numeric state), :641-657 (elimination), :742-760 (back
@@ -54,6 +61,7 @@ struct StaggerArgs {
   const int32_t* probe_row;
   const int32_t* probe_col;
   real* V;
+  real* Vbnd;
   real* state;
   int64_t n_cv_total;
   real* trace;
@@ -80,7 +88,8 @@ __device__ __forceinline__ real clamp_current(const StaggerArgs<real>& a, const 
   return I;
 }
 
-// Linearisation of one CV (phase A).  Returns the Ca pool current (masked).
+// Linearisation of one CV (phase A): G = sum(g_eff), GE = sum(g_eff (E - V)).
+// Returns the Ca pool current (masked to the pool's region).
 template <typename real, uint32_t CH>
 __device__ __forceinline__ real linearise(const bc_chan_const<real>& cc, const real* cp, const real* s, real V,
                                           real& G, real& GE) {
@@ -94,50 +103,50 @@ __device__ __forceinline__ real linearise(const bc_chan_const<real>& cc, const r
     real p = s[bc_slot_na(CH)], q = s[bc_slot_na(CH) + 1];
     real g = cp[BC_CLS_G_NA] * (p * p * p) * q;
     G += g;
-    GE += g * cc.E_Na;
+    GE += g * (cc.E_Na - V);
   }
   if constexpr ((CH & BC_CH_KDR) != 0) {
     real p = s[bc_slot_kdr(CH)];
     real p2 = p * p;
     real g = cp[BC_CLS_G_KDR] * (p2 * p2);
     G += g;
-    GE += g * cc.E_K;
+    GE += g * (cc.E_K - V);
   }
   if constexpr ((CH & BC_CH_IL) != 0) {
     real g = cp[BC_CLS_G_IL];
     G += g;
-    GE += g * cc.E_IL;
+    GE += g * (cc.E_IL - V);
   }
   if constexpr ((CH & BC_CH_KL) != 0) {
     real g = cp[BC_CLS_G_KL];
     G += g;
-    GE += g * cc.E_K;
+    GE += g * (cc.E_K - V);
   }
   if constexpr ((CH & BC_CH_CAL) != 0) {
     real p = s[bc_slot_cal(CH)], q = s[bc_slot_cal(CH) + 1];
     real g = cp[BC_CLS_G_CAL] * (p * p) * q;
     G += g;
-    GE += g * eca;
+    GE += g * (eca - V);
     ica += g * (eca - V);
   }
   if constexpr ((CH & BC_CH_CAT) != 0) {
     real p = s[bc_slot_cat(CH)], q = s[bc_slot_cat(CH) + 1];
     real g = cp[BC_CLS_G_CAT] * (p * p) * q;
     G += g;
-    GE += g * eca;
+    GE += g * (eca - V);
     ica += g * (eca - V);
   }
   if constexpr ((CH & BC_CH_AHP) != 0) {
     real p = s[bc_slot_ahp(CH)];
     real g = cp[BC_CLS_G_AHP] * (p * p);
     G += g;
-    GE += g * cc.E_K;
+    GE += g * (cc.E_K - V);
   }
   if constexpr ((CH & BC_CH_HCN) != 0) {
     real p = s[bc_slot_hcn(CH)];
     real g = cp[BC_CLS_G_HCN] * p;
     G += g;
-    GE += g * cc.hcn_E;
+    GE += g * (cc.hcn_E - V);
   }
   return ica * cp[BC_CLS_CAMASK];
 }
@@ -195,7 +204,8 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   real* bd = reinterpret_cast<real*>(smem_raw);
   real* br = bd + a.max_rows;
-  real* kd = br + a.max_rows;
+  real* xv = br + a.max_rows;  // V_n at every row
+  real* kd = xv + a.max_rows;
   real* kr = kd + a.max_chains;
   real* cls = kr + a.max_chains;
 
@@ -222,6 +232,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       int64_t g = (int64_t)P.cv0 + c;
       info[i] = a.cv_info[g];
       vv[i] = a.V[g];
+      xv[a.cv_info[g] & kRowMask] = vv[i];
 #pragma unroll
       for (int s = 0; s < NS; ++s)
         if (s < bc_nstates(CH)) st[i][s] = a.state[(int64_t)s * a.n_cv_total + g];
@@ -232,6 +243,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       }
     }
   }
+  for (int i = tid; i < P.n_bnd; i += NT) xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu] = a.Vbnd[P.bnd0 + i];
   __syncthreads();
 
   const real dt = a.dt;
@@ -249,7 +261,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         real Icl = (info[i] & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0);
         const real dtcm = cp[BC_CLS_DTCM];
         bd[row] = real(1) + dtcm * G;
-        br[row] = vv[i] + dtcm * (GE + Icl);
+        br[row] = dtcm * (GE + Icl);
       }
     }
     // boundary (algebraic) rows
@@ -267,7 +279,9 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       for (int k = k0 + tid; k < k1; k += NT) {
         const int gk = P.ch0 + k;
         const int j0 = a.chain_off[gk], j1 = a.chain_off[gk + 1];
+        const int tp = a.chain_top[gk];
         int row = a.cr_row[j0];
+        real vr = xv[row];
         real d = a.cr_dg[j0] + bd[row];
         real r = br[row];
         const int q0 = a.kid_off[gk], q1 = a.kid_off[gk + 1];
@@ -276,23 +290,27 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
           d -= kd[kk];
           r -= kr[kk];
         }
-        real inv = real(1) / d;
-        bd[row] = inv;
-        br[row] = r;
-        real m = a.cr_up[j0] * inv;
-        real lo = a.cr_lo[j0];
-        for (int j = j0 + 1; j < j1; ++j) {
-          row = a.cr_row[j];
-          d = (a.cr_dg[j] + bd[row]) - lo * m;
-          r = br[row] - r * m;
-          inv = real(1) / d;
+        real m, lo, up, dl;
+        for (int j = j0;; ) {
+          // axial defect of the edge to the parent row: dt A V_n terms
+          const int pr = (j + 1 < j1) ? a.cr_row[j + 1] : tp;
+          const real vp = pr >= 0 ? xv[pr] : vr;
+          lo = a.cr_lo[j];
+          up = a.cr_up[j];
+          dl = vr - vp;
+          r += lo * dl;
+          const real inv = real(1) / d;
           bd[row] = inv;
           br[row] = r;
-          m = a.cr_up[j] * inv;
-          lo = a.cr_lo[j];
+          m = up * inv;
+          if (++j >= j1) break;
+          row = pr;
+          vr = vp;
+          d = (a.cr_dg[j] + bd[row]) - lo * m;
+          r = (br[row] - r * m) - up * dl;
         }
         kd[k] = lo * m;
-        kr[k] = r * m;
+        kr[k] = r * m + up * dl;
       }
       __syncthreads();
     }
@@ -314,12 +332,17 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       __syncthreads();
     }
 
-    // ---- phase C: Ca pool and gates at V_{n+1}
+    // ---- phase C: V += dV; Ca pool and gates at V_{n+1}
+    for (int i = tid; i < P.n_bnd; i += NT) {
+      const int row = (int)(a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu);
+      xv[row] += br[row];
+    }
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       if (info[i] != 0xFFFFFFFFu) {
         const int row = (int)(info[i] & kRowMask);
-        const real Vn = br[row];
+        const real Vn = vv[i] + br[row];
+        xv[row] = Vn;
         advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt);
         vv[i] = Vn;
         bad |= (Vn == Vn && fabs(Vn) < real(1e30)) ? 0u : 1u;
@@ -338,6 +361,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         if (s < bc_nstates(CH)) a.state[(int64_t)s * a.n_cv_total + g] = st[i][s];
     }
   }
+  for (int i = tid; i < P.n_bnd; i += NT) a.Vbnd[P.bnd0 + i] = xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu];
   if (bad) atomicOr(a.status, 1u);
 }
 
@@ -396,6 +420,7 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->probe_row = d->probe_row;
   a->probe_col = d->probe_col;
   a->V = static_cast<real*>(d->V);
+  a->Vbnd = static_cast<real*>(d->Vbnd);
   a->state = static_cast<real*>(d->state);
   a->n_cv_total = d->n_cv_total;
   a->trace = static_cast<real*>(d->trace);
@@ -423,7 +448,8 @@ int validate(const bc_stagger_desc* d) {
   if (d->max_chains <= 0 || d->max_cls <= 0 || d->max_cls > 16384)
     return bc::fail(BC_INVALID_LAYOUT, "max_chains/max_cls out of range");
   if (!(d->dt > 0.0)) return bc::fail(BC_INVALID_ARG, "dt must be > 0");
-  if (d->n_packs > 0 && (d->packs == nullptr || d->cv_info == nullptr || d->V == nullptr || d->status == nullptr))
+  if (d->n_packs > 0 && (d->packs == nullptr || d->cv_info == nullptr || d->V == nullptr || d->Vbnd == nullptr ||
+                         d->status == nullptr))
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
   return BC_OK;
 }
@@ -439,7 +465,7 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
   a.step0 = step0;
   a.n_steps = n_steps;
   a.trace = static_cast<real*>(trace);
-  size_t lds = (size_t)(2 * d->max_rows + 2 * d->max_chains + d->max_cls * BC_NCLS) * sizeof(real);
+  size_t lds = (size_t)(3 * d->max_rows + 2 * d->max_chains + d->max_cls * BC_NCLS) * sizeof(real);
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "pack needs %zu bytes of LDS (> 160 KiB)", lds);
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (d->n_packs == 0 || n_steps == 0) return BC_OK;

@@ -1,0 +1,93 @@
+"""Integrators registered for the MI355X kernels.
+
+``get_integrator`` / ``register_integrator`` follow
+``braincell/quad/__init__.py:167-214`` and ``quad/_registry.py:355-406``.
+Registered here:
+
+- ``"staggered_mi355x"`` (alias ``"stagger_mi355x"``): the staggered step
+  (``braincell/quad/_staggered.py:47-154``) on the persistent HIP kernel;
+- ``"ind_exp_euler_mi355x"``: independent exp-Euler for point neurons
+  (``braincell/quad/_exp_euler.py:217-320``) on the point-neuron kernel.
+"""
+
+from __future__ import annotations
+
+from typing import Callable, Mapping
+
+from .. import environ
+from ._registry import IntegratorEntry, IntegratorRegistry, get_registry, register_integrator
+
+__all__ = [
+    "IntegratorEntry", "IntegratorRegistry", "get_registry", "register_integrator", "get_integrator",
+    "all_integrators", "staggered_mi355x_step", "ind_exp_euler_mi355x_step",
+]
+
+
+class _RegistryView(Mapping):
+    def __init__(self, reg):
+        self._r = reg
+
+    def __getitem__(self, k):
+        return self._r[k]
+
+    def __iter__(self):
+        return iter(self._r.as_dict(include_aliases=True))
+
+    def __len__(self):
+        return len(self._r.as_dict(include_aliases=True))
+
+    def __contains__(self, k):
+        return k in self._r
+
+
+all_integrators: Mapping = _RegistryView(get_registry())
+
+
+def get_integrator(method) -> Callable:
+    """Resolve a step function from a registered name (or alias) or pass a
+    callable through; unknown names raise ``ValueError`` with a suggestion,
+    other types ``TypeError``."""
+    if callable(method):
+        return method
+    if isinstance(method, str):
+        reg = get_registry()
+        try:
+            return reg[method]
+        except KeyError:
+            sug = reg.suggest(method, n=1)
+            hint = f" Did you mean {sug[0]!r}?" if sug else ""
+            raise ValueError(f"Unknown integrator {method!r}.{hint} Available: {', '.join(reg.names())}.")
+    raise TypeError(f"Integrator method must be a string or callable, got {type(method).__name__}.")
+
+
+@register_integrator("staggered_mi355x", aliases=("stagger_mi355x",), category="staggered",
+                     description="Staggered DHS voltage / ind_exp_euler channel step on MI355X HIP kernels.")
+def staggered_mi355x_step(target, *args):
+    """Advance a multi-compartment cell by one staggered step (``t``/``dt``
+    from :mod:`braincell_amd.environ`); mutates the cell's device state."""
+    from ..cell import Cell
+
+    if not isinstance(target, Cell):
+        raise TypeError(f"The stagger integrator only supports Cell, but we got {type(target)} instead.")
+    order = target.ion_channel_update_order
+    if order not in ("family", "integration"):
+        raise ValueError(f"ion_channel_update_order must be 'family' or 'integration', got {order!r}.")
+    if order == "integration":
+        raise NotImplementedError("the MI355X kernel implements the default 'family' update order")
+    t = environ.get("t", 0.0)
+    dt = environ.get_dt()
+    return target.advance(t, dt, 1)
+
+
+@register_integrator("ind_exp_euler_mi355x", category="exponential",
+                     description="Independent exponential Euler for point neurons on the MI355X HIP kernel.")
+def ind_exp_euler_mi355x_step(target, *args):
+    """Advance a :class:`~braincell_amd.single.SingleCompartment` population
+    by one step with the (Jacobi) independent exp-Euler rule."""
+    from ..single import SingleCompartment
+
+    if not isinstance(target, SingleCompartment):
+        raise TypeError(f"ind_exp_euler_mi355x supports SingleCompartment, got {type(target)}.")
+    t = environ.get("t", 0.0)
+    dt = environ.get_dt()
+    return target.advance(t, dt, 1, *args)

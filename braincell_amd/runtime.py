@@ -235,6 +235,7 @@ class StaggerLayout:
         cr_row, cr_lo, cr_up, cr_dg = [], [], [], []
         cls_tab, clamp_rows, clamp_t, clamp_amp_raw, clamp_kind, clamp_area = [], [], [], [], [], []
         probe_row, probe_col = [], []
+        bnd_cell = []
         self.cell_cv0 = np.zeros(n_cells, dtype=np.int64)
         self.cell_ncv = np.zeros(n_cells, dtype=np.int64)
         self.cell_probe0 = np.zeros(n_cells + 1, dtype=np.int64)
@@ -282,6 +283,7 @@ class StaggerLayout:
                 brows = np.nonzero(~g.row_is_mid)[0]
                 bflag = np.asarray([r in clamp_bnd_rows for r in brows.tolist()], dtype=np.uint64)
                 bnd_info.append(((brows + row_off).astype(np.uint64) | (bflag << np.uint64(31))).astype(np.uint32))
+                bnd_cell.append(np.full(len(brows), i, dtype=np.int64))
                 for lv in range(c.chains.n_levels):
                     for k in range(c.chains.level_off[lv], c.chains.level_off[lv + 1]):
                         per_level[lv].append((i, k))
@@ -339,6 +341,7 @@ class StaggerLayout:
         self.n_rows_total = grow
         self.cv_info = np.concatenate(cv_info)
         self.bnd_info = np.concatenate(bnd_info) if bnd_info else np.zeros(0, np.uint32)
+        self.bnd_cell = np.concatenate(bnd_cell) if bnd_cell else np.zeros(0, np.int64)
         self.chain_off = np.asarray(chain_off, dtype=np.int32)
         self.chain_top = np.asarray(chain_top, dtype=np.int32)
         self.kid_off = np.concatenate(([0], np.cumsum(kid_cnt))).astype(np.int32)
@@ -383,11 +386,16 @@ class StaggerLayout:
     # ---------------------------------------------------------- init state
     def initial_state(self, V0=None):
         """reset_state: V at V0 (per cell model unless given), gates at their
-        steady state, Ci at its initializer.  Returns (V[n_cv], S[ns, n_cv])."""
+        steady state, Ci at its initializer.  Returns (V[n_cv], S[ns, n_cv],
+        Vbnd[n_bnd]); boundary rows start at their cell's V0 (any start value
+        is consistent: every solve re-imposes the boundary constraints)."""
         V = np.empty(self.n_cv_total)
+        v0 = np.empty(len(self.models))
         for i, m in enumerate(self.models):
             v = m.V0 if V0 is None else (V0[i] if np.ndim(V0) else V0)
+            v0[i] = v
             V[self.cell_cv0[i]:self.cell_cv0[i] + self.cell_ncv[i]] = v
+        Vbnd = v0[self.bnd_cell]
         S = np.zeros((len(self.slots), self.n_cv_total))
         cc = self.cc
         pools = [m.ca for m in self.models if m.ca is not None]
@@ -407,7 +415,7 @@ class StaggerLayout:
                 S[s] = rates.hcn_steady(V)
             elif name == "Ca":
                 S[s] = ci0
-        return V, S
+        return V, S, Vbnd
 
     def cell_slice(self, i: int) -> slice:
         return slice(int(self.cell_cv0[i]), int(self.cell_cv0[i] + self.cell_ncv[i]))
@@ -454,9 +462,10 @@ class StaggerRuntime:
         self._set_dt_tables()
         CC = chan_const_type(precision)
         self._cc = CC(**{k: float(v) for k, v in layout.cc.items()})
-        V, S = layout.initial_state()
+        V, S, Vb = layout.initial_state()
         self.V = t(V, self.dtype)
         self.S = t(S.reshape(-1) if S.size else np.zeros(1), self.dtype)
+        self.Vbnd = t(Vb if Vb.size else np.zeros(1), self.dtype)
         self.step_index = 0
         self._graphs = {}
 
@@ -476,10 +485,12 @@ class StaggerRuntime:
         self.clamp_amp = t(tb["clamp_amp"] if tb["clamp_amp"].size else np.zeros(n), self.dtype)
 
     def reset_state(self, V0=None):
-        V, S = self.L.initial_state(V0)
+        V, S, Vb = self.L.initial_state(V0)
         self.V.copy_(self._t(V, self.dtype))
         if S.size:
             self.S.copy_(self._t(S.reshape(-1), self.dtype))
+        if Vb.size:
+            self.Vbnd.copy_(self._t(Vb, self.dtype))
         self.step_index = 0
         self.status.zero_()
 
@@ -503,7 +514,7 @@ class StaggerRuntime:
         d.clamp_row, d.clamp_on, d.clamp_off, d.clamp_amp = p(self.clamp_row), p(self.clamp_on), p(self.clamp_off), p(self.clamp_amp)
         d.probe_row, d.probe_col = p(self.probe_row), p(self.probe_col)
         d.chan_const = ctypes.cast(ctypes.pointer(self._cc), ctypes.c_void_p)
-        d.V, d.state = p(self.V), p(self.S)
+        d.V, d.Vbnd, d.state = p(self.V), p(self.Vbnd), p(self.S)
         if trace is not None:
             if trace.dtype != self.dtype or trace.shape[-1] != L.n_probes or not trace.is_contiguous():
                 raise ValueError("trace must be a contiguous [n_steps, n_probes] tensor of the runtime dtype")
@@ -545,9 +556,13 @@ class StaggerRuntime:
         current step index; replays always restart from that index."""
         d = self.desc(steps_per_chunk, trace)
         h = ctypes.c_void_p()
-        check(lib().bc_stagger_graph_create(ctypes.byref(d), n_chunks, steps_per_chunk, self.stream_handle(),
-                                            ctypes.byref(h)))
-        return _Graph(h, d)
+        # capture needs a non-default stream (default priority)
+        cap = self.torch.cuda.Stream(device=self.device)
+        self.torch.cuda.synchronize(self.device)
+        check(lib().bc_stagger_graph_create(ctypes.byref(d), n_chunks, steps_per_chunk,
+                                            ctypes.c_void_p(cap.cuda_stream), ctypes.byref(h)))
+        cap.synchronize()
+        return _Graph(h, d, cap)
 
     def soma_trace_cols(self):
         return self.L.cell_probe0[:-1]
@@ -566,9 +581,10 @@ class StaggerRuntime:
 
 
 class _Graph:
-    def __init__(self, handle, desc):
+    def __init__(self, handle, desc, stream):
         self.h = handle
         self.desc = desc  # keep chan_const pointer alive
+        self.stream = stream
 
     def launch(self, stream):
         check(lib().bc_graph_launch(self.h, stream))

@@ -73,6 +73,7 @@ typedef struct bc_pack {
  *                 boundary row)
  *  probe_row/col[i] local CV row and trace column
  *  V[cv], state[s * n_cv_total + cv]  voltage and gate/pool states (in/out)
+ *  Vbnd[i]        voltage of boundary row i (bnd_info order; in/out)
  *  trace[step * trace_ld + col]  probe voltages after each step (out)
  *  status         OR-ed with 1 when a non-finite voltage appears (out)
  * real = float (precision 32) or double (precision 64). */
@@ -105,6 +106,7 @@ typedef struct bc_stagger_desc {
   const int32_t* probe_col;
   const void* chan_const; /* host pointer to bc_chan_const<real> (passed by value) */
   void* V;
+  void* Vbnd;
   void* state;
   void* trace;
   int64_t trace_ld;

@@ -1,0 +1,112 @@
+"""Product layout vs the oracle's restatement, chain invariants, and the CPU
+transcription of the kernel's Hines phase vs the oracle DHS (CPU only)."""
+
+import numpy as np
+import pytest
+
+from braincell_amd import workloads as W
+from braincell_amd.layout import build_chains, discretize_cell
+from braincell_amd.runtime import StaggerLayout, first_step_at
+from oracle import discretize as OD
+from oracle.dhs import dhs_solve
+from oracle.staggered import StaggeredOracle
+from tests.kernel_emulator import bnd_rows, cv_rows, hines_delta
+
+CELLS = [lambda: W.single_tree_cell(0)] + [lambda i=i: W.uneven_tree_cell(i, cmin=200, cmax=700) for i in range(4)]
+
+
+@pytest.mark.parametrize("maker", CELLS)
+def test_rows_and_coefficients_match_oracle(maker):
+    m = maker()
+    g = discretize_cell(m.morph, m.n_cv_branch(), m.per_branch(m.cm), m.per_branch(m.ra))
+    cvs, _, tree, sched, st = OD.discretize(m.morph, 4, 1.0, 100.0)
+    assert np.array_equal(g.row_to_point_id, sched.row_to_point_id)  # bit-identical node ordering
+    assert np.array_equal(g.parent_rows, sched.parent_rows)
+    assert np.array_equal(g.dynamic_rows, st.dynamic_rows)
+    for a, b in [(g.diag_axial, st.diag), (g.lowers, st.lowers), (g.uppers, st.uppers), (g.row_cap_uF, st.row_cap_uF)]:
+        np.testing.assert_allclose(a, b, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(g.cv_area_um2, [c.area_um2 for c in cvs], rtol=1e-14)
+
+
+def test_multisegment_branch_geometry_matches_oracle():
+    from braincell_amd.morph import Branch, Morphology
+
+    soma = Branch.from_lengths(lengths=[10.0, 0.0, 12.0], radii_proximal=[5.0, 5.0, 3.0],
+                               radii_distal=[5.0, 3.0, 2.0], type="soma")
+    m = Morphology.from_root(soma)
+    m.attach(0, Branch.from_lengths(lengths=[30.0, 40.0], radii=[1.0, 0.8, 0.5], type="dendrite"))
+    m.attach(0, Branch.from_lengths(lengths=[25.0], radii=[1.2, 0.7], type="dendrite"))
+    g = discretize_cell(m, np.asarray([3, 5, 2]), np.ones(3), np.full(3, 120.0))
+    cvs, _, _, sched, st = OD.discretize(m, [3, 5, 2], 1.0, 120.0)
+    assert np.array_equal(g.row_to_point_id, sched.row_to_point_id)
+    np.testing.assert_allclose(g.cv_area_um2, [c.area_um2 for c in cvs], rtol=1e-13)
+    np.testing.assert_allclose(g.diag_axial, st.diag, rtol=1e-12)
+
+
+@pytest.mark.parametrize("maker", CELLS)
+def test_chain_invariants(maker):
+    m = maker()
+    g = discretize_cell(m.morph, m.n_cv_branch(), m.per_branch(m.cm), m.per_branch(m.ra))
+    ch = build_chains(g.parent_rows)
+    assert sorted(ch.rows.tolist()) == list(range(g.n_point))  # each row in exactly one chain
+    for k in range(ch.n_chains):
+        rows = ch.rows[ch.off[k]:ch.off[k + 1]]
+        for a, b in zip(rows[:-1], rows[1:]):
+            assert g.parent_rows[a] == b  # consecutive rows are child -> parent
+        assert ch.top_parent[k] == g.parent_rows[rows[-1]]
+        for kid in ch.kids[ch.kid_off[k]:ch.kid_off[k + 1]]:
+            assert ch.top_parent[kid] == rows[0]  # kids hang from the bottom row
+            assert ch.level[kid] < ch.level[k]
+    assert np.all(np.diff(ch.level) >= 0)
+    assert ch.level_off[-1] == ch.n_chains
+
+
+def test_packed_hines_emulation_matches_oracle_dhs():
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=400) for i in range(5)] + [W.single_tree_cell(0)]
+    L = StaggerLayout(models, rows_per_pack=1200, cv_per_thread=4)
+    assert L.n_packs < len(models)  # several cells share a pack
+    dt = 0.025
+    rng = np.random.default_rng(0)
+    G = rng.uniform(0, 1, L.n_cv_total)
+    V = rng.uniform(-80, 0, L.n_cv_total)
+    I = rng.uniform(-5, 5, L.n_cv_total)
+    rows, brow = cv_rows(L), bnd_rows(L)
+    bd = np.zeros(L.n_rows_total)
+    br = np.zeros(L.n_rows_total)
+    xv = np.zeros(L.n_rows_total)
+    bd[rows] = 1 + dt * G
+    br[rows] = dt * I
+    xv[rows] = V
+    xv[brow] = rng.uniform(-80, 0, len(brow))  # any start value: constraints are re-imposed
+    dV = hines_delta(L, dt, bd, br, xv)
+    for i, m in enumerate(models):
+        st = StaggeredOracle(m).st
+        sl = L.cell_slice(i)
+        Vn = dhs_solve(st, V[sl][None], (-G[sl])[None], (I[sl] + G[sl] * V[sl])[None], dt)[0]
+        np.testing.assert_allclose(V[sl] + dV[rows[sl]], Vn, rtol=0, atol=1e-9)
+
+
+def test_pack_plan_respects_budgets():
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=2000) for i in range(12)]
+    L = StaggerLayout(models, threads=256, rows_per_pack=2600)
+    P = L.pack_array
+    assert np.all(P[:, 3] <= 2600)
+    assert np.all(P[:, 1] <= L.threads * L.cv_per_thread)
+    assert sorted(c for cells in L.pack_cells for c in cells) == list(range(12))
+    assert P[:, 1].sum() == L.n_cv_total
+
+
+def test_clamp_windows_follow_float64_times():
+    dt = 0.025
+    n_on = first_step_at(10.0, dt)
+    assert n_on * dt >= 10.0 and (n_on - 1) * dt < 10.0
+    assert first_step_at(0.0, dt) == 0
+    assert first_step_at(1e9, dt) == 2 ** 31 - 1
+
+
+def test_probes_and_soma_location():
+    m = W.single_tree_cell(0)
+    L = StaggerLayout([m], probes="soma")
+    # x = 0.5 on a 4-CV soma lands on an internal boundary -> owner CV 2 midpoint
+    assert L.cells[0].probes.tolist() == [2]
+    assert L.clamp_kind == ["mid"]

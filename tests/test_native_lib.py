@@ -1,0 +1,58 @@
+"""The C-ABI library loads, exports what include/braincell_amd.h declares and
+rejects bad descriptors before touching the GPU (CPU only)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from braincell_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = []
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", f)).read()
+            names += re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(bc_\w+)\s*\(", txt, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _native.lib()
+    names = declared_functions()
+    assert "bc_stagger_run" in names and len(names) >= 7
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_version_and_support_queries():
+    L = _native.lib()
+    assert L.bc_version() >= 100
+    assert L.bc_stagger_supported(32, 0x117, 8) == 1
+    assert L.bc_stagger_supported(64, 0x7, 1) == 1
+    assert L.bc_stagger_supported(16, 0x7, 1) == 0
+    assert L.bc_stagger_supported(32, 0x7, 3) == 0
+
+
+def test_bad_descriptors_are_rejected_without_gpu():
+    L = _native.lib()
+    assert L.bc_stagger_run(None, None) == 1
+    assert b"NULL" in L.bc_last_error()
+    d = _native.StaggerDesc()
+    d.precision = 16
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    assert b"precision" in L.bc_last_error()
+    d.precision = 32
+    d.threads = 100
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    d.threads = 256
+    d.max_rows = 0
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 2
+    with pytest.raises(_native.NativeError):
+        _native.check(L.bc_stagger_run(ctypes.byref(d), None))
+    assert L.bc_graph_launch(None, None) == 1
+    assert L.bc_graph_destroy(None) == 0
