@@ -11,7 +11,7 @@ import os
 
 from .build import library_path
 
-__all__ = ["lib", "check", "BcPack", "StaggerDesc", "chan_const_type", "NativeError"]
+__all__ = ["lib", "check", "BcPack", "StaggerDesc", "AdjointDesc", "chan_const_type", "NativeError"]
 
 BC_OK = 0
 STATUS_NAMES = {1: "INVALID_ARG", 2: "INVALID_LAYOUT", 3: "HIP_ERROR", 4: "NONFINITE", 5: "UNSUPPORTED"}
@@ -42,9 +42,15 @@ class StaggerDesc(ctypes.Structure):
         ("clamp_off", ctypes.c_void_p), ("clamp_amp", ctypes.c_void_p), ("probe_row", ctypes.c_void_p),
         ("probe_col", ctypes.c_void_p), ("chan_const", ctypes.c_void_p), ("V", ctypes.c_void_p), ("Vbnd", ctypes.c_void_p),
         ("state", ctypes.c_void_p), ("trace", ctypes.c_void_p), ("trace_ld", ctypes.c_int64),
-        ("status", ctypes.c_void_p), ("step0", ctypes.c_int32), ("n_steps", ctypes.c_int32),
+        ("rec", ctypes.c_void_p), ("status", ctypes.c_void_p), ("step0", ctypes.c_int32), ("n_steps", ctypes.c_int32),
         ("dt", ctypes.c_double),
     ]
+
+
+class AdjointDesc(ctypes.Structure):
+    _fields_ = [("layout", ctypes.c_void_p), ("n_steps", ctypes.c_int32), ("rec", ctypes.c_void_p),
+                ("trace_bar", ctypes.c_void_p), ("Vbar", ctypes.c_void_p), ("Sbar", ctypes.c_void_p),
+                ("cls_grad", ctypes.c_void_p)]
 
 
 CHAN_CONST_FIELDS = (
@@ -85,6 +91,14 @@ def lib():
     L.bc_graph_launch.restype = ctypes.c_int
     L.bc_graph_destroy.argtypes = [ctypes.c_void_p]
     L.bc_graph_destroy.restype = ctypes.c_int
+    L.bc_adjoint_supported.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_int]
+    L.bc_adjoint_supported.restype = ctypes.c_int
+    L.bc_stagger_adjoint.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.bc_stagger_adjoint.restype = ctypes.c_int
+    L.bc_point_supported.argtypes = [ctypes.c_int]
+    L.bc_point_supported.restype = ctypes.c_int
+    L.bc_point_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.bc_point_run.restype = ctypes.c_int
     _LIB = L
     return L
 

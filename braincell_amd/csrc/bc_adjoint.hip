@@ -1,0 +1,430 @@
+// Hand-written adjoint of the staggered step (reverse mode, one workgroup
+// per pack, steps of one checkpoint segment in reverse order).
+//
+// Forward step n (braincell/quad/_staggered.py:126-154, family order):
+//   X      = M(G_n)^-1 R_n        DHS solve; V_{n+1} = X at CV rows
+//            M_ii = 1 + a_ii + dt/Cm G_i,  R_i = V_i + dt/Cm (sum g_eff E + I_clamp)
+//   Ci'    = Ci + dt exprel(-dt/tau) (max(k I_Ca(V_n, gates_n, Ci_n), 0) + (C_rest - Ci)/tau)
+//   x'     = x + dt exprel(u(V_{n+1}, Ci')) f(x, V_{n+1}, Ci')      every gate
+// Reverse (given adjoints of V_{n+1}, gates_{n+1}, Ci_{n+1}):
+//   gates: xbar_n = xbar' e^u;  Xbar += xbar' dx'/dV;  Cibar' += xbar'_AHP dx'/dCi'
+//   pool:  Cibar_n = Cibar' e^{-dt/tau};  I_Ca bar = Cibar' dt exprel(-dt/tau) k [k I_Ca > 0]
+//   solve: Rbar = M^-T Xbar (same tree, lowers and uppers swapped);
+//          Vbar_n += Rbar;  g_eff bar = dt/Cm Rbar (E - X);  E_Ca bar -> Cibar_n
+//   g_eff = g_max prod(gates^p): -> gate adjoints and d loss / d g_max (per
+//          mechanism class, i.e. per painted region of each cell).
+// Step-start states come from the forward kernel's recording (rec).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bc_channels.h"
+#include "bc_common.h"
+
+namespace {
+
+template <typename real>
+struct AdjArgs {
+  const bc_pack* packs;
+  const uint32_t* cv_info;
+  const uint32_t* bnd_info;
+  const int32_t* chain_off;
+  const int32_t* chain_top;
+  const int32_t* kid_off;
+  const int32_t* kids;
+  const int32_t* level_off;
+  const int32_t* cr_row;
+  const real* cr_lo;
+  const real* cr_up;
+  const real* cr_dg;
+  const real* cls_tab;
+  const int32_t* probe_row;
+  const int32_t* probe_col;
+  int64_t n_cv_total;
+  const real* rec;
+  const real* trace_bar;
+  int64_t trace_ld;
+  real* Vbar;
+  real* Sbar;
+  real* cls_grad;
+  int32_t n_steps;
+  real dt;
+  int32_t max_rows, max_chains, max_cls;
+  bc_chan_const<real> cc;
+};
+
+constexpr uint32_t kRowMask = 0xFFFFu;
+constexpr uint32_t kProbeBit = 1u << 30;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+template <typename real, uint32_t CH, int MAXC>
+__global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
+  constexpr int NS = bc_nstates(CH) > 0 ? bc_nstates(CH) : 1;
+  constexpr int NG = 4;  // gradient columns: Na, KDR, IL, CaL
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  real* bd = reinterpret_cast<real*>(smem_raw);
+  real* br = bd + a.max_rows;
+  real* kd = br + a.max_rows;
+  real* kr = kd + a.max_chains;
+  real* cls = kr + a.max_chains;
+  real* gacc = cls + a.max_cls * BC_NCLS;
+
+  const bc_pack P = a.packs[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int NT = blockDim.x;
+  for (int i = tid; i < P.n_cls * BC_NCLS; i += NT) {
+    cls[i] = a.cls_tab[(int64_t)P.cls0 * BC_NCLS + i];
+    gacc[i] = real(0);
+  }
+  const bc_chan_const<real>& cc = a.cc;
+  const real dt = a.dt;
+  const int64_t ncv = a.n_cv_total;
+  const int64_t slab = (int64_t)(bc_nstates(CH) + 1) * ncv;
+
+  uint32_t info[MAXC];
+  int pcol[MAXC];
+  real vb[MAXC];
+  real sb[MAXC][NS];
+  real gg[MAXC][NG];
+#pragma unroll
+  for (int i = 0; i < MAXC; ++i) {
+    const int c = tid + i * NT;
+    info[i] = kNone;
+    pcol[i] = -1;
+    vb[i] = real(0);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) sb[i][s] = real(0);
+#pragma unroll
+    for (int k = 0; k < NG; ++k) gg[i][k] = real(0);
+    if (c < P.n_cv) {
+      const int64_t g = (int64_t)P.cv0 + c;
+      info[i] = a.cv_info[g];
+      vb[i] = a.Vbar[g];
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (s < bc_nstates(CH)) sb[i][s] = a.Sbar[(int64_t)s * ncv + g];
+      if (info[i] & kProbeBit) {
+        const int row = (int)(info[i] & kRowMask);
+        for (int e = 0; e < P.n_probe; ++e)
+          if (a.probe_row[P.probe0 + e] == row) pcol[i] = a.probe_col[P.probe0 + e];
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int n = a.n_steps - 1; n >= 0; --n) {
+    const real* r0 = a.rec + (int64_t)n * slab;        // state at the start of step n
+    const real* r1 = a.rec + (int64_t)(n + 1) * slab;  // state after step n
+    // ---- phase 1: gates and Ca pool backward; transposed-solve inputs
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      if (info[i] == kNone) continue;
+      const int64_t g = (int64_t)P.cv0 + tid + i * NT;
+      const int row = (int)(info[i] & kRowMask);
+      const real* cp = cls + ((info[i] >> 16) & 0x3FFFu) * BC_NCLS;
+      const real X = r1[g];
+      real s0[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) s0[s] = (s < bc_nstates(CH)) ? r0[(int64_t)(s + 1) * ncv + g] : real(0);
+      if (pcol[i] >= 0) vb[i] += a.trace_bar[(int64_t)n * a.trace_ld + pcol[i]];
+      real Xbar = vb[i];
+      if constexpr ((CH & BC_CH_NA) != 0) {
+        real ap, bp, aq, bq, dap, dbp, daq, dbq, dxx, dxv;
+        bc_na_rates_d<real>(X, cc.na_vsh, ap, bp, aq, bq, dap, dbp, daq, dbq);
+        bc_gate_ab_grad<real>(s0[bc_slot_na(CH)], ap, bp, dap, dbp, cc.na_phi, dt, dxx, dxv);
+        Xbar += sb[i][bc_slot_na(CH)] * dxv;
+        sb[i][bc_slot_na(CH)] *= dxx;
+        bc_gate_ab_grad<real>(s0[bc_slot_na(CH) + 1], aq, bq, daq, dbq, cc.na_phi, dt, dxx, dxv);
+        Xbar += sb[i][bc_slot_na(CH) + 1] * dxv;
+        sb[i][bc_slot_na(CH) + 1] *= dxx;
+      }
+      if constexpr ((CH & BC_CH_KDR) != 0) {
+        real ap, bp, dap, dbp, dxx, dxv;
+        bc_kdr_rates_d<real>(X, cc.kdr_vsh, ap, bp, dap, dbp);
+        bc_gate_ab_grad<real>(s0[bc_slot_kdr(CH)], ap, bp, dap, dbp, cc.kdr_phi, dt, dxx, dxv);
+        Xbar += sb[i][bc_slot_kdr(CH)] * dxv;
+        sb[i][bc_slot_kdr(CH)] *= dxx;
+      }
+      if constexpr ((CH & BC_CH_CAL) != 0) {
+        real pi, pt, qi, qt, dpi, dpt, dqi, dqt, dxx, dxv;
+        bc_cal_rates_d<real>(X, cc.cal_vsh, pi, pt, qi, qt, dpi, dpt, dqi, dqt);
+        bc_gate_it_grad<real>(s0[bc_slot_cal(CH)], pi, pt, dpi, dpt, cc.cal_phi_p, dt, dxx, dxv);
+        Xbar += sb[i][bc_slot_cal(CH)] * dxv;
+        sb[i][bc_slot_cal(CH)] *= dxx;
+        bc_gate_it_grad<real>(s0[bc_slot_cal(CH) + 1], qi, qt, dqi, dqt, cc.cal_phi_q, dt, dxx, dxv);
+        Xbar += sb[i][bc_slot_cal(CH) + 1] * dxv;
+        sb[i][bc_slot_cal(CH) + 1] *= dxx;
+      }
+      const real V0 = r0[g];
+      if constexpr ((CH & BC_CH_POOL) != 0) {
+        const int sc = bc_slot_ca(CH);
+        const real ci = s0[sc];
+        const real cib = sb[i][sc];  // adjoint of Ci_{n+1}
+        const real u = -dt * cc.ca_tau_inv;
+        const real ec = real(1) + u * bc_exprel<real>(u);
+        const real bcf = dt * bc_exprel<real>(u);
+        const real eca = cc.ca_rtzf * bc_math<real>::lg(cc.ca_Co / ci);
+        real ica = real(0), gcal = real(0), pp = real(0), qq = real(0), gmax = real(0);
+        if constexpr ((CH & BC_CH_CAL) != 0) {
+          pp = s0[bc_slot_cal(CH)];
+          qq = s0[bc_slot_cal(CH) + 1];
+          gmax = cp[BC_CLS_G_CAL];
+          gcal = gmax * (pp * pp) * qq;
+          ica = gcal * (eca - V0);
+        }
+        const real mask = cp[BC_CLS_CAMASK];
+        const real raw = mask * ica * cc.ca_drive;
+        const real Ib = raw > real(0) ? cib * bcf * cc.ca_drive * mask : real(0);
+        real cin = cib * ec;
+        if constexpr ((CH & BC_CH_CAL) != 0) {
+          vb[i] = -Ib * gcal;  // start of Vbar_n (the solve adds Rbar below)
+          cin += Ib * gcal * (-cc.ca_rtzf / ci);
+          const real drv = Ib * (eca - V0);  // d/d g_eff
+          sb[i][bc_slot_cal(CH)] += drv * gmax * real(2) * pp * qq;
+          sb[i][bc_slot_cal(CH) + 1] += drv * gmax * pp * pp;
+          gg[i][3] += drv * (pp * pp) * qq;
+        } else {
+          vb[i] = real(0);
+        }
+        sb[i][sc] = cin;
+      } else {
+        vb[i] = real(0);
+      }
+      // transposed solve inputs (M has the same diagonal)
+      real G = real(0);
+      if constexpr ((CH & BC_CH_NA) != 0) {
+        const real p = s0[bc_slot_na(CH)];
+        G += cp[BC_CLS_G_NA] * (p * p * p) * s0[bc_slot_na(CH) + 1];
+      }
+      if constexpr ((CH & BC_CH_KDR) != 0) {
+        const real p2 = s0[bc_slot_kdr(CH)] * s0[bc_slot_kdr(CH)];
+        G += cp[BC_CLS_G_KDR] * (p2 * p2);
+      }
+      if constexpr ((CH & BC_CH_IL) != 0) G += cp[BC_CLS_G_IL];
+      if constexpr ((CH & BC_CH_CAL) != 0) {
+        const real p = s0[bc_slot_cal(CH)];
+        G += cp[BC_CLS_G_CAL] * (p * p) * s0[bc_slot_cal(CH) + 1];
+      }
+      bd[row] = real(1) + cp[BC_CLS_DTCM] * G;
+      br[row] = Xbar;
+    }
+    for (int i = tid; i < P.n_bnd; i += NT) {
+      const int row = (int)(a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu);
+      bd[row] = real(0);
+      br[row] = real(0);
+    }
+    __syncthreads();
+
+    // ---- phase 2: Rbar = M^-T Xbar (lowers <-> uppers)
+    for (int l = 0; l < P.n_lv; ++l) {
+      const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
+      for (int k = k0 + tid; k < k1; k += NT) {
+        const int gk = P.ch0 + k;
+        const int j0 = a.chain_off[gk], j1 = a.chain_off[gk + 1];
+        int row = a.cr_row[j0];
+        real d = a.cr_dg[j0] + bd[row];
+        real r = br[row];
+        for (int q = a.kid_off[gk]; q < a.kid_off[gk + 1]; ++q) {
+          d -= kd[a.kids[q]];
+          r -= kr[a.kids[q]];
+        }
+        real inv = real(1) / d;
+        bd[row] = inv;
+        br[row] = r;
+        real m = a.cr_lo[j0] * inv;  // transposed: the parent's coefficient is lowers[row]
+        real lo = a.cr_up[j0];
+        for (int j = j0 + 1; j < j1; ++j) {
+          row = a.cr_row[j];
+          d = (a.cr_dg[j] + bd[row]) - lo * m;
+          r = br[row] - r * m;
+          inv = real(1) / d;
+          bd[row] = inv;
+          br[row] = r;
+          m = a.cr_lo[j] * inv;
+          lo = a.cr_up[j];
+        }
+        kd[k] = lo * m;
+        kr[k] = r * m;
+      }
+      __syncthreads();
+    }
+    for (int l = P.n_lv - 1; l >= 0; --l) {
+      const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
+      for (int k = k0 + tid; k < k1; k += NT) {
+        const int gk = P.ch0 + k;
+        const int j0 = a.chain_off[gk], j1 = a.chain_off[gk + 1];
+        const int tp = a.chain_top[gk];
+        real xp = tp >= 0 ? br[tp] : real(0);
+        for (int j = j1 - 1; j >= j0; --j) {
+          const int row = a.cr_row[j];
+          const real x = (br[row] - a.cr_up[j] * xp) * bd[row];
+          br[row] = x;
+          xp = x;
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- phase 3: linearisation backward
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      if (info[i] == kNone) continue;
+      const int64_t g = (int64_t)P.cv0 + tid + i * NT;
+      const int row = (int)(info[i] & kRowMask);
+      const real* cp = cls + ((info[i] >> 16) & 0x3FFFu) * BC_NCLS;
+      const real X = r1[g];
+      const real Rb = br[row];
+      const real w = cp[BC_CLS_DTCM] * Rb;  // d loss / d (dt/Cm * g_eff * E) per unit
+      vb[i] += Rb;
+      if constexpr ((CH & BC_CH_NA) != 0) {
+        const real p = r0[(int64_t)(bc_slot_na(CH) + 1) * ncv + g], q = r0[(int64_t)(bc_slot_na(CH) + 2) * ncv + g];
+        const real gbar = w * (cc.E_Na - X);
+        const real gm = cp[BC_CLS_G_NA];
+        sb[i][bc_slot_na(CH)] += gbar * gm * real(3) * p * p * q;
+        sb[i][bc_slot_na(CH) + 1] += gbar * gm * p * p * p;
+        gg[i][0] += gbar * p * p * p * q;
+      }
+      if constexpr ((CH & BC_CH_KDR) != 0) {
+        const real p = r0[(int64_t)(bc_slot_kdr(CH) + 1) * ncv + g];
+        const real gbar = w * (cc.E_K - X);
+        sb[i][bc_slot_kdr(CH)] += gbar * cp[BC_CLS_G_KDR] * real(4) * p * p * p;
+        gg[i][1] += gbar * (p * p) * (p * p);
+      }
+      if constexpr ((CH & BC_CH_IL) != 0) gg[i][2] += w * (cc.E_IL - X);
+      if constexpr ((CH & BC_CH_CAL) != 0) {
+        const real p = r0[(int64_t)(bc_slot_cal(CH) + 1) * ncv + g], q = r0[(int64_t)(bc_slot_cal(CH) + 2) * ncv + g];
+        real eca = cc.E_Ca_fixed;
+        if constexpr ((CH & BC_CH_POOL) != 0) {
+          const real ci = r0[(int64_t)(bc_slot_ca(CH) + 1) * ncv + g];
+          eca = cc.ca_rtzf * bc_math<real>::lg(cc.ca_Co / ci);
+          const real geff = cp[BC_CLS_G_CAL] * (p * p) * q;
+          sb[i][bc_slot_ca(CH)] += w * geff * (-cc.ca_rtzf / ci);
+        }
+        const real gbar = w * (eca - X);
+        const real gm = cp[BC_CLS_G_CAL];
+        sb[i][bc_slot_cal(CH)] += gbar * gm * real(2) * p * q;
+        sb[i][bc_slot_cal(CH) + 1] += gbar * gm * p * p;
+        gg[i][3] += gbar * (p * p) * q;
+      }
+    }
+  }
+
+  // carried adjoints out; parameter gradients reduced per class
+#pragma unroll
+  for (int i = 0; i < MAXC; ++i) {
+    if (info[i] == kNone) continue;
+    const int64_t g = (int64_t)P.cv0 + tid + i * NT;
+    a.Vbar[g] = vb[i];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (s < bc_nstates(CH)) a.Sbar[(int64_t)s * ncv + g] = sb[i][s];
+    const int c = (int)((info[i] >> 16) & 0x3FFFu);
+    atomicAdd(&gacc[c * BC_NCLS + BC_CLS_G_NA], gg[i][0]);
+    atomicAdd(&gacc[c * BC_NCLS + BC_CLS_G_KDR], gg[i][1]);
+    atomicAdd(&gacc[c * BC_NCLS + BC_CLS_G_IL], gg[i][2]);
+    atomicAdd(&gacc[c * BC_NCLS + BC_CLS_G_CAL], gg[i][3]);
+  }
+  __syncthreads();
+  for (int i = tid; i < P.n_cls * BC_NCLS; i += NT)
+    if (gacc[i] != real(0)) atomicAdd(&a.cls_grad[(int64_t)P.cls0 * BC_NCLS + i], gacc[i]);
+}
+
+constexpr uint32_t kSetPassive = BC_CH_IL;
+constexpr uint32_t kSetHH = BC_CH_NA | BC_CH_KDR | BC_CH_IL;
+constexpr uint32_t kSetHHCaL = BC_CH_NA | BC_CH_KDR | BC_CH_IL | BC_CH_CAL | BC_CH_POOL;
+
+template <typename real, uint32_t CH>
+void* adj_pick_maxc(int maxc) {
+  switch (maxc) {
+    case 1: return (void*)&stagger_adjoint_kernel<real, CH, 1>;
+    case 2: return (void*)&stagger_adjoint_kernel<real, CH, 2>;
+    case 4: return (void*)&stagger_adjoint_kernel<real, CH, 4>;
+    case 8: return (void*)&stagger_adjoint_kernel<real, CH, 8>;
+    default: return nullptr;
+  }
+}
+
+template <typename real>
+void* adj_pick(uint32_t ch, int maxc) {
+  switch (ch) {
+    case kSetPassive: return adj_pick_maxc<real, kSetPassive>(maxc);
+    case kSetHH: return adj_pick_maxc<real, kSetHH>(maxc);
+    case kSetHHCaL: return adj_pick_maxc<real, kSetHHCaL>(maxc);
+    default: return nullptr;
+  }
+}
+
+template <typename real>
+int adj_launch(const bc_adjoint_desc* d, hipStream_t s) {
+  const bc_stagger_desc* f = d->layout;
+  void* fn = adj_pick<real>(f->channels, f->cv_per_thread);
+  if (fn == nullptr)
+    return bc::fail(BC_UNSUPPORTED, "no adjoint kernel for channels=0x%x cv_per_thread=%d", f->channels,
+                    f->cv_per_thread);
+  AdjArgs<real> a;
+  a.packs = f->packs;
+  a.cv_info = f->cv_info;
+  a.bnd_info = f->bnd_info;
+  a.chain_off = f->chain_off;
+  a.chain_top = f->chain_top;
+  a.kid_off = f->kid_off;
+  a.kids = f->kids;
+  a.level_off = f->level_off;
+  a.cr_row = f->cr_row;
+  a.cr_lo = static_cast<const real*>(f->cr_lo);
+  a.cr_up = static_cast<const real*>(f->cr_up);
+  a.cr_dg = static_cast<const real*>(f->cr_dg);
+  a.cls_tab = static_cast<const real*>(f->cls_tab);
+  a.probe_row = f->probe_row;
+  a.probe_col = f->probe_col;
+  a.n_cv_total = f->n_cv_total;
+  a.rec = static_cast<const real*>(d->rec);
+  a.trace_bar = static_cast<const real*>(d->trace_bar);
+  a.trace_ld = f->trace_ld;
+  a.Vbar = static_cast<real*>(d->Vbar);
+  a.Sbar = static_cast<real*>(d->Sbar);
+  a.cls_grad = static_cast<real*>(d->cls_grad);
+  a.n_steps = d->n_steps;
+  a.dt = static_cast<real>(f->dt);
+  a.max_rows = f->max_rows;
+  a.max_chains = f->max_chains;
+  a.max_cls = f->max_cls;
+  a.cc = *static_cast<const bc_chan_const<real>*>(f->chan_const);
+  const size_t lds = (size_t)(2 * f->max_rows + 2 * f->max_chains + 2 * f->max_cls * BC_NCLS) * sizeof(real);
+  if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "adjoint pack needs %zu bytes of LDS", lds);
+  if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (f->n_packs == 0 || d->n_steps == 0) return BC_OK;
+  void* args[] = {&a};
+  BC_HIP(hipLaunchKernel(fn, dim3(f->n_packs), dim3(f->threads), args, lds, s));
+  BC_HIP(hipGetLastError());
+  return BC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bc_stagger_adjoint(const bc_adjoint_desc* d, void* stream) {
+  if (d == nullptr || d->layout == nullptr) return bc::fail(BC_INVALID_ARG, "desc/layout is NULL");
+  const bc_stagger_desc* f = d->layout;
+  if (f->precision != 32 && f->precision != 64) return bc::fail(BC_INVALID_ARG, "precision must be 32 or 64");
+  if (f->threads < 64 || f->threads > 256 || f->threads % 64 != 0)
+    return bc::fail(BC_INVALID_ARG, "threads must be 64, 128, 192 or 256");
+  if (d->n_steps < 0) return bc::fail(BC_INVALID_ARG, "n_steps < 0");
+  if (f->chan_const == nullptr) return bc::fail(BC_INVALID_ARG, "chan_const is NULL");
+  if (f->n_packs > 0 && (d->rec == nullptr || d->Vbar == nullptr || d->Sbar == nullptr || d->cls_grad == nullptr ||
+                         (f->trace_ld > 0 && d->trace_bar == nullptr)))
+    return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
+  int rc = f->precision == 32 ? adj_launch<float>(d, static_cast<hipStream_t>(stream))
+                              : adj_launch<double>(d, static_cast<hipStream_t>(stream));
+  if (rc != BC_OK) return rc;
+  return bc::ok();
+}
+
+int bc_adjoint_supported(int precision, uint32_t channels, int cv_per_thread) {
+  if (precision == 32) return adj_pick<float>(channels, cv_per_thread) != nullptr;
+  if (precision == 64) return adj_pick<double>(channels, cv_per_thread) != nullptr;
+  return 0;
+}
+
+}  // extern "C"

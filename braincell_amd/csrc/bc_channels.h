@@ -166,3 +166,94 @@ __host__ __device__ __forceinline__ void bc_hcn_rates(real V, real& pi, real& pt
   pi = real(1) / (real(1) + M::ex((V + real(75)) / real(5.5)));
   pt = real(1) / (M::ex(real(-0.086) * V - real(14.59)) + M::ex(real(0.0701) * V - real(1.87)));
 }
+
+// ----------------------------------------------------------------------------
+// Derivatives used by the hand-written adjoint (reverse of one staggered step).
+
+// d/dx exprel(x) = sum_k k x^(k-1) / (k+1)!  =  (x e^x - (e^x - 1)) / x^2
+template <typename real>
+__host__ __device__ __forceinline__ real bc_exprel_d(real x) {
+  const real eps = sizeof(real) == 4 ? real(0.05) : real(1e-3);
+  if (fabs(x) < eps)
+    return real(0.5) + x * (real(1.0 / 3.0) + x * (real(0.125) + x * (real(1.0 / 30.0) + x * real(1.0 / 144.0))));
+  const real em = bc_math<real>::em1(x);
+  return (x * (em + real(1)) - em) / (x * x);
+}
+
+// Alpha/beta gate update x' = x + dt exprel(u) f, u = -dt phi (a + b):
+// returns dx'/dx = e^u and dx'/dV given da/dV, db/dV.
+template <typename real>
+__host__ __device__ __forceinline__ void bc_gate_ab_grad(real x, real a, real b, real da, real db, real phi, real dt,
+                                                         real& dxx, real& dxv) {
+  const real u = -dt * phi * (a + b);
+  const real er = bc_exprel<real>(u), erd = bc_exprel_d<real>(u);
+  const real f = phi * (a * (real(1) - x) - b * x);
+  const real common = dt * erd * (-dt * phi) * f;
+  const real dA = common + dt * er * phi * (real(1) - x);
+  const real dB = common - dt * er * phi * x;
+  dxx = real(1) + u * er;
+  dxv = dA * da + dB * db;
+}
+
+// Inf/tau gate update: x' = x + dt exprel(u) phi (inf - x) w, w = 1/tau, u = -dt phi w.
+template <typename real>
+__host__ __device__ __forceinline__ void bc_gate_it_grad(real x, real inf, real tau, real dinf, real dtau, real phi,
+                                                         real dt, real& dxx, real& dxv) {
+  const real w = real(1) / tau;
+  const real u = -dt * phi * w;
+  const real er = bc_exprel<real>(u), erd = bc_exprel_d<real>(u);
+  const real f = phi * (inf - x) * w;
+  const real dI = dt * er * phi * w;
+  const real dW = dt * erd * (-dt * phi) * f + dt * er * phi * (inf - x);
+  dxx = real(1) + u * er;
+  dxv = dI * dinf + dW * (-dtau * w * w);
+}
+
+template <typename real>
+__host__ __device__ __forceinline__ void bc_na_rates_d(real V, real vsh, real& ap, real& bp, real& aq, real& bq,
+                                                       real& dap, real& dbp, real& daq, real& dbq) {
+  typedef bc_math<real> M;
+  const real Vp = V - vsh;
+  const real xa = -(Vp - real(13)) * real(0.25), xb = (Vp - real(40)) * real(0.2);
+  const real ea = bc_exprel<real>(xa), eb = bc_exprel<real>(xb);
+  ap = real(1.28) / ea;
+  bp = real(1.4) / eb;
+  dap = real(0.32) * bc_exprel_d<real>(xa) / (ea * ea);
+  dbp = real(-0.28) * bc_exprel_d<real>(xb) / (eb * eb);
+  aq = real(0.128) * M::ex(-(Vp - real(17)) / real(18));
+  daq = -aq / real(18);
+  const real e = M::ex(-(Vp - real(40)) * real(0.2));
+  bq = real(4) / (real(1) + e);
+  dbq = bq * (e / (real(1) + e)) * real(0.2);
+}
+
+template <typename real>
+__host__ __device__ __forceinline__ void bc_kdr_rates_d(real V, real vsh, real& ap, real& bp, real& dap, real& dbp) {
+  typedef bc_math<real> M;
+  const real Vp = V - vsh;
+  const real x = -(Vp - real(15)) * real(0.2);
+  const real e = bc_exprel<real>(x);
+  ap = real(0.16) / e;
+  dap = real(0.032) * bc_exprel_d<real>(x) / (e * e);
+  bp = real(0.5) * M::ex(-(Vp - real(10)) / real(40));
+  dbp = -bp / real(40);
+}
+
+template <typename real>
+__host__ __device__ __forceinline__ void bc_cal_rates_d(real V, real vsh, real& pi, real& pt, real& qi, real& qt,
+                                                        real& dpi, real& dpt, real& dqi, real& dqt) {
+  typedef bc_math<real> M;
+  const real Vp = V - vsh;
+  pi = real(1) / (real(1) + M::ex(-(Vp + real(10)) * real(0.25)));
+  dpi = pi * (real(1) - pi) * real(0.25);
+  const real e1 = M::ex((Vp + real(5)) / real(15));
+  const real s1 = real(1) / e1 + e1;
+  pt = real(0.4) + real(0.7) / s1;
+  dpt = real(-0.7) * ((e1 - real(1) / e1) / real(15)) / (s1 * s1);
+  qi = real(1) / (real(1) + M::ex((Vp + real(25)) * real(0.5)));
+  dqi = -qi * (real(1) - qi) * real(0.5);
+  const real e2 = M::ex((Vp + real(40)) / real(9.5));
+  const real s2 = e2 + real(1) / e2;
+  qt = real(300) + real(100) / s2;
+  dqt = real(-100) * ((e2 - real(1) / e2) / real(9.5)) / (s2 * s2);
+}

@@ -1,0 +1,175 @@
+// Independent exponential-Euler for a population of thalamocortical point
+// neurons (one thread per neuron, every state in registers across steps).
+//
+// Restates ind_exp_euler_step (braincell/quad/_exp_euler.py:217-320,
+// :330-478) on SingleCompartment.update (braincell/_single_compartment/base.py:252-278):
+// every state y_k (V, nine gates, Ci) takes y_k + dt exprel(dt lam_k) f_k
+// with f_k and lam_k = df_k/dy_k evaluated at the old state (Jacobi).  The
+// Ca pool's current is evaluated at the old state (no step-start cache on
+// this path, braincell/ion/_base.py:490-506), so lam_Ci includes the Nernst
+// dependence of E_Ca on Ci while the drive is positive.
+//
+// State slots: Na p,q | KDR p | CaL p,q | CaT p,q | AHP p | HCN p | Ci.
+// Conductance columns: Na, K_Leak, KDR, CaL, CaT, AHP, HCN, IL (mS/cm^2).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bc_channels.h"
+#include "bc_common.h"
+
+namespace {
+
+template <typename real>
+struct PointArgs {
+  int64_t n;
+  real* V;
+  real* S;          // [10 * n]
+  const real* g;    // [8 * n]
+  const real* Iext; // [n] uA/cm^2
+  real* trace;      // [n_steps * n_trace] or null
+  int64_t n_trace;
+  uint32_t* spikes; // [n] or null
+  uint32_t* status;
+  int32_t n_steps;
+  real dt, Cm, V_th;
+  bc_chan_const<real> cc;
+};
+
+template <typename real>
+__global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
+  typedef bc_math<real> M;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int64_t n = a.n;
+  real V = a.V[i];
+  real s[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) s[k] = a.S[k * n + i];
+  real g[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) g[k] = a.g[k * n + i];
+  const real Iext = a.Iext[i];
+  const real dt = a.dt;
+  const bc_chan_const<real>& cc = a.cc;
+  const real invC = real(1) / a.Cm;
+  uint32_t nspk = 0, bad = 0;
+  for (int step = 0; step < a.n_steps; ++step) {
+    const real ci = s[9];
+    const real eca = cc.ca_rtzf * M::lg(cc.ca_Co / ci);
+    // conductances at the old state
+    const real g_na = g[0] * (s[0] * s[0] * s[0]) * s[1];
+    const real g_kl = g[1];
+    const real p2 = s[2] * s[2];
+    const real g_kd = g[2] * (p2 * p2);
+    const real g_cl = g[3] * (s[3] * s[3]) * s[4];
+    const real g_ct = g[4] * (s[5] * s[5]) * s[6];
+    const real g_ah = g[5] * (s[7] * s[7]);
+    const real g_hc = g[6] * s[8];
+    const real g_il = g[7];
+    const real G = g_na + g_kl + g_kd + g_cl + g_ct + g_ah + g_hc + g_il;
+    const real I = Iext + g_na * (cc.E_Na - V) + (g_kl + g_kd + g_ah) * (cc.E_K - V) + (g_cl + g_ct) * (eca - V) +
+                   g_hc * (cc.hcn_E - V) + g_il * (cc.E_IL - V);
+    const real Vn = bc_expeuler<real>(V, I * invC, -G * invC, dt);
+    // Ca pool with its own (old-state) current
+    {
+      const real gca = g_cl + g_ct;
+      const real raw = gca * (eca - V) * cc.ca_drive;
+      const real drive = raw > real(0) ? raw : real(0);
+      const real f = drive + (cc.ca_rest - ci) * cc.ca_tau_inv;
+      real lam = -cc.ca_tau_inv;
+      if (raw > real(0)) lam -= cc.ca_drive * gca * cc.ca_rtzf / ci;
+      s[9] = bc_expeuler<real>(ci, f, lam, dt);
+    }
+    {
+      real ap, bp, aq, bq;
+      bc_na_rates<real>(V, cc.na_vsh, ap, bp, aq, bq);
+      s[0] = bc_gate_ab<real>(s[0], ap, bp, cc.na_phi, dt);
+      s[1] = bc_gate_ab<real>(s[1], aq, bq, cc.na_phi, dt);
+    }
+    {
+      real ap, bp;
+      bc_kdr_rates<real>(V, cc.kdr_vsh, ap, bp);
+      s[2] = bc_gate_ab<real>(s[2], ap, bp, cc.kdr_phi, dt);
+    }
+    {
+      real pi, pt, qi, qt;
+      bc_cal_rates<real>(V, cc.cal_vsh, pi, pt, qi, qt);
+      s[3] = bc_gate_it<real>(s[3], pi, pt, cc.cal_phi_p, dt);
+      s[4] = bc_gate_it<real>(s[4], qi, qt, cc.cal_phi_q, dt);
+    }
+    {
+      real pi, pt, qi, qt;
+      bc_cat_rates<real>(V, cc.cat_vsh, pi, pt, qi, qt);
+      s[5] = bc_gate_it<real>(s[5], pi, pt, cc.cat_phi_p, dt);
+      s[6] = bc_gate_it<real>(s[6], qi, qt, cc.cat_phi_q, dt);
+    }
+    {
+      const real al = cc.ahp_alpha * M::pw(ci, cc.ahp_n);
+      s[7] = bc_gate_ab<real>(s[7], al, cc.ahp_beta, cc.ahp_phi, dt);
+    }
+    {
+      real pi, pt;
+      bc_hcn_rates<real>(V, pi, pt);
+      s[8] = bc_gate_it<real>(s[8], pi, pt, cc.hcn_phi, dt);
+    }
+    nspk += (V < a.V_th && Vn >= a.V_th) ? 1u : 0u;
+    bad |= (Vn == Vn && fabs(Vn) < real(1e30)) ? 0u : 1u;
+    V = Vn;
+    if (i < a.n_trace) a.trace[(int64_t)step * a.n_trace + i] = V;
+  }
+  a.V[i] = V;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) a.S[k * n + i] = s[k];
+  if (a.spikes) a.spikes[i] += nspk;
+  if (bad) atomicOr(a.status, 1u);
+}
+
+template <typename real>
+int launch_point(const bc_point_desc* d, hipStream_t st) {
+  PointArgs<real> a;
+  a.n = d->n;
+  a.V = static_cast<real*>(d->V);
+  a.S = static_cast<real*>(d->state);
+  a.g = static_cast<const real*>(d->g);
+  a.Iext = static_cast<const real*>(d->I_ext);
+  a.trace = static_cast<real*>(d->trace);
+  a.n_trace = d->trace ? d->n_trace : 0;
+  a.spikes = d->spikes;
+  a.status = d->status;
+  a.n_steps = d->n_steps;
+  a.dt = static_cast<real>(d->dt);
+  a.Cm = static_cast<real>(d->Cm);
+  a.V_th = static_cast<real>(d->V_th);
+  a.cc = *static_cast<const bc_chan_const<real>*>(d->chan_const);
+  if (d->n == 0 || d->n_steps == 0) return BC_OK;
+  const int bs = 256;
+  dim3 grid((unsigned)((d->n + bs - 1) / bs));
+  hipLaunchKernelGGL(point_kernel<real>, grid, dim3(bs), 0, st, a);
+  BC_HIP(hipGetLastError());
+  return BC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bc_point_supported(int precision) { return precision == 32 || precision == 64; }
+
+int bc_point_run(const bc_point_desc* d, void* stream) {
+  if (d == nullptr) return bc::fail(BC_INVALID_ARG, "desc is NULL");
+  if (d->precision != 32 && d->precision != 64) return bc::fail(BC_INVALID_ARG, "precision must be 32 or 64");
+  if (d->n < 0 || d->n_steps < 0) return bc::fail(BC_INVALID_ARG, "n and n_steps must be >= 0");
+  if (!(d->dt > 0.0) || !(d->Cm > 0.0)) return bc::fail(BC_INVALID_ARG, "dt and Cm must be > 0");
+  if (d->n > 0 && (d->V == nullptr || d->state == nullptr || d->g == nullptr || d->I_ext == nullptr ||
+                   d->status == nullptr || d->chan_const == nullptr))
+    return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
+  if (d->trace != nullptr && (d->n_trace < 0 || d->n_trace > d->n))
+    return bc::fail(BC_INVALID_ARG, "n_trace must lie in [0, n]");
+  int rc = d->precision == 32 ? launch_point<float>(d, static_cast<hipStream_t>(stream))
+                              : launch_point<double>(d, static_cast<hipStream_t>(stream));
+  if (rc != BC_OK) return rc;
+  return bc::ok();
+}
+
+}  // extern "C"

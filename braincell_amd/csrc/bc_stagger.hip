@@ -66,6 +66,7 @@ struct StaggerArgs {
   int64_t n_cv_total;
   real* trace;
   int64_t trace_ld;
+  real* rec;  // optional: [(n_steps + 1) * (NS + 1) * n_cv_total] step-start V and states
   uint32_t* status;
   int32_t step0;
   int32_t n_steps;
@@ -198,6 +199,23 @@ __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, re
   }
 }
 
+// Step-start snapshot for checkpoint-and-recompute: slot 0 = V, slots 1.. =
+// gate/pool states, laid out [step][slot][cv].
+template <typename real, int MAXC, int NS>
+__device__ __forceinline__ void record_state(const StaggerArgs<real>& a, const bc_pack& P, int tid, int NT, int step,
+                                             const real* vv, const real (*st)[NS], const uint32_t* info) {
+  const int64_t base = (int64_t)step * (NS + 1) * a.n_cv_total;
+#pragma unroll
+  for (int i = 0; i < MAXC; ++i) {
+    if (info[i] != 0xFFFFFFFFu) {
+      const int64_t g = (int64_t)P.cv0 + tid + i * NT;
+      a.rec[base + g] = vv[i];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) a.rec[base + (int64_t)(s + 1) * a.n_cv_total + g] = st[i][s];
+    }
+  }
+}
+
 template <typename real, uint32_t CH, int MAXC>
 __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   constexpr int NS = bc_nstates(CH) > 0 ? bc_nstates(CH) : 1;
@@ -250,6 +268,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   uint32_t bad = 0;
   for (int step = 0; step < a.n_steps; ++step) {
     const int gstep = a.step0 + step;
+    if (a.rec) record_state<real, MAXC, NS>(a, P, tid, NT, step, vv, st, info);
     // ---- phase A: linearise at V_n (midpoint rows)
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
@@ -351,6 +370,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
   }
 
+  if (a.rec) record_state<real, MAXC, NS>(a, P, tid, NT, a.n_steps, vv, st, info);
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
     if (info[i] != 0xFFFFFFFFu) {
@@ -425,6 +445,7 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->n_cv_total = d->n_cv_total;
   a->trace = static_cast<real*>(d->trace);
   a->trace_ld = d->trace_ld;
+  a->rec = static_cast<real*>(d->rec);
   a->status = d->status;
   a->step0 = d->step0;
   a->n_steps = d->n_steps;

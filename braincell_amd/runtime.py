@@ -31,6 +31,7 @@ KERNEL_CHANNEL_SETS = (
     sum(B.values()) | CA_POOL_BIT,
 )
 INT32_MAX = 2 ** 31 - 1
+REGION_CODES = ("soma", "basal_dendrite", "apical_dendrite", "dendrite", "axon", "custom")
 
 
 def choose_channel_set(mask: int) -> int:
@@ -92,10 +93,11 @@ class StaggerLayout:
     """Host-side packed layout of a batch of cells."""
 
     def __init__(self, models, *, threads: int = 256, cv_per_thread: int = None, rows_per_pack: int = None,
-                 probes="soma"):
+                 probes="soma", region_classes: bool = False):
         if len(models) == 0:
             raise ValueError("need at least one cell")
         self.models = list(models)
+        self.region_classes = bool(region_classes)
         mask = 0
         for m in self.models:
             m.validate()
@@ -188,6 +190,8 @@ class StaggerLayout:
             table[:, CLS_COL[n]] = np.asarray(spec["g_branch"], dtype=np.float64) * 1e3  # mS/cm^2
         if m.ca is not None:
             table[:, 9] = np.asarray(m.ca["mask_branch"], dtype=np.float64)
+        if self.region_classes:  # keep painted regions apart (one class per branch type)
+            table[:, 10] = [REGION_CODES.index(b.type) for b in m.morph.branches]
         uniq, inv = np.unique(table, axis=0, return_inverse=True)
         cv_cls = inv.reshape(-1)[geom.cv_branch]
         clamps = []
@@ -236,6 +240,7 @@ class StaggerLayout:
         cls_tab, clamp_rows, clamp_t, clamp_amp_raw, clamp_kind, clamp_area = [], [], [], [], [], []
         probe_row, probe_col = [], []
         bnd_cell = []
+        self.cell_region_cls = np.full((n_cells, len(REGION_CODES)), -1, dtype=np.int64)
         self.cell_cv0 = np.zeros(n_cells, dtype=np.int64)
         self.cell_ncv = np.zeros(n_cells, dtype=np.int64)
         self.cell_probe0 = np.zeros(n_cells + 1, dtype=np.int64)
@@ -288,6 +293,10 @@ class StaggerLayout:
                     for k in range(c.chains.level_off[lv], c.chains.level_off[lv + 1]):
                         per_level[lv].append((i, k))
                 cls_tab.append(c.cls_rows)
+                if self.region_classes:
+                    for code in range(len(REGION_CODES)):
+                        hit = np.nonzero(c.cls_rows[:, 10] == code)[0]
+                        self.cell_region_cls[i, code] = gcls + cls_off + hit[0] if hit.size else -1
                 row_off += g.n_point
                 cv_off += g.n_cv
                 cls_off += c.cls_rows.shape[0]
@@ -494,7 +503,7 @@ class StaggerRuntime:
         self.step_index = 0
         self.status.zero_()
 
-    def desc(self, n_steps: int, trace=None, step0: int = None) -> StaggerDesc:
+    def desc(self, n_steps: int, trace=None, step0: int = None, rec=None) -> StaggerDesc:
         L = self.L
         d = StaggerDesc()
         d.precision = self.precision
@@ -524,6 +533,7 @@ class StaggerRuntime:
         elif L.n_probes > 0 and n_steps > 0:
             raise ValueError("a trace buffer is required when probes exist")
         d.trace_ld = L.n_probes
+        d.rec = p(rec) if rec is not None else None
         d.status = p(self.status)
         d.step0 = self.step_index if step0 is None else int(step0)
         d.n_steps = int(n_steps)
@@ -536,15 +546,18 @@ class StaggerRuntime:
     def new_trace(self, n_steps: int):
         return self.torch.empty((n_steps, self.L.n_probes), dtype=self.dtype, device=self.device)
 
-    def run(self, n_steps: int, trace=None, chunk: int = None):
-        """Advance ``n_steps`` steps; returns the probe trace [n_steps, n_probes]."""
+    def run(self, n_steps: int, trace=None, chunk: int = None, rec=None):
+        """Advance ``n_steps`` steps; returns the probe trace [n_steps, n_probes].
+        ``rec`` (one launch only) receives the step-start snapshots."""
         if trace is None:
             trace = self.new_trace(n_steps)
         chunk = n_steps if chunk is None else int(chunk)
+        if rec is not None and chunk < n_steps:
+            raise ValueError("recording needs a single launch (chunk >= n_steps)")
         done = 0
         while done < n_steps:
             k = min(chunk, n_steps - done)
-            d = self.desc(k, trace[done:] if self.L.n_probes else None)
+            d = self.desc(k, trace[done:] if self.L.n_probes else None, rec=rec)
             self._keep = d
             check(lib().bc_stagger_run(ctypes.byref(d), self.stream_handle()))
             done += k

@@ -75,6 +75,10 @@ typedef struct bc_pack {
  *  V[cv], state[s * n_cv_total + cv]  voltage and gate/pool states (in/out)
  *  Vbnd[i]        voltage of boundary row i (bnd_info order; in/out)
  *  trace[step * trace_ld + col]  probe voltages after each step (out)
+ *  rec            optional (NULL to skip): step-start snapshots for the
+ *                 adjoint, rec[((step * (NS + 1)) + slot) * n_cv_total + cv],
+ *                 slot 0 = V, slots 1..NS = states, steps 0 .. n_steps
+ *                 (the last one is the final state)
  *  status         OR-ed with 1 when a non-finite voltage appears (out)
  * real = float (precision 32) or double (precision 64). */
 typedef struct bc_stagger_desc {
@@ -110,6 +114,7 @@ typedef struct bc_stagger_desc {
   void* state;
   void* trace;
   int64_t trace_ld;
+  void* rec;
   uint32_t* status;
   int32_t step0;   /* global index of the first step (clamp windows) */
   int32_t n_steps;
@@ -140,6 +145,62 @@ int bc_stagger_graph_create(const bc_stagger_desc* desc, int32_t n_chunks, int32
                             void* stream, void** graph_out);
 int bc_graph_launch(void* graph, void* stream);
 int bc_graph_destroy(void* graph);
+
+/* Reverse pass over one checkpoint segment of n_steps steps (steps are
+ * processed last to first).  layout: the forward descriptor (same arrays,
+ * dt and channel set; its state pointers are not read).
+ *  rec        forward recording of the segment (see bc_stagger_desc.rec)
+ *  trace_bar  d loss / d trace for the segment's steps [n_steps * trace_ld]
+ *  Vbar, Sbar adjoints of V and of the states at the segment end (in) and
+ *             at its start (out), laid out like V / state
+ *  cls_grad   += d loss / d cls_tab (per mechanism class; columns of the
+ *             channel densities, mS/cm^2)
+ * Channel sets: IL; Na+KDR+IL; Na+KDR+IL+CaL+pool.  No counterpart in the
+ * reference, which differentiates its JAX scan automatically; this is the
+ * hand-written reverse of _staggered.py:126-154. */
+typedef struct bc_adjoint_desc {
+  const bc_stagger_desc* layout;
+  int32_t n_steps;
+  const void* rec;
+  const void* trace_bar;
+  void* Vbar;
+  void* Sbar;
+  void* cls_grad;
+} bc_adjoint_desc;
+
+int bc_adjoint_supported(int precision, uint32_t channels, int cv_per_thread);
+int bc_stagger_adjoint(const bc_adjoint_desc* desc, void* stream);
+
+/* Point-neuron population (thalamocortical template: Na_Ba2002, K_Leak,
+ * KDR_Ba2002, CaL_IS2008, CaT_HM1992, AHP_De1994, HCN_HM1992, IL and a
+ * CalciumDetailed pool) integrated by independent exponential Euler.
+ *  state[k * n + i]  k = Na p, Na q, KDR p, CaL p, CaL q, CaT p, CaT q,
+ *                    AHP p, HCN p, Ci
+ *  g[k * n + i]      k = Na, K_Leak, KDR, CaL, CaT, AHP, HCN, IL (mS/cm^2)
+ *  I_ext[i]          injected current density (uA/cm^2)
+ *  trace[step * n_trace + i]  V of neurons 0 .. n_trace-1 after each step
+ *  spikes[i]         += upward crossings of V_th */
+typedef struct bc_point_desc {
+  int32_t precision;
+  int64_t n;
+  int32_t n_steps;
+  double dt, Cm, V_th;
+  const void* chan_const; /* host pointer to bc_chan_const<real> */
+  void* V;
+  void* state;
+  const void* g;
+  const void* I_ext;
+  void* trace;
+  int64_t n_trace;
+  uint32_t* spikes;
+  uint32_t* status;
+} bc_point_desc;
+
+int bc_point_supported(int precision);
+/* Replaces SingleCompartment.update with solver ind_exp_euler
+ * (braincell/_single_compartment/base.py:252-278 ->
+ * braincell/quad/_exp_euler.py:217-320) for n_steps steps. */
+int bc_point_run(const bc_point_desc* desc, void* stream);
 
 #ifdef __cplusplus
 }
