@@ -36,6 +36,34 @@ struct PointArgs {
   bc_chan_const<real> cc;
 };
 
+// Adds d to the pair (hi, lo) without losing the low bits (fast two-sum,
+// |hi| >= |d| for state increments): slow gates change by less than half an
+// FP32 ulp per step, and plain accumulation would drop those increments.
+template <typename real>
+__device__ __forceinline__ void acc2(real& hi, real& lo, real d) {
+  const real s = hi + d;
+  lo += d - (s - hi);
+  const real h = s + lo;
+  lo -= h - s;
+  hi = h;
+}
+
+template <typename real>
+__device__ __forceinline__ real incr(real y, real f, real lam, real dt) {
+  return dt * bc_exprel<real>(dt * lam) * f;
+}
+
+template <typename real>
+__device__ __forceinline__ real incr_ab(real x, real a, real b, real phi, real dt) {
+  return incr<real>(x, phi * (a * (real(1) - x) - b * x), -phi * (a + b), dt);
+}
+
+template <typename real>
+__device__ __forceinline__ real incr_it(real x, real inf, real tau, real phi, real dt) {
+  const real it = real(1) / tau;
+  return incr<real>(x, phi * (inf - x) * it, -phi * it, dt);
+}
+
 template <typename real>
 __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
   typedef bc_math<real> M;
@@ -43,9 +71,12 @@ __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
   if (i >= a.n) return;
   const int64_t n = a.n;
   real V = a.V[i];
-  real s[10];
+  real s[10], lo[10], vlo = real(0);
 #pragma unroll
-  for (int k = 0; k < 10; ++k) s[k] = a.S[k * n + i];
+  for (int k = 0; k < 10; ++k) {
+    s[k] = a.S[k * n + i];
+    lo[k] = real(0);
+  }
   real g[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) g[k] = a.g[k * n + i];
@@ -70,7 +101,8 @@ __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
     const real G = g_na + g_kl + g_kd + g_cl + g_ct + g_ah + g_hc + g_il;
     const real I = Iext + g_na * (cc.E_Na - V) + (g_kl + g_kd + g_ah) * (cc.E_K - V) + (g_cl + g_ct) * (eca - V) +
                    g_hc * (cc.hcn_E - V) + g_il * (cc.E_IL - V);
-    const real Vn = bc_expeuler<real>(V, I * invC, -G * invC, dt);
+    const real dV = incr<real>(V, I * invC, -G * invC, dt);
+    real d[10];
     // Ca pool with its own (old-state) current
     {
       const real gca = g_cl + g_ct;
@@ -79,40 +111,44 @@ __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
       const real f = drive + (cc.ca_rest - ci) * cc.ca_tau_inv;
       real lam = -cc.ca_tau_inv;
       if (raw > real(0)) lam -= cc.ca_drive * gca * cc.ca_rtzf / ci;
-      s[9] = bc_expeuler<real>(ci, f, lam, dt);
+      d[9] = incr<real>(ci, f, lam, dt);
     }
     {
       real ap, bp, aq, bq;
       bc_na_rates<real>(V, cc.na_vsh, ap, bp, aq, bq);
-      s[0] = bc_gate_ab<real>(s[0], ap, bp, cc.na_phi, dt);
-      s[1] = bc_gate_ab<real>(s[1], aq, bq, cc.na_phi, dt);
+      d[0] = incr_ab<real>(s[0], ap, bp, cc.na_phi, dt);
+      d[1] = incr_ab<real>(s[1], aq, bq, cc.na_phi, dt);
     }
     {
       real ap, bp;
       bc_kdr_rates<real>(V, cc.kdr_vsh, ap, bp);
-      s[2] = bc_gate_ab<real>(s[2], ap, bp, cc.kdr_phi, dt);
+      d[2] = incr_ab<real>(s[2], ap, bp, cc.kdr_phi, dt);
     }
     {
       real pi, pt, qi, qt;
       bc_cal_rates<real>(V, cc.cal_vsh, pi, pt, qi, qt);
-      s[3] = bc_gate_it<real>(s[3], pi, pt, cc.cal_phi_p, dt);
-      s[4] = bc_gate_it<real>(s[4], qi, qt, cc.cal_phi_q, dt);
+      d[3] = incr_it<real>(s[3], pi, pt, cc.cal_phi_p, dt);
+      d[4] = incr_it<real>(s[4], qi, qt, cc.cal_phi_q, dt);
     }
     {
       real pi, pt, qi, qt;
       bc_cat_rates<real>(V, cc.cat_vsh, pi, pt, qi, qt);
-      s[5] = bc_gate_it<real>(s[5], pi, pt, cc.cat_phi_p, dt);
-      s[6] = bc_gate_it<real>(s[6], qi, qt, cc.cat_phi_q, dt);
+      d[5] = incr_it<real>(s[5], pi, pt, cc.cat_phi_p, dt);
+      d[6] = incr_it<real>(s[6], qi, qt, cc.cat_phi_q, dt);
     }
     {
       const real al = cc.ahp_alpha * M::pw(ci, cc.ahp_n);
-      s[7] = bc_gate_ab<real>(s[7], al, cc.ahp_beta, cc.ahp_phi, dt);
+      d[7] = incr_ab<real>(s[7], al, cc.ahp_beta, cc.ahp_phi, dt);
     }
     {
       real pi, pt;
       bc_hcn_rates<real>(V, pi, pt);
-      s[8] = bc_gate_it<real>(s[8], pi, pt, cc.hcn_phi, dt);
+      d[8] = incr_it<real>(s[8], pi, pt, cc.hcn_phi, dt);
     }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) acc2<real>(s[k], lo[k], d[k]);
+    real Vn = V;
+    acc2<real>(Vn, vlo, dV);
     nspk += (V < a.V_th && Vn >= a.V_th) ? 1u : 0u;
     bad |= (Vn == Vn && fabs(Vn) < real(1e30)) ? 0u : 1u;
     V = Vn;
