@@ -36,8 +36,8 @@ class StaggerDesc(ctypes.Structure):
         ("n_cv_total", ctypes.c_int64),
         ("packs", ctypes.c_void_p), ("cv_info", ctypes.c_void_p), ("bnd_info", ctypes.c_void_p),
         ("chain_off", ctypes.c_void_p), ("chain_top", ctypes.c_void_p), ("kid_off", ctypes.c_void_p),
-        ("kids", ctypes.c_void_p), ("level_off", ctypes.c_void_p), ("cr_row", ctypes.c_void_p),
-        ("cr_lo", ctypes.c_void_p), ("cr_up", ctypes.c_void_p), ("cr_dg", ctypes.c_void_p),
+        ("kids", ctypes.c_void_p), ("level_off", ctypes.c_void_p),
+        ("lo", ctypes.c_void_p), ("up", ctypes.c_void_p),
         ("cls_tab", ctypes.c_void_p), ("clamp_row", ctypes.c_void_p), ("clamp_on", ctypes.c_void_p),
         ("clamp_off", ctypes.c_void_p), ("clamp_amp", ctypes.c_void_p), ("probe_row", ctypes.c_void_p),
         ("probe_col", ctypes.c_void_p), ("chan_const", ctypes.c_void_p), ("V", ctypes.c_void_p), ("Vbnd", ctypes.c_void_p),

@@ -33,10 +33,8 @@ struct AdjArgs {
   const int32_t* kid_off;
   const int32_t* kids;
   const int32_t* level_off;
-  const int32_t* cr_row;
-  const real* cr_lo;
-  const real* cr_up;
-  const real* cr_dg;
+  const real* lo;
+  const real* up;
   const real* cls_tab;
   const int32_t* probe_row;
   const int32_t* probe_col;
@@ -64,7 +62,9 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   real* bd = reinterpret_cast<real*>(smem_raw);
   real* br = bd + a.max_rows;
-  real* kd = br + a.max_rows;
+  real* LO = br + a.max_rows;
+  real* UP = LO + a.max_rows;
+  real* kd = UP + a.max_rows;
   real* kr = kd + a.max_chains;
   real* cls = kr + a.max_chains;
   real* gacc = cls + a.max_cls * BC_NCLS;
@@ -75,6 +75,10 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   for (int i = tid; i < P.n_cls * BC_NCLS; i += NT) {
     cls[i] = a.cls_tab[(int64_t)P.cls0 * BC_NCLS + i];
     gacc[i] = real(0);
+  }
+  for (int i = tid; i < P.n_rows; i += NT) {
+    LO[i] = a.lo[(int64_t)P.row0 + i];
+    UP[i] = a.up[(int64_t)P.row0 + i];
   }
   const bc_chan_const<real>& cc = a.cc;
   const real dt = a.dt;
@@ -215,36 +219,34 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
     }
     __syncthreads();
 
-    // ---- phase 2: Rbar = M^-T Xbar (lowers <-> uppers)
+    // ---- phase 2: Rbar = M^-T Xbar.  M^T has the same diagonal (rebuilt as
+    // e - lo, as in the forward solve); the off-diagonals swap, so the
+    // child's rhs contribution uses lowers and the back substitution uppers.
     for (int l = 0; l < P.n_lv; ++l) {
       const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
       for (int k = k0 + tid; k < k1; k += NT) {
         const int gk = P.ch0 + k;
-        const int j0 = a.chain_off[gk], j1 = a.chain_off[gk + 1];
-        int row = a.cr_row[j0];
-        real d = a.cr_dg[j0] + bd[row];
-        real r = br[row];
+        const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
+        real e = bd[j0];
+        real r = br[j0];
         for (int q = a.kid_off[gk]; q < a.kid_off[gk + 1]; ++q) {
-          d -= kd[a.kids[q]];
+          e -= kd[a.kids[q]];
           r -= kr[a.kids[q]];
         }
-        real inv = real(1) / d;
-        bd[row] = inv;
-        br[row] = r;
-        real m = a.cr_lo[j0] * inv;  // transposed: the parent's coefficient is lowers[row]
-        real lo = a.cr_up[j0];
-        for (int j = j0 + 1; j < j1; ++j) {
-          row = a.cr_row[j];
-          d = (a.cr_dg[j] + bd[row]) - lo * m;
-          r = br[row] - r * m;
-          inv = real(1) / d;
-          bd[row] = inv;
-          br[row] = r;
-          m = a.cr_lo[j] * inv;
-          lo = a.cr_up[j];
+        real kdc, krc;
+        for (int j = j0;;) {
+          const real lo = LO[j], up = UP[j];
+          const real inv = bc_rcp<real>(e - lo);
+          bd[j] = inv;
+          br[j] = r;
+          kdc = up * e * inv;
+          krc = lo * r * inv;
+          if (++j >= j1) break;
+          e = bd[j] - kdc;
+          r = br[j] - krc;
         }
-        kd[k] = lo * m;
-        kr[k] = r * m;
+        kd[k] = kdc;
+        kr[k] = krc;
       }
       __syncthreads();
     }
@@ -252,13 +254,12 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
       const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
       for (int k = k0 + tid; k < k1; k += NT) {
         const int gk = P.ch0 + k;
-        const int j0 = a.chain_off[gk], j1 = a.chain_off[gk + 1];
+        const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
         const int tp = a.chain_top[gk];
         real xp = tp >= 0 ? br[tp] : real(0);
         for (int j = j1 - 1; j >= j0; --j) {
-          const int row = a.cr_row[j];
-          const real x = (br[row] - a.cr_up[j] * xp) * bd[row];
-          br[row] = x;
+          const real x = (br[j] - UP[j] * xp) * bd[j];
+          br[j] = x;
           xp = x;
         }
       }
@@ -370,10 +371,8 @@ int adj_launch(const bc_adjoint_desc* d, hipStream_t s) {
   a.kid_off = f->kid_off;
   a.kids = f->kids;
   a.level_off = f->level_off;
-  a.cr_row = f->cr_row;
-  a.cr_lo = static_cast<const real*>(f->cr_lo);
-  a.cr_up = static_cast<const real*>(f->cr_up);
-  a.cr_dg = static_cast<const real*>(f->cr_dg);
+  a.lo = static_cast<const real*>(f->lo);
+  a.up = static_cast<const real*>(f->up);
   a.cls_tab = static_cast<const real*>(f->cls_tab);
   a.probe_row = f->probe_row;
   a.probe_col = f->probe_col;
@@ -390,7 +389,7 @@ int adj_launch(const bc_adjoint_desc* d, hipStream_t s) {
   a.max_chains = f->max_chains;
   a.max_cls = f->max_cls;
   a.cc = *static_cast<const bc_chan_const<real>*>(f->chan_const);
-  const size_t lds = (size_t)(2 * f->max_rows + 2 * f->max_chains + 2 * f->max_cls * BC_NCLS) * sizeof(real);
+  const size_t lds = (size_t)(4 * f->max_rows + 2 * f->max_chains + 2 * f->max_cls * BC_NCLS) * sizeof(real);
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "adjoint pack needs %zu bytes of LDS", lds);
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (f->n_packs == 0 || d->n_steps == 0) return BC_OK;

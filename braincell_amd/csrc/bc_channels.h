@@ -90,6 +90,12 @@ template <> struct bc_math<double> {
   static constexpr double exprel_eps = 1e-8;
 };
 
+// Reciprocal used by the Hines elimination: the hardware v_rcp_f32 (1 ulp)
+// in FP32, an IEEE division in FP64.
+template <typename real> __device__ __forceinline__ real bc_rcp(real x);
+template <> __device__ __forceinline__ float bc_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double bc_rcp<double>(double x) { return 1.0 / x; }
+
 // (e^x - 1) / x with its removable singularity filled by the series.
 template <typename real>
 __host__ __device__ __forceinline__ real bc_exprel(real x) {

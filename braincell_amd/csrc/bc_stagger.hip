@@ -49,10 +49,8 @@ struct StaggerArgs {
   const int32_t* kid_off;
   const int32_t* kids;
   const int32_t* level_off;
-  const int32_t* cr_row;
-  const real* cr_lo;
-  const real* cr_up;
-  const real* cr_dg;
+  const real* lo;
+  const real* up;
   const real* cls_tab;
   const int32_t* clamp_row;
   const int32_t* clamp_on;
@@ -222,8 +220,10 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   real* bd = reinterpret_cast<real*>(smem_raw);
   real* br = bd + a.max_rows;
-  real* xv = br + a.max_rows;  // V_n at every row
-  real* kd = xv + a.max_rows;
+  real* xv = br + a.max_rows;  // V_n at every slot
+  real* LO = xv + a.max_rows;  // dt * A[slot, parent]
+  real* UP = LO + a.max_rows;  // dt * A[parent, slot]
+  real* kd = UP + a.max_rows;
   real* kr = kd + a.max_chains;
   real* cls = kr + a.max_chains;
 
@@ -231,6 +231,10 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   const int tid = threadIdx.x;
   const int NT = blockDim.x;
   for (int i = tid; i < P.n_cls * BC_NCLS; i += NT) cls[i] = a.cls_tab[(int64_t)P.cls0 * BC_NCLS + i];
+  for (int i = tid; i < P.n_rows; i += NT) {
+    LO[i] = a.lo[(int64_t)P.row0 + i];
+    UP[i] = a.up[(int64_t)P.row0 + i];
+  }
 
   real st[MAXC][NS];
   real vv[MAXC];
@@ -292,44 +296,46 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
     __syncthreads();
 
-    // ---- phase B: Hines elimination, leaves to root, one chain per thread
+    // ---- phase B: Hines elimination, leaves to root, one chain per thread.
+    // Diagonal rebuilt from the zero row sums of the axial operator:
+    // d_j = e_j - lo_j with e_j = diag_j - sum over children c of
+    // up_c e_c / d_c >= 0, so no step subtracts nearly equal numbers.
     for (int l = 0; l < P.n_lv; ++l) {
       const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
       for (int k = k0 + tid; k < k1; k += NT) {
         const int gk = P.ch0 + k;
-        const int j0 = a.chain_off[gk], j1 = a.chain_off[gk + 1];
+        const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
         const int tp = a.chain_top[gk];
-        int row = a.cr_row[j0];
-        real vr = xv[row];
-        real d = a.cr_dg[j0] + bd[row];
-        real r = br[row];
+        real e = bd[j0];
+        real r = br[j0];
         const int q0 = a.kid_off[gk], q1 = a.kid_off[gk + 1];
         for (int q = q0; q < q1; ++q) {
           const int kk = a.kids[q];
-          d -= kd[kk];
+          e -= kd[kk];
           r -= kr[kk];
         }
-        real m, lo, up, dl;
-        for (int j = j0;; ) {
-          // axial defect of the edge to the parent row: dt A V_n terms
-          const int pr = (j + 1 < j1) ? a.cr_row[j + 1] : tp;
+        real vr = xv[j0];
+        real kdc, krc, up, dl;
+        for (int j = j0;;) {
+          // axial defect of the edge to the parent: dt A V_n terms
+          const int pr = (j + 1 < j1) ? j + 1 : tp;
           const real vp = pr >= 0 ? xv[pr] : vr;
-          lo = a.cr_lo[j];
-          up = a.cr_up[j];
+          const real lo = LO[j];
+          up = UP[j];
           dl = vr - vp;
           r += lo * dl;
-          const real inv = real(1) / d;
-          bd[row] = inv;
-          br[row] = r;
-          m = up * inv;
+          const real inv = bc_rcp<real>(e - lo);
+          bd[j] = inv;
+          br[j] = r;
+          kdc = up * e * inv;
+          krc = up * r * inv;
           if (++j >= j1) break;
-          row = pr;
           vr = vp;
-          d = (a.cr_dg[j] + bd[row]) - lo * m;
-          r = (br[row] - r * m) - up * dl;
+          e = bd[j] - kdc;
+          r = (br[j] - krc) - up * dl;
         }
-        kd[k] = lo * m;
-        kr[k] = r * m + up * dl;
+        kd[k] = kdc;
+        kr[k] = krc + up * dl;
       }
       __syncthreads();
     }
@@ -338,13 +344,12 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
       for (int k = k0 + tid; k < k1; k += NT) {
         const int gk = P.ch0 + k;
-        const int j0 = a.chain_off[gk], j1 = a.chain_off[gk + 1];
+        const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
         const int tp = a.chain_top[gk];
         real xp = tp >= 0 ? br[tp] : real(0);
         for (int j = j1 - 1; j >= j0; --j) {
-          const int row = a.cr_row[j];
-          const real x = (br[row] - a.cr_lo[j] * xp) * bd[row];
-          br[row] = x;
+          const real x = (br[j] - LO[j] * xp) * bd[j];
+          br[j] = x;
           xp = x;
         }
       }
@@ -428,10 +433,8 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->kid_off = d->kid_off;
   a->kids = d->kids;
   a->level_off = d->level_off;
-  a->cr_row = d->cr_row;
-  a->cr_lo = static_cast<const real*>(d->cr_lo);
-  a->cr_up = static_cast<const real*>(d->cr_up);
-  a->cr_dg = static_cast<const real*>(d->cr_dg);
+  a->lo = static_cast<const real*>(d->lo);
+  a->up = static_cast<const real*>(d->up);
   a->cls_tab = static_cast<const real*>(d->cls_tab);
   a->clamp_row = d->clamp_row;
   a->clamp_on = d->clamp_on;
@@ -486,7 +489,7 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
   a.step0 = step0;
   a.n_steps = n_steps;
   a.trace = static_cast<real*>(trace);
-  size_t lds = (size_t)(3 * d->max_rows + 2 * d->max_chains + d->max_cls * BC_NCLS) * sizeof(real);
+  size_t lds = (size_t)(5 * d->max_rows + 2 * d->max_chains + d->max_cls * BC_NCLS) * sizeof(real);
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "pack needs %zu bytes of LDS (> 160 KiB)", lds);
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (d->n_packs == 0 || n_steps == 0) return BC_OK;

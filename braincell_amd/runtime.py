@@ -232,48 +232,94 @@ class StaggerLayout:
 
     # ------------------------------------------------------------- assemble
     def _assemble(self):
+        """Flatten the packs.  Inside a pack, rows are renumbered into
+        chain-major *slots*: chain k occupies slots chain_off[k] ..
+        chain_off[k+1]-1 bottom to top, chains sorted by level, so the kernel
+        walks contiguous LDS.  ``slot_rows`` keeps, per pack, the reference
+        row (cell row offset + ``row_to_point_id`` order) of every slot."""
         n_cells = len(self.cells)
         packs = []
-        cv_info, bnd_info = [], []
+        cv_info, bnd_info, bnd_cell = [], [], []
         chain_off, chain_top, kid_cnt, kids, level_off = [0], [], [], [], []
-        cr_row, cr_lo, cr_up, cr_dg = [], [], [], []
+        slot_lo, slot_up, slot_rows_all = [], [], []
         cls_tab, clamp_rows, clamp_t, clamp_amp_raw, clamp_kind, clamp_area = [], [], [], [], [], []
         probe_row, probe_col = [], []
-        bnd_cell = []
         self.cell_region_cls = np.full((n_cells, len(REGION_CODES)), -1, dtype=np.int64)
         self.cell_cv0 = np.zeros(n_cells, dtype=np.int64)
         self.cell_ncv = np.zeros(n_cells, dtype=np.int64)
         self.cell_probe0 = np.zeros(n_cells + 1, dtype=np.int64)
-        probe_counts = np.asarray([len(c.probes) for c in self.cells])
-        self.cell_probe0[1:] = np.cumsum(probe_counts)
+        self.cell_pack = np.zeros(n_cells, dtype=np.int64)
+        self.cell_row_off = np.zeros(n_cells, dtype=np.int64)
+        self.cell_probe0[1:] = np.cumsum([len(c.probes) for c in self.cells])
         gcv = grow = gch = glv = gbnd = gcls = gclamp = gprobe = 0
-        for cells in self.pack_cells:
+        for pi, cells in enumerate(self.pack_cells):
             P = BcPack()
             P.cv0, P.row0, P.ch0, P.lv0, P.bnd0, P.cls0, P.clamp0, P.probe0 = gcv, grow, gch, glv, gbnd, gcls, gclamp, gprobe
-            row_off = cv_off = cls_off = 0
-            nlev = max(self.cells[i].chains.n_levels for i in cells)
-            per_level = [[] for _ in range(nlev)]
-            cell_row_off = {}
-            for i in cells:
+            cs = [self.cells[i] for i in cells]
+            nrow = np.asarray([c.geom.n_point for c in cs])
+            nch = np.asarray([c.chains.n_chains for c in cs])
+            row_off = np.concatenate(([0], np.cumsum(nrow)))
+            ch_base = np.concatenate(([0], np.cumsum(nch)))
+            n_rows, n_ch = int(row_off[-1]), int(ch_base[-1])
+            lev = np.concatenate([c.chains.level for c in cs])
+            cpos = np.repeat(np.arange(len(cs)), nch)
+            kidx = np.concatenate([np.arange(n) for n in nch])
+            perm = np.lexsort((kidx, cpos, lev))
+            inv = np.empty(n_ch, dtype=np.int64)
+            inv[perm] = np.arange(n_ch)
+            lens_old = np.concatenate([np.diff(c.chains.off) for c in cs])
+            seg_old = np.concatenate([c.chains.off[:-1] + np.sum(nrow[:j]) for j, c in enumerate(cs)])
+            allrows = np.concatenate([c.chains.rows + row_off[j] for j, c in enumerate(cs)])
+            lens = lens_old[perm]
+            new_off = np.concatenate(([0], np.cumsum(lens)))
+            idx = np.repeat(seg_old[perm] - new_off[:-1], lens) + np.arange(n_rows)
+            slot_rows = allrows[idx]  # pack-local reference row of each slot
+            slot_of = np.empty(n_rows, dtype=np.int64)
+            slot_of[slot_rows] = np.arange(n_rows)
+            lowers = np.concatenate([c.geom.lowers for c in cs])
+            uppers = np.concatenate([c.geom.uppers for c in cs])
+            slot_lo.append(lowers[slot_rows])
+            slot_up.append(uppers[slot_rows])
+            slot_rows_all.append(slot_rows)
+            tp_old = np.concatenate([np.where(c.chains.top_parent >= 0, c.chains.top_parent + row_off[j], -1)
+                                     for j, c in enumerate(cs)])[perm]
+            tp_slot = np.where(tp_old >= 0, slot_of[np.maximum(tp_old, 0)], -1)
+            chain_top.extend(tp_slot.tolist())
+            chain_off.extend((grow + new_off[1:]).tolist())
+            # parent chain of each chain = the chain whose bottom slot is its top parent
+            chain_at_bottom = np.full(n_rows, -1, dtype=np.int64)
+            chain_at_bottom[new_off[:-1]] = np.arange(n_ch)
+            parent = np.where(tp_slot >= 0, chain_at_bottom[np.maximum(tp_slot, 0)], -1)
+            cnt = np.bincount(parent[parent >= 0], minlength=n_ch)
+            kid_cnt.extend(cnt.tolist())
+            kids.extend(np.argsort(np.where(parent >= 0, parent, n_ch), kind="stable")[: int(cnt.sum())].tolist())
+            lv_sorted = lev[perm]
+            nlev = int(lv_sorted.max()) + 1
+            level_off.extend(np.searchsorted(lv_sorted, np.arange(nlev + 1), side="left").tolist())
+            cv_off = cls_off = 0
+            for j, i in enumerate(cells):
                 c = self.cells[i]
                 g = c.geom
+                ro = row_off[j]
                 self.cell_cv0[i] = gcv + cv_off
                 self.cell_ncv[i] = g.n_cv
-                cell_row_off[i] = row_off
-                rows = g.dynamic_rows + row_off
-                info = rows.astype(np.uint64) | ((c.cv_cls + cls_off).astype(np.uint64) << np.uint64(16))
+                self.cell_pack[i] = pi
+                self.cell_row_off[i] = ro
                 if np.any(c.cv_cls + cls_off >= (1 << 14)):
                     raise ValueError("too many mechanism classes in one pack")
+                cv_slots = slot_of[g.dynamic_rows + ro]
+                info = cv_slots.astype(np.uint64) | ((c.cv_cls + cls_off).astype(np.uint64) << np.uint64(16))
                 clamp_cv = np.zeros(g.n_cv, dtype=bool)
-                clamp_bnd_rows = set()
-                for (kind, idx, t0, t1, amp) in c.clamps:
+                clamp_bnd = set()
+                for (kind, idx_, t0, t1, amp) in c.clamps:
                     if kind == "mid":
-                        clamp_cv[idx] = True
-                        clamp_rows.append(int(g.dynamic_rows[idx] + row_off))
-                        clamp_area.append(g.cv_area_um2[idx] * 1e-8)
+                        clamp_cv[idx_] = True
+                        clamp_rows.append(int(cv_slots[idx_]))
+                        clamp_area.append(g.cv_area_um2[idx_] * 1e-8)
                     else:
-                        clamp_bnd_rows.add(int(idx))
-                        clamp_rows.append(int(idx + row_off))
+                        sl = int(slot_of[idx_ + ro])
+                        clamp_bnd.add(sl)
+                        clamp_rows.append(sl)
                         clamp_area.append(0.0)
                     clamp_kind.append(kind)
                     clamp_t.append((t0, t1))
@@ -283,61 +329,28 @@ class StaggerLayout:
                 info |= (pflag.astype(np.uint64) << np.uint64(30)) | (clamp_cv.astype(np.uint64) << np.uint64(31))
                 cv_info.append(info.astype(np.uint32))
                 for k, cvi in enumerate(c.probes.tolist()):
-                    probe_row.append(int(g.dynamic_rows[cvi] + row_off))
+                    probe_row.append(int(cv_slots[cvi]))
                     probe_col.append(int(self.cell_probe0[i] + k))
-                brows = np.nonzero(~g.row_is_mid)[0]
-                bflag = np.asarray([r in clamp_bnd_rows for r in brows.tolist()], dtype=np.uint64)
-                bnd_info.append(((brows + row_off).astype(np.uint64) | (bflag << np.uint64(31))).astype(np.uint32))
-                bnd_cell.append(np.full(len(brows), i, dtype=np.int64))
-                for lv in range(c.chains.n_levels):
-                    for k in range(c.chains.level_off[lv], c.chains.level_off[lv + 1]):
-                        per_level[lv].append((i, k))
+                bslots = slot_of[np.nonzero(~g.row_is_mid)[0] + ro]
+                bflag = np.asarray([int(b) in clamp_bnd for b in bslots.tolist()], dtype=np.uint64)
+                bnd_info.append((bslots.astype(np.uint64) | (bflag << np.uint64(31))).astype(np.uint32))
+                bnd_cell.append(np.full(len(bslots), i, dtype=np.int64))
                 cls_tab.append(c.cls_rows)
                 if self.region_classes:
                     for code in range(len(REGION_CODES)):
                         hit = np.nonzero(c.cls_rows[:, 10] == code)[0]
                         self.cell_region_cls[i, code] = gcls + cls_off + hit[0] if hit.size else -1
-                row_off += g.n_point
                 cv_off += g.n_cv
                 cls_off += c.cls_rows.shape[0]
-            # merged chain numbering inside the pack
-            local = {}
-            lvl_bounds = [0]
-            for lv in range(nlev):
-                for key in per_level[lv]:
-                    local[key] = len(local)
-                lvl_bounds.append(len(local))
-            ordered = [None] * len(local)
-            for key, idx in local.items():
-                ordered[idx] = key
-            for (i, k) in ordered:
-                ch = self.cells[i].chains
-                g = self.cells[i].geom
-                ro = cell_row_off[i]
-                rws = ch.rows[ch.off[k]:ch.off[k + 1]]
-                cr_row.append(rws + ro)
-                cr_lo.append(g.lowers[rws])
-                cr_up.append(g.uppers[rws])
-                cr_dg.append(g.diag_axial[rws])
-                chain_off.append(chain_off[-1] + len(rws))
-                tp = ch.top_parent[k]
-                chain_top.append(tp + ro if tp >= 0 else -1)
-                kk = ch.kids[ch.kid_off[k]:ch.kid_off[k + 1]]
-                kid_cnt.append(len(kk))
-                kids.extend(local[(i, int(x))] for x in kk.tolist())
-            level_off.extend(lvl_bounds)
-            P.n_cv = cv_off
-            P.n_rows = row_off
-            P.n_ch = len(local)
-            P.n_lv = nlev
+            P.n_cv, P.n_rows, P.n_ch, P.n_lv = cv_off, n_rows, n_ch, nlev
             P.n_bnd = int(sum(len(b) for b in bnd_info)) - gbnd
             P.n_cls = cls_off
             P.n_clamp = len(clamp_rows) - gclamp
             P.n_probe = len(probe_row) - gprobe
             packs.append(P)
             gcv += cv_off
-            grow += row_off
-            gch += len(local)
+            grow += n_rows
+            gch += n_ch
             glv += nlev + 1
             gbnd += P.n_bnd
             gcls += cls_off
@@ -356,10 +369,9 @@ class StaggerLayout:
         self.kid_off = np.concatenate(([0], np.cumsum(kid_cnt))).astype(np.int32)
         self.kids = np.asarray(kids, dtype=np.int32)
         self.level_off = np.asarray(level_off, dtype=np.int32)
-        self.cr_row = np.concatenate(cr_row).astype(np.int32)
-        self.cr_lo = np.concatenate(cr_lo)
-        self.cr_up = np.concatenate(cr_up)
-        self.cr_dg = np.concatenate(cr_dg)
+        self.slot_lo = np.concatenate(slot_lo)
+        self.slot_up = np.concatenate(slot_up)
+        self.slot_rows = slot_rows_all
         self.cls_tab = np.concatenate(cls_tab, axis=0)
         self.clamp_row = np.asarray(clamp_rows, dtype=np.int32)
         self.clamp_t = np.asarray(clamp_t, dtype=np.float64).reshape(-1, 2)
@@ -388,7 +400,7 @@ class StaggerLayout:
                 amp[e] = self.clamp_amp_raw[e] * 1e-3 / self.clamp_area[e]  # uA/cm^2
             else:
                 amp[e] = dt * self.clamp_amp_raw[e] * 1e-3  # mV: dt * I[nA] / 1 uF
-        return {"cr_lo": dt * self.cr_lo, "cr_up": dt * self.cr_up, "cr_dg": dt * self.cr_dg, "cls_tab": cls,
+        return {"lo": dt * self.slot_lo, "up": dt * self.slot_up, "cls_tab": cls,
                 "clamp_on": np.minimum(on, INT32_MAX).astype(np.int32),
                 "clamp_off": np.minimum(off, INT32_MAX).astype(np.int32), "clamp_amp": amp}
 
@@ -463,7 +475,6 @@ class StaggerRuntime:
         self.kid_off = t(layout.kid_off, torch.int32)
         self.kids = t(layout.kids if layout.kids.size else np.zeros(1, np.int32), torch.int32)
         self.level_off = t(layout.level_off, torch.int32)
-        self.cr_row = t(layout.cr_row, torch.int32)
         self.probe_row = t(layout.probe_row if layout.probe_row.size else np.zeros(1, np.int32), torch.int32)
         self.probe_col = t(layout.probe_col if layout.probe_col.size else np.zeros(1, np.int32), torch.int32)
         self.clamp_row = t(layout.clamp_row if layout.clamp_row.size else np.zeros(1, np.int32), torch.int32)
@@ -484,9 +495,8 @@ class StaggerRuntime:
     def _set_dt_tables(self):
         tb = self.L.dt_tables(self.dt)
         t = self._t
-        self.cr_lo = t(tb["cr_lo"], self.dtype)
-        self.cr_up = t(tb["cr_up"], self.dtype)
-        self.cr_dg = t(tb["cr_dg"], self.dtype)
+        self.lo = t(tb["lo"], self.dtype)
+        self.up = t(tb["up"], self.dtype)
         self.cls_tab = t(tb["cls_tab"].reshape(-1), self.dtype)
         n = max(1, tb["clamp_on"].size)
         self.clamp_on = t(tb["clamp_on"] if tb["clamp_on"].size else np.zeros(n, np.int32), self.torch.int32)
@@ -518,8 +528,8 @@ class StaggerRuntime:
         p = lambda x: ctypes.c_void_p(x.data_ptr())
         d.packs, d.cv_info, d.bnd_info = p(self.packs), p(self.cv_info), p(self.bnd_info)
         d.chain_off, d.chain_top, d.kid_off, d.kids = p(self.chain_off), p(self.chain_top), p(self.kid_off), p(self.kids)
-        d.level_off, d.cr_row = p(self.level_off), p(self.cr_row)
-        d.cr_lo, d.cr_up, d.cr_dg, d.cls_tab = p(self.cr_lo), p(self.cr_up), p(self.cr_dg), p(self.cls_tab)
+        d.level_off = p(self.level_off)
+        d.lo, d.up, d.cls_tab = p(self.lo), p(self.up), p(self.cls_tab)
         d.clamp_row, d.clamp_on, d.clamp_off, d.clamp_amp = p(self.clamp_row), p(self.clamp_on), p(self.clamp_off), p(self.clamp_amp)
         d.probe_row, d.probe_col = p(self.probe_row), p(self.probe_col)
         d.chan_const = ctypes.cast(ctypes.pointer(self._cc), ctypes.c_void_p)

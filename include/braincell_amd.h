@@ -57,16 +57,20 @@ typedef struct bc_pack {
 /* Plain staggered run: n_steps steps of the staggered scheme for every pack.
  *
  * Arrays (global index ranges come from bc_pack):
- *  cv_info[cv]    row | class << 16 (14 bits) | probe << 30 | clamp << 31
- *  bnd_info[i]    row | clamp << 31
- *  chain_off[k]   chain k owns chain rows chain_off[k] .. chain_off[k+1]-1
- *                 (size n_chains_total + 1, global chain-row indices)
- *  chain_top[k]   local row the chain's top hangs from, -1 for a root
+ *  cv_info[cv]    slot | class << 16 (14 bits) | probe << 30 | clamp << 31
+ *  bnd_info[i]    slot | clamp << 31
+ *  Rows of a pack are numbered in chain-major "slots": chain k occupies
+ *  slots chain_off[k] - row0 .. chain_off[k+1] - row0 - 1 (bottom to top),
+ *  chains sorted by level.
+ *  chain_off[k]   first slot of chain k as a global row index
+ *                 (size n_chains_total + 1)
+ *  chain_top[k]   local slot the chain's top hangs from, -1 for a root
  *  kid_off[k]     kids of chain k: kids[kid_off[k] .. kid_off[k+1]-1]
  *                 (local chain indices; size n_chains_total + 1)
  *  level_off[.]   per pack n_lv + 1 local chain offsets
- *  cr_row[j]      local row of chain row j (bottom to top)
- *  cr_lo/up/dg[j] dt * lowers, dt * uppers, dt * diag_axial of that row
+ *  lo/up[row]     dt * lowers, dt * uppers (A[row, parent], A[parent, row],
+ *                 ms^-1) per global slot; the axial diagonal is rebuilt
+ *                 from them (zero row sums of the axial operator)
  *  cls_tab[c*12+k] mechanism class table (dt/Cm, g per channel, Ca mask)
  *  clamp_row/on/off/amp[i]  local row, active step range [on, off), and
  *                 amplitude (uA/cm^2 on a CV row; mV rhs increment on a
@@ -97,10 +101,8 @@ typedef struct bc_stagger_desc {
   const int32_t* kid_off;
   const int32_t* kids;
   const int32_t* level_off;
-  const int32_t* cr_row;
-  const void* cr_lo;
-  const void* cr_up;
-  const void* cr_dg;
+  const void* lo;
+  const void* up;
   const void* cls_tab;
   const int32_t* clamp_row;
   const int32_t* clamp_on;

@@ -1,72 +1,115 @@
 """CPU transcription of the HIP staggered kernel's Hines phase (test only).
 
-Runs the chain walker of ``bc_stagger.hip`` (delta-V form) over a packed
-``StaggerLayout`` in NumPy so that layout, packing and the algorithm can be
-checked against the oracle without a GPU.
+Runs the chain walker of ``bc_stagger.hip`` (delta-V form, chain-major slots,
+diagonal rebuilt from the zero row sums) and the transposed walker of
+``bc_adjoint.hip`` over a packed ``StaggerLayout`` in NumPy, so that layout,
+packing and both solves can be checked against the oracle without a GPU.
 """
 
 import numpy as np
 
 
+def _pack_ranges(L, p):
+    P = L.pack_array
+    return P[p, 2], P[p, 3], P[p, 4], P[p, 5], P[p, 6], P[p, 7]
+
+
 def hines_delta(L, dt, bd_in, br_in, xv, dtype=np.float64):
-    """bd_in/br_in/xv: per global row.  Returns dV per global row."""
+    """bd_in/br_in/xv: per global slot.  Returns dV per global slot."""
     tb = L.dt_tables(dt)
-    dg, lo_a, up_a = (tb[k].astype(dtype) for k in ("cr_dg", "cr_lo", "cr_up"))
+    LO, UP = tb["lo"].astype(dtype), tb["up"].astype(dtype)
     bd = bd_in.astype(dtype).copy()
     br = br_in.astype(dtype).copy()
     xv = xv.astype(dtype)
-    P = L.pack_array
     for p in range(L.n_packs):
-        row0, ch0, lv0, nlv, nch = P[p, 2], P[p, 4], P[p, 6], P[p, 7], P[p, 5]
+        row0, nrows, ch0, nch, lv0, nlv = _pack_ranges(L, p)
         kd = np.zeros(nch, dtype)
         kr = np.zeros(nch, dtype)
         for lev in range(nlv):
             for k in range(L.level_off[lv0 + lev], L.level_off[lv0 + lev + 1]):
                 gk = ch0 + k
-                j0, j1 = L.chain_off[gk], L.chain_off[gk + 1]
+                j0, j1 = L.chain_off[gk] - row0, L.chain_off[gk + 1] - row0
                 tp = L.chain_top[gk]
-                row = L.cr_row[j0]
-                vr = xv[row0 + row]
-                d = dg[j0] + bd[row0 + row]
-                r = br[row0 + row]
+                e, r = bd[row0 + j0], br[row0 + j0]
                 for q in range(L.kid_off[gk], L.kid_off[gk + 1]):
-                    d -= kd[L.kids[q]]
+                    e -= kd[L.kids[q]]
                     r -= kr[L.kids[q]]
+                vr = xv[row0 + j0]
                 j = j0
                 while True:
-                    pr = L.cr_row[j + 1] if j + 1 < j1 else tp
+                    pr = j + 1 if j + 1 < j1 else tp
                     vp = xv[row0 + pr] if pr >= 0 else vr
-                    lo, up = lo_a[j], up_a[j]
+                    lo, up = LO[row0 + j], UP[row0 + j]
                     dl = vr - vp
                     r = r + lo * dl
-                    inv = dtype(1) / d
-                    bd[row0 + row] = inv
-                    br[row0 + row] = r
-                    m = up * inv
+                    inv = dtype(1) / (e - lo)
+                    bd[row0 + j] = inv
+                    br[row0 + j] = r
+                    kdc, krc = up * e * inv, up * r * inv
                     j += 1
                     if j >= j1:
                         break
-                    row, vr = pr, vp
-                    d = (dg[j] + bd[row0 + row]) - lo * m
-                    r = (br[row0 + row] - r * m) - up * dl
-                kd[k] = lo * m
-                kr[k] = r * m + up * dl
-        for lev in range(nlv - 1, -1, -1):
-            for k in range(L.level_off[lv0 + lev], L.level_off[lv0 + lev + 1]):
-                gk = ch0 + k
-                j0, j1 = L.chain_off[gk], L.chain_off[gk + 1]
-                tp = L.chain_top[gk]
-                xp = br[row0 + tp] if tp >= 0 else dtype(0)
-                for j in range(j1 - 1, j0 - 1, -1):
-                    row = row0 + L.cr_row[j]
-                    x = (br[row] - lo_a[j] * xp) * bd[row]
-                    br[row] = x
-                    xp = x
+                    vr = vp
+                    e = bd[row0 + j] - kdc
+                    r = (br[row0 + j] - krc) - up * dl
+                kd[k] = kdc
+                kr[k] = krc + up * dl
+        _backsub(L, p, bd, br, LO)
     return br
 
 
+def hines_transposed(L, dt, bd_in, b_in, dtype=np.float64):
+    """Solves M^T y = b with M's diagonal bd_in (+ axial part) per slot."""
+    tb = L.dt_tables(dt)
+    LO, UP = tb["lo"].astype(dtype), tb["up"].astype(dtype)
+    bd = bd_in.astype(dtype).copy()
+    br = b_in.astype(dtype).copy()
+    for p in range(L.n_packs):
+        row0, nrows, ch0, nch, lv0, nlv = _pack_ranges(L, p)
+        kd = np.zeros(nch, dtype)
+        kr = np.zeros(nch, dtype)
+        for lev in range(nlv):
+            for k in range(L.level_off[lv0 + lev], L.level_off[lv0 + lev + 1]):
+                gk = ch0 + k
+                j0, j1 = L.chain_off[gk] - row0, L.chain_off[gk + 1] - row0
+                e, r = bd[row0 + j0], br[row0 + j0]
+                for q in range(L.kid_off[gk], L.kid_off[gk + 1]):
+                    e -= kd[L.kids[q]]
+                    r -= kr[L.kids[q]]
+                j = j0
+                while True:
+                    lo, up = LO[row0 + j], UP[row0 + j]
+                    inv = dtype(1) / (e - lo)
+                    bd[row0 + j] = inv
+                    br[row0 + j] = r
+                    kdc, krc = up * e * inv, lo * r * inv
+                    j += 1
+                    if j >= j1:
+                        break
+                    e = bd[row0 + j] - kdc
+                    r = br[row0 + j] - krc
+                kd[k] = kdc
+                kr[k] = krc
+        _backsub(L, p, bd, br, UP)
+    return br
+
+
+def _backsub(L, p, bd, br, coef):
+    row0, nrows, ch0, nch, lv0, nlv = _pack_ranges(L, p)
+    for lev in range(nlv - 1, -1, -1):
+        for k in range(L.level_off[lv0 + lev], L.level_off[lv0 + lev + 1]):
+            gk = ch0 + k
+            j0, j1 = L.chain_off[gk] - row0, L.chain_off[gk + 1] - row0
+            tp = L.chain_top[gk]
+            xp = br[row0 + tp] if tp >= 0 else 0.0
+            for j in range(j1 - 1, j0 - 1, -1):
+                x = (br[row0 + j] - coef[row0 + j] * xp) * bd[row0 + j]
+                br[row0 + j] = x
+                xp = x
+
+
 def cv_rows(L):
-    """Global row of every CV (pack order)."""
+    """Global slot of every CV (pack order)."""
     info = L.cv_info.astype(np.int64)
     rows = np.empty(L.n_cv_total, dtype=np.int64)
     P = L.pack_array

@@ -10,7 +10,7 @@ from braincell_amd.runtime import StaggerLayout, first_step_at
 from oracle import discretize as OD
 from oracle.dhs import dhs_solve
 from oracle.staggered import StaggeredOracle
-from tests.kernel_emulator import bnd_rows, cv_rows, hines_delta
+from tests.kernel_emulator import bnd_rows, cv_rows, hines_delta, hines_transposed
 
 CELLS = [lambda: W.single_tree_cell(0)] + [lambda i=i: W.uneven_tree_cell(i, cmin=200, cmax=700) for i in range(4)]
 
@@ -110,3 +110,38 @@ def test_probes_and_soma_location():
     # x = 0.5 on a 4-CV soma lands on an internal boundary -> owner CV 2 midpoint
     assert L.cells[0].probes.tolist() == [2]
     assert L.clamp_kind == ["mid"]
+
+
+def test_packed_transposed_solve_matches_dense():
+    from oracle.dhs import assemble_dense, numeric_state
+
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=300) for i in range(3)]
+    L = StaggerLayout(models, rows_per_pack=1200, cv_per_thread=4)
+    dt = 0.025
+    rng = np.random.default_rng(4)
+    G = rng.uniform(0, 1, L.n_cv_total)
+    b = rng.standard_normal(L.n_cv_total)
+    rows = cv_rows(L)
+    bd = np.zeros(L.n_rows_total)
+    bb = np.zeros(L.n_rows_total)
+    bd[rows] = 1 + dt * G
+    bb[rows] = b
+    y = hines_transposed(L, dt, bd, bb)
+    for i, m in enumerate(models):
+        st = StaggeredOracle(m).st
+        sl = L.cell_slice(i)
+        d, s, lo, up = numeric_state(st, np.zeros((1, st.dynamic_rows.size)), (-G[sl])[None],
+                                     np.zeros((1, st.dynamic_rows.size)), dt)
+        M = assemble_dense(st, d[0], lo, up)
+        rhs = np.zeros(st.n_point)
+        rhs[st.dynamic_rows] = b[sl]
+        ref = np.linalg.solve(M.T, rhs)
+        np.testing.assert_allclose(y[rows[sl]], ref[st.dynamic_rows], rtol=0, atol=1e-10)
+
+
+def test_slots_are_a_permutation_of_reference_rows():
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=300) for i in range(4)]
+    L = StaggerLayout(models, rows_per_pack=2000, cv_per_thread=8)
+    for p in range(L.n_packs):
+        sr = L.slot_rows[p]
+        assert sorted(sr.tolist()) == list(range(L.pack_array[p, 3]))
