@@ -33,6 +33,7 @@ class StaggerDesc(ctypes.Structure):
         ("precision", ctypes.c_int32), ("channels", ctypes.c_uint32), ("n_packs", ctypes.c_int32),
         ("threads", ctypes.c_int32), ("cv_per_thread", ctypes.c_int32),
         ("max_rows", ctypes.c_int32), ("max_chains", ctypes.c_int32), ("max_cls", ctypes.c_int32),
+        ("max_levels", ctypes.c_int32),
         ("n_cv_total", ctypes.c_int64),
         ("packs", ctypes.c_void_p), ("cv_info", ctypes.c_void_p), ("bnd_info", ctypes.c_void_p),
         ("chain_off", ctypes.c_void_p), ("chain_top", ctypes.c_void_p), ("kid_off", ctypes.c_void_p),

@@ -90,11 +90,13 @@ template <> struct bc_math<double> {
   static constexpr double exprel_eps = 1e-8;
 };
 
-// Reciprocal used by the Hines elimination: the hardware v_rcp_f32 (1 ulp)
-// in FP32, an IEEE division in FP64.
+// Reciprocal: the hardware v_rcp_f32 (1 ulp) in FP32, an IEEE division in
+// FP64.  bc_div(a, b) = a * rcp(b).
 template <typename real> __device__ __forceinline__ real bc_rcp(real x);
 template <> __device__ __forceinline__ float bc_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <> __device__ __forceinline__ double bc_rcp<double>(double x) { return 1.0 / x; }
+template <typename real>
+__device__ __forceinline__ real bc_div(real a, real b) { return a * bc_rcp<real>(b); }
 
 // (e^x - 1) / x with its removable singularity filled by the series.
 template <typename real>
@@ -103,18 +105,40 @@ __host__ __device__ __forceinline__ real bc_exprel(real x) {
   return bc_math<real>::em1(x) / x;
 }
 
+// x / (e^x - 1) = 1 / exprel(x).  FP32: Bernoulli series for |x| <= 1/2
+// (truncation below 3e-11, far under an FP32 ulp), x * rcp(e^x - 1) (no
+// cancellation there) otherwise.  FP64: the IEEE quotient 1 / exprel(x).
+template <typename real>
+__device__ __forceinline__ real bc_inv_exprel(real x) {
+  if constexpr (sizeof(real) == 8) return real(1) / bc_exprel<real>(x);
+  if (fabs(x) <= real(0.5)) {
+    const real x2 = x * x;
+    return real(1) - real(0.5) * x +
+           x2 * (real(1.0 / 12.0) + x2 * (real(-1.0 / 720.0) + x2 * (real(1.0 / 30240.0) + x2 * real(-1.0 / 1209600.0))));
+  }
+  return bc_div<real>(x, bc_math<real>::ex(x) - real(1));
+}
+
 // y + dt * exprel(dt * lam) * f
 template <typename real>
 __host__ __device__ __forceinline__ real bc_expeuler(real y, real f, real lam, real dt) {
   return y + dt * bc_exprel<real>(dt * lam) * f;
 }
 
-// Gate in alpha/beta form: f = phi (a (1 - x) - b x), lam = -phi (a + b).
+// Gate in alpha/beta form: f = phi (a (1 - x) - b x), lam = -phi (a + b);
+// the exp-Euler step is exact for this linear ODE:
+// x' = x + dt exprel(dt lam) f = x_inf + (x - x_inf) e^{dt lam}.
 template <typename real>
 __host__ __device__ __forceinline__ real bc_gate_ab(real x, real a, real b, real phi, real dt) {
   real f = phi * (a * (real(1) - x) - b * x);
   real lam = -phi * (a + b);
   return bc_expeuler<real>(x, f, lam, dt);
+}
+template <typename real>
+__device__ __forceinline__ real bc_gate_ab_cf(real x, real a, real b, real phi, real dt) {
+  const real s = a + b;
+  const real xi = bc_div<real>(a, s);
+  return xi + (x - xi) * bc_math<real>::ex(-dt * phi * s);
 }
 
 // Gate in inf/tau form: f = phi (inf - x) / tau, lam = -phi / tau.
@@ -125,52 +149,60 @@ __host__ __device__ __forceinline__ real bc_gate_it(real x, real inf, real tau, 
   real lam = -phi * itau;
   return bc_expeuler<real>(x, f, lam, dt);
 }
-
 template <typename real>
-__host__ __device__ __forceinline__ void bc_na_rates(real V, real vsh, real& ap, real& bp, real& aq, real& bq) {
+__device__ __forceinline__ real bc_gate_it_cf(real x, real inf, real tau, real phi, real dt) {
+  return inf + (x - inf) * bc_math<real>::ex(-dt * phi * bc_rcp<real>(tau));
+}
+
+// Rate functions.  Divisions by literal constants are multiplications by
+// their reciprocals; other divisions use bc_div.
+template <typename real>
+__device__ __forceinline__ void bc_na_rates(real V, real vsh, real& ap, real& bp, real& aq, real& bq) {
   typedef bc_math<real> M;
   real Vp = V - vsh;
-  ap = real(0.32 * 4.0) / bc_exprel<real>(-(Vp - real(13)) * real(0.25));
-  bp = real(0.28 * 5.0) / bc_exprel<real>((Vp - real(40)) * real(0.2));
-  aq = real(0.128) * M::ex(-(Vp - real(17)) / real(18));
-  bq = real(4) / (real(1) + M::ex(-(Vp - real(40)) * real(0.2)));
+  ap = real(0.32 * 4.0) * bc_inv_exprel<real>(-(Vp - real(13)) * real(0.25));
+  bp = real(0.28 * 5.0) * bc_inv_exprel<real>((Vp - real(40)) * real(0.2));
+  aq = real(0.128) * M::ex(-(Vp - real(17)) * real(1.0 / 18.0));
+  bq = bc_div<real>(real(4), real(1) + M::ex(-(Vp - real(40)) * real(0.2)));
 }
 
 template <typename real>
-__host__ __device__ __forceinline__ void bc_kdr_rates(real V, real vsh, real& ap, real& bp) {
+__device__ __forceinline__ void bc_kdr_rates(real V, real vsh, real& ap, real& bp) {
   typedef bc_math<real> M;
   real Vp = V - vsh;
-  ap = real(0.032 * 5.0) / bc_exprel<real>(-(Vp - real(15)) * real(0.2));
-  bp = real(0.5) * M::ex(-(Vp - real(10)) / real(40));
+  ap = real(0.032 * 5.0) * bc_inv_exprel<real>(-(Vp - real(15)) * real(0.2));
+  bp = real(0.5) * M::ex(-(Vp - real(10)) * real(1.0 / 40.0));
 }
 
 template <typename real>
-__host__ __device__ __forceinline__ void bc_cal_rates(real V, real vsh, real& pi, real& pt, real& qi, real& qt) {
+__device__ __forceinline__ void bc_cal_rates(real V, real vsh, real& pi, real& pt, real& qi, real& qt) {
   typedef bc_math<real> M;
   real Vp = V - vsh;
-  pi = real(1) / (real(1) + M::ex(-(Vp + real(10)) * real(0.25)));
-  real e1 = M::ex((Vp + real(5)) / real(15));
-  pt = real(0.4) + real(0.7) / (real(1) / e1 + e1);
-  qi = real(1) / (real(1) + M::ex((Vp + real(25)) * real(0.5)));
-  real e2 = M::ex((Vp + real(40)) / real(9.5));
-  qt = real(300) + real(100) / (e2 + real(1) / e2);
+  pi = bc_rcp<real>(real(1) + M::ex(-(Vp + real(10)) * real(0.25)));
+  real e1 = M::ex((Vp + real(5)) * real(1.0 / 15.0));
+  pt = real(0.4) + bc_div<real>(real(0.7), bc_rcp<real>(e1) + e1);
+  qi = bc_rcp<real>(real(1) + M::ex((Vp + real(25)) * real(0.5)));
+  real e2 = M::ex((Vp + real(40)) * real(1.0 / 9.5));
+  qt = real(300) + bc_div<real>(real(100), e2 + bc_rcp<real>(e2));
 }
 
 template <typename real>
-__host__ __device__ __forceinline__ void bc_cat_rates(real V, real vsh, real& pi, real& pt, real& qi, real& qt) {
+__device__ __forceinline__ void bc_cat_rates(real V, real vsh, real& pi, real& pt, real& qi, real& qt) {
   typedef bc_math<real> M;
   real Vp = V - vsh;
-  pi = real(1) / (real(1) + M::ex(-(Vp + real(59)) / real(6.2)));
-  pt = real(1) / (M::ex(-(Vp + real(132)) / real(16.7)) + M::ex((Vp + real(16.8)) / real(18.2))) + real(0.612);
-  qi = real(1) / (real(1) + M::ex((Vp + real(83)) * real(0.25)));
-  qt = (Vp >= real(-80)) ? (M::ex(-(Vp + real(22)) / real(10.5)) + real(28)) : M::ex((Vp + real(467)) / real(66.6));
+  pi = bc_rcp<real>(real(1) + M::ex(-(Vp + real(59)) * real(1.0 / 6.2)));
+  pt = bc_rcp<real>(M::ex(-(Vp + real(132)) * real(1.0 / 16.7)) + M::ex((Vp + real(16.8)) * real(1.0 / 18.2))) +
+       real(0.612);
+  qi = bc_rcp<real>(real(1) + M::ex((Vp + real(83)) * real(0.25)));
+  qt = (Vp >= real(-80)) ? (M::ex(-(Vp + real(22)) * real(1.0 / 10.5)) + real(28))
+                         : M::ex((Vp + real(467)) * real(1.0 / 66.6));
 }
 
 template <typename real>
-__host__ __device__ __forceinline__ void bc_hcn_rates(real V, real& pi, real& pt) {
+__device__ __forceinline__ void bc_hcn_rates(real V, real& pi, real& pt) {
   typedef bc_math<real> M;
-  pi = real(1) / (real(1) + M::ex((V + real(75)) / real(5.5)));
-  pt = real(1) / (M::ex(real(-0.086) * V - real(14.59)) + M::ex(real(0.0701) * V - real(1.87)));
+  pi = bc_rcp<real>(real(1) + M::ex((V + real(75)) * real(1.0 / 5.5)));
+  pt = bc_rcp<real>(M::ex(real(-0.086) * V - real(14.59)) + M::ex(real(0.0701) * V - real(1.87)));
 }
 
 // ----------------------------------------------------------------------------

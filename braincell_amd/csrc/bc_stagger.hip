@@ -97,7 +97,7 @@ __device__ __forceinline__ real linearise(const bc_chan_const<real>& cc, const r
   GE = real(0);
   real ica = real(0);
   real eca = cc.E_Ca_fixed;
-  if constexpr ((CH & BC_CH_POOL) != 0) eca = cc.ca_rtzf * M::lg(cc.ca_Co / s[bc_slot_ca(CH)]);
+  if constexpr ((CH & BC_CH_POOL) != 0) eca = cc.ca_rtzf * M::lg(bc_div<real>(cc.ca_Co, s[bc_slot_ca(CH)]));
   if constexpr ((CH & BC_CH_NA) != 0) {
     real p = s[bc_slot_na(CH)], q = s[bc_slot_na(CH) + 1];
     real g = cp[BC_CLS_G_NA] * (p * p * p) * q;
@@ -151,8 +151,12 @@ __device__ __forceinline__ real linearise(const bc_chan_const<real>& cc, const r
 }
 
 // Phase C: Ca pool then gates at V_{n+1}.
+// The exp-Euler update of every gate is exact for its linear ODE at frozen
+// V, so it is written in closed form x_inf + (x - x_inf) e^{dt lam}.  The Ca
+// pool's lam = -1/tau is constant on this path: ca_b = dt exprel(-dt/tau).
 template <typename real, uint32_t CH>
-__device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, real* s, real V, real ica, real dt) {
+__device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, real* s, real V, real ica, real dt,
+                                               real ca_b) {
   typedef bc_math<real> M;
   real ci = real(0);
   if constexpr ((CH & BC_CH_POOL) != 0) {
@@ -160,40 +164,40 @@ __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, re
     real drive = ica * cc.ca_drive;
     drive = drive > real(0) ? drive : real(0);
     real f = drive + (cc.ca_rest - c) * cc.ca_tau_inv;
-    ci = bc_expeuler<real>(c, f, -cc.ca_tau_inv, dt);
+    ci = c + ca_b * f;
     s[bc_slot_ca(CH)] = ci;
   }
   if constexpr ((CH & BC_CH_NA) != 0) {
     real ap, bp, aq, bq;
     bc_na_rates<real>(V, cc.na_vsh, ap, bp, aq, bq);
-    s[bc_slot_na(CH)] = bc_gate_ab<real>(s[bc_slot_na(CH)], ap, bp, cc.na_phi, dt);
-    s[bc_slot_na(CH) + 1] = bc_gate_ab<real>(s[bc_slot_na(CH) + 1], aq, bq, cc.na_phi, dt);
+    s[bc_slot_na(CH)] = bc_gate_ab_cf<real>(s[bc_slot_na(CH)], ap, bp, cc.na_phi, dt);
+    s[bc_slot_na(CH) + 1] = bc_gate_ab_cf<real>(s[bc_slot_na(CH) + 1], aq, bq, cc.na_phi, dt);
   }
   if constexpr ((CH & BC_CH_KDR) != 0) {
     real ap, bp;
     bc_kdr_rates<real>(V, cc.kdr_vsh, ap, bp);
-    s[bc_slot_kdr(CH)] = bc_gate_ab<real>(s[bc_slot_kdr(CH)], ap, bp, cc.kdr_phi, dt);
+    s[bc_slot_kdr(CH)] = bc_gate_ab_cf<real>(s[bc_slot_kdr(CH)], ap, bp, cc.kdr_phi, dt);
   }
   if constexpr ((CH & BC_CH_CAL) != 0) {
     real pi, pt, qi, qt;
     bc_cal_rates<real>(V, cc.cal_vsh, pi, pt, qi, qt);
-    s[bc_slot_cal(CH)] = bc_gate_it<real>(s[bc_slot_cal(CH)], pi, pt, cc.cal_phi_p, dt);
-    s[bc_slot_cal(CH) + 1] = bc_gate_it<real>(s[bc_slot_cal(CH) + 1], qi, qt, cc.cal_phi_q, dt);
+    s[bc_slot_cal(CH)] = bc_gate_it_cf<real>(s[bc_slot_cal(CH)], pi, pt, cc.cal_phi_p, dt);
+    s[bc_slot_cal(CH) + 1] = bc_gate_it_cf<real>(s[bc_slot_cal(CH) + 1], qi, qt, cc.cal_phi_q, dt);
   }
   if constexpr ((CH & BC_CH_CAT) != 0) {
     real pi, pt, qi, qt;
     bc_cat_rates<real>(V, cc.cat_vsh, pi, pt, qi, qt);
-    s[bc_slot_cat(CH)] = bc_gate_it<real>(s[bc_slot_cat(CH)], pi, pt, cc.cat_phi_p, dt);
-    s[bc_slot_cat(CH) + 1] = bc_gate_it<real>(s[bc_slot_cat(CH) + 1], qi, qt, cc.cat_phi_q, dt);
+    s[bc_slot_cat(CH)] = bc_gate_it_cf<real>(s[bc_slot_cat(CH)], pi, pt, cc.cat_phi_p, dt);
+    s[bc_slot_cat(CH) + 1] = bc_gate_it_cf<real>(s[bc_slot_cat(CH) + 1], qi, qt, cc.cat_phi_q, dt);
   }
   if constexpr ((CH & BC_CH_AHP) != 0) {
     real a = cc.ahp_alpha * M::pw(ci, cc.ahp_n);
-    s[bc_slot_ahp(CH)] = bc_gate_ab<real>(s[bc_slot_ahp(CH)], a, cc.ahp_beta, cc.ahp_phi, dt);
+    s[bc_slot_ahp(CH)] = bc_gate_ab_cf<real>(s[bc_slot_ahp(CH)], a, cc.ahp_beta, cc.ahp_phi, dt);
   }
   if constexpr ((CH & BC_CH_HCN) != 0) {
     real pi, pt;
     bc_hcn_rates<real>(V, pi, pt);
-    s[bc_slot_hcn(CH)] = bc_gate_it<real>(s[bc_slot_hcn(CH)], pi, pt, cc.hcn_phi, dt);
+    s[bc_slot_hcn(CH)] = bc_gate_it_cf<real>(s[bc_slot_hcn(CH)], pi, pt, cc.hcn_phi, dt);
   }
 }
 
@@ -226,6 +230,13 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   real* kd = UP + a.max_rows;
   real* kr = kd + a.max_chains;
   real* cls = kr + a.max_chains;
+  // chain metadata, 16-bit: first slot of each chain (+ end), top parent slot,
+  // kid list offsets (+ end), kid chains, level offsets (+ end)
+  int16_t* m_j0 = reinterpret_cast<int16_t*>(cls + a.max_cls * BC_NCLS);
+  int16_t* m_tp = m_j0 + (a.max_chains + 1);
+  int16_t* m_k0 = m_tp + a.max_chains;
+  int16_t* m_kid = m_k0 + (a.max_chains + 1);
+  int16_t* m_lv = m_kid + a.max_chains;
 
   const bc_pack P = a.packs[blockIdx.x];
   const int tid = threadIdx.x;
@@ -235,6 +246,14 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     LO[i] = a.lo[(int64_t)P.row0 + i];
     UP[i] = a.up[(int64_t)P.row0 + i];
   }
+  const int kbase = a.kid_off[P.ch0];
+  for (int i = tid; i <= P.n_ch; i += NT) {
+    m_j0[i] = (int16_t)(a.chain_off[P.ch0 + i] - P.row0);
+    m_k0[i] = (int16_t)(a.kid_off[P.ch0 + i] - kbase);
+    if (i < P.n_ch) m_tp[i] = (int16_t)a.chain_top[P.ch0 + i];
+  }
+  for (int i = tid; i < a.kid_off[P.ch0 + P.n_ch] - kbase; i += NT) m_kid[i] = (int16_t)a.kids[kbase + i];
+  for (int i = tid; i <= P.n_lv; i += NT) m_lv[i] = (int16_t)a.level_off[P.lv0 + i];
 
   real st[MAXC][NS];
   real vv[MAXC];
@@ -269,6 +288,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   __syncthreads();
 
   const real dt = a.dt;
+  const real ca_b = dt * bc_exprel<real>(-dt * a.cc.ca_tau_inv);
   uint32_t bad = 0;
   for (int step = 0; step < a.n_steps; ++step) {
     const int gstep = a.step0 + step;
@@ -301,16 +321,15 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     // d_j = e_j - lo_j with e_j = diag_j - sum over children c of
     // up_c e_c / d_c >= 0, so no step subtracts nearly equal numbers.
     for (int l = 0; l < P.n_lv; ++l) {
-      const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
+      const int k0 = m_lv[l], k1 = m_lv[l + 1];
       for (int k = k0 + tid; k < k1; k += NT) {
-        const int gk = P.ch0 + k;
-        const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
-        const int tp = a.chain_top[gk];
+        const int j0 = m_j0[k], j1 = m_j0[k + 1];
+        const int tp = m_tp[k];
         real e = bd[j0];
         real r = br[j0];
-        const int q0 = a.kid_off[gk], q1 = a.kid_off[gk + 1];
+        const int q0 = m_k0[k], q1 = m_k0[k + 1];
         for (int q = q0; q < q1; ++q) {
-          const int kk = a.kids[q];
+          const int kk = m_kid[q];
           e -= kd[kk];
           r -= kr[kk];
         }
@@ -341,11 +360,10 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
     // ---- back substitution, root to leaves
     for (int l = P.n_lv - 1; l >= 0; --l) {
-      const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
+      const int k0 = m_lv[l], k1 = m_lv[l + 1];
       for (int k = k0 + tid; k < k1; k += NT) {
-        const int gk = P.ch0 + k;
-        const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
-        const int tp = a.chain_top[gk];
+        const int j0 = m_j0[k], j1 = m_j0[k + 1];
+        const int tp = m_tp[k];
         real xp = tp >= 0 ? br[tp] : real(0);
         for (int j = j1 - 1; j >= j0; --j) {
           const real x = (br[j] - LO[j] * xp) * bd[j];
@@ -367,7 +385,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         const int row = (int)(info[i] & kRowMask);
         const real Vn = vv[i] + br[row];
         xv[row] = Vn;
-        advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt);
+        advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt, ca_b);
         vv[i] = Vn;
         bad |= (Vn == Vn && fabs(Vn) < real(1e30)) ? 0u : 1u;
         if (pcol[i] >= 0) a.trace[(int64_t)step * a.trace_ld + pcol[i]] = Vn;
@@ -469,8 +487,10 @@ int validate(const bc_stagger_desc* d) {
     return bc::fail(BC_INVALID_ARG, "threads must be 64, 128, 192 or 256 (got %d)", d->threads);
   if (d->n_steps < 0) return bc::fail(BC_INVALID_ARG, "n_steps < 0");
   if (d->max_rows <= 0 || d->max_rows > 65536) return bc::fail(BC_INVALID_LAYOUT, "max_rows out of range");
-  if (d->max_chains <= 0 || d->max_cls <= 0 || d->max_cls > 16384)
+  if (d->max_chains <= 0 || d->max_chains > 32767 || d->max_cls <= 0 || d->max_cls > 16384)
     return bc::fail(BC_INVALID_LAYOUT, "max_chains/max_cls out of range");
+  if (d->max_rows > 32767 || d->max_levels <= 0 || d->max_levels > 32767)
+    return bc::fail(BC_INVALID_LAYOUT, "max_rows/max_levels out of range for 16-bit chain metadata");
   if (!(d->dt > 0.0)) return bc::fail(BC_INVALID_ARG, "dt must be > 0");
   if (d->n_packs > 0 && (d->packs == nullptr || d->cv_info == nullptr || d->V == nullptr || d->Vbnd == nullptr ||
                          d->status == nullptr))
@@ -489,7 +509,8 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
   a.step0 = step0;
   a.n_steps = n_steps;
   a.trace = static_cast<real*>(trace);
-  size_t lds = (size_t)(5 * d->max_rows + 2 * d->max_chains + d->max_cls * BC_NCLS) * sizeof(real);
+  size_t lds = (size_t)(5 * d->max_rows + 2 * d->max_chains + d->max_cls * BC_NCLS) * sizeof(real) +
+               (size_t)(4 * d->max_chains + 3 + d->max_levels + 1) * sizeof(int16_t);
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "pack needs %zu bytes of LDS (> 160 KiB)", lds);
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (d->n_packs == 0 || n_steps == 0) return BC_OK;

@@ -385,6 +385,7 @@ class StaggerLayout:
         self.max_rows = int(p[:, 3].max())
         self.max_chains = int(p[:, 5].max())
         self.max_cls = int(p[:, 11].max())
+        self.max_levels = int(p[:, 7].max())
 
     # ------------------------------------------------------------ dt-tables
     def dt_tables(self, dt: float):
@@ -524,6 +525,7 @@ class StaggerRuntime:
         d.max_rows = L.max_rows
         d.max_chains = L.max_chains
         d.max_cls = L.max_cls
+        d.max_levels = L.max_levels
         d.n_cv_total = L.n_cv_total
         p = lambda x: ctypes.c_void_p(x.data_ptr())
         d.packs, d.cv_info, d.bnd_info = p(self.packs), p(self.cv_info), p(self.bnd_info)

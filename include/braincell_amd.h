@@ -92,6 +92,7 @@ typedef struct bc_stagger_desc {
   int32_t threads;      /* workgroup size (64..1024, multiple of 64) */
   int32_t cv_per_thread;/* register slots per thread: 1, 2, 4, 8 or 16 */
   int32_t max_rows, max_chains, max_cls; /* LDS sizing (max over packs) */
+  int32_t max_levels;
   int64_t n_cv_total;
   const bc_pack* packs;
   const uint32_t* cv_info;
