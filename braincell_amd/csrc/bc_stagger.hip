@@ -69,7 +69,7 @@ struct StaggerArgs {
   int32_t step0;
   int32_t n_steps;
   real dt;
-  int32_t max_rows, max_chains, max_cls;
+  int32_t max_rows, max_chains, max_cls, max_levels;
   bc_chan_const<real> cc;
 };
 
@@ -218,39 +218,253 @@ __device__ __forceinline__ void record_state(const StaggerArgs<real>& a, const b
   }
 }
 
+constexpr int kWave = 64;  // gfx950 wavefront
+
+// One DHS row in LDS: after phase A d = diagonal, r = right-hand side; the
+// elimination overwrites d with 1 / pivot and r with the reduced rhs, the
+// back substitution r with the solution.  lo / up = dt A[row, parent] and
+// dt A[parent, row].  One 16-byte (FP32) load fetches a row.
+template <typename real>
+struct alignas(4 * sizeof(real)) HRow {
+  real d, r, lo, up;
+};
+// Contribution of an eliminated chain to the row it hangs from.
+template <typename real>
+struct alignas(2 * sizeof(real)) KPair {
+  real d, r;
+};
+
+// Chain record (16 bytes): x = j0 | j1 << 16 (slot range), y = top | n_kids
+// << 16 (top = slot the chain hangs from, -1 for a root), z = kid0 | kid1
+// << 16, w = offset of the chain's kid list (kids past the second).
+struct ChainRec {
+  int j0, j1, tp, nk, k0, k1, q0;
+};
+__device__ __forceinline__ ChainRec unpack_chain(int4 m) {
+  ChainRec c;
+  c.j0 = m.x & 0xFFFF;
+  c.j1 = (int)((unsigned)m.x >> 16);
+  c.tp = (m.y << 16) >> 16;
+  c.nk = (int)((unsigned)m.y >> 16);
+  c.k0 = m.z & 0xFFFF;
+  c.k1 = (int)((unsigned)m.z >> 16);
+  c.q0 = m.w;
+  return c;
+}
+
+// Orders LDS accesses of the lanes of one wavefront across this point (LDS
+// operations of a wavefront complete in issue order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Axial part of the delta-V right-hand side of a row, dt (A V_n) with the
+// sign of the defect: lo_row (V_row - V_parent) minus up_c (V_c - V_row) for
+// the chain child c = row - 1 (a chain's bottom row gets its kid chains'
+// terms from their elimination instead).  Branch-free: a bottom row reads
+// its own slot in place of row - 1 and discards it.
+template <typename real>
+__device__ __forceinline__ real axial_defect(int row, real V, const HRow<real>* R, const real* xv,
+                                             const int16_t* m_link) {
+  const unsigned u = (unsigned)(uint16_t)m_link[row];
+  const int par = (int)(u & 0x7FFFu);
+  const bool bottom = (u & 0x8000u) != 0;
+  const int below = bottom ? row : row - 1;
+  const real own = R[row].lo * (V - xv[par]);
+  const real kid = R[below].up * (xv[below] - V);
+  return bottom ? own : own - kid;
+}
+
+// Rows of a chain are walked in groups of kGroup: the group's rows are
+// loaded together (indices clamped to the chain, so every load stays inside
+// it) and the next group's loads are issued before the current group is
+// computed, so LDS latency is paid once per group.  Rows past the chain's
+// end are computed on the clamped copies and their results are sent to a
+// per-lane dummy slot, so the group body has no branch.
+constexpr int kGroup = 4;
+
+// Forward elimination of chain k (slots j0 .. j1-1, bottom to top) in the
+// delta-V form.  (sd, sr) is what the rows below subtract from the current
+// row: the kids' contributions at the bottom, then the previous row's.
+template <typename real>
+__device__ __forceinline__ void eliminate_chain(int k, int dummy, const int4* CR, const int16_t* m_kid,
+                                                HRow<real>* R, const real* xv, KPair<real>* KK) {
+  const ChainRec c = unpack_chain(CR[k]);
+  HRow<real> cur[kGroup], nxt[kGroup];
+#pragma unroll
+  for (int i = 0; i < kGroup; ++i) cur[i] = R[min(c.j0 + i, c.j1 - 1)];
+  // the top edge's axial term for the row the chain hangs from (zero for a root)
+  const int tq = c.tp >= 0 ? c.tp : c.j1 - 1;
+  const real top_defect = R[c.j1 - 1].up * (xv[c.j1 - 1] - xv[tq]);
+  real sd = real(0), sr = real(0);
+  if (c.nk > 0) {
+    const KPair<real> t = KK[c.k0];
+    sd = t.d;
+    sr = t.r;
+  }
+  if (c.nk > 1) {
+    const KPair<real> t = KK[c.k1];
+    sd += t.d;
+    sr += t.r;
+  }
+  for (int q = 2; q < c.nk; ++q) {
+    const KPair<real> t = KK[m_kid[c.q0 + q]];
+    sd += t.d;
+    sr += t.r;
+  }
+  for (int j = c.j0;; j += kGroup) {
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) nxt[i] = R[min(j + kGroup + i, c.j1 - 1)];
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) {
+      const bool valid = j + i < c.j1;
+      const HRow<real>& w = cur[i];
+      const real e = w.d - sd;
+      const real r = w.r - sr;
+      const real inv = bc_rcp<real>(e - w.lo);
+      const real ui = w.up * inv;
+      HRow<real>* dst = &R[valid ? j + i : dummy];
+      dst->d = inv;
+      dst->r = r;
+      sd = valid ? ui * e : sd;
+      sr = valid ? ui * r : sr;
+    }
+    if (j + kGroup >= c.j1) break;
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) cur[i] = nxt[i];
+  }
+  KK[k].d = sd;
+  KK[k].r = sr + top_defect;
+}
+
+// Back substitution in affine form.  Inside chain k the solution of row j
+// is x_j = (r_j - lo_j x_{j+1}) / d_j, so x_j = alpha_j + beta_j X where X is
+// the solution of the row the chain hangs from (zero for a root):
+//   alpha_j = (r_j - lo_j alpha_{j+1}) / d_j,  beta_j = -lo_j beta_{j+1} / d_j,
+// starting from alpha = 0, beta = 1 above the top (beta = 0 for a root).
+// Every chain computes its (alpha, beta) at once, independently of the
+// others; the chains' X then follow from pointer jumping over the chain
+// tree.  R[j].d / R[j].r are overwritten with beta_j / alpha_j, and the
+// bottom row's pair is returned through KK[k] = (alpha, beta).
+template <typename real>
+__device__ __forceinline__ void chain_affine(int k, int dummy, const int4* CR, HRow<real>* R, KPair<real>* KK) {
+  const ChainRec c = unpack_chain(CR[k]);
+  real al = real(0), be = c.tp >= 0 ? real(1) : real(0);
+  HRow<real> cur[kGroup], nxt[kGroup];
+#pragma unroll
+  for (int i = 0; i < kGroup; ++i) cur[i] = R[max(c.j1 - 1 - i, c.j0)];
+  for (int j = c.j1 - 1;; j -= kGroup) {
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) nxt[i] = R[max(j - kGroup - i, c.j0)];
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) {
+      const bool valid = j - i >= c.j0;
+      const real a2 = (cur[i].r - cur[i].lo * al) * cur[i].d;
+      const real b2 = -cur[i].lo * be * cur[i].d;
+      HRow<real>* dst = &R[valid ? j - i : dummy];
+      dst->d = b2;
+      dst->r = a2;
+      al = valid ? a2 : al;
+      be = valid ? b2 : be;
+    }
+    if (j - kGroup < c.j0) break;
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) cur[i] = nxt[i];
+  }
+  KK[k].d = al;
+  KK[k].r = be;
+}
+
+// Solutions of chain k's rows from X, the solution of the row it hangs from.
+template <typename real>
+__device__ __forceinline__ void chain_solution(int k, real X, const int4* CR, HRow<real>* R) {
+  const ChainRec c = unpack_chain(CR[k]);
+  for (int j0 = c.j0; j0 < c.j1; j0 += kGroup) {
+    HRow<real> w[kGroup];
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) w[i] = R[min(j0 + i, c.j1 - 1)];
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i)
+      if (j0 + i < c.j1) R[j0 + i].r = w[i].r + w[i].d * X;
+  }
+}
+
+// Dummy rows after the pack's rows: one per thread, written by rows past a
+// chain's end in a branch-free row group and never read.
+constexpr int kDummySlots = 256;
+
+// LDS carve-up (bytes), shared with the launch code: rows and chains are
+// rounded up to multiples of 4 so every array stays naturally aligned.
+struct LdsPlan {
+  size_t rows, recs, kk, kk2, xv, cls, cnt, kid, lv, link, par, ptr, ptr2, total;
+};
+template <typename real>
+__host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int max_cls, int max_levels) {
+  const size_t mr = (size_t)((max_rows + 3) & ~3), mc = (size_t)((max_chains + 3) & ~3);
+  LdsPlan p;
+  p.rows = 0;
+  p.recs = p.rows + (mr + kDummySlots) * sizeof(HRow<real>);
+  p.kk = p.recs + mc * sizeof(int4);
+  p.kk2 = p.kk + mc * sizeof(KPair<real>);
+  p.xv = p.kk2 + mc * sizeof(KPair<real>);
+  p.cls = p.xv + mr * sizeof(real);
+  p.cnt = p.cls + (size_t)max_cls * BC_NCLS * sizeof(real);
+  p.kid = p.cnt + mc * sizeof(int);
+  p.lv = p.kid + mc * sizeof(int16_t);
+  p.link = p.lv + (size_t)((max_levels + 4) & ~3) * sizeof(int16_t);
+  p.par = p.link + mr * sizeof(int16_t);
+  p.ptr = p.par + mc * sizeof(int16_t);
+  p.ptr2 = p.ptr + mc * sizeof(int16_t);
+  p.total = p.ptr2 + mc * sizeof(int16_t);
+  return p;
+}
+
 template <typename real, uint32_t CH, int MAXC>
 __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   constexpr int NS = bc_nstates(CH) > 0 ? bc_nstates(CH) : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  real* bd = reinterpret_cast<real*>(smem_raw);
-  real* br = bd + a.max_rows;
-  real* xv = br + a.max_rows;  // V_n at every slot
-  real* LO = xv + a.max_rows;  // dt * A[slot, parent]
-  real* UP = LO + a.max_rows;  // dt * A[parent, slot]
-  real* kd = UP + a.max_rows;
-  real* kr = kd + a.max_chains;
-  real* cls = kr + a.max_chains;
-  // chain metadata, 16-bit: first slot of each chain (+ end), top parent slot,
-  // kid list offsets (+ end), kid chains, level offsets (+ end)
-  int16_t* m_j0 = reinterpret_cast<int16_t*>(cls + a.max_cls * BC_NCLS);
-  int16_t* m_tp = m_j0 + (a.max_chains + 1);
-  int16_t* m_k0 = m_tp + a.max_chains;
-  int16_t* m_kid = m_k0 + (a.max_chains + 1);
-  int16_t* m_lv = m_kid + a.max_chains;
+  const LdsPlan lp = lds_plan<real>(a.max_rows, a.max_chains, a.max_cls, a.max_levels);
+  HRow<real>* R = reinterpret_cast<HRow<real>*>(smem_raw + lp.rows);
+  int4* CR = reinterpret_cast<int4*>(smem_raw + lp.recs);
+  KPair<real>* KK = reinterpret_cast<KPair<real>*>(smem_raw + lp.kk);
+  real* xv = reinterpret_cast<real*>(smem_raw + lp.xv);  // V_n at every slot
+  real* cls = reinterpret_cast<real*>(smem_raw + lp.cls);
+  int16_t* m_kid = reinterpret_cast<int16_t*>(smem_raw + lp.kid);
+  int16_t* m_lv = reinterpret_cast<int16_t*>(smem_raw + lp.lv);
+  KPair<real>* KK2 = reinterpret_cast<KPair<real>*>(smem_raw + lp.kk2);
+  int* m_cnt = reinterpret_cast<int*>(smem_raw + lp.cnt);  // kids eliminated so far
+  // parent slot of every slot (15 bits; a root's top points at itself) | 1 << 15
+  // for a chain's bottom slot (its children are the kid chains, not slot - 1)
+  int16_t* m_link = reinterpret_cast<int16_t*>(smem_raw + lp.link);
+  int16_t* m_par = reinterpret_cast<int16_t*>(smem_raw + lp.par);    // parent chain, -1 for a root
+  int16_t* m_ptr = reinterpret_cast<int16_t*>(smem_raw + lp.ptr);  // pointer jumping (two buffers)
+  int16_t* m_ptr2 = reinterpret_cast<int16_t*>(smem_raw + lp.ptr2);
 
   const bc_pack P = a.packs[blockIdx.x];
   const int tid = threadIdx.x;
   const int NT = blockDim.x;
   for (int i = tid; i < P.n_cls * BC_NCLS; i += NT) cls[i] = a.cls_tab[(int64_t)P.cls0 * BC_NCLS + i];
   for (int i = tid; i < P.n_rows; i += NT) {
-    LO[i] = a.lo[(int64_t)P.row0 + i];
-    UP[i] = a.up[(int64_t)P.row0 + i];
+    R[i].lo = a.lo[(int64_t)P.row0 + i];
+    R[i].up = a.up[(int64_t)P.row0 + i];
   }
   const int kbase = a.kid_off[P.ch0];
-  for (int i = tid; i <= P.n_ch; i += NT) {
-    m_j0[i] = (int16_t)(a.chain_off[P.ch0 + i] - P.row0);
-    m_k0[i] = (int16_t)(a.kid_off[P.ch0 + i] - kbase);
-    if (i < P.n_ch) m_tp[i] = (int16_t)a.chain_top[P.ch0 + i];
+  for (int i = tid; i < P.n_ch; i += NT) {
+    const int g = P.ch0 + i;
+    const int j0 = a.chain_off[g] - P.row0, j1 = a.chain_off[g + 1] - P.row0;
+    const int q0 = a.kid_off[g] - kbase, nk = a.kid_off[g + 1] - kbase - q0;
+    const int k0 = nk > 0 ? a.kids[kbase + q0] : 0;
+    const int k1 = nk > 1 ? a.kids[kbase + q0 + 1] : 0;
+    const int tp = a.chain_top[g];
+    CR[i] = make_int4(j0 | (j1 << 16), (tp & 0xFFFF) | (nk << 16), k0 | (k1 << 16), q0);
+    for (int j = j0; j < j1; ++j) {
+      const int par = (j + 1 < j1) ? j + 1 : (tp >= 0 ? tp : j);
+      m_link[j] = (int16_t)(par | (j == j0 ? 0x8000 : 0));
+    }
+    for (int q = 0; q < nk; ++q) m_par[a.kids[kbase + q0 + q]] = (int16_t)i;
+    if (tp < 0) m_par[i] = -1;
   }
   for (int i = tid; i < a.kid_off[P.ch0 + P.n_ch] - kbase; i += NT) m_kid[i] = (int16_t)a.kids[kbase + i];
   for (int i = tid; i <= P.n_lv; i += NT) m_lv[i] = (int16_t)a.level_off[P.lv0 + i];
@@ -287,6 +501,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   for (int i = tid; i < P.n_bnd; i += NT) xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu] = a.Vbnd[P.bnd0 + i];
   __syncthreads();
 
+  const int dummy = ((a.max_rows + 3) & ~3) + tid;
   const real dt = a.dt;
   const real ca_b = dt * bc_exprel<real>(-dt * a.cc.ca_tau_inv);
   uint32_t bad = 0;
@@ -303,87 +518,113 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         ica[i] = linearise<real, CH>(a.cc, cp, st[i], vv[i], G, GE);
         real Icl = (info[i] & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0);
         const real dtcm = cp[BC_CLS_DTCM];
-        bd[row] = real(1) + dtcm * G;
-        br[row] = dtcm * (GE + Icl);
+        const real D = axial_defect<real>(row, vv[i], R, xv, m_link);
+        R[row].d = real(1) + dtcm * G;
+        R[row].r = dtcm * (GE + Icl) + D;
       }
     }
     // boundary (algebraic) rows
     for (int i = tid; i < P.n_bnd; i += NT) {
       uint32_t b = a.bnd_info[P.bnd0 + i];
       int row = (int)(b & 0x7FFFFFFFu);
-      bd[row] = real(0);
-      br[row] = (b & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0);
+      const real D = axial_defect<real>(row, xv[row], R, xv, m_link);
+      R[row].d = real(0);
+      R[row].r = ((b & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0)) + D;
     }
+    for (int k = tid; k < P.n_ch; k += NT) m_cnt[k] = 0;
     __syncthreads();
 
-    // ---- phase B: Hines elimination, leaves to root, one chain per thread.
+    // ---- phase B: Hines elimination as a dataflow over chains, no barrier.
     // Diagonal rebuilt from the zero row sums of the axial operator:
     // d_j = e_j - lo_j with e_j = diag_j - sum over children c of
     // up_c e_c / d_c >= 0, so no step subtracts nearly equal numbers.
-    for (int l = 0; l < P.n_lv; ++l) {
-      const int k0 = m_lv[l], k1 = m_lv[l + 1];
-      for (int k = k0 + tid; k < k1; k += NT) {
-        const int j0 = m_j0[k], j1 = m_j0[k + 1];
-        const int tp = m_tp[k];
-        real e = bd[j0];
-        real r = br[j0];
-        const int q0 = m_k0[k], q1 = m_k0[k + 1];
-        for (int q = q0; q < q1; ++q) {
-          const int kk = m_kid[q];
-          e -= kd[kk];
-          r -= kr[kk];
+    // Every thread starts at leaf chains; after eliminating a chain it bumps
+    // the parent's arrival counter and the last kid to arrive goes on with
+    // the parent, so a chain starts as soon as its kids are done and the
+    // critical path is the longest leaf-to-root path.  Chain values do not
+    // depend on which thread takes a chain: results are deterministic.
+    {
+      const int n_leaf = m_lv[1] - m_lv[0];
+      int leaf = tid;
+      int k = leaf < n_leaf ? m_lv[0] + leaf : -1;
+      while (k >= 0) {
+        eliminate_chain<real>(k, dummy, CR, m_kid, R, xv, KK);
+        const int p = m_par[k];
+        bool climb = false;
+        if (p >= 0) {
+          const int nk = (int)((unsigned)CR[p].y >> 16);
+          climb = __hip_atomic_fetch_add(&m_cnt[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == nk - 1;
         }
-        real vr = xv[j0];
-        real kdc, krc, up, dl;
-        for (int j = j0;;) {
-          // axial defect of the edge to the parent: dt A V_n terms
-          const int pr = (j + 1 < j1) ? j + 1 : tp;
-          const real vp = pr >= 0 ? xv[pr] : vr;
-          const real lo = LO[j];
-          up = UP[j];
-          dl = vr - vp;
-          r += lo * dl;
-          const real inv = bc_rcp<real>(e - lo);
-          bd[j] = inv;
-          br[j] = r;
-          kdc = up * e * inv;
-          krc = up * r * inv;
-          if (++j >= j1) break;
-          vr = vp;
-          e = bd[j] - kdc;
-          r = (br[j] - krc) - up * dl;
-        }
-        kd[k] = kdc;
-        kr[k] = krc + up * dl;
-      }
-      __syncthreads();
-    }
-    // ---- back substitution, root to leaves
-    for (int l = P.n_lv - 1; l >= 0; --l) {
-      const int k0 = m_lv[l], k1 = m_lv[l + 1];
-      for (int k = k0 + tid; k < k1; k += NT) {
-        const int j0 = m_j0[k], j1 = m_j0[k + 1];
-        const int tp = m_tp[k];
-        real xp = tp >= 0 ? br[tp] : real(0);
-        for (int j = j1 - 1; j >= j0; --j) {
-          const real x = (br[j] - LO[j] * xp) * bd[j];
-          br[j] = x;
-          xp = x;
+        if (climb) {
+          k = p;
+        } else {
+          leaf += NT;
+          k = leaf < n_leaf ? m_lv[0] + leaf : -1;
         }
       }
-      __syncthreads();
     }
+    __syncthreads();
+    // ---- back substitution: per-chain affine maps, pointer jumping over the
+    // chain tree for the chains' bottom solutions, then every row at once
+    for (int k = tid; k < P.n_ch; k += NT) {
+      chain_affine<real>(k, dummy, CR, R, KK);
+      m_ptr[k] = m_par[k];
+    }
+    __syncthreads();
+    {
+      // (A, B, ptr): x_bottom(k) = A + B x_bottom(ptr); ptr < 0 once resolved
+      KPair<real>* A0 = KK;
+      KPair<real>* A1 = KK2;
+      int16_t* p0 = m_ptr;
+      int16_t* p1 = m_ptr2;
+      // a forest of at most 32767 chains resolves in 16 rounds; the bound
+      // only turns an inconsistent layout into an error instead of a hang
+      for (int round = 0;; ++round) {
+        int open = 0;
+        for (int k = tid; k < P.n_ch; k += NT) {
+          const int q = p0[k];
+          KPair<real> v = A0[k];
+          int qn = q;
+          if (q >= 0) {
+            const KPair<real> u = A0[q];
+            v.d = v.d + v.r * u.d;
+            v.r = v.r * u.r;
+            qn = p0[q];
+            open |= qn >= 0;
+          }
+          A1[k] = v;
+          p1[k] = (int16_t)qn;
+        }
+        const int more = __syncthreads_or(open);
+        KPair<real>* ta = A0;
+        A0 = A1;
+        A1 = ta;
+        int16_t* tq = p0;
+        p0 = p1;
+        p1 = tq;
+        if (!more) break;
+        if (round >= 16) {
+          bad |= 2u;
+          break;
+        }
+      }
+      for (int k = tid; k < P.n_ch; k += NT) {
+        const int p = m_par[k];
+        chain_solution<real>(k, p >= 0 ? A0[p].d : real(0), CR, R);
+      }
+    }
+    __syncthreads();
 
     // ---- phase C: V += dV; Ca pool and gates at V_{n+1}
     for (int i = tid; i < P.n_bnd; i += NT) {
       const int row = (int)(a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu);
-      xv[row] += br[row];
+      xv[row] += R[row].r;
     }
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       if (info[i] != 0xFFFFFFFFu) {
         const int row = (int)(info[i] & kRowMask);
-        const real Vn = vv[i] + br[row];
+        const real Vn = vv[i] + R[row].r;
         xv[row] = Vn;
         advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt, ca_b);
         vv[i] = Vn;
@@ -391,6 +632,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         if (pcol[i] >= 0) a.trace[(int64_t)step * a.trace_ld + pcol[i]] = Vn;
       }
     }
+    __syncthreads();  // phase A reads the voltages of other threads' rows
   }
 
   if (a.rec) record_state<real, MAXC, NS>(a, P, tid, NT, a.n_steps, vv, st, info);
@@ -405,7 +647,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
   }
   for (int i = tid; i < P.n_bnd; i += NT) a.Vbnd[P.bnd0 + i] = xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu];
-  if (bad) atomicOr(a.status, 1u);
+  if (bad) atomicOr(a.status, bad);
 }
 
 // ------------------------------------------------------------------ dispatch
@@ -474,6 +716,7 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->max_rows = d->max_rows;
   a->max_chains = d->max_chains;
   a->max_cls = d->max_cls;
+  a->max_levels = d->max_levels;
   if (d->chan_const == nullptr) return bc::fail(BC_INVALID_ARG, "chan_const is NULL");
   a->cc = *static_cast<const bc_chan_const<real>*>(d->chan_const);
   return BC_OK;
@@ -509,8 +752,7 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
   a.step0 = step0;
   a.n_steps = n_steps;
   a.trace = static_cast<real*>(trace);
-  size_t lds = (size_t)(5 * d->max_rows + 2 * d->max_chains + d->max_cls * BC_NCLS) * sizeof(real) +
-               (size_t)(4 * d->max_chains + 3 + d->max_levels + 1) * sizeof(int16_t);
+  const size_t lds = lds_plan<real>(d->max_rows, d->max_chains, d->max_cls, d->max_levels).total;
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "pack needs %zu bytes of LDS (> 160 KiB)", lds);
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (d->n_packs == 0 || n_steps == 0) return BC_OK;

@@ -600,8 +600,11 @@ class StaggerRuntime:
         return V, S
 
     def check_status(self):
-        if int(self.status.item()) != 0:
+        st = int(self.status.item())
+        if st != 0:
             from ._native import NativeError
+            if st & 2:
+                raise NativeError(4, "chain tree of a pack did not resolve (inconsistent layout)")
             raise NativeError(4, "non-finite membrane voltage detected")
 
 

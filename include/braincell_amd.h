@@ -83,7 +83,9 @@ typedef struct bc_pack {
  *                 adjoint, rec[((step * (NS + 1)) + slot) * n_cv_total + cv],
  *                 slot 0 = V, slots 1..NS = states, steps 0 .. n_steps
  *                 (the last one is the final state)
- *  status         OR-ed with 1 when a non-finite voltage appears (out)
+ *  status         OR-ed with 1 when a non-finite voltage appears, with 2
+ *                 when the chain tree of a pack does not resolve (an
+ *                 inconsistent layout) (out)
  * real = float (precision 32) or double (precision 64). */
 typedef struct bc_stagger_desc {
   int32_t precision;    /* 32 or 64 */

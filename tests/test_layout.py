@@ -59,6 +59,18 @@ def test_chain_invariants(maker):
             assert ch.level[kid] < ch.level[k]
     assert np.all(np.diff(ch.level) >= 0)
     assert ch.level_off[-1] == ch.n_chains
+    # every chain of level h > 0 has a kid of level h - 1, so level widths never
+    # grow towards the root (the kernel runs the narrow tail in one wavefront)
+    assert np.all(np.diff(np.diff(ch.level_off)) <= 0)
+
+
+def test_pack_level_widths_never_grow():
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=600) for i in range(8)]
+    L = StaggerLayout(models, rows_per_pack=1500, cv_per_thread=4)
+    lvo = np.asarray(L.level_off)
+    for pk in L.packs:
+        w = np.diff(lvo[pk.lv0:pk.lv0 + pk.n_lv + 1])
+        assert np.all(w > 0) and np.all(np.diff(w) <= 0)
 
 
 def test_packed_hines_emulation_matches_oracle_dhs():
