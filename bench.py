@@ -30,22 +30,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(models, probe_cvs, n_steps, gpu_soma, dt):
-    """Oracle (FP64 NumPy restatement, one core) on a bounded sample of the
-    same workload; also returns the GPU-vs-oracle soma RMSE on that sample."""
-    from oracle.staggered import StaggeredOracle
+def cpu_baseline(models, soma_cvs, n_steps, gpu_soma, dt):
+    """FP64 C port of the reference algorithm (OpenMP over cells, every host
+    core this process may use) on a bounded sample of the same workload.
+    Returns (compartment-updates/s, seconds, threads, per-cell soma RMSE of
+    the GPU FP32 trace against the port's FP64 trace)."""
+    from oracle.cport_run import PortBatch
 
-    cvs = 0
-    rm = []
+    batch = PortBatch(models, dt, soma_cvs)
+    threads = PortBatch.max_threads()
     t0 = time.perf_counter()
-    for i, m in enumerate(models):
-        o = StaggeredOracle(m)
-        o.reset_state(m.V0)
-        tr = o.run(dt, n_steps, probe_cvs=[int(probe_cvs[i])])
-        cvs += o.C
-        rm.append(float(np.sqrt(np.mean((tr[:, 0] - gpu_soma[:, i]) ** 2))))
+    tr = batch.run(n_steps, sequential=True, threads=threads)
     el = time.perf_counter() - t0
-    return cvs * n_steps / el, el, rm
+    rm = [float(np.sqrt(np.mean((tr[:, i] - gpu_soma[:, i]) ** 2))) for i in range(len(models))]
+    return batch.n_cv * n_steps / el, el, threads, rm
 
 
 def main():
@@ -57,9 +55,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--threads", type=int, default=256)
     ap.add_argument("--rows-per-pack", type=int, default=2600)
-    ap.add_argument("--cpu-sample-cells", type=int, default=2)
-    ap.add_argument("--cpu-sample-steps", type=int, default=800)
+    ap.add_argument("--cpu-sample-cells", type=int, default=128)
+    ap.add_argument("--cpu-sample-steps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="finish with a 2 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
@@ -84,7 +84,6 @@ def main():
     rt.run(Wm, trace=trace[:Wm], chunk=chunk)
     torch.cuda.synchronize()
     rt.check_status()
-    warm_soma = trace[:Wm].double().cpu().numpy()
     # timed: hipGraph of K / chunk persistent launches
     rt.reset_state()
     graph = rt.make_graph(K // chunk, chunk, trace)
@@ -99,13 +98,15 @@ def main():
     log(f"timed {K} steps in {el:.3f}s -> {cu:.3e} compartment-updates/s")
 
     # per-launch duration of the dominant kernel with HIP events on its stream
+    # (own trace buffer: the timed run's trace is the accuracy sample below)
     rt.reset_state()
     n_ev = 5
+    ev_trace = rt.new_trace(chunk)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(n_ev + 1)]
     torch.cuda.synchronize()
     evs[0].record()
     for i in range(n_ev):
-        rt.run(chunk, trace=trace[:chunk], chunk=chunk)
+        rt.run(chunk, trace=ev_trace, chunk=chunk)
         evs[i + 1].record()
     torch.cuda.synchronize()
     durs = [evs[i].elapsed_time(evs[i + 1]) * 1e-3 for i in range(n_ev)]
@@ -127,14 +128,28 @@ def main():
     cpu = None
     acc = None
     if not args.no_cpu:
-        ns = min(args.cpu_sample_steps, Wm)
-        sample = list(range(args.cpu_sample_cells))
-        cvals, cel, rms = cpu_baseline([models[i] for i in sample], [L.cells[i].probes[0] for i in sample], ns,
-                                       warm_soma[:ns, sample], dt)
-        cpu = {"value": float(f"{cvals:.4g}"), "unit": "compartment-updates/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 NumPy oracle, cells {sample} ({sum(L.cells[i].geom.n_cv for i in sample)} CVs) x "
-                         f"{ns} steps, {cel:.1f}s"}
-        acc = {"soma_rmse_mV_max": max(rms), "cells": len(sample), "steps": ns}
+        # the timed run started from the initial state, so its trace is the
+        # GPU side of the accuracy comparison
+        ns = min(args.cpu_sample_steps, K)
+        sample = list(range(min(args.cpu_sample_cells, args.cells)))
+        gpu_soma = trace[:ns, [int(L.cell_probe0[i]) for i in sample]].double().cpu().numpy()
+        t2 = time.perf_counter()
+        cvals, cel, cth, rms = cpu_baseline([models[i] for i in sample], [L.cells[i].probes[0] for i in sample], ns,
+                                            gpu_soma, dt)
+        log(f"cpu baseline {cvals:.3e} compartment-updates/s on {cth} threads ({cel:.1f}s run, "
+            f"{time.perf_counter() - t2:.1f}s with setup)")
+        ncv = sum(L.cells[i].geom.n_cv for i in sample)
+        cpu = {"value": float(f"{cvals:.4g}"), "unit": "compartment-updates/s", "cores": cth, "kind": "port",
+               "sample": f"FP64 C port (OpenMP, {cth} threads), cells 0-{len(sample) - 1} of the same batch "
+                         f"({ncv} CVs) x {ns} steps, {cel:.1f}s"}
+        acc = {"soma_rmse_mV_max": float(f"{max(rms):.3g}"), "soma_rmse_mV_mean": float(f"{np.mean(rms):.3g}"),
+               "cells": len(sample), "steps": ns, "reference": "FP64 C port"}
+    if args.calibrate_copy:
+        a = torch.empty(512 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     out = {
         "metric": "compartment-updates/s", "value": float(f"{cu:.4g}"), "unit": "compartment-updates/s",
         "n_gpus": 1, "steps": K, "warmup": Wm, "ms_per_step": round(el / K * 1e3, 5), "higher_is_better": True,
