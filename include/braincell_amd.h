@@ -67,7 +67,9 @@ typedef struct bc_pack {
  *  chain_top[k]   local slot the chain's top hangs from, -1 for a root
  *  kid_off[k]     kids of chain k: kids[kid_off[k] .. kid_off[k+1]-1]
  *                 (local chain indices; size n_chains_total + 1)
- *  level_off[.]   per pack n_lv + 1 local chain offsets
+ *  level_off[.]   per pack n_lv + 1 local chain offsets of the chain levels
+ *                 (level = chain height; level 0 = the leaf chains, where
+ *                 the kernel's dataflow elimination starts)
  *  lo/up[row]     dt * lowers, dt * uppers (A[row, parent], A[parent, row],
  *                 ms^-1) per global slot; the axial diagonal is rebuilt
  *                 from them (zero row sums of the axial operator)
@@ -91,7 +93,7 @@ typedef struct bc_stagger_desc {
   int32_t precision;    /* 32 or 64 */
   uint32_t channels;    /* BC_CH_* set compiled into the kernel */
   int32_t n_packs;
-  int32_t threads;      /* workgroup size (64..1024, multiple of 64) */
+  int32_t threads;      /* workgroup size (64, 128, 192 or 256) */
   int32_t cv_per_thread;/* register slots per thread: 1, 2, 4, 8 or 16 */
   int32_t max_rows, max_chains, max_cls; /* LDS sizing (max over packs) */
   int32_t max_levels;
