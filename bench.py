@@ -48,6 +48,9 @@ def cpu_baseline(models, soma_cvs, n_steps, gpu_soma, dt):
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=("cells", "point", "adjoint"), default="cells",
+                    help="cells: 4096 uneven trees (default, the headline); point: 100k HTC point neurons; "
+                         "adjoint: 512 region trees, forward + checkpointed adjoint")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -66,6 +69,14 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.workload == "point":
+        return bench_point(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
+    return bench_cells(args)
+
+
+def bench_cells(args):
     import torch
     from braincell_amd import workloads as W
     from braincell_amd.runtime import StaggerLayout, StaggerRuntime
@@ -162,6 +173,106 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "accuracy": acc,
     }
     print(json.dumps(out), flush=True)
+
+
+FP32_PEAK_TFLOPS = 157.3
+POINT_FLOP = 440.0  # ~200 FMA + ~40 exp/log/div per neuron-step
+
+
+def bench_point(args):
+    """100 000 HTC point neurons, FP32, dt 0.025 ms, state in registers for
+    4000-step launches."""
+    import torch
+    from braincell_amd import workloads as W
+    from braincell_amd.single import SingleCompartment
+
+    n, dt, chunk = 100_000, 0.025, 4000
+    pop = W.htc_population(n, seed=1)
+    channels = {nm: dict(W.HTC_CHANNELS[nm], g_max=pop["g"][nm]) for nm in pop["names"]}
+    sc = SingleCompartment(n, channels=channels, E_Na=W.HTC_IONS["E_Na"], E_K=W.HTC_IONS["E_K"],
+                           ca=W.HTC_IONS["ca"], V0=pop["V0"], precision=32)
+    sc.init_state()
+    sc.set_input(pop["I_dc_nA"])
+    K = max(chunk, (args.steps // chunk) * chunk)
+    sc.advance(0.0, dt, chunk)
+    torch.cuda.synchronize()
+    sc.reset_state(pop["V0"])
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(K // chunk + 1)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    evs[0].record()
+    for i in range(K // chunk):
+        sc.advance(i * chunk * dt, dt, chunk)
+        evs[i + 1].record()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    launch_s = float(np.mean([evs[i].elapsed_time(evs[i + 1]) * 1e-3 for i in range(K // chunk)]))
+    val = n * K / el
+    flops = POINT_FLOP * n * chunk / launch_s / 1e12
+    cpu = None
+    if not args.no_cpu:
+        from oracle.single import HTCPopulationOracle
+
+        m, ns = 20_000, 400
+        g = {nm: pop["g"][nm][:m] for nm in pop["names"]}
+        params = {nm: {k: v for k, v in W.HTC_CHANNELS[nm].items() if k != "g_max"} for nm in pop["names"]}
+        o = HTCPopulationOracle(g, pop["I_dc_nA"][:m], params, W.HTC_IONS)
+        o.reset_state(pop["V0"][:m])
+        t2 = time.perf_counter()
+        o.run(dt, ns)
+        cel = time.perf_counter() - t2
+        cpu = {"value": float(f"{m * ns / cel:.4g}"), "unit": "neuron-steps/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 NumPy oracle (vectorised over neurons), {m} neurons x {ns} steps, {cel:.1f}s"}
+    print(json.dumps({
+        "metric": "neuron-steps/s", "value": float(f"{val:.4g}"), "unit": "neuron-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": chunk, "ms_per_step": round(el / K * 1e3, 6), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "100000 HTC point neurons (Na, KDR, K_Leak, CaL, CaT, AHP, HCN, IL, CalciumDetailed), "
+                               "lognormal densities, ind_exp_euler, dt 0.025 ms", "neurons": n,
+                   "steps_per_launch": chunk},
+        "roofline": {"bound": "valu", "achieved": round(flops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(flops / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": "point_kernel<float>",
+                     "launch_ms": round(launch_s * 1e3, 3), "flop_per_neuron_step": POINT_FLOP},
+        "cpu_baseline": cpu}), flush=True)
+
+
+def bench_adjoint(args):
+    """512 region trees (~1000 CVs), FP32, 200 ms: MSE loss against a target
+    trace from hidden per-region densities; forward with checkpoints every
+    200 steps plus the hand-written adjoint."""
+    import torch
+    from braincell_amd import workloads as W
+    from braincell_amd.adjoint import AdjointRuntime
+
+    n_cells, dt, T = 512, 0.025, 8000
+    rng = np.random.default_rng(7)
+    base = np.asarray([W.BASE_DENSITIES[k] for k in ("Na_Ba2002", "KDR_Ba2002", "IL", "CaL_IS2008")])
+    hidden = base[None, None, :] * np.exp(0.3 * rng.standard_normal((n_cells, 3, 4)))
+    models = [W.region_tree_cell(i) for i in range(n_cells)]
+    t0 = time.perf_counter()
+    adj = AdjointRuntime(models, dt, checkpoint_every=200)
+    log(f"adjoint layout {adj.L.summary()} built in {time.perf_counter() - t0:.1f}s")
+    adj.set_params(torch.as_tensor(hidden))
+    target = adj.forward(T).clone()
+    theta = torch.as_tensor(np.broadcast_to(base, (n_cells, 3, 4)).copy())
+    adj.set_params(theta)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    tr = adj.forward(T)
+    bar = 2.0 * (tr - target) / tr.numel()
+    g = adj.theta_grad(adj.backward(bar))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    ncv = adj.L.n_cv_total
+    val = ncv * T / el
+    print(json.dumps({
+        "metric": "compartment-steps/s (forward + adjoint)", "value": float(f"{val:.4g}"),
+        "unit": "compartment-steps/s", "n_gpus": 1, "steps": T, "warmup": 0, "ms_per_step": round(el / T * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "512 region trees (250 branches, ~1000 CVs), Na/KDR/IL/CaL + CalciumDetailed, dt 0.025 ms, "
+                               "200 ms, MSE(soma V) vs hidden lognormal per-region densities, checkpoint every 200 steps",
+                   "cells": n_cells, "compartments": int(ncv), "params_per_cell": 12},
+        "gradient_norm": float(g.norm().item()), "loss": float(((tr - target) ** 2).mean().item())}), flush=True)
 
 
 if __name__ == "__main__":
