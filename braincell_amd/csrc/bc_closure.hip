@@ -1,0 +1,497 @@
+// Learned gating closure: a per-compartment MLP 8 -> 64 -> 64 -> 5 (tanh)
+// that corrects the exponential-Euler gate update of the staggered step at
+// large dt.  No counterpart in the reference.
+//
+// Features of one compartment at a coarse step n -> n+1 of length dt:
+//   f = (V_{n+1}, Na p_n, Na q_n, KDR p_n, CaL p_n, CaL q_n, ln Ci_{n+1}, dt),
+// normalised as (f - mu) / sd.  The network's 5 outputs, times per-output
+// scales, are added to the plain update of the 5 gates (the staggered step's
+// own x_inf + (x - x_inf) e^{dt lam} at V_{n+1}).
+//
+// Parameters (FP32 master copy, 5061 values): W1[8][64] b1[64] W2[64][64]
+// b2[64] W3[64][5] b3[5], W[i][o] = weight from input i to output o.
+//
+// Kernels:
+//   closure_data      training pairs from fine trajectories (features, target
+//                     increments minus the plain update)
+//   closure_grad      FP32 forward + backward of the MSE loss on a minibatch,
+//                     per-block partial gradients (deterministic)
+//   closure_reduce    sums the partials; closure_adam: Adam step
+//   closure_fwd32     FP32 inference (reference for the MFMA path)
+//   closure_apply     BF16 MFMA inference with FP32 accumulation, fused with
+//                     feature extraction and the correction of the state
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "bc_channels.h"
+#include "bc_common.h"
+#include "braincell_amd.h"
+
+namespace {
+
+constexpr int NIN = 8, NH = 64, NOUT = 5;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NIN * NH, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH * NOUT, NPARAM = OFF_B3 + NOUT;
+static_assert(NPARAM == BC_CLOSURE_NPARAM, "parameter count");
+
+// Gate slots of the Na/KDR/IL/CaL/pool state layout.
+constexpr int S_NAP = 0, S_NAQ = 1, S_KDP = 2, S_CLP = 3, S_CLQ = 4, S_CI = 5, NSTATE = 6;
+
+__device__ __forceinline__ float fast_tanh(float x) {
+  // tanh x = 1 - 2 / (e^{2x} + 1); saturates correctly for large |x|
+  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2 log2(e)
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+// Plain exponential-Euler update of the 5 gates at voltage V (the staggered
+// step's phase-C formulas).
+__device__ __forceinline__ void plain_gates(const bc_chan_const<float>& cc, float V, const float* x, float dt,
+                                            float* out) {
+  float ap, bp, aq, bq;
+  bc_na_rates<float>(V, cc.na_vsh, ap, bp, aq, bq);
+  out[0] = bc_gate_ab_cf<float>(x[0], ap, bp, cc.na_phi, dt);
+  out[1] = bc_gate_ab_cf<float>(x[1], aq, bq, cc.na_phi, dt);
+  float kp, kb;
+  bc_kdr_rates<float>(V, cc.kdr_vsh, kp, kb);
+  out[2] = bc_gate_ab_cf<float>(x[2], kp, kb, cc.kdr_phi, dt);
+  float pi, pt, qi, qt;
+  bc_cal_rates<float>(V, cc.cal_vsh, pi, pt, qi, qt);
+  out[3] = bc_gate_it_cf<float>(x[3], pi, pt, cc.cal_phi_p, dt);
+  out[4] = bc_gate_it_cf<float>(x[4], qi, qt, cc.cal_phi_q, dt);
+}
+
+// ------------------------------------------------------------ training data
+__global__ void closure_data(int64_t n, float dt, bc_chan_const<float> cc, const float* __restrict__ V1,
+                             const float* __restrict__ S0, const float* __restrict__ S1, int64_t ld,
+                             float* __restrict__ X, float* __restrict__ Y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float x0[5], plain[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x0[k] = S0[k * ld + i];
+  const float V = V1[i];
+  plain_gates(cc, V, x0, dt, plain);
+  X[i * NIN + 0] = V;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) X[i * NIN + 1 + k] = x0[k];
+  X[i * NIN + 6] = logf(S1[S_CI * ld + i]);
+  X[i * NIN + 7] = dt;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Y[i * NOUT + k] = S1[k * ld + i] - plain[k];
+}
+
+// ------------------------------------------------------------ FP32 training
+constexpr int TB = 128;    // samples (threads) per block
+constexpr int HP = NH + 1; // padded LDS row
+
+__global__ void __launch_bounds__(TB) closure_grad(int64_t B, const float* __restrict__ X,
+                                                    const float* __restrict__ Yn, const int64_t* __restrict__ idx,
+                                                    const float* __restrict__ P, float* __restrict__ partial,
+                                                    float* __restrict__ loss_part) {
+  __shared__ float W[NPARAM];
+  __shared__ float xs[TB][NIN + 1];
+  __shared__ float h1[TB][HP];
+  __shared__ float h2[TB][HP];
+  __shared__ float d3[TB][NOUT + 1];
+  __shared__ float red[TB];
+  const int t = threadIdx.x;
+  for (int p = t; p < NPARAM; p += TB) W[p] = P[p];
+  const int64_t s = (int64_t)blockIdx.x * TB + t;
+  const bool valid = s < B;
+  const int64_t row = valid ? idx[s] : 0;
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) xs[t][i] = valid ? X[row * NIN + i] : 0.0f;
+  __syncthreads();
+  // forward
+  for (int o = 0; o < NH; ++o) {
+    float a = W[OFF_B1 + o];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) a += xs[t][i] * W[OFF_W1 + i * NH + o];
+    h1[t][o] = tanhf(a);
+  }
+  for (int o = 0; o < NH; ++o) {
+    float a = W[OFF_B2 + o];
+    for (int i = 0; i < NH; ++i) a += h1[t][i] * W[OFF_W2 + i * NH + o];
+    h2[t][o] = tanhf(a);
+  }
+  float loss = 0.0f;
+  const float inv_b = 1.0f / (float)B;
+  for (int o = 0; o < NOUT; ++o) {
+    float a = W[OFF_B3 + o];
+    for (int i = 0; i < NH; ++i) a += h2[t][i] * W[OFF_W3 + i * NOUT + o];
+    const float d = valid ? a - Yn[row * NOUT + o] : 0.0f;
+    loss += d * d;
+    d3[t][o] = 2.0f * d * inv_b;
+  }
+  red[t] = loss * inv_b;
+  __syncthreads();
+  float* g = partial + (int64_t)blockIdx.x * NPARAM;
+  // layer 3 gradients
+  for (int p = t; p < NH * NOUT + NOUT; p += TB) {
+    float a = 0.0f;
+    if (p < NH * NOUT) {
+      const int i = p / NOUT, o = p % NOUT;
+      for (int k = 0; k < TB; ++k) a += h2[k][i] * d3[k][o];
+      g[OFF_W3 + p] = a;
+    } else {
+      const int o = p - NH * NOUT;
+      for (int k = 0; k < TB; ++k) a += d3[k][o];
+      g[OFF_B3 + o] = a;
+    }
+  }
+  if (t == 0) {
+    float a = 0.0f;
+    for (int k = 0; k < TB; ++k) a += red[k];
+    loss_part[blockIdx.x] = a;
+  }
+  __syncthreads();
+  // delta2 overwrites h2's row (element-wise in place)
+  for (int i = 0; i < NH; ++i) {
+    float a = 0.0f;
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) a += W[OFF_W3 + i * NOUT + o] * d3[t][o];
+    h2[t][i] = a * (1.0f - h2[t][i] * h2[t][i]);
+  }
+  __syncthreads();
+  // layer 2 gradients
+  for (int p = t; p < NH * NH + NH; p += TB) {
+    float a = 0.0f;
+    if (p < NH * NH) {
+      const int i = p / NH, o = p % NH;
+      for (int k = 0; k < TB; ++k) a += h1[k][i] * h2[k][o];
+      g[OFF_W2 + p] = a;
+    } else {
+      const int o = p - NH * NH;
+      for (int k = 0; k < TB; ++k) a += h2[k][o];
+      g[OFF_B2 + o] = a;
+    }
+  }
+  __syncthreads();
+  // delta1 overwrites h1's row (element-wise in place)
+  for (int i = 0; i < NH; ++i) {
+    float a = 0.0f;
+    for (int o = 0; o < NH; ++o) a += W[OFF_W2 + i * NH + o] * h2[t][o];
+    h1[t][i] = a * (1.0f - h1[t][i] * h1[t][i]);
+  }
+  __syncthreads();
+  // layer 1 gradients
+  for (int p = t; p < NIN * NH + NH; p += TB) {
+    float a = 0.0f;
+    if (p < NIN * NH) {
+      const int i = p / NH, o = p % NH;
+      for (int k = 0; k < TB; ++k) a += xs[k][i] * h1[k][o];
+      g[OFF_W1 + p] = a;
+    } else {
+      const int o = p - NIN * NH;
+      for (int k = 0; k < TB; ++k) a += h1[k][o];
+      g[OFF_B1 + o] = a;
+    }
+  }
+}
+
+__global__ void closure_reduce(int nblk, const float* __restrict__ partial, const float* __restrict__ loss_part,
+                               float* __restrict__ grad, float* __restrict__ loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < NPARAM) {
+    float a = 0.0f;
+    for (int b = 0; b < nblk; ++b) a += partial[(int64_t)b * NPARAM + p];
+    grad[p] = a;
+  } else if (p == NPARAM) {
+    float a = 0.0f;
+    for (int b = 0; b < nblk; ++b) a += loss_part[b];
+    loss[0] = a;
+  }
+}
+
+__global__ void closure_adam(float* __restrict__ P, float* __restrict__ m, float* __restrict__ v,
+                             const float* __restrict__ g, float lr, float b1, float b2, float eps, float c1,
+                             float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NPARAM) return;
+  const float gp = g[p];
+  const float mp = b1 * m[p] + (1.0f - b1) * gp;
+  const float vp = b2 * v[p] + (1.0f - b2) * gp * gp;
+  m[p] = mp;
+  v[p] = vp;
+  P[p] -= lr * (mp * c1) / (sqrtf(vp * c2) + eps);
+}
+
+// ----------------------------------------------------------- FP32 inference
+__global__ void closure_fwd32(int64_t n, const float* __restrict__ X, const float* __restrict__ P,
+                              const float* __restrict__ norm, float* __restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  float x[NIN], h1[NH], h2[NH];
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) x[i] = (X[s * NIN + i] - norm[i]) / norm[NIN + i];
+  for (int o = 0; o < NH; ++o) {
+    float a = P[OFF_B1 + o];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) a += x[i] * P[OFF_W1 + i * NH + o];
+    h1[o] = tanhf(a);
+  }
+  for (int o = 0; o < NH; ++o) {
+    float a = P[OFF_B2 + o];
+    for (int i = 0; i < NH; ++i) a += h1[i] * P[OFF_W2 + i * NH + o];
+    h2[o] = tanhf(a);
+  }
+  for (int o = 0; o < NOUT; ++o) {
+    float a = P[OFF_B3 + o];
+    for (int i = 0; i < NH; ++i) a += h2[i] * P[OFF_W3 + i * NOUT + o];
+    out[s * NOUT + o] = a * norm[2 * NIN + o];
+  }
+}
+
+// ----------------------------------------------------------- MFMA inference
+// v_mfma_f32_16x16x32_bf16, lane l = 16 g + n (g = l / 16, n = l % 16):
+//   A[i][k]: i = n, k = 8 g + j;  B[k][c]: k = 8 g + j, c = n;
+//   D[r'][c]: r' = 4 g + r (r < 4), c = n.
+// The layers are computed transposed, H^T = W^T X^T, so the 16 samples of a
+// tile are the MFMA columns: lane (g, n) ends a layer holding hidden
+// features 16 T + 4 g + r (T = output tile, r < 4) of sample n.  The next
+// layer's B operand for K chunk c takes, in lane group g, the 8 features
+// 16 (2c + j/4) + 4 g + j%4 that the lane already holds; the weights (A)
+// are gathered in the same K order, so no activation moves between lanes.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int hid_feature(int c, int g, int j) { return 16 * (2 * c + j / 4) + 4 * g + (j & 3); }
+
+struct MlpRegs {
+  bf16x8 a1[4];     // layer 1, output tiles T = 0..3 (K = 8 inputs + bias + padding)
+  bf16x8 a2[4][2];  // layer 2, output tile T, K chunk c
+  bf16x8 a3[2];     // layer 3 (5 outputs padded to 16), K chunk c
+  floatx4 b2[4];    // layer-2 bias in the accumulator layout
+  floatx4 b3;
+};
+
+__device__ __forceinline__ void load_mlp(MlpRegs& m, const float* __restrict__ P, int lane) {
+  const int g = lane >> 4, n = lane & 15;
+#pragma unroll
+  for (int T = 0; T < 4; ++T) {
+    const int o = 16 * T + n;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = 8 * g + j;
+      float w = 0.0f;
+      if (k < NIN) w = P[OFF_W1 + k * NH + o];
+      else if (k == NIN) w = P[OFF_B1 + o];
+      m.a1[T][j] = (__bf16)w;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) m.a2[T][c][j] = (__bf16)P[OFF_W2 + hid_feature(c, g, j) * NH + o];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) m.b2[T][r] = P[OFF_B2 + 16 * T + 4 * g + r];
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) m.a3[c][j] = n < NOUT ? (__bf16)P[OFF_W3 + hid_feature(c, g, j) * NOUT + n] : (__bf16)0.0f;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) m.b3[r] = (4 * g + r < NOUT) ? P[OFF_B3 + 4 * g + r] : 0.0f;
+}
+
+// One tile of 16 samples.  xin: in lane group 0, the normalised features of
+// sample n; returns D3 (lane (g, n) holds outputs 4 g + r of sample n).
+__device__ __forceinline__ floatx4 mlp_tile(const MlpRegs& m, int lane, const float* xin) {
+  const int g = lane >> 4;
+  bf16x8 b;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) b[j] = (__bf16)(g == 0 ? xin[j] : (g == 1 && j == 0 ? 1.0f : 0.0f));
+  floatx4 h[4];
+#pragma unroll
+  for (int T = 0; T < 4; ++T) {
+    floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m.a1[T], b, z, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[T][r] = fast_tanh(z[r]);
+  }
+  bf16x8 hb[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) hb[c][j] = (__bf16)h[2 * c + j / 4][j & 3];
+#pragma unroll
+  for (int T = 0; T < 4; ++T) {
+    floatx4 z = m.b2[T];
+    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m.a2[T][0], hb[0], z, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m.a2[T][1], hb[1], z, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[T][r] = fast_tanh(z[r]);
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) hb[c][j] = (__bf16)h[2 * c + j / 4][j & 3];
+  floatx4 y = m.b3;
+  y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m.a3[0], hb[0], y, 0, 0, 0);
+  y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m.a3[1], hb[1], y, 0, 0, 0);
+  return y;
+}
+
+// MFMA inference on precomputed raw features (test / measurement path).
+__global__ void __launch_bounds__(256) closure_fwd_mfma(int64_t n, const float* __restrict__ X,
+                                                        const float* __restrict__ P, const float* __restrict__ norm,
+                                                        float* __restrict__ out) {
+  MlpRegs m;
+  const int lane = threadIdx.x & 63;
+  load_mlp(m, P, lane);
+  const int g = lane >> 4, cn = lane & 15;
+  const int64_t n_tiles = (n + 15) / 16;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t tile = wave; tile < n_tiles; tile += n_waves) {
+    const int64_t s = tile * 16 + cn;
+    float x[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = (g == 0 && s < n) ? (X[s * NIN + i] - norm[i]) / norm[NIN + i] : 0.0f;
+    const floatx4 y = mlp_tile(m, lane, x);
+    if (s < n) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int o = 4 * g + r;
+        if (o < NOUT) out[s * NOUT + o] = y[r] * norm[2 * NIN + o];
+      }
+    }
+  }
+}
+
+// Coarse-step correction, fused with feature extraction: V1, S1 = the state
+// after a plain staggered step (gates already updated), S0 = the gates at
+// the step start (ld = row stride of both state arrays).  Adds the scaled
+// network output to the 5 gates of S1 and clips them to [0, 1].
+__global__ void __launch_bounds__(256) closure_apply(int64_t n, float dt, const float* __restrict__ V1,
+                                                     const float* __restrict__ S0, float* __restrict__ S1,
+                                                     int64_t ld0, int64_t ld1, const float* __restrict__ P,
+                                                     const float* __restrict__ norm) {
+  MlpRegs m;
+  const int lane = threadIdx.x & 63;
+  load_mlp(m, P, lane);
+  const int g = lane >> 4, cn = lane & 15;
+  const int64_t n_tiles = (n + 15) / 16;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  for (int64_t tile = wave; tile < n_tiles; tile += n_waves) {
+    const int64_t s = tile * 16 + cn;
+    const bool ok = s < n;
+    float x[8];
+    if (g == 0 && ok) {
+      float f[8];
+      f[0] = V1[s];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) f[1 + k] = S0[k * ld0 + s];
+      f[6] = logf(S1[S_CI * ld1 + s]);
+      f[7] = dt;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x[i] = (f[i] - norm[i]) / norm[NIN + i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x[i] = 0.0f;
+    }
+    const floatx4 y = mlp_tile(m, lane, x);
+    if (ok) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int o = 4 * g + r;
+        if (o < NOUT) {
+          float* q = &S1[o * ld1 + s];
+          const float v = *q + y[r] * norm[2 * NIN + o];
+          *q = fminf(fmaxf(v, 0.0f), 1.0f);
+        }
+      }
+    }
+  }
+}
+
+int grid_for(int64_t n, int per) {
+  int64_t b = (n + per - 1) / per;
+  return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
+}
+
+}  // namespace
+
+extern "C" {
+
+int bc_closure_data(const bc_closure_data_desc* d, void* stream) {
+  if (d == nullptr || d->chan_const == nullptr) return bc::fail(BC_INVALID_ARG, "desc / chan_const is NULL");
+  if (d->n < 0 || d->ld < d->n) return bc::fail(BC_INVALID_ARG, "need 0 <= n <= ld");
+  if (d->n == 0) return bc::ok();
+  if (!d->V1 || !d->S0 || !d->S1 || !d->X || !d->Y) return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
+  const int64_t nb = (d->n + 255) / 256;
+  hipLaunchKernelGGL(closure_data, dim3((unsigned)nb), dim3(256), 0, static_cast<hipStream_t>(stream), d->n,
+                     (float)d->dt, *static_cast<const bc_chan_const<float>*>(d->chan_const),
+                     static_cast<const float*>(d->V1), static_cast<const float*>(d->S0),
+                     static_cast<const float*>(d->S1), d->ld, static_cast<float*>(d->X), static_cast<float*>(d->Y));
+  BC_HIP(hipGetLastError());
+  return bc::ok();
+}
+
+int bc_closure_grad(const bc_closure_train_desc* d, void* stream) {
+  if (d == nullptr) return bc::fail(BC_INVALID_ARG, "desc is NULL");
+  if (d->batch <= 0) return bc::fail(BC_INVALID_ARG, "batch must be > 0");
+  if (!d->X || !d->Yn || !d->idx || !d->P || !d->partial || !d->loss_part || !d->grad || !d->loss)
+    return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
+  const int nblk = (int)((d->batch + TB - 1) / TB);
+  if (nblk > d->max_blocks) return bc::fail(BC_INVALID_ARG, "batch needs %d partial blocks > max_blocks %d", nblk,
+                                            d->max_blocks);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(closure_grad, dim3(nblk), dim3(TB), 0, st, d->batch, static_cast<const float*>(d->X),
+                     static_cast<const float*>(d->Yn), d->idx, static_cast<const float*>(d->P),
+                     static_cast<float*>(d->partial), static_cast<float*>(d->loss_part));
+  BC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(closure_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, st, nblk,
+                     static_cast<const float*>(d->partial), static_cast<const float*>(d->loss_part),
+                     static_cast<float*>(d->grad), static_cast<float*>(d->loss));
+  BC_HIP(hipGetLastError());
+  return bc::ok();
+}
+
+int bc_closure_adam(const bc_closure_adam_desc* d, void* stream) {
+  if (d == nullptr || !d->P || !d->m || !d->v || !d->grad) return bc::fail(BC_INVALID_ARG, "NULL argument");
+  if (d->step < 1) return bc::fail(BC_INVALID_ARG, "step counts from 1");
+  const double c1 = 1.0 / (1.0 - pow(d->beta1, (double)d->step));
+  const double c2 = 1.0 / (1.0 - pow(d->beta2, (double)d->step));
+  hipLaunchKernelGGL(closure_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     static_cast<float*>(d->P), static_cast<float*>(d->m), static_cast<float*>(d->v),
+                     static_cast<const float*>(d->grad), (float)d->lr, (float)d->beta1, (float)d->beta2,
+                     (float)d->eps, (float)c1, (float)c2);
+  BC_HIP(hipGetLastError());
+  return bc::ok();
+}
+
+int bc_closure_forward(const bc_closure_fwd_desc* d, void* stream) {
+  if (d == nullptr || !d->X || !d->P || !d->norm || !d->out) return bc::fail(BC_INVALID_ARG, "NULL argument");
+  if (d->n < 0) return bc::fail(BC_INVALID_ARG, "n < 0");
+  if (d->n == 0) return bc::ok();
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (d->mfma) {
+    hipLaunchKernelGGL(closure_fwd_mfma, dim3(grid_for((d->n + 15) / 16, 32)), dim3(256), 0, st, d->n,
+                       static_cast<const float*>(d->X), static_cast<const float*>(d->P),
+                       static_cast<const float*>(d->norm), static_cast<float*>(d->out));
+  } else {
+    hipLaunchKernelGGL(closure_fwd32, dim3(grid_for(d->n, 256)), dim3(256), 0, st, d->n,
+                       static_cast<const float*>(d->X), static_cast<const float*>(d->P),
+                       static_cast<const float*>(d->norm), static_cast<float*>(d->out));
+  }
+  BC_HIP(hipGetLastError());
+  return bc::ok();
+}
+
+int bc_closure_apply(const bc_closure_apply_desc* d, void* stream) {
+  if (d == nullptr || !d->V1 || !d->S0 || !d->S1 || !d->P || !d->norm)
+    return bc::fail(BC_INVALID_ARG, "NULL argument");
+  if (d->n < 0 || d->ld0 < d->n || d->ld1 < d->n) return bc::fail(BC_INVALID_ARG, "need 0 <= n <= ld0, ld1");
+  if (d->n == 0) return bc::ok();
+  hipLaunchKernelGGL(closure_apply, dim3(grid_for((d->n + 15) / 16, 32)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), d->n, (float)d->dt, static_cast<const float*>(d->V1),
+                     static_cast<const float*>(d->S0), static_cast<float*>(d->S1), d->ld0, d->ld1,
+                     static_cast<const float*>(d->P), static_cast<const float*>(d->norm));
+  BC_HIP(hipGetLastError());
+  return bc::ok();
+}
+
+}  // extern "C"

@@ -209,6 +209,91 @@ int bc_point_supported(int precision);
  * braincell/quad/_exp_euler.py:217-320) for n_steps steps. */
 int bc_point_run(const bc_point_desc* desc, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Learned gating closure (no counterpart in the reference): an MLP
+ * 8 -> 64 -> 64 -> 5 (tanh) that corrects the exponential-Euler update of
+ * the 5 gates of the Na/KDR/IL/CaL/pool model at large dt.
+ * Features: (V_{n+1}, Na p, Na q, KDR p, CaL p, CaL q at step start,
+ * ln Ci_{n+1}, dt); parameters: W1[8][64] b1[64] W2[64][64] b2[64] W3[64][5]
+ * b3[5] (FP32, BC_CLOSURE_NPARAM values); norm = mu[8], sd[8],
+ * out_scale[5] (features are (f - mu) / sd, corrections out * out_scale).
+ * All arrays FP32 device memory. */
+#define BC_CLOSURE_NPARAM 5061
+
+/* Training pairs from a fine trajectory: for n compartments, V1 = V and
+ * S1 = states at t + dt, S0 = states at t (state arrays [6][ld]: Na p, Na q,
+ * KDR p, CaL p, CaL q, Ci).  X[n][8] = raw features, Y[n][5] = S1 gates minus
+ * the plain update of S0's gates at V1. */
+typedef struct bc_closure_data_desc {
+  int64_t n, ld;
+  double dt;
+  const void* chan_const; /* host pointer to bc_chan_const<float> */
+  const void* V1;
+  const void* S0;
+  const void* S1;
+  void* X;
+  void* Y;
+} bc_closure_data_desc;
+int bc_closure_data(const bc_closure_data_desc* desc, void* stream);
+
+/* Gradient of the minibatch loss mean_b sum_o (net(X[idx[b]])_o - Yn[idx[b]][o])^2
+ * (X normalised, Yn = targets / out_scale) with respect to P: FP32 forward
+ * and backward, per-128-sample partial sums reduced in fixed order (the
+ * result is deterministic).  partial: [max_blocks][BC_CLOSURE_NPARAM],
+ * loss_part: [max_blocks], grad: [BC_CLOSURE_NPARAM], loss: [1]. */
+typedef struct bc_closure_train_desc {
+  int64_t batch;
+  int32_t max_blocks;
+  const void* X;
+  const void* Yn;
+  const int64_t* idx;
+  const void* P;
+  void* partial;
+  void* loss_part;
+  void* grad;
+  void* loss;
+} bc_closure_train_desc;
+int bc_closure_grad(const bc_closure_train_desc* desc, void* stream);
+
+/* One Adam step (bias-corrected, step counts from 1). */
+typedef struct bc_closure_adam_desc {
+  int64_t step;
+  double lr, beta1, beta2, eps;
+  void* P;
+  void* m;
+  void* v;
+  const void* grad;
+} bc_closure_adam_desc;
+int bc_closure_adam(const bc_closure_adam_desc* desc, void* stream);
+
+/* Network corrections out[n][5] = net((X - mu) / sd) * out_scale from raw
+ * features X[n][8]: mfma = 0 FP32, mfma = 1 BF16 MFMA with FP32 accumulation. */
+typedef struct bc_closure_fwd_desc {
+  int64_t n;
+  int32_t mfma;
+  const void* X;
+  const void* P;
+  const void* norm;
+  void* out;
+} bc_closure_fwd_desc;
+int bc_closure_forward(const bc_closure_fwd_desc* desc, void* stream);
+
+/* Applies the closure after a plain staggered step of length dt: V1, S1 =
+ * the state after the step (gates already updated, row stride ld1), S0 =
+ * the state at the step start (row stride ld0; e.g. the step-0 snapshot of
+ * a recording run).  S1's 5 gates += BF16-MFMA network output * out_scale,
+ * clipped to [0, 1]. */
+typedef struct bc_closure_apply_desc {
+  int64_t n, ld0, ld1;
+  double dt;
+  const void* V1;
+  const void* S0;
+  void* S1;
+  const void* P;
+  const void* norm;
+} bc_closure_apply_desc;
+int bc_closure_apply(const bc_closure_apply_desc* desc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,4 +27,7 @@ Parity status per module:
   pinned (no reference trajectory exists to compare against).
 - ``single``: pinned (leak RC closed form, dV/dt = I/C known answer,
   exp-Euler on linear decay = exp(-0.01)).
+- ``closure``: the MLP gradient is pinned by central finite differences; the
+  gate update by the channel pins above.  No reference counterpart exists
+  (the closure is this project's addition).
 """

@@ -1,0 +1,114 @@
+"""GPU checks of the learned-closure kernels against the FP64 oracle."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import closure as oc
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(seed=0, gain=1.0):
+    from braincell_amd.closure import ClosureNet
+
+    P = ClosureNet.init_params(seed, out_gain=gain)
+    rng = np.random.default_rng(seed + 1)
+    norm = np.concatenate([rng.uniform(-1, 1, 8), rng.uniform(0.5, 2.0, 8), rng.uniform(0.01, 0.1, 5)])
+    return ClosureNet(P, norm.astype(np.float32)), P.astype(np.float64), norm
+
+
+def test_gradient_kernel_matches_oracle(cuda):
+    import torch
+    from braincell_amd.closure import NPARAM, _AdamDesc, _ptr, _stream, _TrainDesc
+    from braincell_amd._native import check, lib
+
+    rng = np.random.default_rng(3)
+    n, batch = 3000, 1000
+    Xn = rng.standard_normal((n, 8)).astype(np.float32)
+    Yn = rng.standard_normal((n, 5)).astype(np.float32)
+    idx = rng.choice(n, batch, replace=False).astype(np.int64)
+    net, P, _ = _net(0, gain=1.0)
+    tX, tY, ti = (torch.as_tensor(a, device=cuda) for a in (Xn, Yn, idx))
+    nb = (batch + 127) // 128
+    partial = torch.empty((nb, NPARAM), device=cuda)
+    lp = torch.empty(nb, device=cuda)
+    grad = torch.empty(NPARAM, device=cuda)
+    loss = torch.empty(1, device=cuda)
+    d = _TrainDesc(batch, nb, _ptr(tX), _ptr(tY), _ptr(ti), _ptr(net.P), _ptr(partial), _ptr(lp), _ptr(grad),
+                   _ptr(loss))
+    check(lib().bc_closure_grad(ctypes.byref(d), _stream()))
+    torch.cuda.synchronize()
+    ref_loss, ref_g = oc.loss_and_grad(net.P.double().cpu().numpy(), Xn[idx].astype(np.float64),
+                                       Yn[idx].astype(np.float64))
+    g = grad.double().cpu().numpy()
+    assert abs(loss.item() - ref_loss) <= 1e-5 * abs(ref_loss)
+    assert np.linalg.norm(g - ref_g) <= 1e-4 * np.linalg.norm(ref_g)
+    # the kernel's partial sums are reduced in a fixed order: bitwise repeatable
+    g1 = grad.clone()
+    check(lib().bc_closure_grad(ctypes.byref(d), _stream()))
+    torch.cuda.synchronize()
+    assert torch.equal(g1, grad)
+    # one Adam step against the textbook update
+    m = torch.zeros(NPARAM, device=cuda)
+    v = torch.zeros(NPARAM, device=cuda)
+    P0 = net.P.double().cpu().numpy()
+    a = _AdamDesc(1, 1e-3, 0.9, 0.999, 1e-8, _ptr(net.P), _ptr(m), _ptr(v), _ptr(grad))
+    check(lib().bc_closure_adam(ctypes.byref(a), _stream()))
+    torch.cuda.synchronize()
+    gg = grad.double().cpu().numpy()
+    mh, vh = 0.1 * gg / 0.1, 0.001 * gg * gg / 0.001
+    ref = P0 - 1e-3 * mh / (np.sqrt(vh) + 1e-8)
+    assert np.allclose(net.P.double().cpu().numpy(), ref, rtol=1e-5, atol=1e-7)
+
+
+def test_fp32_and_mfma_inference_match_oracle(cuda):
+    import torch
+
+    net, _, norm = _net(4, gain=1.0)
+    rng = np.random.default_rng(5)
+    n = 4099  # not a multiple of the 16-sample MFMA tile
+    X = np.column_stack([rng.uniform(-80, 30, n), rng.uniform(0, 1, (n, 5)), np.log(rng.uniform(1e-4, 1e-2, n)),
+                         np.full(n, 0.1)]).astype(np.float32)
+    tX = torch.as_tensor(X, device=cuda)
+    f32 = net.forward(tX, mfma=False).double().cpu().numpy()
+    mf = net.forward(tX, mfma=True).double().cpu().numpy()
+    Xn = (X.astype(np.float64) - norm[:8]) / norm[8:16]
+    ref = oc.forward(net.P.double().cpu().numpy(), Xn)[0] * norm[16:]
+    scale = np.abs(ref).max()
+    assert np.abs(f32 - ref).max() <= 1e-5 * scale
+    # BF16 operands (8-bit mantissa) through three layers, FP32 accumulation
+    assert np.abs(mf - ref).max() <= 3e-2 * scale
+    assert np.sqrt(np.mean((mf - ref) ** 2)) <= 6e-3 * scale
+
+
+def test_training_targets_match_oracle(cuda):
+    import torch
+    from braincell_amd.closure import _DataDesc, _ptr, _stream
+    from braincell_amd.runtime import StaggerLayout, StaggerRuntime
+    from braincell_amd import workloads as W
+    from braincell_amd._native import check, lib
+
+    L = StaggerLayout([W.uneven_tree_cell(0, cmin=200, cmax=260)])
+    rt = StaggerRuntime(L, 0.1, precision=32)
+    rng = np.random.default_rng(6)
+    n = 1000
+    V1 = rng.uniform(-90, 40, n)
+    S0 = np.vstack([rng.uniform(0, 1, (5, n)), rng.uniform(1e-4, 1e-2, (1, n))])
+    S1 = np.vstack([rng.uniform(0, 1, (5, n)), rng.uniform(1e-4, 1e-2, (1, n))])
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=cuda)
+    tV, tS0, tS1 = t(V1), t(S0), t(S1)
+    X = torch.empty((n, 8), device=cuda)
+    Y = torch.empty((n, 5), device=cuda)
+    d = _DataDesc(n, n, 0.1, ctypes.cast(ctypes.pointer(rt._cc), ctypes.c_void_p), _ptr(tV), _ptr(tS0), _ptr(tS1),
+                  _ptr(X), _ptr(Y))
+    check(lib().bc_closure_data(ctypes.byref(d), _stream()))
+    torch.cuda.synchronize()
+    V1f, S0f, S1f = (a.double().cpu().numpy() for a in (tV, tS0, tS1))
+    refX = oc.features(V1f, S0f[:5], S1f[5], 0.1)
+    cc = rt._cc
+    refY = (S1f[:5] - oc.plain_gates(V1f, S0f[:5], 0.1, cc.na_vsh, cc.kdr_vsh, cc.cal_vsh, cc.na_phi, cc.kdr_phi,
+                                     cc.cal_phi_p, cc.cal_phi_q)).T
+    assert np.allclose(X.double().cpu().numpy(), refX, rtol=1e-6, atol=1e-6)
+    assert np.abs(Y.double().cpu().numpy() - refY).max() <= 2e-6
