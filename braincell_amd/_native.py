@@ -44,7 +44,7 @@ class StaggerDesc(ctypes.Structure):
         ("probe_col", ctypes.c_void_p), ("chan_const", ctypes.c_void_p), ("V", ctypes.c_void_p), ("Vbnd", ctypes.c_void_p),
         ("state", ctypes.c_void_p), ("trace", ctypes.c_void_p), ("trace_ld", ctypes.c_int64),
         ("rec", ctypes.c_void_p), ("status", ctypes.c_void_p), ("step0", ctypes.c_int32), ("n_steps", ctypes.c_int32),
-        ("dt", ctypes.c_double),
+        ("dt", ctypes.c_double), ("flags", ctypes.c_uint32),
     ]
 
 

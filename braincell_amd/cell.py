@@ -172,6 +172,8 @@ class Cell:
         if rt is None or rt.dt != float(dt):
             old = rt
             rt = StaggerRuntime(self._layout, dt, precision=self.precision, device=self.device)
+            if self.ion_channel_update_order == "integration":
+                rt.flags |= 2  # BC_STEP_INTEGRATION_ORDER
             if old is not None:  # keep the current state across a dt change
                 rt.V.copy_(old.V)
                 rt.S.copy_(old.S)

@@ -2,7 +2,9 @@
 
 A per-compartment MLP 8 -> 64 -> 64 -> 5 (tanh) corrects the exponential-
 Euler update of the five gates (Na p, q; KDR p; CaL p, q) of the
-Na/KDR/IL/CaL/CalciumDetailed model.  Everything numeric runs in this
+Na/KDR/IL/CaL/CalciumDetailed model: with delta the plain increment of a
+gate at V_{n+1} and c the network output, the gate moves by delta (1 + c),
+so a cell at rest (delta = 0) cannot drift whatever the network says.  Everything numeric runs in this
 package's kernels (``bc_closure.hip``):
 
 * training pairs come from plain fine-dt trajectories of the staggered
@@ -42,12 +44,12 @@ CHSET = 0x117  # Na | KDR | IL | CaL | CalciumDetailed
 class _DataDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("ld", ctypes.c_int64), ("dt", ctypes.c_double), ("chan_const", ctypes.c_void_p),
                 ("V1", ctypes.c_void_p), ("S0", ctypes.c_void_p), ("S1", ctypes.c_void_p), ("X", ctypes.c_void_p),
-                ("Y", ctypes.c_void_p)]
+                ("D", ctypes.c_void_p), ("Y", ctypes.c_void_p)]
 
 
 class _TrainDesc(ctypes.Structure):
     _fields_ = [("batch", ctypes.c_int64), ("max_blocks", ctypes.c_int32), ("X", ctypes.c_void_p),
-                ("Yn", ctypes.c_void_p), ("idx", ctypes.c_void_p), ("P", ctypes.c_void_p), ("partial", ctypes.c_void_p),
+                ("Dn", ctypes.c_void_p), ("Yn", ctypes.c_void_p), ("idx", ctypes.c_void_p), ("P", ctypes.c_void_p), ("partial", ctypes.c_void_p),
                 ("loss_part", ctypes.c_void_p), ("grad", ctypes.c_void_p), ("loss", ctypes.c_void_p)]
 
 
@@ -63,9 +65,11 @@ class _FwdDesc(ctypes.Structure):
 
 
 class _ApplyDesc(ctypes.Structure):
-    _fields_ = [("n", ctypes.c_int64), ("ld0", ctypes.c_int64), ("ld1", ctypes.c_int64), ("dt", ctypes.c_double),
-                ("V1", ctypes.c_void_p), ("S0", ctypes.c_void_p), ("S1", ctypes.c_void_p), ("P", ctypes.c_void_p),
-                ("norm", ctypes.c_void_p)]
+    _fields_ = [("n", ctypes.c_int64), ("ld", ctypes.c_int64), ("dt", ctypes.c_double), ("chan_const", ctypes.c_void_p),
+                ("V1", ctypes.c_void_p), ("S", ctypes.c_void_p), ("P", ctypes.c_void_p), ("norm", ctypes.c_void_p)]
+
+
+SKIP_GATES = 1  # BC_STEP_SKIP_GATES
 
 
 def _stream():
@@ -109,7 +113,7 @@ class ClosureNet:
         return P.astype(np.float32)
 
     def forward(self, X, mfma: bool = True):
-        """Corrections [n, 5] for raw features X [n, 8] (FP32 device)."""
+        """Network outputs c [n, 5] for raw features X [n, 8] (FP32 device)."""
         import torch
 
         X = X.contiguous()
@@ -131,7 +135,8 @@ class ClosureNet:
 @dataclasses.dataclass
 class ClosureData:
     X: object  # [n, 8] raw features (FP32 device)
-    Y: object  # [n, 5] target corrections (FP32 device)
+    D: object  # [n, 5] plain exp-Euler increments of the gates
+    Y: object  # [n, 5] fine end state minus the plain update
 
     @property
     def n(self):
@@ -168,32 +173,37 @@ def generate_data(models, *, dt_fine: float = 0.005, dt_coarse: float = 0.1, n_p
         rt.run(w, trace=tr[:w])
     ncv = L.n_cv_total
     X = torch.empty((n_pairs * ncv, NIN), dtype=torch.float32, device=rt.device)
+    D = torch.empty((n_pairs * ncv, NOUT), dtype=torch.float32, device=rt.device)
     Y = torch.empty((n_pairs * ncv, NOUT), dtype=torch.float32, device=rt.device)
     for i in range(n_pairs):
         S0 = rt.S.clone()
         rt.run(k, trace=tr[:k])
         d = _DataDesc(ncv, ncv, float(dt_coarse), ctypes.cast(ctypes.pointer(rt._cc), ctypes.c_void_p), _ptr(rt.V),
                       _ptr(S0), _ptr(rt.S), ctypes.c_void_p(X.data_ptr() + i * ncv * NIN * 4),
+                      ctypes.c_void_p(D.data_ptr() + i * ncv * NOUT * 4),
                       ctypes.c_void_p(Y.data_ptr() + i * ncv * NOUT * 4))
         check(lib().bc_closure_data(ctypes.byref(d), _stream()))
         if gap:
             rt.run(gap, trace=tr[:gap])
     torch.cuda.synchronize()
     rt.check_status()
-    return ClosureData(X, Y)
+    return ClosureData(X, D, Y)
 
 
 # ---------------------------------------------------------------- training
 def train(data: ClosureData, *, epochs: int = 4, batch: int = 8192, lr: float = 2e-3, seed: int = 0,
           net: ClosureNet = None, log=None) -> ClosureNet:
-    """Adam on the normalised one-step target (MSE), all in this package's
-    kernels; the learning rate decays linearly to 10 % over the run."""
+    """Adam on the one-step loss sum_o (delta_o c_o - y_o)^2 (delta and y
+    divided by the per-output scale of y), all in this package's kernels;
+    the learning rate decays linearly to 10 % over the run."""
     import torch
 
     norm = data.stats().to(data.X.device)
     mu, sd, sc = norm[:NIN], norm[NIN:2 * NIN], norm[2 * NIN:]
     Xn = ((data.X - mu) / sd).contiguous()
     Yn = (data.Y / sc).contiguous()
+    Dn = (data.D / sc).contiguous()
+    norm[2 * NIN:] = 1.0  # c is dimensionless: the output scale stays 1
     if net is None:
         net = ClosureNet(ClosureNet.init_params(seed), norm.cpu().numpy(), device=data.X.device)
     else:
@@ -218,7 +228,7 @@ def train(data: ClosureData, *, epochs: int = 4, batch: int = 8192, lr: float = 
         ep_loss = torch.zeros((), dtype=torch.float64, device=P.device)
         for b in range(steps_per_epoch):
             idx = perm[b * batch:(b + 1) * batch].contiguous()
-            d = _TrainDesc(int(idx.numel()), max_blocks, _ptr(Xn), _ptr(Yn), _ptr(idx), _ptr(P), _ptr(partial),
+            d = _TrainDesc(int(idx.numel()), max_blocks, _ptr(Xn), _ptr(Dn), _ptr(Yn), _ptr(idx), _ptr(P), _ptr(partial),
                            _ptr(loss_part), _ptr(grad), _ptr(loss))
             check(lib().bc_closure_grad(ctypes.byref(d), _stream()))
             step += 1
@@ -237,28 +247,24 @@ def one_step_error(net: ClosureNet, data: ClosureData, mfma: bool = True):
     """Mean squared gate-update error per output without and with the
     closure (teacher-forced, on ``data``)."""
     base = (data.Y.double() ** 2).mean(0)
-    corr = net.forward(data.X, mfma=mfma).double()
+    corr = data.D.double() * net.forward(data.X, mfma=mfma).double()
     return base.cpu().numpy(), ((data.Y.double() - corr) ** 2).mean(0).cpu().numpy()
 
 
 # ----------------------------------------------------------------- runtime
 class ClosureStepper:
     """Coarse staggered steps with the closure: each step is a one-step
-    recording launch of the staggered kernel (its step-0 snapshot is the
-    gate state the features need) followed by ``bc_closure_apply``."""
+    launch of the staggered kernel that updates V and the Ca pool but skips
+    the gates (``BC_STEP_SKIP_GATES``), then ``bc_closure_apply`` performs
+    the gate update (plain exp-Euler + network correction) in place."""
 
     def __init__(self, models, dt: float, net: ClosureNet, *, threads: int = 256, layout=None):
-        import torch
-
         self.L = layout or StaggerLayout(models, threads=threads, probes="soma")
         if self.L.chset != CHSET:
             raise ValueError("the closure applies to the Na/KDR/IL/CaL/CalciumDetailed model")
         self.rt = StaggerRuntime(self.L, dt, precision=32)
         self.net = net
         self.dt = float(dt)
-        ncv = self.L.n_cv_total
-        self.ns = len(self.L.slots)
-        self.rec = torch.empty(2 * (self.ns + 1) * ncv, dtype=torch.float32, device=self.rt.device)
 
     def reset_state(self):
         self.rt.reset_state()
@@ -268,13 +274,18 @@ class ClosureStepper:
         ncv = self.L.n_cv_total
         if trace is None:
             trace = rt.new_trace(n_steps)
-        for i in range(n_steps):
-            rt.run(1, trace=trace[i:i + 1], rec=self.rec if closure else None)
-            if closure:
-                S0 = ctypes.c_void_p(self.rec.data_ptr() + ncv * 4)  # snapshot slot 1 = first state
-                d = _ApplyDesc(ncv, ncv, ncv, self.dt, _ptr(rt.V), S0, _ptr(rt.S), _ptr(self.net.P),
-                               _ptr(self.net.norm))
+        if not closure:
+            rt.flags = 0
+            return rt.run(n_steps, trace=trace, chunk=200)
+        cc = ctypes.cast(ctypes.pointer(rt._cc), ctypes.c_void_p)
+        d = _ApplyDesc(ncv, ncv, self.dt, cc, _ptr(rt.V), _ptr(rt.S), _ptr(self.net.P), _ptr(self.net.norm))
+        rt.flags = SKIP_GATES
+        try:
+            for i in range(n_steps):
+                rt.run(1, trace=trace[i:i + 1])
                 check(lib().bc_closure_apply(ctypes.byref(d), _stream()))
+        finally:
+            rt.flags = 0
         return trace
 
 

@@ -4,22 +4,27 @@
 //
 // Features of one compartment at a coarse step n -> n+1 of length dt:
 //   f = (V_{n+1}, Na p_n, Na q_n, KDR p_n, CaL p_n, CaL q_n, ln Ci_{n+1}, dt),
-// normalised as (f - mu) / sd.  The network's 5 outputs, times per-output
-// scales, are added to the plain update of the 5 gates (the staggered step's
-// own x_inf + (x - x_inf) e^{dt lam} at V_{n+1}).
+// normalised as (f - mu) / sd.  The network's 5 outputs c (times per-output
+// scales) rescale the plain exponential-Euler increments of the 5 gates,
+// delta = x_inf + (x - x_inf) e^{dt lam} - x at V_{n+1} (the staggered step's
+// own update): x_{n+1} = x_n + delta (1 + c).  At a fixed point delta = 0,
+// so the closure cannot drift a resting cell.
 //
 // Parameters (FP32 master copy, 5061 values): W1[8][64] b1[64] W2[64][64]
 // b2[64] W3[64][5] b3[5], W[i][o] = weight from input i to output o.
 //
 // Kernels:
-//   closure_data      training pairs from fine trajectories (features, target
-//                     increments minus the plain update)
-//   closure_grad      FP32 forward + backward of the MSE loss on a minibatch,
-//                     per-block partial gradients (deterministic)
+//   closure_data      training triples from fine trajectories (features, the
+//                     plain increment delta, the fine end state minus the
+//                     plain update)
+//   closure_grad      FP32 forward + backward of the delta-weighted MSE loss
+//                     sum_o (delta_o c_o - y_o)^2 on a minibatch, per-block
+//                     partial gradients (deterministic)
 //   closure_reduce    sums the partials; closure_adam: Adam step
 //   closure_fwd32     FP32 inference (reference for the MFMA path)
 //   closure_apply     BF16 MFMA inference with FP32 accumulation, fused with
-//                     feature extraction and the correction of the state
+//                     feature extraction and the plain gate update, in place
+//                     after a staggered step that skipped its gates
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -65,7 +70,7 @@ __device__ __forceinline__ void plain_gates(const bc_chan_const<float>& cc, floa
 // ------------------------------------------------------------ training data
 __global__ void closure_data(int64_t n, float dt, bc_chan_const<float> cc, const float* __restrict__ V1,
                              const float* __restrict__ S0, const float* __restrict__ S1, int64_t ld,
-                             float* __restrict__ X, float* __restrict__ Y) {
+                             float* __restrict__ X, float* __restrict__ D, float* __restrict__ Y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float x0[5], plain[5];
@@ -79,7 +84,10 @@ __global__ void closure_data(int64_t n, float dt, bc_chan_const<float> cc, const
   X[i * NIN + 6] = logf(S1[S_CI * ld + i]);
   X[i * NIN + 7] = dt;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) Y[i * NOUT + k] = S1[k * ld + i] - plain[k];
+  for (int k = 0; k < 5; ++k) {
+    D[i * NOUT + k] = plain[k] - x0[k];
+    Y[i * NOUT + k] = S1[k * ld + i] - plain[k];
+  }
 }
 
 // ------------------------------------------------------------ FP32 training
@@ -87,7 +95,8 @@ constexpr int TB = 128;    // samples (threads) per block
 constexpr int HP = NH + 1; // padded LDS row
 
 __global__ void __launch_bounds__(TB) closure_grad(int64_t B, const float* __restrict__ X,
-                                                    const float* __restrict__ Yn, const int64_t* __restrict__ idx,
+                                                    const float* __restrict__ Dn, const float* __restrict__ Yn,
+                                                    const int64_t* __restrict__ idx,
                                                     const float* __restrict__ P, float* __restrict__ partial,
                                                     float* __restrict__ loss_part) {
   __shared__ float W[NPARAM];
@@ -121,9 +130,10 @@ __global__ void __launch_bounds__(TB) closure_grad(int64_t B, const float* __res
   for (int o = 0; o < NOUT; ++o) {
     float a = W[OFF_B3 + o];
     for (int i = 0; i < NH; ++i) a += h2[t][i] * W[OFF_W3 + i * NOUT + o];
-    const float d = valid ? a - Yn[row * NOUT + o] : 0.0f;
+    const float w = valid ? Dn[row * NOUT + o] : 0.0f;
+    const float d = valid ? w * a - Yn[row * NOUT + o] : 0.0f;
     loss += d * d;
-    d3[t][o] = 2.0f * d * inv_b;
+    d3[t][o] = 2.0f * d * w * inv_b;
   }
   red[t] = loss * inv_b;
   __syncthreads();
@@ -267,7 +277,7 @@ struct MlpRegs {
   floatx4 b3;
 };
 
-__device__ __forceinline__ void load_mlp(MlpRegs& m, const float* __restrict__ P, int lane) {
+__device__ __forceinline__ void load_mlp(MlpRegs& m, const float* P, int lane) {
   const int g = lane >> 4, n = lane & 15;
 #pragma unroll
   for (int T = 0; T < 4; ++T) {
@@ -334,12 +344,17 @@ __device__ __forceinline__ floatx4 mlp_tile(const MlpRegs& m, int lane, const fl
 }
 
 // MFMA inference on precomputed raw features (test / measurement path).
+// The weights are staged once per workgroup in LDS and gathered from there
+// into each wave's operand registers.
 __global__ void __launch_bounds__(256) closure_fwd_mfma(int64_t n, const float* __restrict__ X,
                                                         const float* __restrict__ P, const float* __restrict__ norm,
                                                         float* __restrict__ out) {
+  __shared__ float Ps[NPARAM];
+  for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) Ps[p] = P[p];
+  __syncthreads();
   MlpRegs m;
   const int lane = threadIdx.x & 63;
-  load_mlp(m, P, lane);
+  load_mlp(m, Ps, lane);
   const int g = lane >> 4, cn = lane & 15;
   const int64_t n_tiles = (n + 15) / 16;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -360,49 +375,77 @@ __global__ void __launch_bounds__(256) closure_fwd_mfma(int64_t n, const float* 
   }
 }
 
-// Coarse-step correction, fused with feature extraction: V1, S1 = the state
-// after a plain staggered step (gates already updated), S0 = the gates at
-// the step start (ld = row stride of both state arrays).  Adds the scaled
-// network output to the 5 gates of S1 and clips them to [0, 1].
-__global__ void __launch_bounds__(256) closure_apply(int64_t n, float dt, const float* __restrict__ V1,
-                                                     const float* __restrict__ S0, float* __restrict__ S1,
-                                                     int64_t ld0, int64_t ld1, const float* __restrict__ P,
-                                                     const float* __restrict__ norm) {
+// Closure gate update, in place, after a staggered step run with
+// BC_STEP_SKIP_GATES: S holds the step-start gates and the updated Ci, V1
+// the new voltage.  A wave takes 64 compartments at a time (four MFMA
+// tiles): every lane extracts the features and the plain update of its own
+// compartment (coalesced loads, no idle lanes), the features move to the
+// operand lanes of their tile by shuffles, the four tiles run the network,
+// and the outputs shuffle back.  The gate becomes clip(x + delta (1 + c),
+// 0, 1) with delta the plain increment.
+__global__ void __launch_bounds__(256) closure_apply(int64_t n, float dt, bc_chan_const<float> cc,
+                                                     const float* __restrict__ V1, float* __restrict__ S, int64_t ld,
+                                                     const float* __restrict__ P, const float* __restrict__ norm) {
+  __shared__ float Ps[NPARAM];
+  for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) Ps[p] = P[p];
+  __syncthreads();
   MlpRegs m;
   const int lane = threadIdx.x & 63;
-  load_mlp(m, P, lane);
+  load_mlp(m, Ps, lane);
   const int g = lane >> 4, cn = lane & 15;
-  const int64_t n_tiles = (n + 15) / 16;
+  float mu[8], isd[8], sc[NOUT];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    mu[i] = norm[i];
+    isd[i] = 1.0f / norm[NIN + i];
+  }
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) sc[o] = norm[2 * NIN + o];
+  const int64_t n_quads = (n + 63) / 64;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  for (int64_t tile = wave; tile < n_tiles; tile += n_waves) {
-    const int64_t s = tile * 16 + cn;
+  for (int64_t quad = wave; quad < n_quads; quad += n_waves) {
+    const int64_t s = quad * 64 + lane;
     const bool ok = s < n;
-    float x[8];
-    if (g == 0 && ok) {
-      float f[8];
-      f[0] = V1[s];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) f[1 + k] = S0[k * ld0 + s];
-      f[6] = logf(S1[S_CI * ld1 + s]);
-      f[7] = dt;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) x[i] = (f[i] - norm[i]) / norm[NIN + i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < 8; ++i) x[i] = 0.0f;
-    }
-    const floatx4 y = mlp_tile(m, lane, x);
+    float x0[NOUT], plain[NOUT], f[8];
     if (ok) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int o = 4 * g + r;
-        if (o < NOUT) {
-          float* q = &S1[o * ld1 + s];
-          const float v = *q + y[r] * norm[2 * NIN + o];
-          *q = fminf(fmaxf(v, 0.0f), 1.0f);
-        }
+      for (int k = 0; k < NOUT; ++k) x0[k] = S[k * ld + s];
+      const float V = V1[s];
+      plain_gates(cc, V, x0, dt, plain);
+      f[0] = V;
+#pragma unroll
+      for (int k = 0; k < NOUT; ++k) f[1 + k] = x0[k];
+      f[6] = __logf(S[S_CI * ld + s]);
+      f[7] = dt;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) f[i] = (f[i] - mu[i]) * isd[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) f[i] = 0.0f;
+#pragma unroll
+      for (int k = 0; k < NOUT; ++k) x0[k] = plain[k] = 0.0f;
+    }
+    float c[NOUT];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float xq[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) xq[i] = __shfl(f[i], 16 * q + cn);
+      const floatx4 y = mlp_tile(m, lane, xq);
+      float t[NOUT];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) t[k] = __shfl(y[k], cn);
+      t[4] = __shfl(y[0], 16 + cn);
+      if (g == q) {
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k) c[k] = t[k];
       }
+    }
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < NOUT; ++k)
+        S[k * ld + s] = fminf(fmaxf(plain[k] + (plain[k] - x0[k]) * (c[k] * sc[k]), 0.0f), 1.0f);
     }
   }
 }
@@ -420,12 +463,14 @@ int bc_closure_data(const bc_closure_data_desc* d, void* stream) {
   if (d == nullptr || d->chan_const == nullptr) return bc::fail(BC_INVALID_ARG, "desc / chan_const is NULL");
   if (d->n < 0 || d->ld < d->n) return bc::fail(BC_INVALID_ARG, "need 0 <= n <= ld");
   if (d->n == 0) return bc::ok();
-  if (!d->V1 || !d->S0 || !d->S1 || !d->X || !d->Y) return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
+  if (!d->V1 || !d->S0 || !d->S1 || !d->X || !d->D || !d->Y)
+    return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
   const int64_t nb = (d->n + 255) / 256;
   hipLaunchKernelGGL(closure_data, dim3((unsigned)nb), dim3(256), 0, static_cast<hipStream_t>(stream), d->n,
                      (float)d->dt, *static_cast<const bc_chan_const<float>*>(d->chan_const),
                      static_cast<const float*>(d->V1), static_cast<const float*>(d->S0),
-                     static_cast<const float*>(d->S1), d->ld, static_cast<float*>(d->X), static_cast<float*>(d->Y));
+                     static_cast<const float*>(d->S1), d->ld, static_cast<float*>(d->X), static_cast<float*>(d->D),
+                     static_cast<float*>(d->Y));
   BC_HIP(hipGetLastError());
   return bc::ok();
 }
@@ -433,14 +478,14 @@ int bc_closure_data(const bc_closure_data_desc* d, void* stream) {
 int bc_closure_grad(const bc_closure_train_desc* d, void* stream) {
   if (d == nullptr) return bc::fail(BC_INVALID_ARG, "desc is NULL");
   if (d->batch <= 0) return bc::fail(BC_INVALID_ARG, "batch must be > 0");
-  if (!d->X || !d->Yn || !d->idx || !d->P || !d->partial || !d->loss_part || !d->grad || !d->loss)
+  if (!d->X || !d->Dn || !d->Yn || !d->idx || !d->P || !d->partial || !d->loss_part || !d->grad || !d->loss)
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
   const int nblk = (int)((d->batch + TB - 1) / TB);
   if (nblk > d->max_blocks) return bc::fail(BC_INVALID_ARG, "batch needs %d partial blocks > max_blocks %d", nblk,
                                             d->max_blocks);
   hipStream_t st = static_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(closure_grad, dim3(nblk), dim3(TB), 0, st, d->batch, static_cast<const float*>(d->X),
-                     static_cast<const float*>(d->Yn), d->idx, static_cast<const float*>(d->P),
+                     static_cast<const float*>(d->Dn), static_cast<const float*>(d->Yn), d->idx, static_cast<const float*>(d->P),
                      static_cast<float*>(d->partial), static_cast<float*>(d->loss_part));
   BC_HIP(hipGetLastError());
   hipLaunchKernelGGL(closure_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, st, nblk,
@@ -469,7 +514,7 @@ int bc_closure_forward(const bc_closure_fwd_desc* d, void* stream) {
   if (d->n == 0) return bc::ok();
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (d->mfma) {
-    hipLaunchKernelGGL(closure_fwd_mfma, dim3(grid_for((d->n + 15) / 16, 32)), dim3(256), 0, st, d->n,
+    hipLaunchKernelGGL(closure_fwd_mfma, dim3(grid_for((d->n + 15) / 16, 128)), dim3(256), 0, st, d->n,
                        static_cast<const float*>(d->X), static_cast<const float*>(d->P),
                        static_cast<const float*>(d->norm), static_cast<float*>(d->out));
   } else {
@@ -482,14 +527,15 @@ int bc_closure_forward(const bc_closure_fwd_desc* d, void* stream) {
 }
 
 int bc_closure_apply(const bc_closure_apply_desc* d, void* stream) {
-  if (d == nullptr || !d->V1 || !d->S0 || !d->S1 || !d->P || !d->norm)
+  if (d == nullptr || !d->V1 || !d->S || !d->P || !d->norm || !d->chan_const)
     return bc::fail(BC_INVALID_ARG, "NULL argument");
-  if (d->n < 0 || d->ld0 < d->n || d->ld1 < d->n) return bc::fail(BC_INVALID_ARG, "need 0 <= n <= ld0, ld1");
+  if (d->n < 0 || d->ld < d->n) return bc::fail(BC_INVALID_ARG, "need 0 <= n <= ld");
   if (d->n == 0) return bc::ok();
-  hipLaunchKernelGGL(closure_apply, dim3(grid_for((d->n + 15) / 16, 32)), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), d->n, (float)d->dt, static_cast<const float*>(d->V1),
-                     static_cast<const float*>(d->S0), static_cast<float*>(d->S1), d->ld0, d->ld1,
-                     static_cast<const float*>(d->P), static_cast<const float*>(d->norm));
+  hipLaunchKernelGGL(closure_apply, dim3(grid_for((d->n + 63) / 64, 32)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), d->n, (float)d->dt,
+                     *static_cast<const bc_chan_const<float>*>(d->chan_const), static_cast<const float*>(d->V1),
+                     static_cast<float*>(d->S), d->ld, static_cast<const float*>(d->P),
+                     static_cast<const float*>(d->norm));
   BC_HIP(hipGetLastError());
   return bc::ok();
 }

@@ -70,6 +70,7 @@ struct StaggerArgs {
   int32_t n_steps;
   real dt;
   int32_t max_rows, max_chains, max_cls, max_levels;
+  uint32_t flags;
   bc_chan_const<real> cc;
 };
 
@@ -156,9 +157,9 @@ __device__ __forceinline__ real linearise(const bc_chan_const<real>& cc, const r
 // pool's lam = -1/tau is constant on this path: ca_b = dt exprel(-dt/tau).
 template <typename real, uint32_t CH>
 __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, real* s, real V, real ica, real dt,
-                                               real ca_b) {
+                                               real ca_b, uint32_t flags) {
   typedef bc_math<real> M;
-  real ci = real(0);
+  real ci = real(0), ci_gates = real(0);
   if constexpr ((CH & BC_CH_POOL) != 0) {
     real c = s[bc_slot_ca(CH)];
     real drive = ica * cc.ca_drive;
@@ -166,7 +167,9 @@ __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, re
     real f = drive + (cc.ca_rest - c) * cc.ca_tau_inv;
     ci = c + ca_b * f;
     s[bc_slot_ca(CH)] = ci;
+    ci_gates = (flags & BC_STEP_INTEGRATION_ORDER) ? c : ci;
   }
+  if (flags & BC_STEP_SKIP_GATES) return;
   if constexpr ((CH & BC_CH_NA) != 0) {
     real ap, bp, aq, bq;
     bc_na_rates<real>(V, cc.na_vsh, ap, bp, aq, bq);
@@ -191,7 +194,7 @@ __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, re
     s[bc_slot_cat(CH) + 1] = bc_gate_it_cf<real>(s[bc_slot_cat(CH) + 1], qi, qt, cc.cat_phi_q, dt);
   }
   if constexpr ((CH & BC_CH_AHP) != 0) {
-    real a = cc.ahp_alpha * M::pw(ci, cc.ahp_n);
+    real a = cc.ahp_alpha * M::pw(ci_gates, cc.ahp_n);
     s[bc_slot_ahp(CH)] = bc_gate_ab_cf<real>(s[bc_slot_ahp(CH)], a, cc.ahp_beta, cc.ahp_phi, dt);
   }
   if constexpr ((CH & BC_CH_HCN) != 0) {
@@ -626,7 +629,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         const int row = (int)(info[i] & kRowMask);
         const real Vn = vv[i] + R[row].r;
         xv[row] = Vn;
-        advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt, ca_b);
+        advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt, ca_b, a.flags);
         vv[i] = Vn;
         bad |= (Vn == Vn && fabs(Vn) < real(1e30)) ? 0u : 1u;
         if (pcol[i] >= 0) a.trace[(int64_t)step * a.trace_ld + pcol[i]] = Vn;
@@ -713,6 +716,7 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->step0 = d->step0;
   a->n_steps = d->n_steps;
   a->dt = static_cast<real>(d->dt);
+  a->flags = d->flags;
   a->max_rows = d->max_rows;
   a->max_chains = d->max_chains;
   a->max_cls = d->max_cls;
@@ -735,6 +739,8 @@ int validate(const bc_stagger_desc* d) {
   if (d->max_rows > 32767 || d->max_levels <= 0 || d->max_levels > 32767)
     return bc::fail(BC_INVALID_LAYOUT, "max_rows/max_levels out of range for 16-bit chain metadata");
   if (!(d->dt > 0.0)) return bc::fail(BC_INVALID_ARG, "dt must be > 0");
+  if ((d->flags & ~(BC_STEP_SKIP_GATES | BC_STEP_INTEGRATION_ORDER)) != 0)
+    return bc::fail(BC_INVALID_ARG, "unknown flags 0x%x", d->flags);
   if (d->n_packs > 0 && (d->packs == nullptr || d->cv_info == nullptr || d->V == nullptr || d->Vbnd == nullptr ||
                          d->status == nullptr))
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");

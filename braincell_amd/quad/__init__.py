@@ -72,8 +72,6 @@ def staggered_mi355x_step(target, *args):
     order = target.ion_channel_update_order
     if order not in ("family", "integration"):
         raise ValueError(f"ion_channel_update_order must be 'family' or 'integration', got {order!r}.")
-    if order == "integration":
-        raise NotImplementedError("the MI355X kernel implements the default 'family' update order")
     t = environ.get("t", 0.0)
     dt = environ.get_dt()
     return target.advance(t, dt, 1)

@@ -487,6 +487,7 @@ class StaggerRuntime:
         self.V = t(V, self.dtype)
         self.S = t(S.reshape(-1) if S.size else np.zeros(1), self.dtype)
         self.Vbnd = t(Vb if Vb.size else np.zeros(1), self.dtype)
+        self.flags = 0  # BC_STEP_* bits for every launch
         self.step_index = 0
         self._graphs = {}
 
@@ -550,6 +551,7 @@ class StaggerRuntime:
         d.step0 = self.step_index if step0 is None else int(step0)
         d.n_steps = int(n_steps)
         d.dt = self.dt
+        d.flags = self.flags
         return d
 
     def stream_handle(self):

@@ -126,7 +126,18 @@ typedef struct bc_stagger_desc {
   int32_t step0;   /* global index of the first step (clamp windows) */
   int32_t n_steps;
   double dt;       /* ms */
+  uint32_t flags;  /* BC_STEP_* */
 } bc_stagger_desc;
+
+/* bc_stagger_desc.flags */
+#define BC_STEP_SKIP_GATES (1u << 0) /* update V and the Ca pool only; gates keep
+                                        their step-start values (a closure then
+                                        applies the gate update) */
+#define BC_STEP_INTEGRATION_ORDER (1u << 1) /* ion_channel_update_order
+                                        "integration" (_multi_compartment/cell.py:
+                                        2137-2155): every node updates from the
+                                        old state, so AHP reads Ci_n instead of
+                                        the updated Ci_{n+1} ("family") */
 
 /* Library identity and diagnostics. */
 int bc_version(void);
@@ -212,18 +223,20 @@ int bc_point_run(const bc_point_desc* desc, void* stream);
 /* ------------------------------------------------------------------------
  * Learned gating closure (no counterpart in the reference): an MLP
  * 8 -> 64 -> 64 -> 5 (tanh) that corrects the exponential-Euler update of
- * the 5 gates of the Na/KDR/IL/CaL/pool model at large dt.
+ * the 5 gates of the Na/KDR/IL/CaL/pool model at large dt: with delta the
+ * plain increment and c the network output (times out_scale), a gate moves
+ * by delta (1 + c).
  * Features: (V_{n+1}, Na p, Na q, KDR p, CaL p, CaL q at step start,
  * ln Ci_{n+1}, dt); parameters: W1[8][64] b1[64] W2[64][64] b2[64] W3[64][5]
  * b3[5] (FP32, BC_CLOSURE_NPARAM values); norm = mu[8], sd[8],
- * out_scale[5] (features are (f - mu) / sd, corrections out * out_scale).
+ * out_scale[5] (features are (f - mu) / sd, c = out * out_scale).
  * All arrays FP32 device memory. */
 #define BC_CLOSURE_NPARAM 5061
 
 /* Training pairs from a fine trajectory: for n compartments, V1 = V and
  * S1 = states at t + dt, S0 = states at t (state arrays [6][ld]: Na p, Na q,
- * KDR p, CaL p, CaL q, Ci).  X[n][8] = raw features, Y[n][5] = S1 gates minus
- * the plain update of S0's gates at V1. */
+ * KDR p, CaL p, CaL q, Ci).  X[n][8] = raw features, D[n][5] = the plain
+ * increment of S0's gates at V1, Y[n][5] = S1 gates minus that plain update. */
 typedef struct bc_closure_data_desc {
   int64_t n, ld;
   double dt;
@@ -232,12 +245,14 @@ typedef struct bc_closure_data_desc {
   const void* S0;
   const void* S1;
   void* X;
+  void* D;
   void* Y;
 } bc_closure_data_desc;
 int bc_closure_data(const bc_closure_data_desc* desc, void* stream);
 
-/* Gradient of the minibatch loss mean_b sum_o (net(X[idx[b]])_o - Yn[idx[b]][o])^2
- * (X normalised, Yn = targets / out_scale) with respect to P: FP32 forward
+/* Gradient of the minibatch loss
+ * mean_b sum_o (Dn[i][o] net(X[i])_o - Yn[i][o])^2, i = idx[b] (X normalised,
+ * Dn and Yn = D and Y divided by per-output scales) with respect to P: FP32 forward
  * and backward, per-128-sample partial sums reduced in fixed order (the
  * result is deterministic).  partial: [max_blocks][BC_CLOSURE_NPARAM],
  * loss_part: [max_blocks], grad: [BC_CLOSURE_NPARAM], loss: [1]. */
@@ -245,6 +260,7 @@ typedef struct bc_closure_train_desc {
   int64_t batch;
   int32_t max_blocks;
   const void* X;
+  const void* Dn;
   const void* Yn;
   const int64_t* idx;
   const void* P;
@@ -266,7 +282,7 @@ typedef struct bc_closure_adam_desc {
 } bc_closure_adam_desc;
 int bc_closure_adam(const bc_closure_adam_desc* desc, void* stream);
 
-/* Network corrections out[n][5] = net((X - mu) / sd) * out_scale from raw
+/* Network outputs out[n][5] = c = net((X - mu) / sd) * out_scale from raw
  * features X[n][8]: mfma = 0 FP32, mfma = 1 BF16 MFMA with FP32 accumulation. */
 typedef struct bc_closure_fwd_desc {
   int64_t n;
@@ -278,17 +294,18 @@ typedef struct bc_closure_fwd_desc {
 } bc_closure_fwd_desc;
 int bc_closure_forward(const bc_closure_fwd_desc* desc, void* stream);
 
-/* Applies the closure after a plain staggered step of length dt: V1, S1 =
- * the state after the step (gates already updated, row stride ld1), S0 =
- * the state at the step start (row stride ld0; e.g. the step-0 snapshot of
- * a recording run).  S1's 5 gates += BF16-MFMA network output * out_scale,
- * clipped to [0, 1]. */
+/* Gate update with the closure, in place, after a staggered step of length
+ * dt run with BC_STEP_SKIP_GATES: V1 = the new voltage, S = the state
+ * ([6][ld]; gates still at their step-start values, Ci updated).  Each gate
+ * x becomes clip(x + delta (1 + c), 0, 1): delta = its plain exp-Euler
+ * increment at V1, c = the BF16-MFMA network output * out_scale.
+ * chan_const: host pointer to bc_chan_const<float>. */
 typedef struct bc_closure_apply_desc {
-  int64_t n, ld0, ld1;
+  int64_t n, ld;
   double dt;
+  const void* chan_const;
   const void* V1;
-  const void* S0;
-  void* S1;
+  void* S;
   const void* P;
   const void* norm;
 } bc_closure_apply_desc;

@@ -6,7 +6,8 @@ checked against it: the plain exponential-Euler gate update that the
 training targets subtract (``quad/_exp_euler.py:330-478`` for the update,
 the channel files ``sodium.py:134-170``, ``potassium.py:189-214``,
 ``calcium.py:880-920`` for the rates), the feature vector, and the MLP
-8 -> 64 -> 64 -> 5 (tanh) with its MSE loss and gradient by backpropagation.
+8 -> 64 -> 64 -> 5 (tanh) with its increment-weighted MSE loss
+sum_o (delta_o c_o - y_o)^2 and gradient by backpropagation.
 """
 
 from __future__ import annotations
@@ -42,14 +43,14 @@ def forward(P, Xn):
     return h2 @ W3 + b3, h1, h2
 
 
-def loss_and_grad(P, Xn, Yn):
-    """mean_b sum_o (net - Yn)^2 and its gradient (flat, FP64)."""
+def loss_and_grad(P, Xn, Dn, Yn):
+    """mean_b sum_o (Dn * net - Yn)^2 and its gradient (flat, FP64)."""
     (W1, b1), (W2, b2), (W3, b3) = unpack(P)
     y, h1, h2 = forward(P, Xn)
     B = Xn.shape[0]
-    d = y - Yn
+    d = Dn * y - Yn
     loss = float(np.sum(d * d) / B)
-    d3 = 2.0 * d / B
+    d3 = 2.0 * d * Dn / B
     gW3, gb3 = h2.T @ d3, d3.sum(0)
     d2 = (d3 @ W3.T) * (1.0 - h2 * h2)
     gW2, gb2 = h1.T @ d2, d2.sum(0)

@@ -42,7 +42,16 @@ def _default(params, key, value):
 class StaggeredOracle:
     """Single-cell FP64 staggered integrator over a ``CellModel``."""
 
-    def __init__(self, model):
+    def __init__(self, model, update_order: str = "family"):
+        """``update_order``: the cell's ion_channel_update_order
+        (``quad/_staggered.py:143-154``): "family" updates the pool first and
+        the channels with the updated ion (``cell.py:2157-2217``);
+        "integration" updates every top-level node from the old state
+        (``cell.py:2137-2155``), so concentration-gated channels (AHP) read
+        the step-start Ci."""
+        if update_order not in ("family", "integration"):
+            raise ValueError(f"ion_channel_update_order must be 'family' or 'integration', got {update_order!r}.")
+        self.update_order = update_order
         self.model = model
         cvs, bids, tree, sched, st = discretize(model.morph, model.cv_per_branch, model.cm, model.ra)
         self.cvs, self.bids, self.tree, self.sched, self.st = cvs, bids, tree, sched, st
@@ -93,7 +102,8 @@ class StaggeredOracle:
         self.V = V0
         self.gates = {}
         for name in self.g:
-            self.gates[name] = self._steady(name, V0, None)
+            if name != "AHP_De1994":  # needs Ci, set below
+                self.gates[name] = self._steady(name, V0, None)
         if self.ca is not None:
             self.Ci = np.full(self.C, float(self.ca["Ci0"]))
         else:
@@ -236,7 +246,7 @@ class StaggeredOracle:
             Ci_n = ch.exp_euler(Ci, fc, -1.0 / float(self.ca["tau"]) * np.ones_like(Ci), dt)
         else:
             Ci_n = None
-        self._update_gates(Vn, Ci_n, dt)
+        self._update_gates(Vn, Ci_n if self.update_order == "family" else Ci, dt)
         self.V = Vn
         self.Ci = Ci_n
 

@@ -27,21 +27,22 @@ def test_gradient_kernel_matches_oracle(cuda):
     rng = np.random.default_rng(3)
     n, batch = 3000, 1000
     Xn = rng.standard_normal((n, 8)).astype(np.float32)
+    Dn = rng.standard_normal((n, 5)).astype(np.float32)
     Yn = rng.standard_normal((n, 5)).astype(np.float32)
     idx = rng.choice(n, batch, replace=False).astype(np.int64)
     net, P, _ = _net(0, gain=1.0)
-    tX, tY, ti = (torch.as_tensor(a, device=cuda) for a in (Xn, Yn, idx))
+    tX, tD, tY, ti = (torch.as_tensor(a, device=cuda) for a in (Xn, Dn, Yn, idx))
     nb = (batch + 127) // 128
     partial = torch.empty((nb, NPARAM), device=cuda)
     lp = torch.empty(nb, device=cuda)
     grad = torch.empty(NPARAM, device=cuda)
     loss = torch.empty(1, device=cuda)
-    d = _TrainDesc(batch, nb, _ptr(tX), _ptr(tY), _ptr(ti), _ptr(net.P), _ptr(partial), _ptr(lp), _ptr(grad),
+    d = _TrainDesc(batch, nb, _ptr(tX), _ptr(tD), _ptr(tY), _ptr(ti), _ptr(net.P), _ptr(partial), _ptr(lp), _ptr(grad),
                    _ptr(loss))
     check(lib().bc_closure_grad(ctypes.byref(d), _stream()))
     torch.cuda.synchronize()
     ref_loss, ref_g = oc.loss_and_grad(net.P.double().cpu().numpy(), Xn[idx].astype(np.float64),
-                                       Yn[idx].astype(np.float64))
+                                       Dn[idx].astype(np.float64), Yn[idx].astype(np.float64))
     g = grad.double().cpu().numpy()
     assert abs(loss.item() - ref_loss) <= 1e-5 * abs(ref_loss)
     assert np.linalg.norm(g - ref_g) <= 1e-4 * np.linalg.norm(ref_g)
@@ -100,15 +101,83 @@ def test_training_targets_match_oracle(cuda):
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=cuda)
     tV, tS0, tS1 = t(V1), t(S0), t(S1)
     X = torch.empty((n, 8), device=cuda)
+    D = torch.empty((n, 5), device=cuda)
     Y = torch.empty((n, 5), device=cuda)
     d = _DataDesc(n, n, 0.1, ctypes.cast(ctypes.pointer(rt._cc), ctypes.c_void_p), _ptr(tV), _ptr(tS0), _ptr(tS1),
-                  _ptr(X), _ptr(Y))
+                  _ptr(X), _ptr(D), _ptr(Y))
     check(lib().bc_closure_data(ctypes.byref(d), _stream()))
     torch.cuda.synchronize()
     V1f, S0f, S1f = (a.double().cpu().numpy() for a in (tV, tS0, tS1))
     refX = oc.features(V1f, S0f[:5], S1f[5], 0.1)
     cc = rt._cc
-    refY = (S1f[:5] - oc.plain_gates(V1f, S0f[:5], 0.1, cc.na_vsh, cc.kdr_vsh, cc.cal_vsh, cc.na_phi, cc.kdr_phi,
-                                     cc.cal_phi_p, cc.cal_phi_q)).T
+    plain = oc.plain_gates(V1f, S0f[:5], 0.1, cc.na_vsh, cc.kdr_vsh, cc.cal_vsh, cc.na_phi, cc.kdr_phi, cc.cal_phi_p,
+                           cc.cal_phi_q)
+    refY = (S1f[:5] - plain).T
+    assert np.abs(D.double().cpu().numpy() - (plain - S0f[:5]).T).max() <= 2e-6
     assert np.allclose(X.double().cpu().numpy(), refX, rtol=1e-6, atol=1e-6)
     assert np.abs(Y.double().cpu().numpy() - refY).max() <= 2e-6
+
+
+def _small_models(seeds, amps):
+    from braincell_amd import closure as C
+    from braincell_amd import workloads as W
+
+    return [C.with_clamp(W.uneven_tree_cell(s, cmin=200, cmax=500), a) for s, a in zip(seeds, amps)]
+
+
+def test_stepper_with_a_silent_network_is_the_plain_step(cuda):
+    """Skip-gates step + in-place closure update with a zero output layer
+    reproduces the plain staggered step."""
+    import torch
+    from braincell_amd import closure as C
+    from braincell_amd.runtime import StaggerLayout, StaggerRuntime
+
+    models = _small_models([11, 12, 13], [0.05, 0.2, 0.45])
+    P = C.ClosureNet.init_params(0)
+    P[4736:] = 0.0  # W3, b3
+    net = C.ClosureNet(P, np.concatenate([np.zeros(8), np.ones(8), np.ones(5)]).astype(np.float32))
+    st = C.ClosureStepper(models, 0.1, net)
+    st.reset_state()
+    got = st.run(300).double().cpu().numpy()
+    L = StaggerLayout(models, probes="soma")
+    ref = StaggerRuntime(L, 0.1, precision=32).run(300).double().cpu().numpy()
+    torch.cuda.synchronize()
+    assert np.max(np.abs(got - ref)) < 1e-3
+
+
+def test_trained_network_gains_over_the_plain_update(cuda):
+    """Ablation: on held-out cells the trained closure cuts the one-step gate
+    error of the plain dt = 0.1 ms update (teacher-forced against plain
+    dt = 0.005 ms trajectories) by far."""
+    from braincell_amd import closure as C
+
+    rng = np.random.default_rng(0)
+    train_m = _small_models(range(100, 112), rng.uniform(0, 0.5, 12))
+    test_m = _small_models(range(200, 204), [0.05, 0.15, 0.3, 0.45])
+    data = C.generate_data(train_m, n_pairs=60, warm_ms=3.0, gap_ms=1.0)
+    net = C.train(data, epochs=30, batch=4096, lr=3e-3)
+    tw, tc = C.one_step_error(net, data, mfma=False)
+    held = C.generate_data(test_m, n_pairs=10, warm_ms=3.0, gap_ms=1.0)
+    without, with_ = C.one_step_error(net, held, mfma=True)
+    print("one-step MSE train without", tw, "with", tc)
+    print("one-step MSE held-out without", without, "with", with_, "ratio", with_ / without)
+    assert np.sum(tc / tw) < 0.5 * len(tw)
+    # every gate improves on held-out cells, on average by far
+    assert np.all(with_ < 0.8 * without)
+    assert np.sum(with_ / without) < 0.4 * len(without)
+
+
+def test_calibration_falls_back_on_cells_it_cannot_hold(cuda):
+    from braincell_amd import closure as C
+
+    rng = np.random.default_rng(1)
+    data = C.generate_data(_small_models(range(300, 306), rng.uniform(0, 0.5, 6)), n_pairs=20, warm_ms=3.0)
+    net = C.train(data, epochs=5, batch=4096)
+    models = _small_models([400, 401, 402], [0.02, 0.06, 0.5])
+    cb = C.ClosureBatch(models, net, threshold_mV=0.5)
+    acc = cb.calibrate(window_ms=20.0)
+    print("calibration RMSE", cb.rmse)
+    assert acc[0] and acc[1] and not acc[2]  # the strongly clamped cell spikes
+    out_a, out_r, ia, ir = cb.run(5.0)
+    assert list(ia) == [0, 1] and list(ir) == [2]
+    assert out_a.shape == (50, 2) and out_r.shape == (200, 1)

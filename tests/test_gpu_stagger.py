@@ -88,3 +88,43 @@ def test_chunked_and_graph_runs_equal_one_launch(cuda):
         g.launch(rt.stream_handle())
         torch.cuda.synchronize()
         assert np.array_equal(trace.double().cpu().numpy(), one), f"replay {rep}"
+
+
+def _ahp_cell():
+    """Config 0 plus CaL, AHP and a CalciumDetailed pool on the soma, clamped
+    hard enough to spike (Ca influx drives the AHP gate)."""
+    import dataclasses
+
+    m = W.single_tree_cell(0)
+    nb = m.n_branches
+    soma = np.asarray([b.type == "soma" for b in m.morph.branches])
+    ch = dict(m.channels)
+    ch["CaL_IS2008"] = {"g_branch": np.where(soma, 0.01, 0.0), "params": {}}
+    ch["AHP_De1994"] = {"g_branch": np.where(soma, 0.02, 0.0), "params": {}}
+    return dataclasses.replace(m, channels=ch, ca=W._ca_pool(soma), clamps=[(0, 0.5, 2.0, 60.0, 0.6)]).validate()
+
+
+@pytest.mark.parametrize("order", ["family", "integration"])
+def test_update_orders_match_oracle_fp64(cuda, order):
+    import torch
+    from braincell_amd.runtime import StaggerLayout, StaggerRuntime
+    from oracle.staggered import StaggeredOracle
+
+    m = _ahp_cell()
+    n = 1600
+    L = StaggerLayout([m], probes="soma")
+    rt = StaggerRuntime(L, 0.025, precision=64)
+    rt.flags = 2 if order == "integration" else 0
+    got = rt.run(n, chunk=400).double().cpu().numpy()[:, 0]
+    torch.cuda.synchronize()
+    rt.check_status()
+    o = StaggeredOracle(m, update_order=order)
+    o.reset_state(m.V0)
+    ref = o.run(0.025, n, probe_cvs=[soma_cv(m)])[:, 0]
+    assert len(spike_times(ref, 0.025)) >= 2
+    assert np.max(np.abs(got - ref)) < 1e-6
+    # the order matters on this cell: the other order's oracle is far away
+    other = StaggeredOracle(m, update_order="family" if order == "integration" else "integration")
+    other.reset_state(m.V0)
+    alt = other.run(0.025, n, probe_cvs=[soma_cv(m)])[:, 0]
+    assert np.max(np.abs(alt - ref)) > 1e-3

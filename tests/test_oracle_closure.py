@@ -11,13 +11,14 @@ def _problem(seed=0, n=16):
     rng = np.random.default_rng(seed)
     P = rng.standard_normal(5061) * 0.3
     Xn = rng.standard_normal((n, 8))
+    Dn = rng.standard_normal((n, 5))
     Yn = rng.standard_normal((n, 5))
-    return P, Xn, Yn
+    return P, Xn, Dn, Yn
 
 
 def test_mlp_gradient_matches_central_differences():
-    P, Xn, Yn = _problem()
-    _, g = oc.loss_and_grad(P, Xn, Yn)
+    P, Xn, Dn, Yn = _problem()
+    _, g = oc.loss_and_grad(P, Xn, Dn, Yn)
     rng = np.random.default_rng(1)
     # every block of the parameter vector: W1, b1, W2, b2, W3, b3
     idx = np.concatenate([rng.choice(np.arange(a, b), min(6, b - a), replace=False)
@@ -27,7 +28,7 @@ def test_mlp_gradient_matches_central_differences():
         Pp, Pm = P.copy(), P.copy()
         Pp[i] += h
         Pm[i] -= h
-        fd = (oc.loss_and_grad(Pp, Xn, Yn)[0] - oc.loss_and_grad(Pm, Xn, Yn)[0]) / (2 * h)
+        fd = (oc.loss_and_grad(Pp, Xn, Dn, Yn)[0] - oc.loss_and_grad(Pm, Xn, Dn, Yn)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-7 + 1e-6 * abs(fd), (i, fd, g[i])
 
 
