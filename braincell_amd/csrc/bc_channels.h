@@ -90,6 +90,21 @@ template <> struct bc_math<double> {
   static constexpr double exprel_eps = 1e-8;
 };
 
+// Math policy of the staggered kernel: in FP32 the hardware exponential and
+// logarithm (v_exp_f32 / v_log_f32 on exp2 / log2; relative error about
+// |x| 2^-24 + 1 ulp), in FP64 the IEEE library functions.
+template <typename real> struct bc_fastmath;
+template <> struct bc_fastmath<float> {
+  static __device__ __forceinline__ float ex(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+  static __device__ __forceinline__ float em1(float x) { return expm1f(x); }
+  static __device__ __forceinline__ float lg(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
+  static __device__ __forceinline__ float pw(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+  }
+  static constexpr float exprel_eps = 1e-4f;
+};
+template <> struct bc_fastmath<double> : bc_math<double> {};
+
 // Reciprocal: the hardware v_rcp_f32 (1 ulp) in FP32, an IEEE division in
 // FP64.  bc_div(a, b) = a * rcp(b).
 template <typename real> __device__ __forceinline__ real bc_rcp(real x);
@@ -108,7 +123,7 @@ __host__ __device__ __forceinline__ real bc_exprel(real x) {
 // x / (e^x - 1) = 1 / exprel(x).  FP32: Bernoulli series for |x| <= 1/2
 // (truncation below 3e-11, far under an FP32 ulp), x * rcp(e^x - 1) (no
 // cancellation there) otherwise.  FP64: the IEEE quotient 1 / exprel(x).
-template <typename real>
+template <typename real, typename M = bc_math<real>>
 __device__ __forceinline__ real bc_inv_exprel(real x) {
   if constexpr (sizeof(real) == 8) return real(1) / bc_exprel<real>(x);
   if (fabs(x) <= real(0.5)) {
@@ -116,7 +131,7 @@ __device__ __forceinline__ real bc_inv_exprel(real x) {
     return real(1) - real(0.5) * x +
            x2 * (real(1.0 / 12.0) + x2 * (real(-1.0 / 720.0) + x2 * (real(1.0 / 30240.0) + x2 * real(-1.0 / 1209600.0))));
   }
-  return bc_div<real>(x, bc_math<real>::ex(x) - real(1));
+  return bc_div<real>(x, M::ex(x) - real(1));
 }
 
 // y + dt * exprel(dt * lam) * f
@@ -134,11 +149,11 @@ __host__ __device__ __forceinline__ real bc_gate_ab(real x, real a, real b, real
   real lam = -phi * (a + b);
   return bc_expeuler<real>(x, f, lam, dt);
 }
-template <typename real>
+template <typename real, typename M = bc_math<real>>
 __device__ __forceinline__ real bc_gate_ab_cf(real x, real a, real b, real phi, real dt) {
   const real s = a + b;
   const real xi = bc_div<real>(a, s);
-  return xi + (x - xi) * bc_math<real>::ex(-dt * phi * s);
+  return xi + (x - xi) * M::ex(-dt * phi * s);
 }
 
 // Gate in inf/tau form: f = phi (inf - x) / tau, lam = -phi / tau.
@@ -149,34 +164,31 @@ __host__ __device__ __forceinline__ real bc_gate_it(real x, real inf, real tau, 
   real lam = -phi * itau;
   return bc_expeuler<real>(x, f, lam, dt);
 }
-template <typename real>
+template <typename real, typename M = bc_math<real>>
 __device__ __forceinline__ real bc_gate_it_cf(real x, real inf, real tau, real phi, real dt) {
-  return inf + (x - inf) * bc_math<real>::ex(-dt * phi * bc_rcp<real>(tau));
+  return inf + (x - inf) * M::ex(-dt * phi * bc_rcp<real>(tau));
 }
 
 // Rate functions.  Divisions by literal constants are multiplications by
 // their reciprocals; other divisions use bc_div.
-template <typename real>
+template <typename real, typename M = bc_math<real>>
 __device__ __forceinline__ void bc_na_rates(real V, real vsh, real& ap, real& bp, real& aq, real& bq) {
-  typedef bc_math<real> M;
   real Vp = V - vsh;
-  ap = real(0.32 * 4.0) * bc_inv_exprel<real>(-(Vp - real(13)) * real(0.25));
-  bp = real(0.28 * 5.0) * bc_inv_exprel<real>((Vp - real(40)) * real(0.2));
+  ap = real(0.32 * 4.0) * bc_inv_exprel<real, M>(-(Vp - real(13)) * real(0.25));
+  bp = real(0.28 * 5.0) * bc_inv_exprel<real, M>((Vp - real(40)) * real(0.2));
   aq = real(0.128) * M::ex(-(Vp - real(17)) * real(1.0 / 18.0));
   bq = bc_div<real>(real(4), real(1) + M::ex(-(Vp - real(40)) * real(0.2)));
 }
 
-template <typename real>
+template <typename real, typename M = bc_math<real>>
 __device__ __forceinline__ void bc_kdr_rates(real V, real vsh, real& ap, real& bp) {
-  typedef bc_math<real> M;
   real Vp = V - vsh;
-  ap = real(0.032 * 5.0) * bc_inv_exprel<real>(-(Vp - real(15)) * real(0.2));
+  ap = real(0.032 * 5.0) * bc_inv_exprel<real, M>(-(Vp - real(15)) * real(0.2));
   bp = real(0.5) * M::ex(-(Vp - real(10)) * real(1.0 / 40.0));
 }
 
-template <typename real>
+template <typename real, typename M = bc_math<real>>
 __device__ __forceinline__ void bc_cal_rates(real V, real vsh, real& pi, real& pt, real& qi, real& qt) {
-  typedef bc_math<real> M;
   real Vp = V - vsh;
   pi = bc_rcp<real>(real(1) + M::ex(-(Vp + real(10)) * real(0.25)));
   real e1 = M::ex((Vp + real(5)) * real(1.0 / 15.0));
@@ -186,9 +198,8 @@ __device__ __forceinline__ void bc_cal_rates(real V, real vsh, real& pi, real& p
   qt = real(300) + bc_div<real>(real(100), e2 + bc_rcp<real>(e2));
 }
 
-template <typename real>
+template <typename real, typename M = bc_math<real>>
 __device__ __forceinline__ void bc_cat_rates(real V, real vsh, real& pi, real& pt, real& qi, real& qt) {
-  typedef bc_math<real> M;
   real Vp = V - vsh;
   pi = bc_rcp<real>(real(1) + M::ex(-(Vp + real(59)) * real(1.0 / 6.2)));
   pt = bc_rcp<real>(M::ex(-(Vp + real(132)) * real(1.0 / 16.7)) + M::ex((Vp + real(16.8)) * real(1.0 / 18.2))) +
@@ -198,9 +209,8 @@ __device__ __forceinline__ void bc_cat_rates(real V, real vsh, real& pi, real& p
                          : M::ex((Vp + real(467)) * real(1.0 / 66.6));
 }
 
-template <typename real>
+template <typename real, typename M = bc_math<real>>
 __device__ __forceinline__ void bc_hcn_rates(real V, real& pi, real& pt) {
-  typedef bc_math<real> M;
   pi = bc_rcp<real>(real(1) + M::ex((V + real(75)) * real(1.0 / 5.5)));
   pt = bc_rcp<real>(M::ex(real(-0.086) * V - real(14.59)) + M::ex(real(0.0701) * V - real(1.87)));
 }

@@ -93,7 +93,7 @@ __device__ __forceinline__ real clamp_current(const StaggerArgs<real>& a, const 
 template <typename real, uint32_t CH>
 __device__ __forceinline__ real linearise(const bc_chan_const<real>& cc, const real* cp, const real* s, real V,
                                           real& G, real& GE) {
-  typedef bc_math<real> M;
+  typedef bc_fastmath<real> M;
   G = real(0);
   GE = real(0);
   real ica = real(0);
@@ -158,7 +158,7 @@ __device__ __forceinline__ real linearise(const bc_chan_const<real>& cc, const r
 template <typename real, uint32_t CH>
 __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, real* s, real V, real ica, real dt,
                                                real ca_b, uint32_t flags) {
-  typedef bc_math<real> M;
+  typedef bc_fastmath<real> M;
   real ci = real(0), ci_gates = real(0);
   if constexpr ((CH & BC_CH_POOL) != 0) {
     real c = s[bc_slot_ca(CH)];
@@ -172,35 +172,35 @@ __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, re
   if (flags & BC_STEP_SKIP_GATES) return;
   if constexpr ((CH & BC_CH_NA) != 0) {
     real ap, bp, aq, bq;
-    bc_na_rates<real>(V, cc.na_vsh, ap, bp, aq, bq);
-    s[bc_slot_na(CH)] = bc_gate_ab_cf<real>(s[bc_slot_na(CH)], ap, bp, cc.na_phi, dt);
-    s[bc_slot_na(CH) + 1] = bc_gate_ab_cf<real>(s[bc_slot_na(CH) + 1], aq, bq, cc.na_phi, dt);
+    bc_na_rates<real, M>(V, cc.na_vsh, ap, bp, aq, bq);
+    s[bc_slot_na(CH)] = bc_gate_ab_cf<real, M>(s[bc_slot_na(CH)], ap, bp, cc.na_phi, dt);
+    s[bc_slot_na(CH) + 1] = bc_gate_ab_cf<real, M>(s[bc_slot_na(CH) + 1], aq, bq, cc.na_phi, dt);
   }
   if constexpr ((CH & BC_CH_KDR) != 0) {
     real ap, bp;
-    bc_kdr_rates<real>(V, cc.kdr_vsh, ap, bp);
-    s[bc_slot_kdr(CH)] = bc_gate_ab_cf<real>(s[bc_slot_kdr(CH)], ap, bp, cc.kdr_phi, dt);
+    bc_kdr_rates<real, M>(V, cc.kdr_vsh, ap, bp);
+    s[bc_slot_kdr(CH)] = bc_gate_ab_cf<real, M>(s[bc_slot_kdr(CH)], ap, bp, cc.kdr_phi, dt);
   }
   if constexpr ((CH & BC_CH_CAL) != 0) {
     real pi, pt, qi, qt;
-    bc_cal_rates<real>(V, cc.cal_vsh, pi, pt, qi, qt);
-    s[bc_slot_cal(CH)] = bc_gate_it_cf<real>(s[bc_slot_cal(CH)], pi, pt, cc.cal_phi_p, dt);
-    s[bc_slot_cal(CH) + 1] = bc_gate_it_cf<real>(s[bc_slot_cal(CH) + 1], qi, qt, cc.cal_phi_q, dt);
+    bc_cal_rates<real, M>(V, cc.cal_vsh, pi, pt, qi, qt);
+    s[bc_slot_cal(CH)] = bc_gate_it_cf<real, M>(s[bc_slot_cal(CH)], pi, pt, cc.cal_phi_p, dt);
+    s[bc_slot_cal(CH) + 1] = bc_gate_it_cf<real, M>(s[bc_slot_cal(CH) + 1], qi, qt, cc.cal_phi_q, dt);
   }
   if constexpr ((CH & BC_CH_CAT) != 0) {
     real pi, pt, qi, qt;
-    bc_cat_rates<real>(V, cc.cat_vsh, pi, pt, qi, qt);
-    s[bc_slot_cat(CH)] = bc_gate_it_cf<real>(s[bc_slot_cat(CH)], pi, pt, cc.cat_phi_p, dt);
-    s[bc_slot_cat(CH) + 1] = bc_gate_it_cf<real>(s[bc_slot_cat(CH) + 1], qi, qt, cc.cat_phi_q, dt);
+    bc_cat_rates<real, M>(V, cc.cat_vsh, pi, pt, qi, qt);
+    s[bc_slot_cat(CH)] = bc_gate_it_cf<real, M>(s[bc_slot_cat(CH)], pi, pt, cc.cat_phi_p, dt);
+    s[bc_slot_cat(CH) + 1] = bc_gate_it_cf<real, M>(s[bc_slot_cat(CH) + 1], qi, qt, cc.cat_phi_q, dt);
   }
   if constexpr ((CH & BC_CH_AHP) != 0) {
     real a = cc.ahp_alpha * M::pw(ci_gates, cc.ahp_n);
-    s[bc_slot_ahp(CH)] = bc_gate_ab_cf<real>(s[bc_slot_ahp(CH)], a, cc.ahp_beta, cc.ahp_phi, dt);
+    s[bc_slot_ahp(CH)] = bc_gate_ab_cf<real, M>(s[bc_slot_ahp(CH)], a, cc.ahp_beta, cc.ahp_phi, dt);
   }
   if constexpr ((CH & BC_CH_HCN) != 0) {
     real pi, pt;
-    bc_hcn_rates<real>(V, pi, pt);
-    s[bc_slot_hcn(CH)] = bc_gate_it_cf<real>(s[bc_slot_hcn(CH)], pi, pt, cc.hcn_phi, dt);
+    bc_hcn_rates<real, M>(V, pi, pt);
+    s[bc_slot_hcn(CH)] = bc_gate_it_cf<real, M>(s[bc_slot_hcn(CH)], pi, pt, cc.hcn_phi, dt);
   }
 }
 
@@ -380,28 +380,25 @@ __device__ __forceinline__ void chain_affine(int k, int dummy, const int4* CR, H
   KK[k].r = be;
 }
 
-// Solutions of chain k's rows from X, the solution of the row it hangs from.
+// Solution of one row after the affine pass and the pointer jumping.
 template <typename real>
-__device__ __forceinline__ void chain_solution(int k, real X, const int4* CR, HRow<real>* R) {
-  const ChainRec c = unpack_chain(CR[k]);
-  for (int j0 = c.j0; j0 < c.j1; j0 += kGroup) {
-    HRow<real> w[kGroup];
-#pragma unroll
-    for (int i = 0; i < kGroup; ++i) w[i] = R[min(j0 + i, c.j1 - 1)];
-#pragma unroll
-    for (int i = 0; i < kGroup; ++i)
-      if (j0 + i < c.j1) R[j0 + i].r = w[i].r + w[i].d * X;
-  }
+__device__ __forceinline__ real row_solution(int row, const HRow<real>* R, const int16_t* m_rowch,
+                                             const int16_t* m_par, const KPair<real>* Xb) {
+  const int p = m_par[m_rowch[row]];
+  const real X = p >= 0 ? Xb[p].d : real(0);
+  const HRow<real> w = R[row];
+  return w.r + w.d * X;
 }
 
-// Dummy rows after the pack's rows: one per thread, written by rows past a
-// chain's end in a branch-free row group and never read.
-constexpr int kDummySlots = 256;
+// Dummy row after the pack's rows: rows past a chain's end in a branch-free
+// row group write their (unused) results there; it is never read, so all
+// threads may share it.
+constexpr int kDummySlots = 1;
 
 // LDS carve-up (bytes), shared with the launch code: rows and chains are
 // rounded up to multiples of 4 so every array stays naturally aligned.
 struct LdsPlan {
-  size_t rows, recs, kk, kk2, xv, cls, cnt, kid, lv, link, par, ptr, ptr2, total;
+  size_t rows, recs, kk, kk2, xv, cls, cnt, kid, lv, link, rowch, par, ptr, ptr2, total;
 };
 template <typename real>
 __host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int max_cls, int max_levels) {
@@ -417,7 +414,8 @@ __host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int ma
   p.kid = p.cnt + mc * sizeof(int);
   p.lv = p.kid + mc * sizeof(int16_t);
   p.link = p.lv + (size_t)((max_levels + 4) & ~3) * sizeof(int16_t);
-  p.par = p.link + mr * sizeof(int16_t);
+  p.rowch = p.link + mr * sizeof(int16_t);
+  p.par = p.rowch + mr * sizeof(int16_t);
   p.ptr = p.par + mc * sizeof(int16_t);
   p.ptr2 = p.ptr + mc * sizeof(int16_t);
   p.total = p.ptr2 + mc * sizeof(int16_t);
@@ -441,6 +439,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   // parent slot of every slot (15 bits; a root's top points at itself) | 1 << 15
   // for a chain's bottom slot (its children are the kid chains, not slot - 1)
   int16_t* m_link = reinterpret_cast<int16_t*>(smem_raw + lp.link);
+  int16_t* m_rowch = reinterpret_cast<int16_t*>(smem_raw + lp.rowch);  // chain of every slot
   int16_t* m_par = reinterpret_cast<int16_t*>(smem_raw + lp.par);    // parent chain, -1 for a root
   int16_t* m_ptr = reinterpret_cast<int16_t*>(smem_raw + lp.ptr);  // pointer jumping (two buffers)
   int16_t* m_ptr2 = reinterpret_cast<int16_t*>(smem_raw + lp.ptr2);
@@ -465,6 +464,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     for (int j = j0; j < j1; ++j) {
       const int par = (j + 1 < j1) ? j + 1 : (tp >= 0 ? tp : j);
       m_link[j] = (int16_t)(par | (j == j0 ? 0x8000 : 0));
+      m_rowch[j] = (int16_t)i;
     }
     for (int q = 0; q < nk; ++q) m_par[a.kids[kbase + q0 + q]] = (int16_t)i;
     if (tp < 0) m_par[i] = -1;
@@ -504,7 +504,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   for (int i = tid; i < P.n_bnd; i += NT) xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu] = a.Vbnd[P.bnd0 + i];
   __syncthreads();
 
-  const int dummy = ((a.max_rows + 3) & ~3) + tid;
+  const int dummy = (a.max_rows + 3) & ~3;
   const real dt = a.dt;
   const real ca_b = dt * bc_exprel<real>(-dt * a.cc.ca_tau_inv);
   uint32_t bad = 0;
@@ -574,6 +574,9 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       m_ptr[k] = m_par[k];
     }
     __syncthreads();
+    // solution of row j of chain k: x_j = alpha_j + beta_j X, X = the solution
+    // at the bottom of k's parent chain (Xb[parent].d after the jumping)
+    const KPair<real>* Xb;
     {
       // (A, B, ptr): x_bottom(k) = A + B x_bottom(ptr); ptr < 0 once resolved
       KPair<real>* A0 = KK;
@@ -611,23 +614,20 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
           break;
         }
       }
-      for (int k = tid; k < P.n_ch; k += NT) {
-        const int p = m_par[k];
-        chain_solution<real>(k, p >= 0 ? A0[p].d : real(0), CR, R);
-      }
+      Xb = A0;
     }
-    __syncthreads();
+
 
     // ---- phase C: V += dV; Ca pool and gates at V_{n+1}
     for (int i = tid; i < P.n_bnd; i += NT) {
       const int row = (int)(a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu);
-      xv[row] += R[row].r;
+      xv[row] += row_solution<real>(row, R, m_rowch, m_par, Xb);
     }
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       if (info[i] != 0xFFFFFFFFu) {
         const int row = (int)(info[i] & kRowMask);
-        const real Vn = vv[i] + R[row].r;
+        const real Vn = vv[i] + row_solution<real>(row, R, m_rowch, m_par, Xb);
         xv[row] = Vn;
         advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt, ca_b, a.flags);
         vv[i] = Vn;
