@@ -291,13 +291,36 @@ constexpr int kGroup = 4;
 // Forward elimination of chain k (slots j0 .. j1-1, bottom to top) in the
 // delta-V form.  (sd, sr) is what the rows below subtract from the current
 // row: the kids' contributions at the bottom, then the previous row's.
+// One row group of a chain's elimination: rows j .. j + kGroup - 1 (those
+// past the chain's end write to the dummy row and keep (sd, sr)).
+template <typename real>
+__device__ __forceinline__ void eliminate_group(const HRow<real>* g, int j, int j1, int dummy, HRow<real>* R,
+                                                real& sd, real& sr) {
+#pragma unroll
+  for (int i = 0; i < kGroup; ++i) {
+    const bool valid = j + i < j1;
+    const HRow<real>& w = g[i];
+    const real e = w.d - sd;
+    const real r = w.r - sr;
+    const real inv = bc_rcp<real>(e - w.lo);
+    const real ui = w.up * inv;
+    HRow<real>* dst = &R[valid ? j + i : dummy];
+    dst->d = inv;
+    dst->r = r;
+    sd = valid ? ui * e : sd;
+    sr = valid ? ui * r : sr;
+  }
+}
+
 template <typename real>
 __device__ __forceinline__ void eliminate_chain(int k, int dummy, const int4* CR, const int16_t* m_kid,
                                                 HRow<real>* R, const real* xv, KPair<real>* KK) {
   const ChainRec c = unpack_chain(CR[k]);
-  HRow<real> cur[kGroup], nxt[kGroup];
+  // two row buffers used in turn: the next group is loaded into one while
+  // the other is eliminated
+  HRow<real> ga[kGroup], gb[kGroup];
 #pragma unroll
-  for (int i = 0; i < kGroup; ++i) cur[i] = R[min(c.j0 + i, c.j1 - 1)];
+  for (int i = 0; i < kGroup; ++i) ga[i] = R[c.j0 + i];
   // the top edge's axial term for the row the chain hangs from (zero for a root)
   const int tq = c.tp >= 0 ? c.tp : c.j1 - 1;
   const real top_defect = R[c.j1 - 1].up * (xv[c.j1 - 1] - xv[tq]);
@@ -317,26 +340,17 @@ __device__ __forceinline__ void eliminate_chain(int k, int dummy, const int4* CR
     sd += t.d;
     sr += t.r;
   }
-  for (int j = c.j0;; j += kGroup) {
+  for (int j = c.j0;;) {
 #pragma unroll
-    for (int i = 0; i < kGroup; ++i) nxt[i] = R[min(j + kGroup + i, c.j1 - 1)];
+    for (int i = 0; i < kGroup; ++i) gb[i] = R[j + kGroup + i];
+    eliminate_group<real>(ga, j, c.j1, dummy, R, sd, sr);
+    j += kGroup;
+    if (j >= c.j1) break;
 #pragma unroll
-    for (int i = 0; i < kGroup; ++i) {
-      const bool valid = j + i < c.j1;
-      const HRow<real>& w = cur[i];
-      const real e = w.d - sd;
-      const real r = w.r - sr;
-      const real inv = bc_rcp<real>(e - w.lo);
-      const real ui = w.up * inv;
-      HRow<real>* dst = &R[valid ? j + i : dummy];
-      dst->d = inv;
-      dst->r = r;
-      sd = valid ? ui * e : sd;
-      sr = valid ? ui * r : sr;
-    }
-    if (j + kGroup >= c.j1) break;
-#pragma unroll
-    for (int i = 0; i < kGroup; ++i) cur[i] = nxt[i];
+    for (int i = 0; i < kGroup; ++i) ga[i] = R[j + kGroup + i];
+    eliminate_group<real>(gb, j, c.j1, dummy, R, sd, sr);
+    j += kGroup;
+    if (j >= c.j1) break;
   }
   KK[k].d = sd;
   KK[k].r = sr + top_defect;
@@ -390,10 +404,12 @@ __device__ __forceinline__ real row_solution(int row, const HRow<real>* R, const
   return w.r + w.d * X;
 }
 
-// Dummy row after the pack's rows: rows past a chain's end in a branch-free
-// row group write their (unused) results there; it is never read, so all
-// threads may share it.
-constexpr int kDummySlots = 1;
+// Rows after the pack's rows: the first is the dummy row that rows past a
+// chain's end in a branch-free row group write their (unused) results to
+// (never read, so all threads share it); with the rest it pads the row array
+// so a chain walk may load up to 2 kGroup - 1 rows past its last row
+// without clamping (loads past a chain's end are discarded).
+constexpr int kDummySlots = 8;
 
 // LDS carve-up (bytes), shared with the launch code: rows and chains are
 // rounded up to multiples of 4 so every array stays naturally aligned.
