@@ -366,29 +366,42 @@ __device__ __forceinline__ void eliminate_chain(int k, int dummy, const int4* CR
 // tree.  R[j].d / R[j].r are overwritten with beta_j / alpha_j, and the
 // bottom row's pair is returned through KK[k] = (alpha, beta).
 template <typename real>
+__device__ __forceinline__ void affine_group(const HRow<real>* g, int j, int j0, int dummy, HRow<real>* R, real& al,
+                                             real& be) {
+#pragma unroll
+  for (int i = 0; i < kGroup; ++i) {
+    const bool valid = j - i >= j0;
+    const real a2 = (g[i].r - g[i].lo * al) * g[i].d;
+    const real b2 = -g[i].lo * be * g[i].d;
+    HRow<real>* dst = &R[valid ? j - i : dummy];
+    dst->d = b2;
+    dst->r = a2;
+    al = valid ? a2 : al;
+    be = valid ? b2 : be;
+  }
+}
+
+template <typename real>
 __device__ __forceinline__ void chain_affine(int k, int dummy, const int4* CR, HRow<real>* R, KPair<real>* KK) {
   const ChainRec c = unpack_chain(CR[k]);
   real al = real(0), be = c.tp >= 0 ? real(1) : real(0);
-  HRow<real> cur[kGroup], nxt[kGroup];
+  // top to bottom in groups, two alternating buffers; loads below the
+  // chain's first row land in other chains' rows or the padding before row
+  // 0 and are discarded
+  HRow<real> ga[kGroup], gb[kGroup];
 #pragma unroll
-  for (int i = 0; i < kGroup; ++i) cur[i] = R[max(c.j1 - 1 - i, c.j0)];
-  for (int j = c.j1 - 1;; j -= kGroup) {
+  for (int i = 0; i < kGroup; ++i) ga[i] = R[c.j1 - 1 - i];
+  for (int j = c.j1 - 1;;) {
 #pragma unroll
-    for (int i = 0; i < kGroup; ++i) nxt[i] = R[max(j - kGroup - i, c.j0)];
+    for (int i = 0; i < kGroup; ++i) gb[i] = R[j - kGroup - i];
+    affine_group<real>(ga, j, c.j0, dummy, R, al, be);
+    j -= kGroup;
+    if (j < c.j0) break;
 #pragma unroll
-    for (int i = 0; i < kGroup; ++i) {
-      const bool valid = j - i >= c.j0;
-      const real a2 = (cur[i].r - cur[i].lo * al) * cur[i].d;
-      const real b2 = -cur[i].lo * be * cur[i].d;
-      HRow<real>* dst = &R[valid ? j - i : dummy];
-      dst->d = b2;
-      dst->r = a2;
-      al = valid ? a2 : al;
-      be = valid ? b2 : be;
-    }
-    if (j - kGroup < c.j0) break;
-#pragma unroll
-    for (int i = 0; i < kGroup; ++i) cur[i] = nxt[i];
+    for (int i = 0; i < kGroup; ++i) ga[i] = R[j - kGroup - i];
+    affine_group<real>(gb, j, c.j0, dummy, R, al, be);
+    j -= kGroup;
+    if (j < c.j0) break;
   }
   KK[k].d = al;
   KK[k].r = be;
@@ -410,6 +423,9 @@ __device__ __forceinline__ real row_solution(int row, const HRow<real>* R, const
 // so a chain walk may load up to 2 kGroup - 1 rows past its last row
 // without clamping (loads past a chain's end are discarded).
 constexpr int kDummySlots = 8;
+// Padding rows before row 0: a downward chain walk may load up to
+// 2 kGroup - 1 rows below its first row.
+constexpr int kPadBefore = 8;
 
 // LDS carve-up (bytes), shared with the launch code: rows and chains are
 // rounded up to multiples of 4 so every array stays naturally aligned.
@@ -420,7 +436,7 @@ template <typename real>
 __host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int max_cls, int max_levels) {
   const size_t mr = (size_t)((max_rows + 3) & ~3), mc = (size_t)((max_chains + 3) & ~3);
   LdsPlan p;
-  p.rows = 0;
+  p.rows = kPadBefore * sizeof(HRow<real>);  // row 0; kPadBefore padding rows precede it
   p.recs = p.rows + (mr + kDummySlots) * sizeof(HRow<real>);
   p.kk = p.recs + mc * sizeof(int4);
   p.kk2 = p.kk + mc * sizeof(KPair<real>);
