@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--cpu-sample-cells", type=int, default=128)
     ap.add_argument("--cpu-sample-steps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep --threads / --rows-per-pack as given")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with a 2 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -89,6 +90,12 @@ def bench_cells(args):
     models = W.uneven_tree_batch(args.cells)
     L = StaggerLayout(models, threads=args.threads, rows_per_pack=args.rows_per_pack)
     log(f"layout {L.summary()} built in {time.perf_counter() - t0:.1f}s")
+    if not args.no_autotune:
+        from braincell_amd.autotune import autotune
+
+        t0 = time.perf_counter()
+        L, _ = autotune(L, dt, precision=32, log=log)
+        log(f"autotuned layout {L.summary()} in {time.perf_counter() - t0:.1f}s")
     rt = StaggerRuntime(L, dt, precision=32)
     chunk = args.chunk
     K = (args.steps // chunk) * chunk
@@ -132,7 +139,7 @@ def bench_cells(args):
     achieved = bytes_per_launch / launch_s / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "stagger_kernel<float, NA|KDR|IL|CaL|pool, 8>", "launch_ms": round(launch_s * 1e3, 3),
+            "kernel": f"stagger_kernel<float, NA|KDR|IL|CaL|pool, {L.cv_per_thread}>", "launch_ms": round(launch_s * 1e3, 3),
             "steps_per_launch": chunk}
     tf = os.path.join(ROOT, "profiles", "stagger_c2_traffic.json")
     if os.path.exists(tf):
@@ -172,6 +179,7 @@ def bench_cells(args):
                                f"dendritic CaL/CalciumDetailed, dt 0.025 ms, staggered plain step",
                    "cells": args.cells, "compartments": int(L.n_cv_total), "dhs_rows": int(L.n_rows_total),
                    "packs": L.n_packs, "threads": L.threads, "cv_per_thread": L.cv_per_thread,
+                   "rows_per_pack": L.rows_per_pack, "autotuned": not args.no_autotune,
                    "steps_per_launch": chunk, "parallelism": "1 GPU, one workgroup per pack"},
         "roofline": roof, "cpu_baseline": cpu, "accuracy": acc,
     }

@@ -1,0 +1,136 @@
+"""Autotuner for the packing of cells into workgroups.
+
+A batch of uneven cells is packed whole into packs, one workgroup per pack
+(``StaggerLayout``).  Three knobs decide how the batch maps onto the GPU:
+the workgroup size, the register slots per thread (compartments each
+thread keeps in registers across the steps of a launch) and the row budget
+of a pack (how many cells share a workgroup and its LDS).  Bigger packs
+share the latency of the tree solve between more cells but need more LDS
+per workgroup, which lowers the number of resident workgroups per compute
+unit.  The best trade-off depends on the cell-size mix, so it is measured:
+every candidate plan that fits is timed for a few steps on the device and
+the fastest one is kept, in memory and in a small JSON cache keyed by the
+batch's size signature.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+from .runtime import StaggerLayout, StaggerRuntime
+
+__all__ = ["candidate_plans", "autotune", "cache_path"]
+
+LDS_LIMIT = 160 * 1024
+
+
+def cache_path() -> str:
+    base = os.environ.get("BRAINCELL_AMD_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "braincell_amd"))
+    return os.path.join(base, "autotune.json")
+
+
+def signature(layout: StaggerLayout, precision: int, dt: float) -> str:
+    """Cell-size mix, channel set, precision and device."""
+    import torch
+
+    sizes = np.asarray(sorted((c.geom.n_cv, c.geom.n_point) for c in layout.cells), dtype=np.int64)
+    h = hashlib.sha1(sizes.tobytes())
+    dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
+    h.update(f"{layout.chset}|{precision}|{dt}|{dev}".encode())
+    return h.hexdigest()[:20]
+
+
+def candidate_plans(layout: StaggerLayout, precision: int = 32):
+    """(threads, cv_per_thread, rows_per_pack) plans whose packs fit the LDS."""
+    nrow_max = max(c.geom.n_point for c in layout.cells)
+    ncv_max = max(c.geom.n_cv for c in layout.cells)
+    out = []
+    for threads in (128, 256):
+        for scale in (1.0, 1.25, 1.6, 2.0, 3.0):
+            rows = int(nrow_max * scale)
+            cpt = 1
+            while cpt * threads < max(ncv_max, int(rows * 0.8)) and cpt < 16:
+                cpt *= 2
+            if cpt * threads < ncv_max:
+                continue
+            out.append((threads, cpt, rows))
+    # dedupe, keep order
+    seen, plans = set(), []
+    for p in out:
+        if p not in seen:
+            seen.add(p)
+            plans.append(p)
+    return plans
+
+
+def _time_plan(L: StaggerLayout, dt: float, precision: int, steps: int, reps: int) -> float:
+    import torch
+
+    rt = StaggerRuntime(L, dt, precision=precision)
+    tr = rt.new_trace(steps)
+    rt.run(steps, trace=tr)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for _ in range(reps):
+        e0.record()
+        rt.run(steps, trace=tr)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    rt.check_status()
+    return best / steps
+
+
+def autotune(layout: StaggerLayout, dt: float, *, precision: int = 32, steps: int = 40, reps: int = 2,
+             use_cache: bool = True, log=None):
+    """Times every candidate plan of ``layout``'s cells and returns
+    (best layout, table of (plan, seconds per step)).  Results are cached by
+    the batch's size signature."""
+    sig = signature(layout, precision, dt)
+    path = cache_path()
+    cache = {}
+    if use_cache and os.path.exists(path):
+        try:
+            with open(path) as f:
+                cache = json.load(f)
+        except (OSError, ValueError):
+            cache = {}
+    if use_cache and sig in cache:
+        t, c, r = cache[sig]["plan"]
+        if log:
+            log(f"autotune: cached plan threads={t} cv_per_thread={c} rows_per_pack={r}")
+        return layout.repacked(threads=t, cv_per_thread=c, rows_per_pack=r), cache[sig]["table"]
+    table = []
+    best = None
+    for (t, c, r) in candidate_plans(layout, precision):
+        t0 = time.perf_counter()
+        try:
+            L = layout.repacked(threads=t, cv_per_thread=c, rows_per_pack=r)
+        except ValueError:
+            continue
+        if L.lds_bytes(precision) > LDS_LIMIT:
+            continue
+        sec = _time_plan(L, dt, precision, steps, reps)
+        table.append(((t, c, r), sec))
+        if log:
+            log(f"autotune: threads={t} cv_per_thread={c} rows_per_pack={r} packs={L.n_packs} "
+                f"lds={L.lds_bytes(precision)} -> {sec * 1e3:.4f} ms/step ({time.perf_counter() - t0:.1f}s)")
+        if best is None or sec < best[1]:
+            best = (L, sec, (t, c, r))
+    if best is None:
+        raise RuntimeError("no candidate plan fits the device")
+    if use_cache:
+        cache[sig] = {"plan": list(best[2]), "table": [[list(p), s] for p, s in table]}
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            with open(path, "w") as f:
+                json.dump(cache, f)
+        except OSError:
+            pass
+    return best[0], table

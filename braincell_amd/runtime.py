@@ -110,7 +110,13 @@ class StaggerLayout:
         self.slots = state_slots(self.chset)
         self.cc = self._channel_constants()
         self.cells = [self._lower_cell(m, probes) for m in self.models]
+        self._configure(threads, cv_per_thread, rows_per_pack)
+
+    def _configure(self, threads, cv_per_thread, rows_per_pack):
+        """Workgroup size, register slots and pack plan over the lowered cells."""
         self.threads = int(threads)
+        if self.threads not in (64, 128, 192, 256):
+            raise ValueError("threads must be 64, 128, 192 or 256")
         ncv_max = max(c.geom.n_cv for c in self.cells)
         if cv_per_thread is None:
             cv_per_thread = 1
@@ -126,6 +132,24 @@ class StaggerLayout:
         self.rows_per_pack = int(max(nrow_max, rows_per_pack or nrow_max))
         self.pack_cells = self._plan_packs(cap_cv)
         self._assemble()
+
+    def repacked(self, *, threads: int = None, cv_per_thread: int = None, rows_per_pack: int = None):
+        """The same lowered cells under another workgroup size / pack plan
+        (the cells are not discretised again)."""
+        import copy
+
+        L = copy.copy(self)
+        L._configure(self.threads if threads is None else threads, cv_per_thread, rows_per_pack)
+        return L
+
+    def lds_bytes(self, precision: int = 32) -> int:
+        """Dynamic LDS of one workgroup (the kernel's lds_plan)."""
+        rs = 8 if precision == 64 else 4
+        mr = (self.max_rows + 3) & ~3
+        mc = (self.max_chains + 3) & ~3
+        rows = (8 + mr + 8) * 4 * rs
+        return int(rows + mc * 16 + 2 * mc * 2 * rs + mr * rs + self.max_cls * 12 * rs + mc * 4 + mc * 2
+                   + ((self.max_levels + 4) & ~3) * 2 + 2 * mr * 2 + 3 * mc * 2)
 
     # ------------------------------------------------------------ constants
     def _channel_constants(self):
