@@ -280,47 +280,60 @@ __device__ __forceinline__ real axial_defect(int row, real V, const HRow<real>* 
   return bottom ? own : own - kid;
 }
 
-// Rows of a chain are walked in groups of kGroup: the group's rows are
-// loaded together (indices clamped to the chain, so every load stays inside
-// it) and the next group's loads are issued before the current group is
-// computed, so LDS latency is paid once per group.  Rows past the chain's
-// end are computed on the clamped copies and their results are sent to a
-// per-lane dummy slot, so the group body has no branch.
+// Rows of a chain are walked in groups of kGroup whose rows are loaded
+// together, the next group's loads issued before the current group is
+// computed, so LDS latency is paid once per group.  A chain of len rows is
+// split into a head of len mod kGroup rows, done first, and len / kGroup
+// full groups.  Only the head checks which of its rows belong to the chain
+// (the others are computed on rows of other chains or padding, keep the
+// running values and store to the dummy rows); the full groups run without
+// any check.  Loads may reach up to 2 kGroup - 1 rows past either end of a
+// chain, which the padding rows before row 0 and after the last row cover.
 constexpr int kGroup = 4;
 
-// Forward elimination of chain k (slots j0 .. j1-1, bottom to top) in the
-// delta-V form.  (sd, sr) is what the rows below subtract from the current
-// row: the kids' contributions at the bottom, then the previous row's.
-// One row group of a chain's elimination: rows j .. j + kGroup - 1 (those
-// past the chain's end write to the dummy row and keep (sd, sr)).
+// Forward elimination in the delta-V form.  (sd, sr) is what the rows below
+// subtract from the current row: the kids' contributions at the bottom,
+// then the previous row's.
 template <typename real>
-__device__ __forceinline__ void eliminate_group(const HRow<real>* g, int j, int j1, int dummy, HRow<real>* R,
-                                                real& sd, real& sr) {
+__device__ __forceinline__ void eliminate_row(const HRow<real>& w, HRow<real>* dst, bool valid, real& sd, real& sr) {
+  const real e = w.d - sd;
+  const real r = w.r - sr;
+  const real inv = bc_rcp<real>(e - w.lo);
+  const real ui = w.up * inv;
+  dst->d = inv;
+  dst->r = r;
+  sd = valid ? ui * e : sd;
+  sr = valid ? ui * r : sr;
+}
+
+// Rows j .. j + kGroup - 1, all inside the chain.
+template <typename real>
+__device__ __forceinline__ void eliminate_full(const HRow<real>* g, HRow<real>* Rj, real& sd, real& sr) {
 #pragma unroll
-  for (int i = 0; i < kGroup; ++i) {
-    const bool valid = j + i < j1;
-    const HRow<real>& w = g[i];
-    const real e = w.d - sd;
-    const real r = w.r - sr;
-    const real inv = bc_rcp<real>(e - w.lo);
-    const real ui = w.up * inv;
-    HRow<real>* dst = &R[valid ? j + i : dummy];
-    dst->d = inv;
-    dst->r = r;
-    sd = valid ? ui * e : sd;
-    sr = valid ? ui * r : sr;
-  }
+  for (int i = 0; i < kGroup; ++i) eliminate_row<real>(g[i], Rj + i, true, sd, sr);
+}
+
+// The chain's first rem < kGroup rows (j0 .. j0 + rem - 1).
+template <typename real>
+__device__ __forceinline__ void eliminate_head(const HRow<real>* g, int rem, HRow<real>* Rj, HRow<real>* Rd, real& sd,
+                                               real& sr) {
+#pragma unroll
+  for (int i = 0; i < kGroup - 1; ++i) eliminate_row<real>(g[i], (i < rem ? Rj : Rd) + i, i < rem, sd, sr);
 }
 
 template <typename real>
-__device__ __forceinline__ void eliminate_chain(int k, int dummy, const int4* CR, const int16_t* m_kid,
-                                                HRow<real>* R, const real* xv, KPair<real>* KK) {
-  const ChainRec c = unpack_chain(CR[k]);
-  // two row buffers used in turn: the next group is loaded into one while
-  // the other is eliminated
-  HRow<real> ga[kGroup], gb[kGroup];
+__device__ __forceinline__ void eliminate_chain(int k, int4 rec, int dummy, const int16_t* m_kid, HRow<real>* R,
+                                                const real* xv, KPair<real>* KK) {
+  const ChainRec c = unpack_chain(rec);
+  const int rem = (c.j1 - c.j0) & (kGroup - 1);
+  int j = c.j0 + rem;
+  // the head's rows and the first full group are loaded together; two
+  // buffers for the full groups are used in turn
+  HRow<real> gh[kGroup - 1], ga[kGroup], gb[kGroup];
 #pragma unroll
-  for (int i = 0; i < kGroup; ++i) ga[i] = R[c.j0 + i];
+  for (int i = 0; i < kGroup - 1; ++i) gh[i] = R[c.j0 + i];
+#pragma unroll
+  for (int i = 0; i < kGroup; ++i) ga[i] = R[j + i];
   // the top edge's axial term for the row the chain hangs from (zero for a root)
   const int tq = c.tp >= 0 ? c.tp : c.j1 - 1;
   const real top_defect = R[c.j1 - 1].up * (xv[c.j1 - 1] - xv[tq]);
@@ -340,17 +353,17 @@ __device__ __forceinline__ void eliminate_chain(int k, int dummy, const int4* CR
     sd += t.d;
     sr += t.r;
   }
-  for (int j = c.j0;;) {
+  eliminate_head<real>(gh, rem, R + c.j0, R + dummy, sd, sr);
+  while (j < c.j1) {
 #pragma unroll
     for (int i = 0; i < kGroup; ++i) gb[i] = R[j + kGroup + i];
-    eliminate_group<real>(ga, j, c.j1, dummy, R, sd, sr);
+    eliminate_full<real>(ga, R + j, sd, sr);
     j += kGroup;
     if (j >= c.j1) break;
 #pragma unroll
     for (int i = 0; i < kGroup; ++i) ga[i] = R[j + kGroup + i];
-    eliminate_group<real>(gb, j, c.j1, dummy, R, sd, sr);
+    eliminate_full<real>(gb, R + j, sd, sr);
     j += kGroup;
-    if (j >= c.j1) break;
   }
   KK[k].d = sd;
   KK[k].r = sr + top_defect;
@@ -364,44 +377,46 @@ __device__ __forceinline__ void eliminate_chain(int k, int dummy, const int4* CR
 // Every chain computes its (alpha, beta) at once, independently of the
 // others; the chains' X then follow from pointer jumping over the chain
 // tree.  R[j].d / R[j].r are overwritten with beta_j / alpha_j, and the
-// bottom row's pair is returned through KK[k] = (alpha, beta).
+// bottom row's pair is returned through KK[k] = (alpha, beta).  The walk is
+// top to bottom: a checked head of len mod kGroup rows at the top, then
+// full groups.
 template <typename real>
-__device__ __forceinline__ void affine_group(const HRow<real>* g, int j, int j0, int dummy, HRow<real>* R, real& al,
-                                             real& be) {
-#pragma unroll
-  for (int i = 0; i < kGroup; ++i) {
-    const bool valid = j - i >= j0;
-    const real a2 = (g[i].r - g[i].lo * al) * g[i].d;
-    const real b2 = -g[i].lo * be * g[i].d;
-    HRow<real>* dst = &R[valid ? j - i : dummy];
-    dst->d = b2;
-    dst->r = a2;
-    al = valid ? a2 : al;
-    be = valid ? b2 : be;
-  }
+__device__ __forceinline__ void affine_row(const HRow<real>& w, HRow<real>* dst, bool valid, real& al, real& be) {
+  const real a2 = (w.r - w.lo * al) * w.d;
+  const real b2 = -w.lo * be * w.d;
+  dst->d = b2;
+  dst->r = a2;
+  al = valid ? a2 : al;
+  be = valid ? b2 : be;
 }
 
 template <typename real>
 __device__ __forceinline__ void chain_affine(int k, int dummy, const int4* CR, HRow<real>* R, KPair<real>* KK) {
   const ChainRec c = unpack_chain(CR[k]);
   real al = real(0), be = c.tp >= 0 ? real(1) : real(0);
-  // top to bottom in groups, two alternating buffers; loads below the
-  // chain's first row land in other chains' rows or the padding before row
-  // 0 and are discarded
-  HRow<real> ga[kGroup], gb[kGroup];
+  const int rem = (c.j1 - c.j0) & (kGroup - 1);
+  int j = c.j1 - 1 - rem;  // top row of the first full group
+  HRow<real> gh[kGroup - 1], ga[kGroup], gb[kGroup];
 #pragma unroll
-  for (int i = 0; i < kGroup; ++i) ga[i] = R[c.j1 - 1 - i];
-  for (int j = c.j1 - 1;;) {
+  for (int i = 0; i < kGroup - 1; ++i) gh[i] = R[c.j1 - 1 - i];
+#pragma unroll
+  for (int i = 0; i < kGroup; ++i) ga[i] = R[j - i];
+  HRow<real>* const Rt = R + (c.j1 - 1);
+  HRow<real>* const Rd = R + dummy + (kGroup - 1);
+#pragma unroll
+  for (int i = 0; i < kGroup - 1; ++i) affine_row<real>(gh[i], (i < rem ? Rt : Rd) - i, i < rem, al, be);
+  while (j >= c.j0) {
 #pragma unroll
     for (int i = 0; i < kGroup; ++i) gb[i] = R[j - kGroup - i];
-    affine_group<real>(ga, j, c.j0, dummy, R, al, be);
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) affine_row<real>(ga[i], R + (j - i), true, al, be);
     j -= kGroup;
     if (j < c.j0) break;
 #pragma unroll
     for (int i = 0; i < kGroup; ++i) ga[i] = R[j - kGroup - i];
-    affine_group<real>(gb, j, c.j0, dummy, R, al, be);
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) affine_row<real>(gb[i], R + (j - i), true, al, be);
     j -= kGroup;
-    if (j < c.j0) break;
   }
   KK[k].d = al;
   KK[k].r = be;
@@ -417,11 +432,11 @@ __device__ __forceinline__ real row_solution(int row, const HRow<real>* R, const
   return w.r + w.d * X;
 }
 
-// Rows after the pack's rows: the first is the dummy row that rows past a
-// chain's end in a branch-free row group write their (unused) results to
-// (never read, so all threads share it); with the rest it pads the row array
-// so a chain walk may load up to 2 kGroup - 1 rows past its last row
-// without clamping (loads past a chain's end are discarded).
+// Rows after the pack's rows: the first kGroup are the dummy rows that head
+// rows outside their chain write their (unused) results to (never read, so
+// all threads share them); with the rest they pad the row array so a chain
+// walk may load up to 2 kGroup - 1 rows past its last row without clamping
+// (loads past a chain's end are discarded).
 constexpr int kDummySlots = 8;
 // Padding rows before row 0: a downward chain walk may load up to
 // 2 kGroup - 1 rows below its first row.
@@ -582,19 +597,30 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       const int n_leaf = m_lv[1] - m_lv[0];
       int leaf = tid;
       int k = leaf < n_leaf ? m_lv[0] + leaf : -1;
+      int4 rec = CR[k >= 0 ? k : 0];
       while (k >= 0) {
-        eliminate_chain<real>(k, dummy, CR, m_kid, R, xv, KK);
+        // the records of both possible next chains (the parent, or this
+        // thread's next leaf) are read before the chain is eliminated, so
+        // only the arrival counter's round trip separates two chains of a
+        // climb; chain records and parents are not written in this phase
         const int p = m_par[k];
+        const int nl = leaf + NT;
+        const int kl = nl < n_leaf ? m_lv[0] + nl : -1;
+        const int4 rec_p = CR[p >= 0 ? p : k];
+        const int4 rec_l = CR[kl >= 0 ? kl : k];
+        eliminate_chain<real>(k, rec, dummy, m_kid, R, xv, KK);
         bool climb = false;
         if (p >= 0) {
-          const int nk = (int)((unsigned)CR[p].y >> 16);
+          const int nk = (int)((unsigned)rec_p.y >> 16);
           climb = __hip_atomic_fetch_add(&m_cnt[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == nk - 1;
         }
         if (climb) {
           k = p;
+          rec = rec_p;
         } else {
-          leaf += NT;
-          k = leaf < n_leaf ? m_lv[0] + leaf : -1;
+          leaf = nl;
+          k = kl;
+          rec = rec_l;
         }
       }
     }
