@@ -269,9 +269,8 @@ __device__ __forceinline__ void wave_sync() {
 // terms from their elimination instead).  Branch-free: a bottom row reads
 // its own slot in place of row - 1 and discards it.
 template <typename real>
-__device__ __forceinline__ real axial_defect(int row, real V, const HRow<real>* R, const real* xv,
-                                             const int16_t* m_link) {
-  const unsigned u = (unsigned)(uint16_t)m_link[row];
+__device__ __forceinline__ real axial_defect(int row, real V, const HRow<real>* R, const real* xv, uint32_t link) {
+  const unsigned u = link & 0xFFFFu;
   const int par = (int)(u & 0x7FFFu);
   const bool bottom = (u & 0x8000u) != 0;
   const int below = bottom ? row : row - 1;
@@ -290,6 +289,10 @@ __device__ __forceinline__ real axial_defect(int row, real V, const HRow<real>* 
 // any check.  Loads may reach up to 2 kGroup - 1 rows past either end of a
 // chain, which the padding rows before row 0 and after the last row cover.
 constexpr int kGroup = 4;
+
+// Boundary rows per thread whose metadata is kept in registers (more are
+// handled by a loop that reads it from global memory).
+constexpr int kBndRegs = 1;
 
 // Forward elimination in the delta-V form.  (sd, sr) is what the rows below
 // subtract from the current row: the kids' contributions at the bottom,
@@ -313,12 +316,20 @@ __device__ __forceinline__ void eliminate_full(const HRow<real>* g, HRow<real>* 
   for (int i = 0; i < kGroup; ++i) eliminate_row<real>(g[i], Rj + i, true, sd, sr);
 }
 
-// The chain's first rem < kGroup rows (j0 .. j0 + rem - 1).
+// The chain's first rem < kGroup rows (j0 .. j0 + rem - 1).  The wavefront
+// computes as many head rows as the largest rem among its active lanes (a
+// uniform count, so a lane alone on the critical path computes only its own
+// head rows).
 template <typename real>
 __device__ __forceinline__ void eliminate_head(const HRow<real>* g, int rem, HRow<real>* Rj, HRow<real>* Rd, real& sd,
                                                real& sr) {
-#pragma unroll
-  for (int i = 0; i < kGroup - 1; ++i) eliminate_row<real>(g[i], (i < rem ? Rj : Rd) + i, i < rem, sd, sr);
+  static_assert(kGroup == 4, "the head holds at most three rows");
+  const bool any1 = __ballot(rem >= 1) != 0;
+  const bool any2 = __ballot(rem >= 2) != 0;
+  const bool any3 = __ballot(rem >= 3) != 0;
+  if (any1) eliminate_row<real>(g[0], (rem > 0 ? Rj : Rd) + 0, rem > 0, sd, sr);
+  if (any2) eliminate_row<real>(g[1], (rem > 1 ? Rj : Rd) + 1, rem > 1, sd, sr);
+  if (any3) eliminate_row<real>(g[2], (rem > 2 ? Rj : Rd) + 2, rem > 2, sd, sr);
 }
 
 template <typename real>
@@ -423,11 +434,11 @@ __device__ __forceinline__ void chain_affine(int k, int dummy, const int4* CR, H
 }
 
 // Solution of one row after the affine pass and the pointer jumping.
+// meta's high half is the chain the row's chain hangs from (0xFFFF: none).
 template <typename real>
-__device__ __forceinline__ real row_solution(int row, const HRow<real>* R, const int16_t* m_rowch,
-                                             const int16_t* m_par, const KPair<real>* Xb) {
-  const int p = m_par[m_rowch[row]];
-  const real X = p >= 0 ? Xb[p].d : real(0);
+__device__ __forceinline__ real row_solution(int row, const HRow<real>* R, uint32_t meta, const KPair<real>* Xb) {
+  const uint32_t p = meta >> 16;
+  const real X = p != 0xFFFFu ? Xb[p].d : real(0);
   const HRow<real> w = R[row];
   return w.r + w.d * X;
 }
@@ -551,6 +562,35 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   for (int i = tid; i < P.n_bnd; i += NT) xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu] = a.Vbnd[P.bnd0 + i];
   __syncthreads();
 
+  // Static per-row metadata the steps need, kept in registers: for every CV
+  // its slot link (low half) and the chain its row's solution hangs from
+  // (high half, 0xFFFF for a root chain); for the first kBndRegs boundary
+  // rows of the thread the link (bl) and the row (bits 0-14), the clamp bit
+  // (bit 15) and the chain its solution hangs from (high half) (bx).  The steps then read each operand in one LDS round trip for
+  // all of the thread's rows.
+  uint32_t aux[MAXC];
+#pragma unroll
+  for (int i = 0; i < MAXC; ++i) {
+    aux[i] = 0;
+    if (info[i] != 0xFFFFFFFFu) {
+      const int row = (int)(info[i] & kRowMask);
+      aux[i] = (uint32_t)(uint16_t)m_link[row] | ((uint32_t)(uint16_t)m_par[m_rowch[row]] << 16);
+    }
+  }
+  uint32_t bx[kBndRegs], bl[kBndRegs];
+#pragma unroll
+  for (int b = 0; b < kBndRegs; ++b) {
+    const int i = tid + b * NT;
+    bx[b] = 0;
+    bl[b] = 0;
+    if (i < P.n_bnd) {
+      const uint32_t u = a.bnd_info[P.bnd0 + i];
+      const int row = (int)(u & 0x7FFFFFFFu);
+      bx[b] = (uint32_t)row | ((u & kClampBit) ? 0x8000u : 0u) | ((uint32_t)(uint16_t)m_par[m_rowch[row]] << 16);
+      bl[b] = (uint32_t)(uint16_t)m_link[row];
+    }
+  }
+
   const int dummy = (a.max_rows + 3) & ~3;
   const real dt = a.dt;
   const real ca_b = dt * bc_exprel<real>(-dt * a.cc.ca_tau_inv);
@@ -558,7 +598,17 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   for (int step = 0; step < a.n_steps; ++step) {
     const int gstep = a.step0 + step;
     if (a.rec) record_state<real, MAXC, NS>(a, P, tid, NT, step, vv, st, info);
-    // ---- phase A: linearise at V_n (midpoint rows)
+    // ---- phase A: linearise at V_n.  Every operand is read before the
+    // first row is written, so the reads of all rows share their latency.
+    real dA[MAXC], dB[kBndRegs];
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i)
+      dA[i] = info[i] != 0xFFFFFFFFu ? axial_defect<real>((int)(info[i] & kRowMask), vv[i], R, xv, aux[i]) : real(0);
+#pragma unroll
+    for (int b = 0; b < kBndRegs; ++b) {
+      const int row = (int)(bx[b] & 0x7FFFu);
+      dB[b] = tid + b * NT < P.n_bnd ? axial_defect<real>(row, xv[row], R, xv, bl[b]) : real(0);
+    }
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       if (info[i] != 0xFFFFFFFFu) {
@@ -566,18 +616,25 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         const real* cp = cls + ((info[i] >> 16) & 0x3FFFu) * BC_NCLS;
         real G, GE;
         ica[i] = linearise<real, CH>(a.cc, cp, st[i], vv[i], G, GE);
-        real Icl = (info[i] & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0);
+        const real Icl = (info[i] & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0);
         const real dtcm = cp[BC_CLS_DTCM];
-        const real D = axial_defect<real>(row, vv[i], R, xv, m_link);
         R[row].d = real(1) + dtcm * G;
-        R[row].r = dtcm * (GE + Icl) + D;
+        R[row].r = dtcm * (GE + Icl) + dA[i];
       }
     }
     // boundary (algebraic) rows
-    for (int i = tid; i < P.n_bnd; i += NT) {
+#pragma unroll
+    for (int b = 0; b < kBndRegs; ++b) {
+      if (tid + b * NT < P.n_bnd) {
+        const int row = (int)(bx[b] & 0x7FFFu);
+        R[row].d = real(0);
+        R[row].r = ((bx[b] & 0x8000u) ? clamp_current<real>(a, P, row, gstep) : real(0)) + dB[b];
+      }
+    }
+    for (int i = tid + kBndRegs * NT; i < P.n_bnd; i += NT) {
       uint32_t b = a.bnd_info[P.bnd0 + i];
       int row = (int)(b & 0x7FFFFFFFu);
-      const real D = axial_defect<real>(row, xv[row], R, xv, m_link);
+      const real D = axial_defect<real>(row, xv[row], R, xv, (uint32_t)(uint16_t)m_link[row]);
       R[row].d = real(0);
       R[row].r = ((b & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0)) + D;
     }
@@ -594,20 +651,26 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     // critical path is the longest leaf-to-root path.  Chain values do not
     // depend on which thread takes a chain: results are deterministic.
     {
-      const int n_leaf = m_lv[1] - m_lv[0];
+      const int lv0 = m_lv[0];
+      const int n_leaf = m_lv[1] - lv0;
       int leaf = tid;
-      int k = leaf < n_leaf ? m_lv[0] + leaf : -1;
+      int k = leaf < n_leaf ? lv0 + leaf : -1;
       int4 rec = CR[k >= 0 ? k : 0];
+      int p = m_par[k >= 0 ? k : 0];
       while (k >= 0) {
-        // the records of both possible next chains (the parent, or this
-        // thread's next leaf) are read before the chain is eliminated, so
-        // only the arrival counter's round trip separates two chains of a
-        // climb; chain records and parents are not written in this phase
-        const int p = m_par[k];
+        // the record and parent of both possible next chains (the parent,
+        // or this thread's next leaf) are read before the chain is
+        // eliminated, so only the arrival counter's round trip separates two
+        // chains of a climb; chain records and parents are not written in
+        // this phase
         const int nl = leaf + NT;
-        const int kl = nl < n_leaf ? m_lv[0] + nl : -1;
-        const int4 rec_p = CR[p >= 0 ? p : k];
-        const int4 rec_l = CR[kl >= 0 ? kl : k];
+        const int kl = nl < n_leaf ? lv0 + nl : -1;
+        const int pq = p >= 0 ? p : k;
+        const int lq = kl >= 0 ? kl : k;
+        const int4 rec_p = CR[pq];
+        const int pp = m_par[pq];
+        const int4 rec_l = CR[lq];
+        const int pl = m_par[lq];
         eliminate_chain<real>(k, rec, dummy, m_kid, R, xv, KK);
         bool climb = false;
         if (p >= 0) {
@@ -617,10 +680,12 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         if (climb) {
           k = p;
           rec = rec_p;
+          p = pp;
         } else {
           leaf = nl;
           k = kl;
           rec = rec_l;
+          p = pl;
         }
       }
     }
@@ -676,17 +741,28 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
 
 
-    // ---- phase C: V += dV; Ca pool and gates at V_{n+1}
-    for (int i = tid; i < P.n_bnd; i += NT) {
-      const int row = (int)(a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu);
-      xv[row] += row_solution<real>(row, R, m_rowch, m_par, Xb);
+    // ---- phase C: V += dV; Ca pool and gates at V_{n+1}.  All row
+    // solutions are read before the new voltages are written.
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i)
+      dA[i] = info[i] != 0xFFFFFFFFu ? row_solution<real>((int)(info[i] & kRowMask), R, aux[i], Xb) : real(0);
+#pragma unroll
+    for (int b = 0; b < kBndRegs; ++b) {
+      const int row = (int)(bx[b] & 0x7FFFu);
+      dB[b] = tid + b * NT < P.n_bnd ? xv[row] + row_solution<real>(row, R, bx[b], Xb) : real(0);
     }
+    for (int i = tid + kBndRegs * NT; i < P.n_bnd; i += NT) {
+      const int row = (int)(a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu);
+      xv[row] += row_solution<real>(row, R, (uint32_t)(uint16_t)m_par[m_rowch[row]] << 16, Xb);
+    }
+#pragma unroll
+    for (int b = 0; b < kBndRegs; ++b)
+      if (tid + b * NT < P.n_bnd) xv[bx[b] & 0x7FFFu] = dB[b];
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       if (info[i] != 0xFFFFFFFFu) {
-        const int row = (int)(info[i] & kRowMask);
-        const real Vn = vv[i] + row_solution<real>(row, R, m_rowch, m_par, Xb);
-        xv[row] = Vn;
+        const real Vn = vv[i] + dA[i];
+        xv[info[i] & kRowMask] = Vn;
         advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt, ca_b, a.flags);
         vv[i] = Vn;
         bad |= (Vn == Vn && fabs(Vn) < real(1e30)) ? 0u : 1u;
