@@ -22,22 +22,10 @@ from .channels import IONS
 from .filter import resolve_region
 from .mech import CableProperty, Channel, CurrentClamp, Ion
 from .model import CellModel
+from .policy import CVPerBranch, CVPolicy
 from .quad import get_integrator
 
 __all__ = ["Cell", "CellBatch", "CVPerBranch", "RunResult"]
-
-
-@dataclass(frozen=True)
-class CVPerBranch:
-    """Same number of CVs on every branch (``_discretization/policy.py:84-122``)."""
-
-    cv_per_branch: int = 1
-
-    def __post_init__(self):
-        if isinstance(self.cv_per_branch, bool) or not isinstance(self.cv_per_branch, int):
-            raise TypeError(f"cv_per_branch must be integer, got {self.cv_per_branch!r}.")
-        if self.cv_per_branch <= 0:
-            raise ValueError(f"cv_per_branch must be > 0, got {self.cv_per_branch!r}.")
 
 
 @dataclass
@@ -53,6 +41,8 @@ class Cell:
                  V_th: float = 0.0, ion_channel_update_order: str = "family", device="cuda", name=None):
         self.morpho = morpho
         self.cv_policy = cv_policy or CVPerBranch(1)
+        if not isinstance(self.cv_policy, CVPolicy):
+            raise TypeError(f"cv_policy must be a CVPolicy, got {type(self.cv_policy).__name__}")
         self._solver = get_integrator(solver)
         self.precision = int(precision)
         self.V_th = float(V_th)
@@ -142,7 +132,8 @@ class Cell:
         ca = None
         if ca_params is not None:
             ca = {"mask_branch": ca_mask, **{k: ca_params[k] for k in ("d", "tau", "C_rest", "Co", "Ci0", "temp")}}
-        return CellModel(morph=mo, cv_per_branch=self.cv_policy.cv_per_branch, cm=cm, ra=ra, V0=float(v0[0]),
+        counts = self.cv_policy.counts(mo, cm_branch=cm, ra_branch=ra)
+        return CellModel(morph=mo, cv_per_branch=counts, cm=cm, ra=ra, V0=float(v0[0]),
                          channels=channels, ca=ca, clamps=clamps, **E).validate()
 
     # ----------------------------------------------------------------- state

@@ -168,6 +168,19 @@ def _cv_geometry_general(br, n, ra):
     return area, rprox, rdist
 
 
+def _owning_cv(x: float, n: int) -> int:
+    """CV k of n uniform CVs owns [k/n, (k+1)/n) with the reference's
+    tolerance (``node_build.py:396-433``)."""
+    if x <= EPS_PARAM:
+        return 0
+    if x >= 1.0 - EPS_PARAM:
+        return n - 1
+    for k in range(n):
+        if float(k) / n - EPS_PARAM <= x < float(k + 1) / n - EPS_PARAM:
+            return k
+    raise ValueError(f"x={x} lies in no CV")
+
+
 def discretize_cell(morph, n_cv_branch, cm_branch, ra_branch) -> CellGeometry:
     """Discretise one cell and build its reference-ordered DHS rows."""
     nb = morph.n_branches
@@ -177,10 +190,6 @@ def discretize_cell(morph, n_cv_branch, cm_branch, ra_branch) -> CellGeometry:
     for b in range(1, nb):
         if morph.parent[b] >= b:
             raise ValueError("branches must be inserted after their parent")
-        px = morph.parent_x[b]
-        if EPS_PARAM < px < 1.0 - EPS_PARAM:
-            raise NotImplementedError("attachment at a branch interior (0 < parent_x < 1) is not supported "
-                                      "by the flat layout; attach children at x=0 or x=1")
     C = int(ncv.sum())
     cv_first = np.concatenate(([0], np.cumsum(ncv)[:-1]))
     cv_branch = np.repeat(np.arange(nb), ncv)
@@ -199,7 +208,10 @@ def discretize_cell(morph, n_cv_branch, cm_branch, ra_branch) -> CellGeometry:
         a, rp_, rd_ = _cv_geometry_general(morph.branches[b], n, float(ra_branch[b]))
         area[s_:s_ + n], rprox[s_:s_ + n], rdist[s_:s_ + n] = a, rp_, rd_
 
-    # ---- node tree (node_build.py:58-284), endpoint attachments only
+    # ---- node tree (node_build.py:58-284).  A child attached inside its
+    # parent branch (0 < parent_x < 1) hangs from the midpoint node of the
+    # parent CV that owns parent_x under the half-open tiling
+    # (node_build.py:396-433); that node stays the CV's dynamic row.
     P = 1 + C + nb
     node_parent = np.full(P, -1, dtype=np.int64)
     node_is_mid = np.zeros(P, dtype=bool)
@@ -207,7 +219,9 @@ def discretize_cell(morph, n_cv_branch, cm_branch, ra_branch) -> CellGeometry:
     cx = np.asarray(morph.child_x, dtype=np.float64)
     px = np.asarray(morph.parent_x, dtype=np.float64)
     par = np.asarray(morph.parent, dtype=np.int64)
-    if np.all(cx <= EPS_PARAM):
+    interior = (px > EPS_PARAM) & (px < 1.0 - EPS_PARAM)
+    interior[0] = False
+    if np.all(cx <= EPS_PARAM) and not np.any(interior):
         base = 1 + np.concatenate(([0], np.cumsum(ncv + 1)[:-1]))
         term = base + ncv
         mid0 = base
@@ -241,7 +255,10 @@ def discretize_cell(morph, n_cv_branch, cm_branch, ra_branch) -> CellGeometry:
             if par[b] < 0:
                 a_node, ax = 0, 0.0
             else:
-                a_node = endpoint[par[b], 0] if px[b] <= EPS_PARAM else endpoint[par[b], 1]
+                if interior[b]:
+                    a_node = int(cv_to_node[cv_first[par[b]] + _owning_cv(float(px[b]), int(ncv[par[b]]))])
+                else:
+                    a_node = endpoint[par[b], 0] if px[b] <= EPS_PARAM else endpoint[par[b], 1]
                 ax = float(cx[b])
             ordered = ids if ax <= EPS_PARAM else ids[::-1]
             r_entry = rprox if ax <= EPS_PARAM else rdist
@@ -272,7 +289,15 @@ def discretize_cell(morph, n_cv_branch, cm_branch, ra_branch) -> CellGeometry:
         peel[t_node] = pt
         n = int(ncv[b])
         nodes = np.arange(mid0[b], mid0[b] + n)
-        peel[nodes] = pt + (n - np.arange(n))
+        if any(int(m) in best_child for m in nodes.tolist()):
+            # children attached at this branch's midpoints
+            v = pt
+            for i in range(n - 1, -1, -1):
+                m = int(nodes[i])
+                v = max(v, best_child.get(m, -1)) + 1
+                peel[m] = v
+        else:
+            peel[nodes] = pt + (n - np.arange(n))
         a = int(attach[b])
         best_child[a] = max(best_child.get(a, -1), int(peel[mid0[b]]))
     peel[0] = best_child[0] + 1 if 0 in best_child else 0

@@ -94,6 +94,14 @@ class Morphology:
     def from_root(cls, branch: Branch) -> "Morphology":
         return cls([branch], [-1], [0.0], [0.0])
 
+    @classmethod
+    def from_swc(cls, path, *, mode: str = "neuron") -> "Morphology":
+        """Read an SWC file (``morph/morphology.py`` ``from_swc``; the reader
+        is :mod:`braincell_amd.swc`)."""
+        from .swc import read_swc
+
+        return read_swc(path, mode=mode)
+
     def attach(self, parent: int, child: Branch, *, parent_x: float = 1.0, child_x: float = 0.0) -> int:
         """Append ``child`` under branch ``parent``; returns the new branch index."""
         if not (0 <= int(parent) < len(self.branches)):

@@ -128,3 +128,58 @@ def test_update_orders_match_oracle_fp64(cuda, order):
     other.reset_state(m.V0)
     alt = other.run(0.025, n, probe_cvs=[soma_cv(m)])[:, 0]
     assert np.max(np.abs(alt - ref)) > 1e-3
+
+
+@pytest.mark.parametrize("n_cv", [1, 3, 4])
+def test_interior_attachment_fp64_matches_oracle(cuda, n_cv):
+    """Children attached inside their parent branch (0 < parent_x < 1),
+    at x = 0 and with a reversed walk: all CVs against the oracle."""
+    from tests.helpers import interior_cell
+
+    models = [interior_cell(s, n_cv) for s in range(3)]
+    steps = 1200
+    got, _ = run_gpu(models, steps, precision=64, probes="all")
+    col = 0
+    for m in models:
+        ref, _ = run_oracle(m, steps)
+        n = ref.shape[1]
+        err = np.abs(got[:, col:col + n] - ref).max()
+        print(f"interior cell ({n} CVs): fp64 max |dV| {err:.2e} mV, V range {ref.min():.1f}..{ref.max():.1f}")
+        assert err < 1e-5  # FP64 rounding amplified by spike upstrokes, as above
+        col += n
+    assert ref.max() > 0.0  # the clamped soma spikes
+
+
+def _n144_cell():
+    import os
+
+    from braincell_amd.model import CellModel
+    from braincell_amd.swc import read_swc
+
+    morph = read_swc(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "swc", "n144.swc"))
+    nb = morph.n_branches
+    return CellModel(
+        morph=morph, cv_per_branch=4, cm=1.0, ra=1000.0, V0=-65.0,
+        channels={"Na_Ba2002": {"g_branch": np.full(nb, 0.12), "params": {}},
+                  "KDR_Ba2002": {"g_branch": np.full(nb, 0.036), "params": {}},
+                  "IL": {"g_branch": np.full(nb, 0.0003), "params": {"E": -54.3}}},
+        clamps=[(0, 0.85, 3.0, 2.0, 0.7)],
+    ).validate()
+
+
+def test_swc_cell_matches_oracle(cuda):
+    """The reference benchmark's 868-CV morphology (n144.swc, 4 CVs per
+    branch, children at the soma's midpoint) with Na/KDR/leak: FP64 all CVs
+    and FP32 soma against the oracle over 20 ms."""
+    m = _n144_cell()
+    steps = 800
+    ref, _ = run_oracle(m, steps)
+    assert ref.shape[1] == 868
+    got64, _ = run_gpu([m], steps, precision=64, probes="all")
+    err = np.abs(got64 - ref).max()
+    cv = soma_cv(m)
+    got32, _ = run_gpu([m], steps, precision=32, probes="soma")
+    e32 = rmse(got32[:, 0], ref[:, cv])
+    print(f"n144: fp64 max |dV| {err:.2e} mV, fp32 soma RMSE {e32:.2e} mV, soma peak {ref[:, cv].max():.1f} mV")
+    assert err < 1e-5
+    assert e32 <= 0.05

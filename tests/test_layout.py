@@ -157,3 +157,49 @@ def test_slots_are_a_permutation_of_reference_rows():
     for p in range(L.n_packs):
         sr = L.slot_rows[p]
         assert sorted(sr.tolist()) == list(range(L.pack_array[p, 3]))
+
+
+@pytest.mark.parametrize("n_cv", [1, 3, 4])
+def test_interior_attachment_matches_oracle(n_cv):
+    from tests.helpers import interior_cell
+
+    m = interior_cell(0, n_cv)
+    g = discretize_cell(m.morph, m.n_cv_branch(), m.per_branch(m.cm), m.per_branch(m.ra))
+    cvs, _, tree, sched, st = OD.discretize(m.morph, n_cv, 1.0, 100.0)
+    assert np.array_equal(g.row_to_point_id, sched.row_to_point_id)
+    assert np.array_equal(g.parent_rows, sched.parent_rows)
+    assert np.array_equal(g.dynamic_rows, st.dynamic_rows)
+    parent, children = OD.node_parent_children(tree)
+    assert np.array_equal(g.peel, OD.compute_peel_levels(parent, children))
+    for a, b in [(g.diag_axial, st.diag), (g.lowers, st.lowers), (g.uppers, st.uppers), (g.row_cap_uF, st.row_cap_uF)]:
+        np.testing.assert_allclose(a, b, rtol=1e-13, atol=0)
+    # the interior attachment points are CV midpoints, which stay dynamic rows
+    assert g.row_is_mid.sum() == g.n_cv
+
+
+def test_interior_attachment_packed_hines_matches_oracle_dhs():
+    """The kernel's Hines phase (CPU transcription) on packs that mix
+    interior-attachment cells with others equals the oracle's DHS solve."""
+    from tests.helpers import interior_cell
+
+    models = [interior_cell(s, n) for s, n in ((0, 4), (1, 3), (2, 1))] + [W.uneven_tree_cell(3, cmin=200, cmax=300)]
+    L = StaggerLayout(models, rows_per_pack=600, cv_per_thread=2)
+    dt = 0.025
+    rng = np.random.default_rng(1)
+    G = rng.uniform(0, 1, L.n_cv_total)
+    V = rng.uniform(-80, 0, L.n_cv_total)
+    I = rng.uniform(-5, 5, L.n_cv_total)
+    rows, brow = cv_rows(L), bnd_rows(L)
+    bd = np.zeros(L.n_rows_total)
+    br = np.zeros(L.n_rows_total)
+    xv = np.zeros(L.n_rows_total)
+    bd[rows] = 1 + dt * G
+    br[rows] = dt * I
+    xv[rows] = V
+    xv[brow] = rng.uniform(-80, 0, len(brow))
+    dV = hines_delta(L, dt, bd, br, xv)
+    for i, m in enumerate(models):
+        st = StaggeredOracle(m).st
+        sl = L.cell_slice(i)
+        Vn = dhs_solve(st, V[sl][None], (-G[sl])[None], (I[sl] + G[sl] * V[sl])[None], dt)[0]
+        np.testing.assert_allclose(V[sl] + dV[rows[sl]], Vn, rtol=0, atol=1e-9)
