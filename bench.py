@@ -48,7 +48,7 @@ def cpu_baseline(models, soma_cvs, n_steps, gpu_soma, dt):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("cells", "point", "adjoint", "closure"), default="cells",
+    ap.add_argument("--workload", choices=("cells", "point", "adjoint", "closure", "n144"), default="cells",
                     help="cells: 4096 uneven trees (default, the headline); point: 100k HTC point neurons; "
                          "adjoint: 512 region trees, forward + checkpointed adjoint; "
                          "closure: learned gating closure at dt 0.1 vs plain dt 0.025 on the 4096 trees")
@@ -77,6 +77,8 @@ def main():
         return bench_adjoint(args)
     if args.workload == "closure":
         return bench_closure(args)
+    if args.workload == "n144":
+        return bench_n144(args)
     return bench_cells(args)
 
 
@@ -372,6 +374,80 @@ def bench_closure(args):
                      "one_step_mse_ratio": [float(f"{x:.3g}") for x in e1 / e0]},
         "timing": {"closure_s_per_sim_ms": t_closure, "plain_dt0.025_s_per_sim_ms": t_plain,
                    "data_s": round(t_data, 2), "train_s": round(t_train, 2)}}), flush=True)
+
+
+
+def bench_n144(args):
+    """The reference's own benchmark workload (``examples/profiling/
+    simulator_compare``): N copies of the 868-CV n144 Hodgkin-Huxley cell with
+    clamp amplitudes 0.69 ... 0.71 nA, 800 steps of 0.025 ms, three voltage
+    probes per cell (branches 0, 34, 54 at x = 0.85); metric
+    cell_steps_per_second = N * 800 / seconds (``common.py:85-88``), timing
+    state restoration, the 800 updates and the trace."""
+    import torch
+    from braincell_amd import workloads as W
+    from braincell_amd.layout import discretize_cell
+    from braincell_amd.runtime import StaggerLayout, StaggerRuntime
+
+    n = args.cells if args.cells != 4096 else 10000
+    dt, steps, chunk = 0.025, 800, 200
+    amps = W.n144_amplitudes(n)
+    models = [W.n144_hh_cell(a) for a in amps]
+    m0 = models[0]
+    g = discretize_cell(m0.morph, m0.n_cv_branch(), m0.per_branch(m0.cm), m0.per_branch(m0.ra))
+    probes = [g.locate(b, x, m0.n_cv_branch())[1] for b, x in W.N144_PROBES]
+    t0 = time.perf_counter()
+    L = StaggerLayout(models, probes=probes, threads=256)
+    log(f"layout {L.summary()} built in {time.perf_counter() - t0:.1f}s")
+    if not args.no_autotune:
+        from braincell_amd.autotune import autotune
+
+        L, _ = autotune(L, dt, precision=32, log=log)
+    rt = StaggerRuntime(L, dt, precision=32)
+    trace = rt.new_trace(steps)
+    graph = rt.make_graph(steps // chunk, chunk, trace)
+    stream = rt.stream_handle()
+    rt.reset_state()
+    rt.save_state()
+    samples = []
+    for rep_ in range(1 + 5):  # one warm-up, then the median of five
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        rt.restore_state()  # state restoration on the device
+        graph.launch(stream)
+        torch.cuda.synchronize()
+        if rep_:
+            samples.append(time.perf_counter() - t1)
+    rt.check_status()
+    med = float(np.median(samples))
+    csps = n * steps / med
+    cu = L.n_cv_total * steps / med
+    log(f"{n} cells x {steps} steps: median {med * 1e3:.2f} ms -> {csps:.4g} cell-steps/s, {cu:.4g} compartment-updates/s")
+    # accuracy: the first cell's three probes against the FP64 oracle
+    from oracle.staggered import StaggeredOracle
+
+    o = StaggeredOracle(m0)
+    o.reset_state(m0.V0)
+    t2 = time.perf_counter()
+    ref = o.run(dt, steps, probe_cvs=probes)
+    cpu_s = time.perf_counter() - t2
+    got = trace[:, :3].double().cpu().numpy()
+    rms = [float(np.sqrt(np.mean((got[:, k] - ref[:, k]) ** 2))) for k in range(3)]
+    out = {
+        "metric": "cell_steps_per_second", "value": float(f"{csps:.4g}"), "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": steps, "ms_per_step": round(med / steps * 1e3, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"{n} x n144.swc Hodgkin-Huxley cells (868 CVs each), 6.3 degC, dt 0.025 ms, 800 steps, "
+                               f"3 probes per cell", "cells": n, "compartments": int(L.n_cv_total),
+                   "packs": L.n_packs, "threads": L.threads, "cv_per_thread": L.cv_per_thread,
+                   "rows_per_pack": L.rows_per_pack, "compartment_updates_per_s": float(f"{cu:.4g}"),
+                   "parallelism": "1 GPU, one workgroup per pack"},
+        "cpu_baseline": {"value": float(f"{steps / cpu_s:.4g}"), "unit": "cell-steps/s", "cores": 1, "kind": "port",
+                         "sample": f"NumPy FP64 oracle, one cell x {steps} steps, {cpu_s:.2f}s"},
+        "accuracy": {"probe_rmse_mV": [float(f"{r:.3g}") for r in rms], "reference": "FP64 oracle, cell 0",
+                     "gate": "RMSE <= 2 mV (simulator_compare/common.py:41-45)"},
+    }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

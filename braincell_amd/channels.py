@@ -56,6 +56,14 @@ CHANNELS = {
         1 << 6, "KCa", (("p", 2, "ab"),),
         {"g_max": 0.01, "n": 2.0, "alpha": 48.0, "beta": 0.09, "phi": 1.0},
         "braincell/channel/potassium_calcium.py:78-185"),
+    "Na_HH1952": ChannelInfo(
+        1 << 9, "Na", (("p", 3, "ab"), ("q", 1, "ab")),
+        {"g_max": 0.12, "temp": celsius2kelvin(36.0), "q10": 3.0, "temp_ref": celsius2kelvin(36.0), "V_sh": -45.0},
+        "braincell/channel/sodium.py:293-398"),
+    "K_HH1952": ChannelInfo(
+        1 << 10, "K", (("p", 4, "ab"),),
+        {"g_max": 0.01, "temp": celsius2kelvin(36.0), "q10": 3.0, "temp_ref": celsius2kelvin(36.0), "V_sh": -45.0},
+        "braincell/channel/potassium.py:341-456"),
     "HCN_HM1992": ChannelInfo(
         1 << 7, "none", (("p", 1, "it"),),
         {"g_max": 0.01, "E": 43.0, "temp": celsius2kelvin(36.0), "q10": 1.0, "temp_ref": celsius2kelvin(36.0)},

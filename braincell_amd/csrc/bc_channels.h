@@ -27,6 +27,10 @@
 #define BC_CH_AHP  (1u << 6)
 #define BC_CH_HCN  (1u << 7)
 #define BC_CH_POOL (1u << 8)
+#define BC_CH_NAHH (1u << 9)  // Na_HH1952: the Na gate slots, density column and constants
+#define BC_CH_KHH  (1u << 10) // K_HH1952: the KDR gate slot, density column and constants
+#define BC_CH_NA_ANY (BC_CH_NA | BC_CH_NAHH)
+#define BC_CH_K4_ANY (BC_CH_KDR | BC_CH_KHH)
 
 // Mechanism-class table columns (per painted region of a cell).
 #define BC_CLS_DTCM   0   // dt / Cm  (ms cm^2 / uF)
@@ -43,13 +47,13 @@
 
 // Number of gate/pool states a channel set carries, and their slots.
 __host__ __device__ constexpr int bc_nstates(uint32_t ch) {
-  return ((ch & BC_CH_NA) ? 2 : 0) + ((ch & BC_CH_KDR) ? 1 : 0) + ((ch & BC_CH_CAL) ? 2 : 0) +
+  return ((ch & BC_CH_NA_ANY) ? 2 : 0) + ((ch & BC_CH_K4_ANY) ? 1 : 0) + ((ch & BC_CH_CAL) ? 2 : 0) +
          ((ch & BC_CH_CAT) ? 2 : 0) + ((ch & BC_CH_AHP) ? 1 : 0) + ((ch & BC_CH_HCN) ? 1 : 0) +
          ((ch & BC_CH_POOL) ? 1 : 0);
 }
 __host__ __device__ constexpr int bc_slot_na(uint32_t) { return 0; }
-__host__ __device__ constexpr int bc_slot_kdr(uint32_t ch) { return ((ch & BC_CH_NA) ? 2 : 0); }
-__host__ __device__ constexpr int bc_slot_cal(uint32_t ch) { return bc_slot_kdr(ch) + ((ch & BC_CH_KDR) ? 1 : 0); }
+__host__ __device__ constexpr int bc_slot_kdr(uint32_t ch) { return ((ch & BC_CH_NA_ANY) ? 2 : 0); }
+__host__ __device__ constexpr int bc_slot_cal(uint32_t ch) { return bc_slot_kdr(ch) + ((ch & BC_CH_K4_ANY) ? 1 : 0); }
 __host__ __device__ constexpr int bc_slot_cat(uint32_t ch) { return bc_slot_cal(ch) + ((ch & BC_CH_CAL) ? 2 : 0); }
 __host__ __device__ constexpr int bc_slot_ahp(uint32_t ch) { return bc_slot_cat(ch) + ((ch & BC_CH_CAT) ? 2 : 0); }
 __host__ __device__ constexpr int bc_slot_hcn(uint32_t ch) { return bc_slot_ahp(ch) + ((ch & BC_CH_AHP) ? 1 : 0); }
@@ -185,6 +189,24 @@ __device__ __forceinline__ void bc_kdr_rates(real V, real vsh, real& ap, real& b
   real Vp = V - vsh;
   ap = real(0.032 * 5.0) * bc_inv_exprel<real, M>(-(Vp - real(15)) * real(0.2));
   bp = real(0.5) * M::ex(-(Vp - real(10)) * real(1.0 / 40.0));
+}
+
+// Hodgkin-Huxley 1952 pair (channel/sodium.py:384-398,
+// channel/potassium.py:450-456); V' = V - V_sh with V_sh = -45 mV by default.
+template <typename real, typename M = bc_math<real>>
+__device__ __forceinline__ void bc_nahh_rates(real V, real vsh, real& ap, real& bp, real& aq, real& bq) {
+  real Vp = V - vsh;
+  ap = bc_inv_exprel<real, M>(-(Vp - real(5)) * real(0.1));
+  bp = real(4) * M::ex(-(Vp + real(20)) * real(1.0 / 18.0));
+  aq = real(0.07) * M::ex(-(Vp + real(20)) * real(0.05));
+  bq = bc_rcp<real>(real(1) + M::ex(-(Vp - real(10)) * real(0.1)));
+}
+
+template <typename real, typename M = bc_math<real>>
+__device__ __forceinline__ void bc_khh_rates(real V, real vsh, real& ap, real& bp) {
+  real Vp = V - vsh;
+  ap = real(0.1) * bc_inv_exprel<real, M>(-(Vp + real(10)) * real(0.1));
+  bp = real(0.125) * M::ex(-(Vp + real(20)) * real(1.0 / 80.0));
 }
 
 template <typename real, typename M = bc_math<real>>

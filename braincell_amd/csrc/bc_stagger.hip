@@ -99,13 +99,13 @@ __device__ __forceinline__ real linearise(const bc_chan_const<real>& cc, const r
   real ica = real(0);
   real eca = cc.E_Ca_fixed;
   if constexpr ((CH & BC_CH_POOL) != 0) eca = cc.ca_rtzf * M::lg(bc_div<real>(cc.ca_Co, s[bc_slot_ca(CH)]));
-  if constexpr ((CH & BC_CH_NA) != 0) {
+  if constexpr ((CH & BC_CH_NA_ANY) != 0) {
     real p = s[bc_slot_na(CH)], q = s[bc_slot_na(CH) + 1];
     real g = cp[BC_CLS_G_NA] * (p * p * p) * q;
     G += g;
     GE += g * (cc.E_Na - V);
   }
-  if constexpr ((CH & BC_CH_KDR) != 0) {
+  if constexpr ((CH & BC_CH_K4_ANY) != 0) {
     real p = s[bc_slot_kdr(CH)];
     real p2 = p * p;
     real g = cp[BC_CLS_G_KDR] * (p2 * p2);
@@ -179,6 +179,17 @@ __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, re
   if constexpr ((CH & BC_CH_KDR) != 0) {
     real ap, bp;
     bc_kdr_rates<real, M>(V, cc.kdr_vsh, ap, bp);
+    s[bc_slot_kdr(CH)] = bc_gate_ab_cf<real, M>(s[bc_slot_kdr(CH)], ap, bp, cc.kdr_phi, dt);
+  }
+  if constexpr ((CH & BC_CH_NAHH) != 0) {
+    real ap, bp, aq, bq;
+    bc_nahh_rates<real, M>(V, cc.na_vsh, ap, bp, aq, bq);
+    s[bc_slot_na(CH)] = bc_gate_ab_cf<real, M>(s[bc_slot_na(CH)], ap, bp, cc.na_phi, dt);
+    s[bc_slot_na(CH) + 1] = bc_gate_ab_cf<real, M>(s[bc_slot_na(CH) + 1], aq, bq, cc.na_phi, dt);
+  }
+  if constexpr ((CH & BC_CH_KHH) != 0) {
+    real ap, bp;
+    bc_khh_rates<real, M>(V, cc.kdr_vsh, ap, bp);
     s[bc_slot_kdr(CH)] = bc_gate_ab_cf<real, M>(s[bc_slot_kdr(CH)], ap, bp, cc.kdr_phi, dt);
   }
   if constexpr ((CH & BC_CH_CAL) != 0) {
@@ -793,6 +804,7 @@ constexpr uint32_t kSetHH = BC_CH_NA | BC_CH_KDR | BC_CH_IL;
 constexpr uint32_t kSetHHCaL = BC_CH_NA | BC_CH_KDR | BC_CH_IL | BC_CH_CAL | BC_CH_POOL;
 constexpr uint32_t kSetAll = BC_CH_NA | BC_CH_KDR | BC_CH_IL | BC_CH_KL | BC_CH_CAL | BC_CH_CAT | BC_CH_AHP |
                              BC_CH_HCN | BC_CH_POOL;
+constexpr uint32_t kSetHH1952 = BC_CH_NAHH | BC_CH_KHH | BC_CH_IL;
 
 typedef void (*kernel_f32)(StaggerArgs<float>);
 typedef void (*kernel_f64)(StaggerArgs<double>);
@@ -816,6 +828,7 @@ void* pick_kernel(uint32_t ch, int maxc) {
     case kSetHH: return pick_maxc<real, kSetHH>(maxc);
     case kSetHHCaL: return pick_maxc<real, kSetHHCaL>(maxc);
     case kSetAll: return pick_maxc<real, kSetAll>(maxc);
+    case kSetHH1952: return pick_maxc<real, kSetHH1952>(maxc);
     default: return nullptr;
   }
 }

@@ -51,3 +51,21 @@ def ahp_steady(Ci, alpha, beta, n):
 
 def hcn_steady(V):
     return 1.0 / (1.0 + np.exp((V + 75.0) / 5.5))
+
+
+def nahh_steady(V, vsh):
+    """Na_HH1952 (``channel/sodium.py:384-398``)."""
+    Vp = V - vsh
+    ap = 1.0 / exprel(-(Vp - 5.0) / 10.0)
+    bp = 4.0 * np.exp(-(Vp + 20.0) / 18.0)
+    aq = 0.07 * np.exp(-(Vp + 20.0) / 20.0)
+    bq = 1.0 / (1.0 + np.exp(-(Vp - 10.0) / 10.0))
+    return ap / (ap + bp), aq / (aq + bq)
+
+
+def khh_steady(V, vsh):
+    """K_HH1952 (``channel/potassium.py:450-456``)."""
+    Vp = V - vsh
+    ap = 0.1 / exprel(-(Vp + 10.0) / 10.0)
+    bp = 0.125 * np.exp(-(Vp + 20.0) / 80.0)
+    return ap / (ap + bp)

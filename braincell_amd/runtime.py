@@ -22,13 +22,18 @@ __all__ = ["StaggerLayout", "StaggerRuntime", "KERNEL_CHANNEL_SETS", "choose_cha
 
 CH_ORDER = ("Na_Ba2002", "KDR_Ba2002", "IL", "K_Leak", "CaL_IS2008", "CaT_HM1992", "AHP_De1994", "HCN_HM1992")
 CLS_COL = {n: i + 1 for i, n in enumerate(CH_ORDER)}  # column in the class table
+# the Hodgkin-Huxley 1952 pair occupies the Na / KDR columns, state slots and
+# constants: the compiled sets never hold both pairs
+CLS_COL.update({"Na_HH1952": CLS_COL["Na_Ba2002"], "K_HH1952": CLS_COL["KDR_Ba2002"]})
 NCLS = 12
 B = {n: CHANNELS[n].bit for n in CH_ORDER}
+B_HH = {n: CHANNELS[n].bit for n in ("Na_HH1952", "K_HH1952")}
 KERNEL_CHANNEL_SETS = (
     B["IL"],
     B["Na_Ba2002"] | B["KDR_Ba2002"] | B["IL"],
     B["Na_Ba2002"] | B["KDR_Ba2002"] | B["IL"] | B["CaL_IS2008"] | CA_POOL_BIT,
     sum(B.values()) | CA_POOL_BIT,
+    B_HH["Na_HH1952"] | B_HH["K_HH1952"] | B["IL"],
 )
 INT32_MAX = 2 ** 31 - 1
 REGION_CODES = ("soma", "basal_dendrite", "apical_dendrite", "dendrite", "axon", "custom")
@@ -46,8 +51,12 @@ def state_slots(chset: int) -> list:
     out = []
     if chset & B["Na_Ba2002"]:
         out += [("Na_Ba2002", 0), ("Na_Ba2002", 1)]
+    if chset & B_HH["Na_HH1952"]:
+        out += [("Na_HH1952", 0), ("Na_HH1952", 1)]
     if chset & B["KDR_Ba2002"]:
         out += [("KDR_Ba2002", 0)]
+    if chset & B_HH["K_HH1952"]:
+        out += [("K_HH1952", 0)]
     if chset & B["CaL_IS2008"]:
         out += [("CaL_IS2008", 0), ("CaL_IS2008", 1)]
     if chset & B["CaT_HM1992"]:
@@ -109,7 +118,9 @@ class StaggerLayout:
         self.chset = choose_channel_set(mask)
         self.slots = state_slots(self.chset)
         self.cc = self._channel_constants()
+        self._geom_cache = {}
         self.cells = [self._lower_cell(m, probes) for m in self.models]
+        del self._geom_cache
         self._configure(threads, cv_per_thread, rows_per_pack)
 
     def _configure(self, threads, cv_per_thread, rows_per_pack):
@@ -177,11 +188,15 @@ class StaggerLayout:
                                "ahp_beta", "ahp_n", "ahp_phi", "hcn_E", "hcn_phi", "ca_drive", "ca_tau_inv",
                                "ca_rest", "ca_Co", "ca_rtzf", "E_Ca_fixed")}
         cc["E_Na"], cc["E_K"], cc["E_Ca_fixed"] = ref.E_Na, ref.E_K, ref.E_Ca
-        dflt = {n: dict(CHANNELS[n].params) for n in CH_ORDER}
-        for n in CH_ORDER:
+        names = CH_ORDER + tuple(B_HH)
+        dflt = {n: dict(CHANNELS[n].params) for n in names}
+        for n in names:
             dflt[n].update(params.get(n, {}))
-        na, kd, cl, ct, ah, hc = (dflt[n] for n in ("Na_Ba2002", "KDR_Ba2002", "CaL_IS2008", "CaT_HM1992",
-                                                   "AHP_De1994", "HCN_HM1992"))
+        hh = "Na_HH1952" in params or "K_HH1952" in params
+        if hh and ("Na_Ba2002" in params or "KDR_Ba2002" in params):
+            raise ValueError("Na_HH1952/K_HH1952 and Na_Ba2002/KDR_Ba2002 cannot share a batch")
+        na, kd, cl, ct, ah, hc = (dflt[n] for n in (("Na_HH1952", "K_HH1952") if hh else ("Na_Ba2002", "KDR_Ba2002"))
+                                  + ("CaL_IS2008", "CaT_HM1992", "AHP_De1994", "HCN_HM1992"))
         cc["E_IL"] = dflt["IL"]["E"]
         cc["na_vsh"], cc["na_phi"] = na["V_sh"], _phi(na["q10"], na["temp"], na["temp_ref"])
         cc["kdr_vsh"], cc["kdr_phi"] = kd["V_sh"], _phi(kd["q10"], kd["temp"], kd["temp_ref"])
@@ -205,8 +220,17 @@ class StaggerLayout:
     # ---------------------------------------------------------------- cells
     def _lower_cell(self, m, probes) -> _CellHost:
         ncvb = m.n_cv_branch()
-        geom = discretize_cell(m.morph, ncvb, m.per_branch(m.cm), m.per_branch(m.ra))
-        chains = build_chains(geom.parent_rows)
+        cmb, rab = m.per_branch(m.cm), m.per_branch(m.ra)
+        # cells that share one morphology object and cable constants share
+        # their discretisation (read-only from here on)
+        key = (id(m.morph), ncvb.tobytes(), np.asarray(cmb, np.float64).tobytes(), np.asarray(rab, np.float64).tobytes())
+        hit = self._geom_cache.get(key)
+        if hit is not None and hit[0] is m.morph:
+            geom, chains = hit[1], hit[2]
+        else:
+            geom = discretize_cell(m.morph, ncvb, cmb, rab)
+            chains = build_chains(geom.parent_rows)
+            self._geom_cache[key] = (m.morph, geom, chains)
         nb = m.n_branches
         table = np.zeros((nb, NCLS))
         table[:, 0] = m.per_branch(m.cm)
@@ -449,8 +473,12 @@ class StaggerLayout:
         for s, (name, k) in enumerate(self.slots):
             if name == "Na_Ba2002":
                 S[s] = rates.na_steady(V, cc["na_vsh"])[k]
+            elif name == "Na_HH1952":
+                S[s] = rates.nahh_steady(V, cc["na_vsh"])[k]
             elif name == "KDR_Ba2002":
                 S[s] = rates.kdr_steady(V, cc["kdr_vsh"])
+            elif name == "K_HH1952":
+                S[s] = rates.khh_steady(V, cc["kdr_vsh"])
             elif name == "CaL_IS2008":
                 S[s] = rates.cal_steady(V, cc["cal_vsh"])[k]
             elif name == "CaT_HM1992":
@@ -537,6 +565,19 @@ class StaggerRuntime:
         if Vb.size:
             self.Vbnd.copy_(self._t(Vb, self.dtype))
         self.step_index = 0
+        self.status.zero_()
+
+    def save_state(self):
+        """Device-side snapshot of V, the states and the boundary voltages."""
+        self._saved = (self.V.clone(), self.S.clone(), self.Vbnd.clone(), self.step_index)
+
+    def restore_state(self):
+        """Device-to-device restore of the last :meth:`save_state`."""
+        V, S, Vb, n = self._saved
+        self.V.copy_(V)
+        self.S.copy_(S)
+        self.Vbnd.copy_(Vb)
+        self.step_index = n
         self.status.zero_()
 
     def desc(self, n_steps: int, trace=None, step0: int = None, rec=None) -> StaggerDesc:

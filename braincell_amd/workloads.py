@@ -23,9 +23,18 @@ below is part of each workload's definition.
   same growth; Na/KDR/leak/CaL densities per region (soma, basal, apical),
   i.e. 12 parameters per cell, and a CalciumDetailed pool everywhere.
 - ``htc_population``: the thalamocortical single-compartment population.
+- ``n144_hh_cell(amp)``: the reference's simulator-comparison cell
+  (``examples/profiling/simulator_compare/common.py:30-56``,
+  ``backend_braincell.py:62-113``): ``data/n144.swc`` (217 branches, 4 CVs
+  each = 868 CVs), classic Hodgkin-Huxley (Na_HH1952 120, K_HH1952 36,
+  leak 0.3 mS/cm^2 at -54.3 mV; E_Na 50, E_K -77 mV) at 6.3 degC with
+  T = T_ref, Cm 1 uF/cm^2, Ra 1000 ohm cm, V0 -62 mV, a clamp of ``amp``
+  nA at branch 0, x = 0.85, from 3 to 5 ms.
 """
 
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
@@ -35,8 +44,12 @@ from .morph import Branch, Morphology
 
 __all__ = [
     "single_tree_cell", "uneven_tree_cell", "region_tree_cell", "uneven_tree_batch",
-    "htc_population", "HTC_CHANNELS", "BASE_DENSITIES",
+    "htc_population", "HTC_CHANNELS", "BASE_DENSITIES", "N144_SWC", "n144_hh_cell", "n144_amplitudes",
+    "N144_PROBES",
 ]
+
+N144_SWC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "n144.swc")
+N144_PROBES = ((0, 0.85), (34, 0.85), (54, 0.85))  # (branch, x) of the recorded voltages
 
 BASE_DENSITIES = {"Na_Ba2002": 0.12, "KDR_Ba2002": 0.036, "IL": 0.0003, "CaL_IS2008": 0.002}
 CLAMP_FOREVER = 1.0e9
@@ -187,3 +200,37 @@ def htc_population(n: int = 100_000, seed: int = 1):
     V0 = rng.uniform(-70.0, -60.0, n)
     dc = rng.uniform(0.0, 0.3, n)
     return {"g": g, "V0": V0, "I_dc_nA": dc, "names": names}
+
+
+_N144_MORPH = None
+
+
+def n144_amplitudes(n: int):
+    """Clamp amplitudes in nA: 0.7 for one cell, else 0.69 ... 0.71 evenly
+    (``simulator_compare/common.py:58-64``)."""
+    if n <= 0:
+        raise ValueError("n must be positive")
+    if n == 1:
+        return [0.7]
+    return [0.69 + 0.02 * i / (n - 1) for i in range(n)]
+
+
+def n144_hh_cell(amp: float = 0.7) -> CellModel:
+    global _N144_MORPH
+    from .swc import read_swc
+
+    if _N144_MORPH is None:
+        _N144_MORPH = read_swc(N144_SWC, mode="neuron")
+    morph = _N144_MORPH
+    nb = morph.n_branches
+    t = celsius2kelvin(6.3)
+    hh = {"temp": t, "temp_ref": t}
+    return CellModel(
+        morph=morph, cv_per_branch=4, cm=1.0, ra=1000.0, V0=-62.0, E_Na=50.0, E_K=-77.0,
+        channels={
+            "Na_HH1952": {"g_branch": np.full(nb, 0.12), "params": dict(hh)},
+            "K_HH1952": {"g_branch": np.full(nb, 0.036), "params": dict(hh)},
+            "IL": {"g_branch": np.full(nb, 0.0003), "params": {"E": -54.3}},
+        },
+        clamps=[(0, 0.85, 3.0, 2.0, float(amp))],
+    ).validate()

@@ -39,6 +39,8 @@ enum {
 #define BC_CH_AHP  (1u << 6) /* AHP_De1994 */
 #define BC_CH_HCN  (1u << 7) /* HCN_HM1992 */
 #define BC_CH_POOL (1u << 8) /* CalciumDetailed */
+#define BC_CH_NAHH (1u << 9) /* Na_HH1952 (channel/sodium.py:293-398); never with BC_CH_NA */
+#define BC_CH_KHH  (1u << 10) /* K_HH1952 (channel/potassium.py:341-456); never with BC_CH_KDR */
 
 /* Per-pack header: one workgroup integrates one pack (a forest of whole
  * cells).  All *0 fields are global offsets into the corresponding arrays;

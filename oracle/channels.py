@@ -10,6 +10,8 @@ Reference sources, per function below:
   ``:550-558`` (conductance_factor), ``:560-573`` (reset_state),
   ``:575-588`` (compute_derivative), ``:658-674`` (OhmicHH.current)
 - ``Na_Ba2002`` ``channel/sodium.py:134-170``
+- ``Na_HH1952`` ``channel/sodium.py:293-398``
+- ``K_HH1952`` ``channel/potassium.py:341-456``
 - ``KDR_Ba2002`` ``channel/potassium.py:189-214``
 - ``K_Leak`` ``channel/potassium.py:1216-1235``
 - ``IL`` ``channel/leaky.py:184-200``
@@ -63,6 +65,24 @@ def kdr_ba2002(V, V_sh=-50.0):
     t = Vp - 15.0
     ap = 0.032 * 5.0 / exprel(-t / 5.0)
     bp = 0.5 * np.exp(-(Vp - 10.0) / 40.0)
+    return ap, bp
+
+
+def na_hh1952(V, V_sh=-45.0):
+    """sodium.py:384-398 -> (alpha_p, beta_p, alpha_q, beta_q)."""
+    Vp = V - V_sh
+    ap = 1.0 / exprel(-(Vp - 5.0) / 10.0)
+    bp = 4.0 * np.exp(-(Vp + 20.0) / 18.0)
+    aq = 0.07 * np.exp(-(Vp + 20.0) / 20.0)
+    bq = 1.0 / (1.0 + np.exp(-(Vp - 10.0) / 10.0))
+    return ap, bp, aq, bq
+
+
+def k_hh1952(V, V_sh=-45.0):
+    """potassium.py:450-456 -> (alpha_p, beta_p)."""
+    Vp = V - V_sh
+    ap = 0.1 / exprel(-(Vp + 10.0) / 10.0)
+    bp = 0.125 * np.exp(-(Vp + 20.0) / 80.0)
     return ap, bp
 
 

@@ -113,7 +113,7 @@ class StaggeredOracle:
 
     def _phi(self, name, which=""):
         p = self.params[name]
-        if name in ("Na_Ba2002", "KDR_Ba2002"):
+        if name in ("Na_Ba2002", "KDR_Ba2002", "Na_HH1952", "K_HH1952"):
             return ch.q10_phi(_default(p, "q10", 3.0), _default(p, "temp", ch.c2k(36.0)),
                               _default(p, "temp_ref", ch.c2k(36.0)))
         if name in ("CaL_IS2008", "CaT_HM1992"):
@@ -132,6 +132,10 @@ class StaggeredOracle:
             return ch.na_ba2002(V, _default(p, "V_sh", -50.0))
         if name == "KDR_Ba2002":
             return ch.kdr_ba2002(V, _default(p, "V_sh", -50.0))
+        if name == "Na_HH1952":
+            return ch.na_hh1952(V, _default(p, "V_sh", -45.0))
+        if name == "K_HH1952":
+            return ch.k_hh1952(V, _default(p, "V_sh", -45.0))
         if name == "CaL_IS2008":
             return ch.cal_is2008(V, _default(p, "V_sh", 0.0))
         if name == "CaT_HM1992":
@@ -144,10 +148,10 @@ class StaggeredOracle:
 
     def _steady(self, name, V, Ci):
         r = self._rates(name, V, Ci)
-        if name == "Na_Ba2002":
+        if name in ("Na_Ba2002", "Na_HH1952"):
             ap, bp, aq, bq = r
             return [ap / (ap + bp), aq / (aq + bq)]
-        if name == "KDR_Ba2002" or name == "AHP_De1994":
+        if name in ("KDR_Ba2002", "K_HH1952", "AHP_De1994"):
             a, b = r
             return [a / (a + b)]
         if name in ("CaL_IS2008", "CaT_HM1992"):
@@ -158,9 +162,9 @@ class StaggeredOracle:
 
     def _E(self, name, Ci):
         p = self.params[name]
-        if name == "Na_Ba2002":
+        if name in ("Na_Ba2002", "Na_HH1952"):
             return self.E_Na
-        if name in ("KDR_Ba2002", "K_Leak", "AHP_De1994"):
+        if name in ("KDR_Ba2002", "K_HH1952", "K_Leak", "AHP_De1994"):
             return self.E_K
         if name in CA_CHANNELS:
             return self._E_ca(Ci)
@@ -178,9 +182,9 @@ class StaggeredOracle:
     def _g_eff(self, name):
         """conductance_factor (_base.py:550-558) times g_max, in mS/cm^2."""
         gs = self.gates.get(name, [])
-        if name == "Na_Ba2002":
+        if name in ("Na_Ba2002", "Na_HH1952"):
             f = gs[0] ** 3 * gs[1]
-        elif name == "KDR_Ba2002":
+        elif name in ("KDR_Ba2002", "K_HH1952"):
             f = gs[0] ** 4
         elif name in ("CaL_IS2008", "CaT_HM1992"):
             f = gs[0] ** 2 * gs[1]
@@ -256,11 +260,11 @@ class StaggeredOracle:
             if not gs:
                 continue
             r = self._rates(name, V, Ci)
-            if name == "Na_Ba2002":
+            if name in ("Na_Ba2002", "Na_HH1952"):
                 ap, bp, aq, bq = r
                 phi = self._phi(name)
                 self.gates[name] = [ch.ab_update(gs[0], ap, bp, phi, dt), ch.ab_update(gs[1], aq, bq, phi, dt)]
-            elif name in ("KDR_Ba2002", "AHP_De1994"):
+            elif name in ("KDR_Ba2002", "K_HH1952", "AHP_De1994"):
                 a, b = r
                 self.gates[name] = [ch.ab_update(gs[0], a, b, self._phi(name), dt)]
             elif name in ("CaL_IS2008", "CaT_HM1992"):

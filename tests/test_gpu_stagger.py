@@ -151,12 +151,12 @@ def test_interior_attachment_fp64_matches_oracle(cuda, n_cv):
 
 
 def _n144_cell():
-    import os
-
     from braincell_amd.model import CellModel
     from braincell_amd.swc import read_swc
 
-    morph = read_swc(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "swc", "n144.swc"))
+    from braincell_amd.workloads import N144_SWC
+
+    morph = read_swc(N144_SWC)
     nb = morph.n_branches
     return CellModel(
         morph=morph, cv_per_branch=4, cm=1.0, ra=1000.0, V0=-65.0,
@@ -183,3 +183,28 @@ def test_swc_cell_matches_oracle(cuda):
     print(f"n144: fp64 max |dV| {err:.2e} mV, fp32 soma RMSE {e32:.2e} mV, soma peak {ref[:, cv].max():.1f} mV")
     assert err < 1e-5
     assert e32 <= 0.05
+
+
+def test_n144_hh_benchmark_cell_matches_oracle(cuda):
+    """The reference's simulator-comparison cell (n144.swc, 868 CVs,
+    Hodgkin-Huxley 1952 at 6.3 degC): FP64 all CVs, and an FP32 batch of
+    three clamp amplitudes at the soma, against the oracle over 20 ms."""
+    from braincell_amd import workloads as W
+
+    steps = 800
+    m = W.n144_hh_cell(0.7)
+    ref, _ = run_oracle(m, steps)
+    got64, _ = run_gpu([m], steps, precision=64, probes="all")
+    err = np.abs(got64 - ref).max()
+    print(f"n144 HH: fp64 max |dV| {err:.2e} mV")
+    assert err < 1e-5
+    amps = W.n144_amplitudes(3)
+    models = [W.n144_hh_cell(a) for a in amps]
+    got32, _ = run_gpu(models, steps, precision=32, probes="soma")
+    for i, mi in enumerate(models):
+        r, _ = run_oracle(mi, steps, probe_cvs=[soma_cv(mi)])
+        e = rmse(got32[:, i], r[:, 0])
+        sg, sr = spike_times(got32[:, i], 0.025), spike_times(r[:, 0], 0.025)
+        print(f"amp {amps[i]:.3f} nA: fp32 soma RMSE {e:.2e} mV, spikes {sg} / {sr}")
+        assert e <= 0.05
+        assert len(sg) == len(sr) and np.all(np.abs(sg - sr) <= 0.025 + 1e-9)

@@ -14,6 +14,7 @@ import pytest
 from braincell_amd.swc import SwcReader, read_swc
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "swc")
+N144 = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "braincell_amd", "data", "n144.swc")
 
 
 def _read(tmp_path, body, **kw):
@@ -23,7 +24,7 @@ def _read(tmp_path, body, **kw):
 
 
 def test_n144_matches_the_reference_benchmark_figures():
-    path = os.path.join(HERE, "n144.swc")
+    path = N144
     with open(path, "rb") as f:
         assert hashlib.sha256(f.read()).hexdigest() == "68af99cf829e22005b2177b3d2ff4316bdaec2fcf0f1e4b85f81cdd2a8c01069"
     m = read_swc(path, mode="neuron")
@@ -255,7 +256,7 @@ def test_n144_discretisation_matches_the_oracle():
     from braincell_amd.layout import discretize_cell
     from oracle import discretize as OD
 
-    m = read_swc(os.path.join(HERE, "n144.swc"))
+    m = read_swc(N144)
     g = discretize_cell(m, np.full(m.n_branches, 4), np.ones(m.n_branches), np.full(m.n_branches, 1000.0))
     assert g.n_cv == 868
     cvs, _, _, sched, st = OD.discretize(m, 4, 1.0, 1000.0)
