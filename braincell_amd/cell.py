@@ -20,12 +20,14 @@ import numpy as np
 
 from .channels import IONS
 from .filter import resolve_region
-from .mech import CableProperty, Channel, CurrentClamp, Ion
+from .mech import CableProperty, Channel, CurrentClamp, CurrentProbe, Ion, MechanismProbe, StateProbe
 from .model import CellModel
 from .policy import CVPerBranch, CVPolicy
 from .quad import get_integrator
 
 __all__ = ["Cell", "CellBatch", "CVPerBranch", "RunResult"]
+
+VOLTAGE_ONLY = 4  # BC_STEP_VOLTAGE_ONLY
 
 
 @dataclass
@@ -51,6 +53,7 @@ class Cell:
         self.name = name
         self._paint = []
         self._place = []
+        self._probes = []
         self._runtime = None
         self._initialized = False
 
@@ -74,10 +77,20 @@ class Cell:
         return self
 
     def place(self, locset, *mechanisms) -> "Cell":
+        probes = tuple(m for m in mechanisms if isinstance(m, (StateProbe, MechanismProbe, CurrentProbe)))
+        clamps = tuple(m for m in mechanisms if isinstance(m, CurrentClamp))
         for m in mechanisms:
-            if not isinstance(m, CurrentClamp):
-                raise TypeError(f"place() takes CurrentClamp, got {type(m).__name__}")
-        self._place.append((locset, mechanisms))
+            if not isinstance(m, (CurrentClamp, StateProbe, MechanismProbe, CurrentProbe)):
+                raise TypeError(f"place() takes CurrentClamp or a probe, got {type(m).__name__}")
+        names = {p.name for _, ps in self._probes for p in ps}
+        for p in probes:
+            if p.name in names:
+                raise ValueError(f"Multiple probes share the same name {p.name!r}; probe names must be unique.")
+            names.add(p.name)
+        if clamps:
+            self._place.append((locset, clamps))
+        if probes:
+            self._probes.append((locset, probes))
         self._runtime = None
         return self
 
@@ -137,12 +150,14 @@ class Cell:
                          channels=channels, ca=ca, clamps=clamps, **E).validate()
 
     # ----------------------------------------------------------------- state
-    def init_state(self, *, probes="soma", threads: int = 256):
+    def init_state(self, *, probes=None, threads: int = 256):
         """Build the discretisation, layout and device state (gates start at
         the reference's ``init_state`` value only after ``reset_state``)."""
         from .runtime import StaggerLayout
 
         self._model = self.lower()
+        if probes is None:
+            probes = self._probe_cvs() if self._probes else "soma"
         self._layout = StaggerLayout([self._model], threads=threads, probes=probes)
         self._runtime = None
         self._initialized = True
@@ -185,12 +200,20 @@ class Cell:
         return self._runtime.V
 
     # ------------------------------------------------------------------ run
-    def advance(self, t: float, dt: float, n_steps: int = 1):
-        """Native advance used by the registered step functions."""
+    def advance(self, t: float, dt: float, n_steps: int = 1, *, voltage_only: bool = False):
+        """Native advance used by the registered step functions;
+        ``voltage_only`` runs the DHS voltage solve alone (gates and the Ca
+        pool keep their values)."""
         rt = self._runtime_for(dt)
         n0 = int(round(float(t) / float(dt)))
         rt.step_index = n0
-        tr = rt.run(n_steps)
+        flags = rt.flags
+        if voltage_only:
+            rt.flags = flags | VOLTAGE_ONLY
+        try:
+            tr = rt.run(n_steps)
+        finally:
+            rt.flags = flags
         rt.step_index = n0 + n_steps
         self.last_trace = tr
         return tr
@@ -210,10 +233,82 @@ class Cell:
         if n == 0:
             raise ValueError("Cell.run(...) produced no timesteps; ensure duration > 0 and dt > 0.")
         start = rt.step_index * float(dt)
-        tr = rt.run(n, chunk=chunk)
+        if not self._probes:  # no probes placed: the layout's voltage trace
+            tr = rt.run(n, chunk=chunk)
+            torch.cuda.synchronize()
+            rt.check_status()
+            return RunResult(time=start + times, traces={"V": tr.double().cpu().numpy()})
+        return RunResult(time=start + times, traces=self._run_probes(rt, n, chunk))
+
+    # ---------------------------------------------------------------- probes
+    def _probe_points(self):
+        """(probe, [CV per location]) for every placed probe."""
+        from .layout import discretize_cell
+        from .probes import probe_cv
+
+        m = self._model if self._initialized else self.lower()
+        ncvb = m.n_cv_branch()
+        geom = discretize_cell(m.morph, ncvb, m.per_branch(m.cm), m.per_branch(m.ra))
+        out = []
+        for locset, probes in self._probes:
+            cvs = [probe_cv(geom, ncvb, b, x) for (b, x) in locset.locations(m.morph)]
+            for p in probes:
+                out.append((p, cvs))
+        return out
+
+    def _probe_cvs(self):
+        return sorted({c for _, cvs in self._probe_points() for c in cvs})
+
+    def _run_probes(self, rt, n, chunk):
+        """Run ``n`` steps and sample every probe after each step
+        (``run.py:108-134``: traces keyed by probe name, sorted)."""
+        import torch
+
+        from .probes import channel_current, gate_slot
+
+        points = self._probe_points()
+        col = {int(c): k for k, c in enumerate(self._layout.cells[0].probes)}
+        need_states = any(not isinstance(p, StateProbe) for p, _ in points)
+        L = self._layout
+        ns = len(L.slots)
+        ncv = L.n_cv_total
+        rec = None
+        if need_states:
+            rec = torch.empty((n + 1) * (ns + 1) * ncv, dtype=rt.dtype, device=rt.V.device)
+            tr = rt.run(n, chunk=n, rec=rec)
+        else:
+            tr = rt.run(n, chunk=chunk)
         torch.cuda.synchronize()
         rt.check_status()
-        return RunResult(time=start + times, traces={"V": tr.double().cpu().numpy()})
+        trace = tr.double().cpu().numpy()
+        snaps = rec.double().cpu().numpy().reshape(n + 1, ns + 1, ncv)[1:] if need_states else None
+        m = self._model
+        cc = L.cc
+        out = {}
+        for p, cvs in points:
+            cvs = np.asarray(cvs, dtype=np.int64)
+            if isinstance(p, StateProbe):
+                if any(int(c) not in col for c in cvs):
+                    raise ValueError(f"probe {p.name!r}: its CVs are not in the layout's probe set; "
+                                     "call init_state() after placing probes")
+                out[p.name] = trace[:, [col[int(c)] for c in cvs]]
+            elif isinstance(p, MechanismProbe):
+                out[p.name] = snaps[:, 1 + gate_slot(L.slots, p.mechanism, p.field)][:, cvs]
+            else:
+                V = snaps[:, 0][:, cvs]
+                st = {key: snaps[:, 1 + s][:, cvs] for s, key in enumerate(L.slots)}
+                Ci = st.get(("Ca", 0))
+                names = [p.mechanism] if p.mechanism is not None else (
+                    [c for c in ("CaL_IS2008", "CaT_HM1992") if c in m.channels] if p.ion == "Ca" else None)
+                if names is None:
+                    raise ValueError(f"probe {p.name!r}: only ion='Ca' is supported")
+                if p.mechanism is not None and p.mechanism not in m.channels:
+                    raise KeyError(f"Probe {p.name!r} could not find a mechanism named {p.mechanism!r}.")
+                I = np.zeros(V.shape)
+                for nm in names:
+                    I = I + channel_current(m, nm, cvs, V, st, Ci, cc["E_Ca_fixed"], cc["ca_rtzf"], cc["ca_Co"])
+                out[p.name] = I
+        return {k: out[k] for k in sorted(out)}
 
 
 class CellBatch:

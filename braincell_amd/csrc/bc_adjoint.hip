@@ -411,8 +411,9 @@ int bc_stagger_adjoint(const bc_adjoint_desc* d, void* stream) {
     return bc::fail(BC_INVALID_ARG, "threads must be 64, 128, 192 or 256");
   if (d->n_steps < 0) return bc::fail(BC_INVALID_ARG, "n_steps < 0");
   if (f->chan_const == nullptr) return bc::fail(BC_INVALID_ARG, "chan_const is NULL");
-  if (f->flags & BC_STEP_SKIP_GATES)
-    return bc::fail(BC_UNSUPPORTED, "the adjoint differentiates the full step: BC_STEP_SKIP_GATES is not supported");
+  if (f->flags & (BC_STEP_SKIP_GATES | BC_STEP_VOLTAGE_ONLY))
+    return bc::fail(BC_UNSUPPORTED, "the adjoint differentiates the full step: BC_STEP_SKIP_GATES and "
+                                    "BC_STEP_VOLTAGE_ONLY are not supported");
   if (f->n_packs > 0 && (d->rec == nullptr || d->Vbar == nullptr || d->Sbar == nullptr || d->cls_grad == nullptr ||
                          (f->trace_ld > 0 && d->trace_bar == nullptr)))
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");

@@ -159,6 +159,7 @@ template <typename real, uint32_t CH>
 __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, real* s, real V, real ica, real dt,
                                                real ca_b, uint32_t flags) {
   typedef bc_fastmath<real> M;
+  if (flags & BC_STEP_VOLTAGE_ONLY) return;
   real ci = real(0), ci_gates = real(0);
   if constexpr ((CH & BC_CH_POOL) != 0) {
     real c = s[bc_slot_ca(CH)];
@@ -886,7 +887,7 @@ int validate(const bc_stagger_desc* d) {
   if (d->max_rows > 32767 || d->max_levels <= 0 || d->max_levels > 32767)
     return bc::fail(BC_INVALID_LAYOUT, "max_rows/max_levels out of range for 16-bit chain metadata");
   if (!(d->dt > 0.0)) return bc::fail(BC_INVALID_ARG, "dt must be > 0");
-  if ((d->flags & ~(BC_STEP_SKIP_GATES | BC_STEP_INTEGRATION_ORDER)) != 0)
+  if ((d->flags & ~(BC_STEP_SKIP_GATES | BC_STEP_INTEGRATION_ORDER | BC_STEP_VOLTAGE_ONLY)) != 0)
     return bc::fail(BC_INVALID_ARG, "unknown flags 0x%x", d->flags);
   if (d->n_packs > 0 && (d->packs == nullptr || d->cv_info == nullptr || d->V == nullptr || d->Vbnd == nullptr ||
                          d->status == nullptr))

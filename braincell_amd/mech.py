@@ -16,7 +16,7 @@ from dataclasses import dataclass, field
 
 from .channels import CHANNELS, IONS, celsius2kelvin
 
-__all__ = ["Channel", "Ion", "CableProperty", "CurrentClamp"]
+__all__ = ["Channel", "Ion", "CableProperty", "CurrentClamp", "StateProbe", "MechanismProbe", "CurrentProbe"]
 
 
 @dataclass(frozen=True)
@@ -116,3 +116,43 @@ class CurrentClamp:
             out.append((s, s + d, a))
             s += d
         return out
+
+
+@dataclass(frozen=True)
+class StateProbe:
+    """Records the membrane voltage at its locations after every step
+    (``mech/_point.py:314``, ``_multi_compartment/probes.py:124-134``)."""
+
+    name: str
+    field: str = "v"
+
+    def __post_init__(self):
+        if self.field != "v":
+            raise ValueError(f"Unsupported StateProbe field {self.field!r}.")
+
+
+@dataclass(frozen=True)
+class MechanismProbe:
+    """Records a state of a painted mechanism after every step
+    (``probes.py:137-180``): ``mechanism`` is a channel class name with
+    ``field`` one of its gates (``"p"``, ``"q"``), or ``"CalciumDetailed"``
+    with ``field="C"`` (the pool concentration, mM)."""
+
+    name: str
+    mechanism: str
+    field: str
+
+
+@dataclass(frozen=True)
+class CurrentProbe:
+    """Records a membrane current density (uA/cm^2, outward negative as in
+    the reference's ``g (E - V)``) after every step (``probes.py:183-236``):
+    one channel's (``mechanism``) or an ion's total (``ion="Ca"``)."""
+
+    name: str
+    mechanism: str = None
+    ion: str = None
+
+    def __post_init__(self):
+        if (self.mechanism is None) == (self.ion is None):
+            raise ValueError(f"Probe {self.name!r} must define exactly one of 'mechanism' and 'ion'.")

@@ -77,6 +77,19 @@ def staggered_mi355x_step(target, *args):
     return target.advance(t, dt, 1)
 
 
+@register_integrator("dhs_voltage_mi355x", category="voltage",
+                     description="Implicit-Euler DHS voltage step alone on the MI355X staggered kernel.")
+def dhs_voltage_mi355x_step(target, t, dt, *args):
+    """``dhs_voltage`` (``quad/_staggered.py:187-320``): one implicit-Euler
+    voltage solve of a multi-compartment cell at time ``t`` (ms) with step
+    ``dt`` (ms); gates and the Ca pool are left as they are."""
+    from ..cell import Cell
+
+    if not isinstance(target, Cell):
+        raise TypeError(f"dhs_voltage_step(...) requires a node-tree aware target, got {type(target)}.")
+    return target.advance(float(t), float(dt), 1, voltage_only=True)
+
+
 @register_integrator("ind_exp_euler_mi355x", category="exponential",
                      description="Independent exponential Euler for point neurons on the MI355X HIP kernel.")
 def ind_exp_euler_mi355x_step(target, *args):

@@ -135,6 +135,9 @@ typedef struct bc_stagger_desc {
 #define BC_STEP_SKIP_GATES (1u << 0) /* update V and the Ca pool only; gates keep
                                         their step-start values (a closure then
                                         applies the gate update) */
+#define BC_STEP_VOLTAGE_ONLY (1u << 2) /* dhs_voltage (quad/_staggered.py:187-320):
+                                         the voltage solve alone; gates and the
+                                         Ca pool keep their values */
 #define BC_STEP_INTEGRATION_ORDER (1u << 1) /* ion_channel_update_order
                                         "integration" (_multi_compartment/cell.py:
                                         2137-2155): every node updates from the

@@ -235,6 +235,16 @@ class StaggeredOracle:
         return cur
 
     # ------------------------------------------------------------------- step
+    def voltage_step(self, t, dt):
+        """dhs_voltage (``_staggered.py:187-320``): the implicit-Euler
+        voltage solve alone; gates and Ci are left unchanged."""
+        V = self.V
+        I, G = self.membrane_current(V, self.Ci)
+        I = I + self.clamp_density(t)
+        lin = -G / self.cm
+        const = I / self.cm - V * lin
+        self.V = dhs_solve(self.st, V[None, :], lin[None, :], const[None, :], dt, self.edge_current(t))[0]
+
     def step(self, t, dt):
         V = self.V
         Ci = self.Ci

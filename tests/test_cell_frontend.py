@@ -126,3 +126,77 @@ def test_cell_run_and_registry_steps_match_the_oracle(cuda):
     # a run continues from where the steps stopped (clamp windows follow t)
     more = stepper.run(dt=0.025, duration=(n - 60) * 0.025)
     assert np.max(np.abs(more.traces["V"][:, 0] - res.traces["V"][60:, 0])) < 1e-10
+
+
+def _probe_cell():
+    from braincell_amd.filter import BranchLocation, RootLocation
+    from braincell_amd.mech import CurrentProbe, MechanismProbe, StateProbe
+
+    m = W.uneven_tree_cell(7, cmin=200, cmax=300)
+    cell = Cell(m.morph, cv_policy=CVPerBranch(4))
+    cell.lower = lambda: m  # the workload's lowered model as is
+    cell.place(RootLocation(0.5), StateProbe("v_soma"), MechanismProbe("na_p", "Na_Ba2002", "p"))
+    cell.place(BranchLocation(3, 0.4), StateProbe("v_dend"), MechanismProbe("ci", "CalciumDetailed", "C"),
+               MechanismProbe("cal_q", "CaL_IS2008", "q"), CurrentProbe("i_cal", mechanism="CaL_IS2008"),
+               CurrentProbe("i_ca", ion="Ca"), CurrentProbe("i_kdr", mechanism="KDR_Ba2002"))
+    cell.place(BranchLocation(2, 1.0), StateProbe("v_tip"))
+    return cell, m
+
+
+def test_probe_placement_and_locations():
+    from braincell_amd.mech import CurrentProbe, StateProbe
+
+    cell, m = _probe_cell()
+    pts = {p.name: cvs for p, cvs in cell._probe_points()}
+    assert pts["v_soma"] == pts["na_p"] and len(pts["v_soma"]) == 1
+    # branch 2, x = 1: a boundary node, sampled at the branch's last CV
+    first = int(np.sum(m.n_cv_branch()[:2]))
+    assert pts["v_tip"] == [first + m.n_cv_branch()[2] - 1]
+    with pytest.raises(ValueError, match="unique"):
+        cell.place(RootLocation(0.5), StateProbe("v_soma"))
+    with pytest.raises(ValueError):
+        CurrentProbe("x")
+    with pytest.raises(ValueError):
+        StateProbe("x", field="m")
+
+
+@pytest.mark.gpu
+def test_probes_match_the_oracle(cuda):
+    """Voltage, gate, pool and current probes of Cell.run against the FP64
+    oracle's values at the same CVs after every step."""
+    from oracle.staggered import StaggeredOracle
+
+    cell, m = _probe_cell()
+    cell.precision = 64
+    cell.init_state()
+    n = 480
+    res = cell.run(dt=0.025, duration=n * 0.025)
+    assert sorted(res.traces) == list(res.traces)
+    pts = {p.name: cvs for p, cvs in cell._probe_points()}
+    o = StaggeredOracle(m)
+    o.reset_state(m.V0)
+    ref = {k: np.zeros((n, len(v))) for k, v in pts.items()}
+    for s in range(n):
+        o.step(s * 0.025, 0.025)
+        for k, cvs in pts.items():
+            if k.startswith("v_"):
+                val = o.V[cvs]
+            elif k == "na_p":
+                val = o.gates["Na_Ba2002"][0][cvs]
+            elif k == "cal_q":
+                val = o.gates["CaL_IS2008"][1][cvs]
+            elif k == "ci":
+                val = o.Ci[cvs]
+            elif k == "i_kdr":
+                val = (o._g_eff("KDR_Ba2002") * (o._E("KDR_Ba2002", o.Ci) - o.V))[cvs]
+            else:  # i_cal, i_ca (CaL is the only Ca channel)
+                val = (o._g_eff("CaL_IS2008") * (o._E("CaL_IS2008", o.Ci) - o.V))[cvs]
+            ref[k][s] = val
+    for k in pts:
+        got = res.traces[k]
+        scale = np.abs(ref[k]).max()
+        err = np.abs(got - ref[k]).max() / scale
+        print(f"{k}: max rel err {err:.2e} (scale {scale:.3g}, range {np.ptp(ref[k]):.3g})")
+        assert got.shape == (n, len(pts[k]))
+        assert np.ptp(ref[k]) > 0.0, k  # the probed quantity moves during the run
+        assert err < 1e-8, k

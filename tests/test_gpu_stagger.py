@@ -208,3 +208,33 @@ def test_n144_hh_benchmark_cell_matches_oracle(cuda):
         print(f"amp {amps[i]:.3f} nA: fp32 soma RMSE {e:.2e} mV, spikes {sg} / {sr}")
         assert e <= 0.05
         assert len(sg) == len(sr) and np.all(np.abs(sg - sr) <= 0.025 + 1e-9)
+
+
+def test_dhs_voltage_integrator_matches_oracle(cuda):
+    """dhs_voltage_mi355x: the voltage solve alone (gates and Ci untouched)
+    through the registry, against the oracle's voltage-only step."""
+    import torch
+    from braincell_amd.cell import Cell
+    from braincell_amd.quad import get_integrator
+    from oracle.staggered import StaggeredOracle
+
+    m = W.uneven_tree_cell(5, cmin=200, cmax=400)
+    cell = Cell(m.morph, solver="dhs_voltage_mi355x", precision=64)
+    cell.lower = lambda: m  # the lowered model as is
+    cell.init_state(probes="all")
+    step = get_integrator("dhs_voltage_mi355x")
+    o = StaggeredOracle(m)
+    o.reset_state(m.V0)
+    dt = 0.025
+    S0 = None
+    for n in range(450):  # the clamp starts at 10 ms = step 400
+        step(cell, n * dt, dt)
+        o.voltage_step(n * dt, dt)
+        if S0 is None:
+            S0 = cell.runtime.S.clone()
+    torch.cuda.synchronize()
+    V, _ = cell.runtime.state_of_cell(0)
+    err = np.abs(V - o.V).max()
+    print(f"dhs_voltage: fp64 max |dV| after 450 steps {err:.2e} mV, V range {o.V.min():.2f}..{o.V.max():.2f}")
+    assert err < 1e-9
+    assert torch.equal(S0, cell.runtime.S)  # gates and Ci unchanged

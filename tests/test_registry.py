@@ -91,3 +91,14 @@ def test_global_registry_has_the_mi355x_solvers():
 def test_step_rejects_wrong_target():
     with pytest.raises(TypeError):
         staggered_mi355x_step(object())
+
+
+def test_voltage_integrator_is_registered():
+    from braincell_amd.quad import get_integrator
+    from braincell_amd.quad._registry import get_registry
+
+    f = get_integrator("dhs_voltage_mi355x")
+    assert callable(f)
+    assert get_registry().entry("dhs_voltage_mi355x").category == "voltage"
+    with pytest.raises(TypeError):
+        f(object(), 0.0, 0.025)
