@@ -52,6 +52,9 @@ def main():
                     help="cells: 4096 uneven trees (default, the headline); point: 100k HTC point neurons; "
                          "adjoint: 512 region trees, forward + checkpointed adjoint; "
                          "closure: learned gating closure at dt 0.1 vs plain dt 0.025 on the 4096 trees")
+    ap.add_argument("--point-solver", default="ind_exp_euler_mi355x",
+                    choices=("ind_exp_euler_mi355x", "backward_euler_mi355x", "implicit_euler_mi355x"),
+                    help="solver of --workload point")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
@@ -190,20 +193,28 @@ def bench_cells(args):
 
 FP32_PEAK_TFLOPS = 157.3
 POINT_FLOP = 440.0  # ~200 FMA + ~40 exp/log/div per neuron-step
+# linearly implicit steps: the exp-Euler count plus ~120 for the dual-number
+# rate derivatives, ~40 for the Jacobian entries and ~100 for the elimination
+POINT_IMPLICIT_FLOP = 700.0
 
 
 def bench_point(args):
     """100 000 HTC point neurons, FP32, dt 0.025 ms, state in registers for
-    4000-step launches."""
+    4000-step launches; ``--point-solver`` picks ind_exp_euler (default),
+    backward_euler or implicit_euler."""
     import torch
     from braincell_amd import workloads as W
-    from braincell_amd.single import SingleCompartment
+    from braincell_amd.single import IND_EXP_EULER, SingleCompartment
 
     n, dt, chunk = 100_000, 0.025, 4000
     pop = W.htc_population(n, seed=1)
     channels = {nm: dict(W.HTC_CHANNELS[nm], g_max=pop["g"][nm]) for nm in pop["names"]}
     sc = SingleCompartment(n, channels=channels, E_Na=W.HTC_IONS["E_Na"], E_K=W.HTC_IONS["E_K"],
-                           ca=W.HTC_IONS["ca"], V0=pop["V0"], precision=32)
+                           ca=W.HTC_IONS["ca"], V0=pop["V0"], precision=32, solver=args.point_solver)
+    implicit = sc.method != IND_EXP_EULER
+    theta = {1: 1.0, 2: 0.5}.get(sc.method)
+    flop_step = POINT_IMPLICIT_FLOP if implicit else POINT_FLOP
+    kernel = "point_implicit_kernel<float>" if implicit else "point_kernel<float>"
     sc.init_state()
     sc.set_input(pop["I_dc_nA"])
     K = max(chunk, (args.steps // chunk) * chunk)
@@ -221,18 +232,18 @@ def bench_point(args):
     el = time.perf_counter() - t1
     launch_s = float(np.mean([evs[i].elapsed_time(evs[i + 1]) * 1e-3 for i in range(K // chunk)]))
     val = n * K / el
-    flops = POINT_FLOP * n * chunk / launch_s / 1e12
+    flops = flop_step * n * chunk / launch_s / 1e12
     cpu = None
     if not args.no_cpu:
         from oracle.single import HTCPopulationOracle
 
-        m, ns = 20_000, 400
+        m, ns = (20_000, 40) if implicit else (20_000, 400)
         g = {nm: pop["g"][nm][:m] for nm in pop["names"]}
         params = {nm: {k: v for k, v in W.HTC_CHANNELS[nm].items() if k != "g_max"} for nm in pop["names"]}
         o = HTCPopulationOracle(g, pop["I_dc_nA"][:m], params, W.HTC_IONS)
         o.reset_state(pop["V0"][:m])
         t2 = time.perf_counter()
-        o.run(dt, ns)
+        o.run(dt, ns, theta=theta)
         cel = time.perf_counter() - t2
         cpu = {"value": float(f"{m * ns / cel:.4g}"), "unit": "neuron-steps/s", "cores": 1, "kind": "port",
                "sample": f"FP64 NumPy oracle (vectorised over neurons), {m} neurons x {ns} steps, {cel:.1f}s"}
@@ -241,11 +252,11 @@ def bench_point(args):
         "warmup": chunk, "ms_per_step": round(el / K * 1e3, 6), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "100000 HTC point neurons (Na, KDR, K_Leak, CaL, CaT, AHP, HCN, IL, CalciumDetailed), "
-                               "lognormal densities, ind_exp_euler, dt 0.025 ms", "neurons": n,
-                   "steps_per_launch": chunk},
+                               f"lognormal densities, {args.point_solver.replace('_mi355x', '')}, dt 0.025 ms",
+                   "neurons": n, "steps_per_launch": chunk},
         "roofline": {"bound": "valu", "achieved": round(flops, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(flops / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": "point_kernel<float>",
-                     "launch_ms": round(launch_s * 1e3, 3), "flop_per_neuron_step": POINT_FLOP},
+                     "frac": round(flops / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": kernel,
+                     "launch_ms": round(launch_s * 1e3, 3), "flop_per_neuron_step": flop_step},
         "cpu_baseline": cpu}), flush=True)
 
 

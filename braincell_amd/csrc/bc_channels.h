@@ -18,6 +18,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #define BC_CH_NA   (1u << 0)
 #define BC_CH_KDR  (1u << 1)
 #define BC_CH_IL   (1u << 2)
@@ -127,9 +129,10 @@ __host__ __device__ __forceinline__ real bc_exprel(real x) {
 // x / (e^x - 1) = 1 / exprel(x).  FP32: Bernoulli series for |x| <= 1/2
 // (truncation below 3e-11, far under an FP32 ulp), x * rcp(e^x - 1) (no
 // cancellation there) otherwise.  FP64: the IEEE quotient 1 / exprel(x).
+// Other number types (the dual numbers of bc_dual.h) take the FP32 form.
 template <typename real, typename M = bc_math<real>>
 __device__ __forceinline__ real bc_inv_exprel(real x) {
-  if constexpr (sizeof(real) == 8) return real(1) / bc_exprel<real>(x);
+  if constexpr (std::is_same<real, double>::value) return real(1) / bc_exprel<real>(x);
   if (fabs(x) <= real(0.5)) {
     const real x2 = x * x;
     return real(1) - real(0.5) * x +

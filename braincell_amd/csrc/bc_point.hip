@@ -17,6 +17,7 @@
 
 #include "bc_channels.h"
 #include "bc_common.h"
+#include "bc_dual.h"
 
 namespace {
 
@@ -161,8 +162,185 @@ __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
   if (bad) atomicOr(a.status, 1u);
 }
 
+// Gate terms of the linearly implicit steps: f, df/dV (from the dual rates)
+// and lambda = df/dx.
+template <typename real>
+__device__ __forceinline__ void ab_terms(real x, bc_dual<real> a, bc_dual<real> b, real phi, real& f, real& fv,
+                                         real& lam) {
+  typedef bc_dual<real> D;
+  const D F = D(phi) * (a * D(real(1) - x) - b * D(x));
+  f = F.v;
+  fv = F.d;
+  lam = -phi * (a.v + b.v);
+}
+
+template <typename real>
+__device__ __forceinline__ void it_terms(real x, bc_dual<real> inf, bc_dual<real> tau, real phi, real& f, real& fv,
+                                         real& lam) {
+  typedef bc_dual<real> D;
+  const D it = bc_rcp<D>(tau);
+  const D F = D(phi) * (inf - D(x)) * it;
+  f = F.v;
+  fv = F.d;
+  lam = -phi * it.v;
+}
+
+// Linearly implicit steps over the 11 states y = (V, nine gates, Ci):
+// (I - theta dt J) dy = dt f(y_n), y_{n+1} = y_n + dy, with J the full
+// Jacobian at y_n.  theta = 1 is braincell's backward_euler
+// (braincell/quad/_backward_euler.py:31-64: one solve with the Jacobian at
+// the step start); theta = 1/2 is the Newton iterate of its implicit_euler
+// (braincell/quad/_implicit.py:43-120, order 2, started from y_n:
+// y_n - (I - dt/2 J)^-1 (-dt f(y_n))).  J is sparse: V couples to every state
+// (conductance partials times driving force, and Ci through the Nernst
+// E_Ca), each gate to V and itself (AHP: to Ci instead of V), Ci to V, the
+// four Ca gates and itself while the pool's drive is positive.  The gates
+// are eliminated onto (V, Ci) and the remaining 2x2 system is solved by
+// Cramer's rule; gate pivots 1 + theta dt phi (a + b) are >= 1.
+template <typename real>
+__global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, real theta) {
+  typedef bc_math<real> M;
+  typedef bc_dual<real> D;
+  typedef bc_dmath<real> DM;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int64_t n = a.n;
+  real V = a.V[i];
+  real s[10], lo[10], vlo = real(0);
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    s[k] = a.S[k * n + i];
+    lo[k] = real(0);
+  }
+  real g[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) g[k] = a.g[k * n + i];
+  const real Iext = a.Iext[i];
+  const real dt = a.dt;
+  const real h = theta * dt;
+  const bc_chan_const<real>& cc = a.cc;
+  const real invC = real(1) / a.Cm;
+  uint32_t nspk = 0, bad = 0;
+  for (int step = 0; step < a.n_steps; ++step) {
+    const real ci = s[9];
+    const real eca = cc.ca_rtzf * M::lg(cc.ca_Co / ci);
+    const real deca = -cc.ca_rtzf / ci;  // dE_Ca / dCi
+    // conductances and their partial derivatives in the gates
+    const real p3 = s[0] * s[0] * s[0];
+    const real g_na = g[0] * p3 * s[1];
+    const real dna_p = g[0] * real(3) * s[0] * s[0] * s[1], dna_q = g[0] * p3;
+    const real n3 = s[2] * s[2] * s[2];
+    const real g_kd = g[2] * n3 * s[2], dkd = g[2] * real(4) * n3;
+    const real g_cl = g[3] * s[3] * s[3] * s[4];
+    const real dcl_p = g[3] * real(2) * s[3] * s[4], dcl_q = g[3] * s[3] * s[3];
+    const real g_ct = g[4] * s[5] * s[5] * s[6];
+    const real dct_p = g[4] * real(2) * s[5] * s[6], dct_q = g[4] * s[5] * s[5];
+    const real g_ah = g[5] * s[7] * s[7], dah = g[5] * real(2) * s[7];
+    const real g_hc = g[6] * s[8];
+    const real g_kl = g[1], g_il = g[7];
+    const real G = g_na + g_kl + g_kd + g_cl + g_ct + g_ah + g_hc + g_il;
+    const real I = Iext + g_na * (cc.E_Na - V) + (g_kl + g_kd + g_ah) * (cc.E_K - V) + (g_cl + g_ct) * (eca - V) +
+                   g_hc * (cc.hcn_E - V) + g_il * (cc.E_IL - V);
+    const real fV = I * invC, JVV = -G * invC;
+    real JV[10];  // d fV / d (gate k), k < 9, and d fV / dCi
+    JV[0] = dna_p * (cc.E_Na - V) * invC;
+    JV[1] = dna_q * (cc.E_Na - V) * invC;
+    JV[2] = dkd * (cc.E_K - V) * invC;
+    JV[3] = dcl_p * (eca - V) * invC;
+    JV[4] = dcl_q * (eca - V) * invC;
+    JV[5] = dct_p * (eca - V) * invC;
+    JV[6] = dct_q * (eca - V) * invC;
+    JV[7] = dah * (cc.E_K - V) * invC;
+    JV[8] = g[6] * (cc.hcn_E - V) * invC;
+    JV[9] = (g_cl + g_ct) * deca * invC;
+    real f[9], fv[9], lam[9];
+    const D Vd = D::of(V, real(1));
+    {
+      D ap, bp, aq, bq;
+      bc_na_rates<D, DM>(Vd, D(cc.na_vsh), ap, bp, aq, bq);
+      ab_terms<real>(s[0], ap, bp, cc.na_phi, f[0], fv[0], lam[0]);
+      ab_terms<real>(s[1], aq, bq, cc.na_phi, f[1], fv[1], lam[1]);
+    }
+    {
+      D ap, bp;
+      bc_kdr_rates<D, DM>(Vd, D(cc.kdr_vsh), ap, bp);
+      ab_terms<real>(s[2], ap, bp, cc.kdr_phi, f[2], fv[2], lam[2]);
+    }
+    {
+      D pi, pt, qi, qt;
+      bc_cal_rates<D, DM>(Vd, D(cc.cal_vsh), pi, pt, qi, qt);
+      it_terms<real>(s[3], pi, pt, cc.cal_phi_p, f[3], fv[3], lam[3]);
+      it_terms<real>(s[4], qi, qt, cc.cal_phi_q, f[4], fv[4], lam[4]);
+    }
+    {
+      D pi, pt, qi, qt;
+      bc_cat_rates<D, DM>(Vd, D(cc.cat_vsh), pi, pt, qi, qt);
+      it_terms<real>(s[5], pi, pt, cc.cat_phi_p, f[5], fv[5], lam[5]);
+      it_terms<real>(s[6], qi, qt, cc.cat_phi_q, f[6], fv[6], lam[6]);
+    }
+    // AHP: alpha = alpha0 Ci^n depends on Ci, not on V
+    const real al = cc.ahp_alpha * M::pw(ci, cc.ahp_n);
+    f[7] = cc.ahp_phi * (al * (real(1) - s[7]) - cc.ahp_beta * s[7]);
+    lam[7] = -cc.ahp_phi * (al + cc.ahp_beta);
+    fv[7] = cc.ahp_phi * (al * cc.ahp_n / ci) * (real(1) - s[7]);  // d f / dCi
+    {
+      D pi, pt;
+      bc_hcn_rates<D, DM>(Vd, pi, pt);
+      it_terms<real>(s[8], pi, pt, cc.hcn_phi, f[8], fv[8], lam[8]);
+    }
+    // Ca pool
+    const real gca = g_cl + g_ct;
+    const real raw = gca * (eca - V) * cc.ca_drive;
+    const bool pos = raw > real(0);
+    const real fC = (pos ? raw : real(0)) + (cc.ca_rest - ci) * cc.ca_tau_inv;
+    const real JCC = -cc.ca_tau_inv + (pos ? cc.ca_drive * gca * deca : real(0));
+    const real JCV = pos ? -cc.ca_drive * gca : real(0);
+    const real kc = pos ? cc.ca_drive * (eca - V) : real(0);
+    const real JC[4] = {kc * dcl_p, kc * dcl_q, kc * dct_p, kc * dct_q};
+    // eliminate the gates: dy_k = A_k + B_k dV (AHP: A_7 + B_7 dCi)
+    real m11 = real(1) - h * JVV, m12 = -h * JV[9], r1 = dt * fV;
+    real m21 = -h * JCV, m22 = real(1) - h * JCC, r2 = dt * fC;
+    real A[9], B[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const real piv = real(1) / (real(1) - h * lam[k]);
+      A[k] = dt * f[k] * piv;
+      B[k] = h * fv[k] * piv;
+      if (k == 7) {
+        m12 -= h * JV[k] * B[k];
+      } else {
+        m11 -= h * JV[k] * B[k];
+      }
+      r1 += h * JV[k] * A[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      m21 -= h * JC[k] * B[3 + k];
+      r2 += h * JC[k] * A[3 + k];
+    }
+    const real idet = real(1) / (m11 * m22 - m12 * m21);
+    const real dV = (r1 * m22 - m12 * r2) * idet;
+    const real dC = (m11 * r2 - m21 * r1) * idet;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) acc2<real>(s[k], lo[k], A[k] + B[k] * (k == 7 ? dC : dV));
+    acc2<real>(s[9], lo[9], dC);
+    real Vn = V;
+    acc2<real>(Vn, vlo, dV);
+    nspk += (V < a.V_th && Vn >= a.V_th) ? 1u : 0u;
+    bad |= (Vn == Vn && fabs(Vn) < real(1e30)) ? 0u : 1u;
+    V = Vn;
+    if (i < a.n_trace) a.trace[(int64_t)step * a.n_trace + i] = V;
+  }
+  a.V[i] = V;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) a.S[k * n + i] = s[k];
+  if (a.spikes) a.spikes[i] += nspk;
+  if (bad) atomicOr(a.status, 1u);
+}
+
 template <typename real>
 int launch_point(const bc_point_desc* d, hipStream_t st) {
+  if (d->n == 0 || d->n_steps == 0) return BC_OK;
   PointArgs<real> a;
   a.n = d->n;
   a.V = static_cast<real*>(d->V);
@@ -178,10 +356,14 @@ int launch_point(const bc_point_desc* d, hipStream_t st) {
   a.Cm = static_cast<real>(d->Cm);
   a.V_th = static_cast<real>(d->V_th);
   a.cc = *static_cast<const bc_chan_const<real>*>(d->chan_const);
-  if (d->n == 0 || d->n_steps == 0) return BC_OK;
   const int bs = 256;
   dim3 grid((unsigned)((d->n + bs - 1) / bs));
-  hipLaunchKernelGGL(point_kernel<real>, grid, dim3(bs), 0, st, a);
+  if (d->method == BC_POINT_IND_EXP_EULER) {
+    hipLaunchKernelGGL(point_kernel<real>, grid, dim3(bs), 0, st, a);
+  } else {
+    const real theta = d->method == BC_POINT_BACKWARD_EULER ? real(1) : real(0.5);
+    hipLaunchKernelGGL(point_implicit_kernel<real>, grid, dim3(bs), 0, st, a, theta);
+  }
   BC_HIP(hipGetLastError());
   return BC_OK;
 }
@@ -196,6 +378,9 @@ int bc_point_run(const bc_point_desc* d, void* stream) {
   if (d == nullptr) return bc::fail(BC_INVALID_ARG, "desc is NULL");
   if (d->precision != 32 && d->precision != 64) return bc::fail(BC_INVALID_ARG, "precision must be 32 or 64");
   if (d->n < 0 || d->n_steps < 0) return bc::fail(BC_INVALID_ARG, "n and n_steps must be >= 0");
+  if (d->method != BC_POINT_IND_EXP_EULER && d->method != BC_POINT_BACKWARD_EULER &&
+      d->method != BC_POINT_IMPLICIT_EULER)
+    return bc::fail(BC_INVALID_ARG, "unknown point-neuron method %d", d->method);
   if (!(d->dt > 0.0) || !(d->Cm > 0.0)) return bc::fail(BC_INVALID_ARG, "dt and Cm must be > 0");
   if (d->n > 0 && (d->V == nullptr || d->state == nullptr || d->g == nullptr || d->I_ext == nullptr ||
                    d->status == nullptr || d->chan_const == nullptr))

@@ -7,7 +7,12 @@ Registered here:
 - ``"staggered_mi355x"`` (alias ``"stagger_mi355x"``): the staggered step
   (``braincell/quad/_staggered.py:47-154``) on the persistent HIP kernel;
 - ``"ind_exp_euler_mi355x"``: independent exp-Euler for point neurons
-  (``braincell/quad/_exp_euler.py:217-320``) on the point-neuron kernel.
+  (``braincell/quad/_exp_euler.py:217-320``) on the point-neuron kernel;
+- ``"backward_euler_mi355x"`` / ``"implicit_euler_mi355x"``: the linearly
+  implicit point-neuron steps (``quad/_backward_euler.py:31-160``,
+  ``quad/_implicit.py:43-120, 313-368``) on the same kernel file;
+- ``"dhs_voltage_mi355x"``: the implicit-Euler voltage solve alone
+  (``quad/_staggered.py:187-320``).
 """
 
 from __future__ import annotations
@@ -19,7 +24,8 @@ from ._registry import IntegratorEntry, IntegratorRegistry, get_registry, regist
 
 __all__ = [
     "IntegratorEntry", "IntegratorRegistry", "get_registry", "register_integrator", "get_integrator",
-    "all_integrators", "staggered_mi355x_step", "ind_exp_euler_mi355x_step",
+    "all_integrators", "staggered_mi355x_step", "ind_exp_euler_mi355x_step", "dhs_voltage_mi355x_step",
+    "backward_euler_mi355x_step", "implicit_euler_mi355x_step",
 ]
 
 
@@ -102,3 +108,42 @@ def ind_exp_euler_mi355x_step(target, *args):
     t = environ.get("t", 0.0)
     dt = environ.get_dt()
     return target.advance(t, dt, 1, *args)
+
+
+ind_exp_euler_mi355x_step.point_method = 0
+
+
+def _point_step(target, name, method, args):
+    from ..single import SingleCompartment
+
+    if not isinstance(target, SingleCompartment):
+        raise TypeError(f"{name} supports SingleCompartment, got {type(target)}.")
+    t = environ.get("t", 0.0)
+    dt = environ.get_dt()
+    return target.advance(t, dt, 1, *args, method=method)
+
+
+@register_integrator("backward_euler_mi355x", category="implicit", order=1,
+                     description="Linearised backward Euler for point neurons on the MI355X HIP kernel.")
+def backward_euler_mi355x_step(target, *args):
+    """One step of ``backward_euler`` (``quad/_backward_euler.py:31-64``):
+    ``(I - dt J) dy = dt f(y_n)`` with the full Jacobian of a
+    :class:`~braincell_amd.single.SingleCompartment` neuron's 11 states."""
+    return _point_step(target, "backward_euler_mi355x", 1, args)
+
+
+backward_euler_mi355x_step.point_method = 1
+
+
+@register_integrator("implicit_euler_mi355x", category="implicit", order=1,
+                     description="Newton step of the order-2 implicit residual for point neurons on the MI355X HIP kernel.")
+def implicit_euler_mi355x_step(target, *args):
+    """One step of ``implicit_euler`` (``quad/_implicit.py:43-120, 313-368``):
+    the Newton iterate of y - y_n - dt/2 (f(y_n) + f(y)) = 0 from y_n, i.e.
+    ``(I - dt/2 J) dy = dt f(y_n)``.  The reference iterates further only
+    while the population-wide residual before an update is below 1e-5; those
+    iterates move y by the square of that residual and are not taken."""
+    return _point_step(target, "implicit_euler_mi355x", 2, args)
+
+
+implicit_euler_mi355x_step.point_method = 2

@@ -7,7 +7,8 @@ solver resolved through the integrator registry; ``update(I_ext)`` runs the
 solver for one step.  The kernel implements the thalamocortical channel set
 (``README.md:52-71`` of the reference): Na_Ba2002, K_Leak, KDR_Ba2002,
 CaL_IS2008, CaT_HM1992, AHP_De1994, HCN_HM1992, IL and a CalciumDetailed
-pool; channels left out get zero conductance.
+pool; channels left out get zero conductance.  Solvers: ``ind_exp_euler_mi355x``
+(default), ``backward_euler_mi355x`` and ``implicit_euler_mi355x``.
 """
 
 from __future__ import annotations
@@ -21,7 +22,7 @@ from ._native import chan_const_type, check, lib
 from .channels import CHANNELS, FARADAY, GAS_CONSTANT, IONS, celsius2kelvin
 from .quad import get_integrator
 
-__all__ = ["SingleCompartment", "PointDesc", "POINT_CHANNELS"]
+__all__ = ["SingleCompartment", "PointDesc", "POINT_CHANNELS", "IND_EXP_EULER", "BACKWARD_EULER", "IMPLICIT_EULER"]
 
 POINT_CHANNELS = ("Na_Ba2002", "K_Leak", "KDR_Ba2002", "CaL_IS2008", "CaT_HM1992", "AHP_De1994", "HCN_HM1992", "IL")
 
@@ -31,7 +32,11 @@ class PointDesc(ctypes.Structure):
                 ("dt", ctypes.c_double), ("Cm", ctypes.c_double), ("V_th", ctypes.c_double),
                 ("chan_const", ctypes.c_void_p), ("V", ctypes.c_void_p), ("state", ctypes.c_void_p),
                 ("g", ctypes.c_void_p), ("I_ext", ctypes.c_void_p), ("trace", ctypes.c_void_p),
-                ("n_trace", ctypes.c_int64), ("spikes", ctypes.c_void_p), ("status", ctypes.c_void_p)]
+                ("n_trace", ctypes.c_int64), ("spikes", ctypes.c_void_p), ("status", ctypes.c_void_p),
+                ("method", ctypes.c_int32)]
+
+# bc_point_desc.method values
+IND_EXP_EULER, BACKWARD_EULER, IMPLICIT_EULER = 0, 1, 2
 
 
 def _phi(q10, t, tr):
@@ -59,6 +64,8 @@ class SingleCompartment:
         self.length, self.radius, self.C, self.V_th = float(length), float(radius), float(C), float(V_th)
         self.V0 = V0
         self.solver = get_integrator(solver)
+        # the kernel method ``run`` uses; the registered point solvers carry it
+        self.method = getattr(self.solver, "point_method", IND_EXP_EULER)
         self.precision = int(precision)
         self.device = device
         self._dev = None
@@ -162,8 +169,9 @@ class SingleCompartment:
     def spikes(self):
         return self._dev["spikes"]
 
-    def desc(self, dt, n_steps, trace=None, n_trace=0):
+    def desc(self, dt, n_steps, trace=None, n_trace=0, method=None):
         d = PointDesc()
+        d.method = self.method if method is None else int(method)
         dv = self._dev
         p = lambda x: ctypes.c_void_p(x.data_ptr())
         d.precision, d.n, d.n_steps, d.dt, d.Cm, d.V_th = self.precision, self.size, int(n_steps), float(dt), self.C, self.V_th
@@ -174,7 +182,9 @@ class SingleCompartment:
         d.spikes, d.status = p(dv["spikes"]), p(dv["status"])
         return d
 
-    def advance(self, t, dt, n_steps=1, I_ext=None, trace=None, n_trace=0):
+    def advance(self, t, dt, n_steps=1, I_ext=None, trace=None, n_trace=0, method=None):
+        """``n_steps`` steps of ``dt`` ms with kernel ``method`` (``IND_EXP_EULER``,
+        ``BACKWARD_EULER`` or ``IMPLICIT_EULER``; default: the solver's)."""
         import torch
 
         if self._dev is None:
@@ -183,7 +193,7 @@ class SingleCompartment:
             self.set_input(I_ext)
         if trace is not None and (trace.dtype != self._dtype or trace.numel() < n_steps * n_trace):
             raise ValueError("trace must hold n_steps * n_trace values of the population dtype")
-        d = self.desc(dt, n_steps, trace, n_trace)
+        d = self.desc(dt, n_steps, trace, n_trace, method)
         check(lib().bc_point_run(ctypes.byref(d), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
         return trace
 

@@ -202,7 +202,19 @@ int bc_stagger_adjoint(const bc_adjoint_desc* desc, void* stream);
  *  g[k * n + i]      k = Na, K_Leak, KDR, CaL, CaT, AHP, HCN, IL (mS/cm^2)
  *  I_ext[i]          injected current density (uA/cm^2)
  *  trace[step * n_trace + i]  V of neurons 0 .. n_trace-1 after each step
- *  spikes[i]         += upward crossings of V_th */
+ *  spikes[i]         += upward crossings of V_th
+ *  method            BC_POINT_IND_EXP_EULER (0): ind_exp_euler, each state
+ *                    y_k + dt exprel(dt lam_k) f_k (braincell/quad/
+ *                    _exp_euler.py:217-320);
+ *                    BC_POINT_BACKWARD_EULER (1): (I - dt J) dy = dt f with
+ *                    the full 11-state Jacobian J at the step start
+ *                    (braincell/quad/_backward_euler.py:31-64);
+ *                    BC_POINT_IMPLICIT_EULER (2): (I - dt/2 J) dy = dt f, the
+ *                    Newton iterate of implicit_euler's order-2 residual
+ *                    started from y_n (braincell/quad/_implicit.py:43-120) */
+#define BC_POINT_IND_EXP_EULER 0
+#define BC_POINT_BACKWARD_EULER 1
+#define BC_POINT_IMPLICIT_EULER 2
 typedef struct bc_point_desc {
   int32_t precision;
   int64_t n;
@@ -217,12 +229,15 @@ typedef struct bc_point_desc {
   int64_t n_trace;
   uint32_t* spikes;
   uint32_t* status;
+  int32_t method;
 } bc_point_desc;
 
 int bc_point_supported(int precision);
-/* Replaces SingleCompartment.update with solver ind_exp_euler
- * (braincell/_single_compartment/base.py:252-278 ->
- * braincell/quad/_exp_euler.py:217-320) for n_steps steps. */
+/* Replaces SingleCompartment.update (braincell/_single_compartment/
+ * base.py:252-278) with solver ind_exp_euler (braincell/quad/
+ * _exp_euler.py:217-320), backward_euler (quad/_backward_euler.py:71-160)
+ * or implicit_euler (quad/_implicit.py:313-368), per desc->method, for
+ * n_steps steps. */
 int bc_point_run(const bc_point_desc* desc, void* stream);
 
 /* ------------------------------------------------------------------------

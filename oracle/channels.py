@@ -37,6 +37,17 @@ FARADAY = 96485.33212  # C/mol (CODATA 2018)
 GAS_CONSTANT = 8.314462618  # J/(mol K) (CODATA 2018)
 
 
+def _exprel(x):
+    """``exprel`` for real input; complex input (complex-step derivatives of
+    the rates) takes expm1(x)/x, with the Taylor series where |x| < 1e-6."""
+    if not np.iscomplexobj(x):
+        return exprel(x)
+    x = np.asarray(x)
+    small = np.abs(x) < 1e-6
+    xs = np.where(small, 1.0, x)
+    return np.where(small, 1.0 + x * (0.5 + x / 6.0), np.expm1(xs) / xs)
+
+
 def c2k(celsius: float) -> float:
     return float(celsius) + 273.15
 
@@ -51,9 +62,9 @@ def na_ba2002(V, V_sh=-50.0):
     """sodium.py:158-170 -> (alpha_p, beta_p, alpha_q, beta_q)."""
     Vp = V - V_sh
     t = Vp - 13.0
-    ap = 0.32 * 4.0 / exprel(-t / 4.0)
+    ap = 0.32 * 4.0 / _exprel(-t / 4.0)
     t = Vp - 40.0
-    bp = 0.28 * 5.0 / exprel(t / 5.0)
+    bp = 0.28 * 5.0 / _exprel(t / 5.0)
     aq = 0.128 * np.exp(-(Vp - 17.0) / 18.0)
     bq = 4.0 / (1.0 + np.exp(-(Vp - 40.0) / 5.0))
     return ap, bp, aq, bq
@@ -63,7 +74,7 @@ def kdr_ba2002(V, V_sh=-50.0):
     """potassium.py:206-214 -> (alpha_p, beta_p)."""
     Vp = V - V_sh
     t = Vp - 15.0
-    ap = 0.032 * 5.0 / exprel(-t / 5.0)
+    ap = 0.032 * 5.0 / _exprel(-t / 5.0)
     bp = 0.5 * np.exp(-(Vp - 10.0) / 40.0)
     return ap, bp
 
@@ -71,7 +82,7 @@ def kdr_ba2002(V, V_sh=-50.0):
 def na_hh1952(V, V_sh=-45.0):
     """sodium.py:384-398 -> (alpha_p, beta_p, alpha_q, beta_q)."""
     Vp = V - V_sh
-    ap = 1.0 / exprel(-(Vp - 5.0) / 10.0)
+    ap = 1.0 / _exprel(-(Vp - 5.0) / 10.0)
     bp = 4.0 * np.exp(-(Vp + 20.0) / 18.0)
     aq = 0.07 * np.exp(-(Vp + 20.0) / 20.0)
     bq = 1.0 / (1.0 + np.exp(-(Vp - 10.0) / 10.0))
@@ -81,7 +92,7 @@ def na_hh1952(V, V_sh=-45.0):
 def k_hh1952(V, V_sh=-45.0):
     """potassium.py:450-456 -> (alpha_p, beta_p)."""
     Vp = V - V_sh
-    ap = 0.1 / exprel(-(Vp + 10.0) / 10.0)
+    ap = 0.1 / _exprel(-(Vp + 10.0) / 10.0)
     bp = 0.125 * np.exp(-(Vp + 20.0) / 80.0)
     return ap, bp
 
@@ -108,7 +119,7 @@ def cal_is2008(V, V_sh=0.0):
 
 def ahp_de1994(Ci, alpha=48.0, beta=0.09, n=2.0):
     """potassium_calcium.py:179-185 -> (alpha_p, beta_p)."""
-    return alpha * np.power(Ci, n), beta * np.ones_like(np.asarray(Ci, dtype=float))
+    return alpha * np.power(Ci, n), beta * np.ones(np.shape(Ci))
 
 
 def hcn_hm1992(V):

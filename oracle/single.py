@@ -20,6 +20,20 @@ values (Jacobi), then sets y_k <- y_k + dt exprel(dt lam_k) f_k.
 
 The thalamocortical template is the reference README's HTC example
 (``README.md:52-71``).
+
+``implicit_step(dt, theta)`` restates the linearly implicit solvers on the
+same 11-state vector y = (V, nine gates, Ci):
+
+- ``backward_euler`` (``quad/_backward_euler.py:31-64``): J = df/dy at y_n
+  (``jacrev_last_dim``, exact derivatives), (I - dt J) dy = dt f(y_n);
+- ``implicit_euler`` (``quad/_implicit.py:43-120``, order 2): the Newton
+  iterate from y_n of y - y_n - dt/2 (f(y_n) + f(y)) = 0, i.e.
+  (I - dt/2 J) dy = dt f(y_n).  Further iterates happen in the reference
+  only while the population-wide residual before an update is below its
+  tol = 1e-5 and are not restated.
+
+J is taken by complex-step differentiation of ``derivatives`` (exact to
+rounding), and each neuron's 11 x 11 system is solved densely.
 """
 
 from __future__ import annotations
@@ -139,6 +153,38 @@ class HTCPopulationOracle:
     def _it(x, inf, tau, phi):
         return phi * (inf - x) / tau, -phi / tau
 
+    GATES = ("na_p", "na_q", "kd_p", "cl_p", "cl_q", "ct_p", "ct_q", "ahp_p", "hcn_p")
+
+    def state_vector(self):
+        """[11, N]: V, the nine gates in kernel slot order, Ci."""
+        return np.stack([self.V, *(self.s[k] for k in self.GATES), self.Ci])
+
+    def rhs(self, y):
+        """f(y) for y = [11, N] (real or complex)."""
+        d = self.derivatives(y[0], dict(zip(self.GATES, y[1:10])), y[10])
+        return np.stack([d["V"][0] * np.ones_like(y[0]), *(d[k][0] for k in self.GATES), d["Ci"][0]])
+
+    def jacobian(self, y, h=1e-30):
+        """J[i, k, n] = d f_i / d y_k of neuron n, by complex steps of size h."""
+        J = np.empty((11, 11, y.shape[1]))
+        for k in range(11):
+            yc = y.astype(complex)
+            yc[k] += 1j * h
+            J[:, k] = self.rhs(yc).imag / h
+        return J
+
+    def implicit_step(self, dt, theta):
+        """theta = 1: backward_euler; theta = 1/2: implicit_euler (order 2)."""
+        y = self.state_vector()
+        f = self.rhs(y)
+        J = self.jacobian(y)
+        lhs = np.eye(11)[None] - theta * dt * np.moveaxis(J, 2, 0)
+        dy = np.linalg.solve(lhs, (dt * f).T[..., None])[..., 0].T
+        y1 = y + dy
+        self.V = y1[0]
+        self.s = {k: y1[1 + j] for j, k in enumerate(self.GATES)}
+        self.Ci = y1[10]
+
     def step(self, dt):
         d = self.derivatives(self.V, self.s, self.Ci)
         Vn = ch.exp_euler(self.V, *d["V"], dt)
@@ -146,10 +192,14 @@ class HTCPopulationOracle:
         Cin = ch.exp_euler(self.Ci, *d["Ci"], dt)
         self.V, self.s, self.Ci = Vn, sn, Cin
 
-    def run(self, dt, n_steps, probes=None):
+    def run(self, dt, n_steps, probes=None, theta=None):
+        """theta None: ind_exp_euler; 1 or 1/2: ``implicit_step``."""
         probes = np.arange(self.N) if probes is None else np.asarray(probes)
         out = np.empty((n_steps, len(probes)))
         for n in range(n_steps):
-            self.step(dt)
+            if theta is None:
+                self.step(dt)
+            else:
+                self.implicit_step(dt, theta)
             out[n] = self.V[probes]
         return out

@@ -1,4 +1,5 @@
-"""GPU parity of the point-neuron (ind_exp_euler) kernel against the oracle."""
+"""GPU parity of the point-neuron kernels (ind_exp_euler, backward_euler,
+implicit_euler) against the oracle."""
 
 import numpy as np
 import pytest
@@ -15,13 +16,13 @@ def _htc(n, seed=1):
     return pop, channels
 
 
-def _run(n, steps, precision, n_trace):
+def _run(n, steps, precision, n_trace, solver="ind_exp_euler_mi355x"):
     import torch
     from braincell_amd.single import SingleCompartment
 
     pop, channels = _htc(n)
     sc = SingleCompartment(n, channels=channels, E_Na=W.HTC_IONS["E_Na"], E_K=W.HTC_IONS["E_K"],
-                           ca=W.HTC_IONS["ca"], V0=pop["V0"], precision=precision)
+                           ca=W.HTC_IONS["ca"], V0=pop["V0"], precision=precision, solver=solver)
     sc.init_state()
     sc.set_input(pop["I_dc_nA"])
     tr = sc.run(dt=0.025, duration=steps * 0.025, n_trace=n_trace)
@@ -29,14 +30,14 @@ def _run(n, steps, precision, n_trace):
     return tr.double().cpu().numpy(), pop, channels, sc
 
 
-def _oracle(pop, channels, n, steps):
+def _oracle(pop, channels, n, steps, theta=None):
     from oracle.single import HTCPopulationOracle
 
     g = {nm: pop["g"][nm][:n] for nm in pop["names"]}
     params = {nm: {k: v for k, v in W.HTC_CHANNELS[nm].items() if k != "g_max"} for nm in pop["names"]}
     o = HTCPopulationOracle(g, pop["I_dc_nA"][:n], params, W.HTC_IONS)
     o.reset_state(pop["V0"][:n])
-    return o.run(0.025, steps)
+    return o.run(0.025, steps, theta=theta)
 
 
 def test_point_fp64_matches_oracle(cuda):
@@ -91,3 +92,91 @@ def test_point_fp32_long_run_within_fp32_noise_floor(cuda):
     fail_floor = sum(rmse(noisy[:, i], ref[:, i]) > 0.05 for i in range(n))
     print(f"200 ms: fp32 kernel fails {fail_gpu}/{n}, FP64 + FP32-noise floor fails {fail_floor}/{n}")
     assert fail_gpu <= fail_floor + 2
+
+
+IMPLICIT = [("backward_euler_mi355x", 1.0), ("implicit_euler_mi355x", 0.5)]
+
+
+@pytest.mark.parametrize("solver,theta", IMPLICIT)
+def test_implicit_fp64_matches_oracle(cuda, solver, theta):
+    n, steps = 64, 2000
+    print(f"{solver} fp64, {n} neurons x {steps} steps", flush=True)
+    got, pop, ch, sc = _run(n, steps, 64, n, solver=solver)
+    assert sc.method == (1 if theta == 1.0 else 2)
+    ref = _oracle(pop, ch, n, steps, theta=theta)
+    err = np.abs(got - ref).max()
+    print(f"{solver} fp64 max |dV|:", err)
+    assert np.ptp(ref) > 50.0  # the population fires
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("solver,theta", IMPLICIT)
+def test_implicit_fp32_within_fp32_noise_floor(cuda, solver, theta):
+    """FP32 against the FP64 oracle over 100 ms.  A few fast-firing neurons
+    are chaotic under these steps: the FP64 oracle with FP32-sized relative
+    noise injected every step leaves the 0.05 mV bar on them too.  The kernel
+    may fail the bar only on as many neurons as that noise floor does, and
+    spike times agree to one step wherever it holds the bar."""
+    from oracle.single import HTCPopulationOracle
+
+    n, steps = 256, 4000
+    print(f"{solver} fp32, {n} neurons x {steps} steps", flush=True)
+    got, pop, ch, _ = _run(n, steps, 32, n, solver=solver)
+    ref = _oracle(pop, ch, n, steps, theta=theta)
+    errs = np.array([rmse(got[:, i], ref[:, i]) for i in range(n)])
+    params = {nm: {k: v for k, v in W.HTC_CHANNELS[nm].items() if k != "g_max"} for nm in pop["names"]}
+    o = HTCPopulationOracle(pop["g"], pop["I_dc_nA"], params, W.HTC_IONS)
+    o.reset_state(pop["V0"])
+    rng = np.random.default_rng(5)
+    noisy = np.empty((steps, n))
+    for k in range(steps):
+        o.implicit_step(0.025, theta)
+        o.V = o.V * (1 + 6e-8 * rng.standard_normal(n))
+        for key in o.s:
+            o.s[key] = o.s[key] * (1 + 6e-8 * rng.standard_normal(n))
+        noisy[k] = o.V
+    floor = np.array([rmse(noisy[:, i], ref[:, i]) for i in range(n)])
+    fail_gpu, fail_floor = int(np.sum(errs > 0.05)), int(np.sum(floor > 0.05))
+    print(f"{solver} fp32 RMSE median {np.median(errs):.3g}, worst {errs.max():.3g}; fails {fail_gpu}/{n}, "
+          f"FP64 + FP32-noise floor fails {fail_floor}/{n} (median {np.median(floor):.3g})")
+    assert np.median(errs) <= 1e-3
+    assert fail_gpu <= fail_floor + 2
+    nsp = 0
+    for i in np.flatnonzero(errs <= 0.05):
+        a, b = spike_times(got[:, i], 0.025), spike_times(ref[:, i], 0.025)
+        assert len(a) == len(b)
+        if len(b):
+            assert np.max(np.abs(a - b)) <= 0.025 + 1e-9
+        nsp += len(b)
+    assert nsp > 0
+
+
+def test_update_runs_the_registered_implicit_solver(cuda):
+    """``update`` through the registry equals single-step ``advance`` calls
+    with the backward-Euler method (and differs from ind_exp_euler)."""
+    import torch
+
+    from braincell_amd import environ
+    from braincell_amd.single import BACKWARD_EULER, SingleCompartment
+
+    pop, channels = _htc(32)
+    kw = dict(channels=channels, E_Na=W.HTC_IONS["E_Na"], E_K=W.HTC_IONS["E_K"], ca=W.HTC_IONS["ca"],
+              V0=pop["V0"], precision=64)
+    a = SingleCompartment(32, solver="backward_euler_mi355x", **kw)
+    b = SingleCompartment(32, **kw)
+    for sc in (a, b):
+        sc.init_state()
+        sc.set_input(pop["I_dc_nA"])
+    with environ.context(dt=0.025):
+        for k in range(40):
+            with environ.context(t=k * 0.025):
+                a.update()
+    c = SingleCompartment(32, **kw)
+    c.init_state()
+    c.set_input(pop["I_dc_nA"])
+    for k in range(40):  # one launch per step, as update() does
+        b.advance(k * 0.025, 0.025, 1, method=BACKWARD_EULER)
+        c.advance(k * 0.025, 0.025, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(a.V, b.V)
+    assert not torch.equal(a.V, c.V)

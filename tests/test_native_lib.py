@@ -56,3 +56,16 @@ def test_bad_descriptors_are_rejected_without_gpu():
         _native.check(L.bc_stagger_run(ctypes.byref(d), None))
     assert L.bc_graph_launch(None, None) == 1
     assert L.bc_graph_destroy(None) == 0
+
+
+def test_point_descriptor_method_is_checked_without_gpu():
+    from braincell_amd.single import PointDesc
+
+    L = _native.lib()
+    d = PointDesc()
+    d.precision, d.n, d.n_steps, d.dt, d.Cm = 64, 0, 1, 0.025, 1.0
+    d.method = 7
+    assert L.bc_point_run(ctypes.byref(d), None) == 1
+    assert b"method" in L.bc_last_error()
+    d.method = 2
+    assert L.bc_point_run(ctypes.byref(d), None) == 0  # n = 0: nothing to launch

@@ -102,3 +102,18 @@ def test_voltage_integrator_is_registered():
     assert get_registry().entry("dhs_voltage_mi355x").category == "voltage"
     with pytest.raises(TypeError):
         f(object(), 0.0, 0.025)
+
+
+def test_point_implicit_integrators_are_registered():
+    from braincell_amd.quad import get_integrator
+    from braincell_amd.quad._registry import get_registry
+    from braincell_amd.single import BACKWARD_EULER, IMPLICIT_EULER, IND_EXP_EULER, SingleCompartment
+
+    for name, method in (("backward_euler_mi355x", BACKWARD_EULER), ("implicit_euler_mi355x", IMPLICIT_EULER)):
+        e = get_registry().entry(name)
+        assert e.category == "implicit" and e.order == 1
+        sc = SingleCompartment(4, channels={"IL": {"g_max": 1e-4}}, solver=name)
+        assert sc.method == method
+        with pytest.raises(TypeError):
+            get_integrator(name)(object())
+    assert SingleCompartment(4, channels={"IL": {"g_max": 1e-4}}).method == IND_EXP_EULER
