@@ -44,7 +44,18 @@ class StaggerDesc(ctypes.Structure):
         ("probe_col", ctypes.c_void_p), ("chan_const", ctypes.c_void_p), ("V", ctypes.c_void_p), ("Vbnd", ctypes.c_void_p),
         ("state", ctypes.c_void_p), ("trace", ctypes.c_void_p), ("trace_ld", ctypes.c_int64),
         ("rec", ctypes.c_void_p), ("status", ctypes.c_void_p), ("step0", ctypes.c_int32), ("n_steps", ctypes.c_int32),
-        ("dt", ctypes.c_double), ("flags", ctypes.c_uint32),
+        ("dt", ctypes.c_double), ("flags", ctypes.c_uint32), ("syn", ctypes.c_void_p),
+    ]
+
+
+class SynapseDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_cells", ctypes.c_int32), ("spk_len", ctypes.c_int32), ("min_delay", ctypes.c_int32),
+        ("max_delay", ctypes.c_int32), ("v_th", ctypes.c_double),
+        ("pack_off", ctypes.c_void_p), ("row", ctypes.c_void_p), ("row_off", ctypes.c_void_p),
+        ("par", ctypes.c_void_p), ("state", ctypes.c_void_p), ("con_off", ctypes.c_void_p),
+        ("con_pre", ctypes.c_void_p), ("con_delay", ctypes.c_void_p), ("con_w", ctypes.c_void_p),
+        ("cv_cell", ctypes.c_void_p), ("spk", ctypes.c_void_p),
     ]
 
 

@@ -20,7 +20,7 @@ import numpy as np
 
 from .channels import IONS
 from .filter import resolve_region
-from .mech import CableProperty, Channel, CurrentClamp, CurrentProbe, Ion, MechanismProbe, StateProbe
+from .mech import CableProperty, Channel, CurrentClamp, CurrentProbe, Ion, MechanismProbe, StateProbe, Synapse
 from .model import CellModel
 from .policy import CVPerBranch, CVPolicy
 from .quad import get_integrator
@@ -40,7 +40,8 @@ class Cell:
     """A multi-compartment cell integrated by a registry-selected solver."""
 
     def __init__(self, morpho, *, cv_policy=None, solver="staggered_mi355x", precision: int = 32,
-                 V_th: float = 0.0, ion_channel_update_order: str = "family", device="cuda", name=None):
+                 V_th: float = 0.0, ion_channel_update_order: str = "family", device="cuda", name=None,
+                 pop_size=(1,)):
         self.morpho = morpho
         self.cv_policy = cv_policy or CVPerBranch(1)
         if not isinstance(self.cv_policy, CVPolicy):
@@ -54,8 +55,13 @@ class Cell:
         self._paint = []
         self._place = []
         self._probes = []
+        self._synapses = []
         self._runtime = None
         self._initialized = False
+        ps = (pop_size,) if isinstance(pop_size, int) else tuple(pop_size)
+        if len(ps) != 1 or int(ps[0]) <= 0:
+            raise ValueError(f"pop_size must be one positive int, got {pop_size!r}")
+        self.pop_size = (int(ps[0]),)
 
     # ---------------------------------------------------------- declarations
     @property
@@ -79,9 +85,16 @@ class Cell:
     def place(self, locset, *mechanisms) -> "Cell":
         probes = tuple(m for m in mechanisms if isinstance(m, (StateProbe, MechanismProbe, CurrentProbe)))
         clamps = tuple(m for m in mechanisms if isinstance(m, CurrentClamp))
+        synapses = tuple(m for m in mechanisms if isinstance(m, Synapse))
         for m in mechanisms:
-            if not isinstance(m, (CurrentClamp, StateProbe, MechanismProbe, CurrentProbe)):
-                raise TypeError(f"place() takes CurrentClamp or a probe, got {type(m).__name__}")
+            if not isinstance(m, (CurrentClamp, StateProbe, MechanismProbe, CurrentProbe, Synapse)):
+                raise TypeError(f"place() takes CurrentClamp, Synapse or a probe, got {type(m).__name__}")
+        have = {syn.instance_name for _, syn in self._synapses}
+        for syn in synapses:
+            if syn.instance_name in have:
+                raise ValueError(f"Multiple synapses share the instance name {syn.instance_name!r}.")
+            have.add(syn.instance_name)
+            self._synapses.append((locset, syn))
         names = {p.name for _, ps in self._probes for p in ps}
         for p in probes:
             if p.name in names:
@@ -142,12 +155,13 @@ class Cell:
                 for m in mechs:
                     for (t0, t1, amp) in m.segments():
                         clamps.append((b, x, t0, t1 - t0, amp))
+        synapses = [(syn.instance_name, syn, list(locset.locations(mo))) for locset, syn in self._synapses]
         ca = None
         if ca_params is not None:
             ca = {"mask_branch": ca_mask, **{k: ca_params[k] for k in ("d", "tau", "C_rest", "Co", "Ci0", "temp")}}
         counts = self.cv_policy.counts(mo, cm_branch=cm, ra_branch=ra)
         return CellModel(morph=mo, cv_per_branch=counts, cm=cm, ra=ra, V0=float(v0[0]),
-                         channels=channels, ca=ca, clamps=clamps, **E).validate()
+                         channels=channels, ca=ca, clamps=clamps, synapses=synapses, **E).validate()
 
     # ----------------------------------------------------------------- state
     def init_state(self, *, probes=None, threads: int = 256):

@@ -414,6 +414,7 @@ int bc_stagger_adjoint(const bc_adjoint_desc* d, void* stream) {
   if (f->flags & (BC_STEP_SKIP_GATES | BC_STEP_VOLTAGE_ONLY))
     return bc::fail(BC_UNSUPPORTED, "the adjoint differentiates the full step: BC_STEP_SKIP_GATES and "
                                     "BC_STEP_VOLTAGE_ONLY are not supported");
+  if (f->syn != nullptr) return bc::fail(BC_UNSUPPORTED, "the adjoint does not differentiate synapses");
   if (f->n_packs > 0 && (d->rec == nullptr || d->Vbar == nullptr || d->Sbar == nullptr || d->cls_grad == nullptr ||
                          (f->trace_ld > 0 && d->trace_bar == nullptr)))
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");

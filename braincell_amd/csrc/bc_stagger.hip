@@ -72,6 +72,20 @@ struct StaggerArgs {
   int32_t max_rows, max_chains, max_cls, max_levels;
   uint32_t flags;
   bc_chan_const<real> cc;
+  // synapses and spike delivery (network runs; unused by the plain kernels)
+  const int32_t* syn_pack_off;
+  const int32_t* syn_row;
+  const int32_t* syn_row_off;
+  const real* syn_par;
+  real* syn_state;
+  const int32_t* con_off;
+  const int32_t* con_pre;
+  const int32_t* con_delay;
+  const real* con_w;
+  const int32_t* cv_cell;
+  uint8_t* spk;
+  int32_t spk_len, n_cells;
+  real v_th;
 };
 
 constexpr uint32_t kRowMask = 0xFFFFu;
@@ -492,7 +506,39 @@ __host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int ma
   return p;
 }
 
-template <typename real, uint32_t CH, int MAXC>
+// Synapse pass (network runs), after phase A has written every row: each
+// synapse takes its due arrivals (a gather over its incoming contacts from
+// the spike history of earlier launches), adds its conductance and current
+// at V_n to its row and decays its states.  A thread owns whole synapse
+// rows, so the per-row sums are formed in a fixed order.
+template <typename real>
+__device__ __forceinline__ void synapse_pass(const StaggerArgs<real>& a, HRow<real>* R, const real* xv, int gstep,
+                                             int tid, int NT) {
+  const int j1 = a.syn_pack_off[blockIdx.x + 1];
+  for (int j = a.syn_pack_off[blockIdx.x] + tid; j < j1; j += NT) {
+    const int row = a.syn_row[j];
+    real gs = real(0), gse = real(0);
+    for (int s = a.syn_row_off[j]; s < a.syn_row_off[j + 1]; ++s) {
+      real drive = real(0);
+      for (int c = a.con_off[s]; c < a.con_off[s + 1]; ++c) {
+        const int src = gstep - a.con_delay[c];
+        if (src >= 0 && a.spk[(int64_t)(src % a.spk_len) * a.n_cells + a.con_pre[c]]) drive += a.con_w[c];
+      }
+      const real* q = a.syn_par + (int64_t)s * 6;
+      const real A = a.syn_state[2 * (int64_t)s] + drive * q[2];
+      const real B = a.syn_state[2 * (int64_t)s + 1] + drive * q[3];
+      const real kg = q[0] * (B - A);
+      gs += kg;
+      gse += kg * q[1];
+      a.syn_state[2 * (int64_t)s] = A - q[4] * A;
+      a.syn_state[2 * (int64_t)s + 1] = B - q[5] * B;
+    }
+    R[row].d += gs;
+    R[row].r += gse - gs * xv[row];
+  }
+}
+
+template <typename real, uint32_t CH, int MAXC, bool SYN>
 __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   constexpr int NS = bc_nstates(CH) > 0 ? bc_nstates(CH) : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -652,6 +698,10 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
     for (int k = tid; k < P.n_ch; k += NT) m_cnt[k] = 0;
     __syncthreads();
+    if constexpr (SYN) {
+      synapse_pass<real>(a, R, xv, gstep, tid, NT);
+      __syncthreads();
+    }
 
     // ---- phase B: Hines elimination as a dataflow over chains, no barrier.
     // Diagonal rebuilt from the zero row sums of the axial operator:
@@ -776,6 +826,10 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         const real Vn = vv[i] + dA[i];
         xv[info[i] & kRowMask] = Vn;
         advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt, ca_b, a.flags);
+        if constexpr (SYN) {
+          if (vv[i] < a.v_th && Vn >= a.v_th)
+            a.spk[(int64_t)(gstep % a.spk_len) * a.n_cells + a.cv_cell[(int64_t)P.cv0 + tid + i * NT]] = 1;
+        }
         vv[i] = Vn;
         bad |= (Vn == Vn && fabs(Vn) < real(1e30)) ? 0u : 1u;
         if (pcol[i] >= 0) a.trace[(int64_t)step * a.trace_ld + pcol[i]] = Vn;
@@ -811,27 +865,44 @@ typedef void (*kernel_f32)(StaggerArgs<float>);
 typedef void (*kernel_f64)(StaggerArgs<double>);
 
 template <typename real, uint32_t CH>
-void* pick_maxc(int maxc) {
+void* pick_maxc(int maxc, bool syn) {
+  if (syn) {  // network runs: up to 8 CVs per thread
+    switch (maxc) {
+      case 1: return (void*)&stagger_kernel<real, CH, 1, true>;
+      case 2: return (void*)&stagger_kernel<real, CH, 2, true>;
+      case 4: return (void*)&stagger_kernel<real, CH, 4, true>;
+      case 8: return (void*)&stagger_kernel<real, CH, 8, true>;
+      default: return nullptr;
+    }
+  }
   switch (maxc) {
-    case 1: return (void*)&stagger_kernel<real, CH, 1>;
-    case 2: return (void*)&stagger_kernel<real, CH, 2>;
-    case 4: return (void*)&stagger_kernel<real, CH, 4>;
-    case 8: return (void*)&stagger_kernel<real, CH, 8>;
-    case 16: return (void*)&stagger_kernel<real, CH, 16>;
+    case 1: return (void*)&stagger_kernel<real, CH, 1, false>;
+    case 2: return (void*)&stagger_kernel<real, CH, 2, false>;
+    case 4: return (void*)&stagger_kernel<real, CH, 4, false>;
+    case 8: return (void*)&stagger_kernel<real, CH, 8, false>;
+    case 16: return (void*)&stagger_kernel<real, CH, 16, false>;
     default: return nullptr;
   }
 }
 
 template <typename real>
-void* pick_kernel(uint32_t ch, int maxc) {
+void* pick_kernel(uint32_t ch, int maxc, bool syn = false) {
   switch (ch) {
-    case kSetPassive: return pick_maxc<real, kSetPassive>(maxc);
-    case kSetHH: return pick_maxc<real, kSetHH>(maxc);
-    case kSetHHCaL: return pick_maxc<real, kSetHHCaL>(maxc);
-    case kSetAll: return pick_maxc<real, kSetAll>(maxc);
-    case kSetHH1952: return pick_maxc<real, kSetHH1952>(maxc);
+    case kSetPassive: return pick_maxc<real, kSetPassive>(maxc, syn);
+    case kSetHH: return pick_maxc<real, kSetHH>(maxc, syn);
+    case kSetHHCaL: return pick_maxc<real, kSetHHCaL>(maxc, syn);
+    case kSetAll: return pick_maxc<real, kSetAll>(maxc, syn);
+    case kSetHH1952: return pick_maxc<real, kSetHH1952>(maxc, syn);
     default: return nullptr;
   }
+}
+
+// Clears the spike-history slots of steps step0 .. step0 + n_steps - 1.
+__global__ void spike_clear(uint8_t* spk, int spk_len, int n_cells, int step0, int n_steps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_steps * n_cells) return;
+  const int slot = (int)((step0 + i / n_cells) % spk_len);
+  spk[(int64_t)slot * n_cells + i % n_cells] = 0;
 }
 
 template <typename real>
@@ -871,6 +942,21 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->max_levels = d->max_levels;
   if (d->chan_const == nullptr) return bc::fail(BC_INVALID_ARG, "chan_const is NULL");
   a->cc = *static_cast<const bc_chan_const<real>*>(d->chan_const);
+  const bc_synapse_desc* y = d->syn;
+  a->syn_pack_off = y ? y->pack_off : nullptr;
+  a->syn_row = y ? y->row : nullptr;
+  a->syn_row_off = y ? y->row_off : nullptr;
+  a->syn_par = y ? static_cast<const real*>(y->par) : nullptr;
+  a->syn_state = y ? static_cast<real*>(y->state) : nullptr;
+  a->con_off = y ? y->con_off : nullptr;
+  a->con_pre = y ? y->con_pre : nullptr;
+  a->con_delay = y ? y->con_delay : nullptr;
+  a->con_w = y ? static_cast<const real*>(y->con_w) : nullptr;
+  a->cv_cell = y ? y->cv_cell : nullptr;
+  a->spk = y ? y->spk : nullptr;
+  a->spk_len = y ? y->spk_len : 1;
+  a->n_cells = y ? y->n_cells : 0;
+  a->v_th = y ? static_cast<real>(y->v_th) : real(0);
   return BC_OK;
 }
 
@@ -892,14 +978,23 @@ int validate(const bc_stagger_desc* d) {
   if (d->n_packs > 0 && (d->packs == nullptr || d->cv_info == nullptr || d->V == nullptr || d->Vbnd == nullptr ||
                          d->status == nullptr))
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
+  if (const bc_synapse_desc* y = d->syn) {
+    if (y->n_cells <= 0 || y->min_delay < 1 || y->max_delay < y->min_delay)
+      return bc::fail(BC_INVALID_ARG, "synapse desc: n_cells > 0 and 1 <= min_delay <= max_delay required");
+    if (y->pack_off == nullptr || y->row == nullptr || y->row_off == nullptr || y->par == nullptr ||
+        y->state == nullptr || y->con_off == nullptr || y->con_pre == nullptr || y->con_delay == nullptr ||
+        y->con_w == nullptr || y->cv_cell == nullptr || y->spk == nullptr)
+      return bc::fail(BC_INVALID_ARG, "synapse desc: required pointer is NULL");
+  }
   return BC_OK;
 }
 
 template <typename real>
 int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_steps, void* trace) {
-  void* fn = pick_kernel<real>(d->channels, d->cv_per_thread);
+  void* fn = pick_kernel<real>(d->channels, d->cv_per_thread, d->syn != nullptr);
   if (fn == nullptr)
-    return bc::fail(BC_UNSUPPORTED, "no kernel for channels=0x%x cv_per_thread=%d", d->channels, d->cv_per_thread);
+    return bc::fail(BC_UNSUPPORTED, "no kernel for channels=0x%x cv_per_thread=%d%s", d->channels, d->cv_per_thread,
+                    d->syn ? " with synapses" : "");
   StaggerArgs<real> a;
   int rc = fill_args<real>(d, &a);
   if (rc != BC_OK) return rc;
@@ -910,6 +1005,16 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "pack needs %zu bytes of LDS (> 160 KiB)", lds);
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (d->n_packs == 0 || n_steps == 0) return BC_OK;
+  if (const bc_synapse_desc* y = d->syn) {
+    if (n_steps > y->min_delay || y->spk_len < y->max_delay + n_steps || step0 < 0)
+      return bc::fail(BC_INVALID_ARG,
+                      "network launch of %d steps from step %d: need n_steps <= min_delay (%d), "
+                      "spk_len (%d) >= max_delay (%d) + n_steps and step0 >= 0",
+                      n_steps, step0, y->min_delay, y->spk_len, y->max_delay);
+    const int64_t n = (int64_t)n_steps * y->n_cells;
+    hipLaunchKernelGGL(spike_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y->spk, y->spk_len,
+                       y->n_cells, step0, n_steps);
+  }
   void* args[] = {&a};
   BC_HIP(hipLaunchKernel(fn, dim3(d->n_packs), dim3(d->threads), args, lds, s));
   return BC_OK;

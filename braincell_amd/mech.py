@@ -1,10 +1,11 @@
 """Mechanism declarations painted or placed on a :class:`~braincell_amd.cell.Cell`.
 
 Mirrors ``braincell.mech``: ``Channel`` / ``Ion`` (``mech/_density.py:315-496``),
-``CableProperty`` (``mech/_cable.py:42``) and ``CurrentClamp``
-(``mech/_point.py:86-141``).  Values are plain floats in fixed units:
+``CableProperty`` (``mech/_cable.py:42``), ``CurrentClamp``
+(``mech/_point.py:86-141``) and ``Synapse`` (``mech/_point.py:389-440``).  Values are plain floats in fixed units:
 
 - conductance densities ``g_max`` in S/cm^2 (as the reference README paints),
+  synaptic conductances and weights in uS,
 - voltages in mV, times in ms, currents in nA,
 - capacitance in uF/cm^2, axial resistivity in ohm cm,
 - concentrations in mM, lengths in um, temperatures in K.
@@ -16,7 +17,8 @@ from dataclasses import dataclass, field
 
 from .channels import CHANNELS, IONS, celsius2kelvin
 
-__all__ = ["Channel", "Ion", "CableProperty", "CurrentClamp", "StateProbe", "MechanismProbe", "CurrentProbe"]
+__all__ = ["Channel", "Ion", "CableProperty", "CurrentClamp", "StateProbe", "MechanismProbe", "CurrentProbe",
+           "Synapse"]
 
 
 @dataclass(frozen=True)
@@ -156,3 +158,83 @@ class CurrentProbe:
     def __post_init__(self):
         if (self.mechanism is None) == (self.ion is None):
             raise ValueError(f"Probe {self.name!r} must define exactly one of 'mechanism' and 'ion'.")
+
+
+_SYNAPSE_DEFAULTS = {
+    "ExpSyn": {"tau": 0.1, "e": 0.0, "weight": 1.0},
+    "Exp2Syn": {"tau1": 0.1, "tau2": 10.0, "e": 0.0, "weight": 1.0},
+}
+
+
+class Synapse:
+    """Registry-keyed synapse placed with ``Cell.place`` (``mech/_point.py:
+    389-440``): ``Synapse("ExpSyn", tau=2.0, e=0.0, weight=1.0, name="exp")``.
+
+    Types (``synapse/markov.py:65-200``, NEURON's expsyn / exp2syn):
+
+    - ``ExpSyn``: g' = -g / tau, an event adds its weight to g;
+    - ``Exp2Syn``: A' = -A / tau1, B' = -B / tau2, g = B - A, an event adds
+      weight * factor to A and B (factor normalises the peak of g to the
+      weight; tau1 is kept in [1e-9, 0.9999] * tau2).
+
+    The current is g (V - e) (outward positive).  A placement over several
+    locations is one pool; a contact's ``synapse_index`` picks the location.
+    """
+
+    __slots__ = ("synapse_type", "params", "name")
+
+    def __init__(self, synapse_type: str, /, *, name: str = None, **params):
+        if synapse_type not in _SYNAPSE_DEFAULTS:
+            raise ValueError(f"Unknown synapse type {synapse_type!r}; supported: {sorted(_SYNAPSE_DEFAULTS)}.")
+        unknown = set(params) - set(_SYNAPSE_DEFAULTS[synapse_type])
+        if unknown:
+            raise ValueError(f"{synapse_type} has no parameters {sorted(unknown)}.")
+        self.synapse_type = synapse_type
+        self.params = {k: float(v) for k, v in params.items()}
+        self.name = name
+
+    @property
+    def instance_name(self) -> str:
+        return self.name if self.name is not None else self.synapse_type
+
+    def resolved(self) -> dict:
+        return {**_SYNAPSE_DEFAULTS[self.synapse_type], **self.params}
+
+    def effective_tau1(self) -> float:
+        """``Exp2Syn._effective_tau1`` (``synapse/markov.py:152-156``)."""
+        p = self.resolved()
+        ratio = p["tau1"] / p["tau2"]
+        if ratio > 0.9999:
+            return p["tau2"] * 0.9999
+        if ratio < 1e-9:
+            return p["tau2"] * 1e-9
+        return p["tau1"]
+
+    def factor(self) -> float:
+        """``Exp2Syn._compute_factor`` (``synapse/markov.py:158-162``)."""
+        import math
+
+        p = self.resolved()
+        t1, t2 = self.effective_tau1(), p["tau2"]
+        tp = (t1 * t2) / (t2 - t1) * math.log(t2 / t1)
+        return 1.0 / (-math.exp(-(tp / t1)) + math.exp(-(tp / t2)))
+
+    def kernel_terms(self, dt: float):
+        """(e, fa, fb, ka, kb): event gains of the states A and B and their
+        exponential-Euler decay fractions over ``dt`` (y -= k y with k =
+        dt exprel(-dt / tau) / tau, ``quad/_exp_euler.py:462-465``)."""
+        import math
+
+        def k(tau):
+            x = -dt / tau
+            ex = 1.0 + x * (0.5 + x / 6.0) if abs(x) < 1e-8 else math.expm1(x) / x
+            return dt * ex / tau
+
+        p = self.resolved()
+        if self.synapse_type == "ExpSyn":
+            return p["e"], 0.0, 1.0, 0.0, k(p["tau"])
+        f = self.factor()
+        return p["e"], f, f, k(self.effective_tau1()), k(p["tau2"])
+
+    def __repr__(self):
+        return f"Synapse({self.synapse_type!r}, name={self.name!r}, **{self.params!r})"

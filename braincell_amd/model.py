@@ -32,6 +32,7 @@ class CellModel:
     ca : None or dict with "mask_branch" and CalciumDetailed parameters
     E_Na, E_K, E_Ca : fixed reversal potentials (mV)
     clamps : list of (branch, x, start_ms, duration_ms, amplitude_nA)
+    synapses : list of (instance name, mech.Synapse, [(branch, x), ...])
     """
 
     morph: object
@@ -45,6 +46,7 @@ class CellModel:
     E_K: float = IONS["PotassiumFixed"]["E"]
     E_Ca: float = IONS["CalciumFixed"]["E"]
     clamps: list = field(default_factory=list)
+    synapses: list = field(default_factory=list)
 
     @property
     def n_branches(self) -> int:

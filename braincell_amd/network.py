@@ -1,0 +1,370 @@
+"""Networks of multi-compartment cell populations: synapses and delayed
+spike delivery around the staggered step.
+
+Mirrors ``braincell.network`` (``network/core.py:28-182``,
+``engine.py:67-548``, ``delivery.py:255-410``, ``lowering.py:185-228``):
+named populations of identical cells (``Cell(pop_size=(n,))``), contact
+tables (``Connection``) from a presynaptic population to a named synapse
+pool of the postsynaptic cells with a weight (uS) and a delay per contact,
+and ``Network.run(dt=..., duration=...)`` returning
+``NetworkRunResult(time, traces, spikes)``.
+
+Step semantics (``engine.py:474-548``): at the start of step n each synapse
+receives the events that are due, i.e. sent at step n - delay; its states
+jump, the voltage solve sees its conductance, then its states decay (the
+runtime-synapse phase of the staggered step).  A cell spikes at step n when
+any of its CVs crosses V_th upwards during the step; the event arrives at
+step n + delay, with delays of ceil(delay / dt) steps and at least one
+(``lowering.py:185-228``).
+
+On the GPU all populations share one packed launch.  A launch runs at most
+min_delay steps, so every arrival it needs comes from spikes of earlier
+launches; the kernel gathers them per synapse, in a fixed order, from a
+spike history ring of at least max_delay + launch steps.  No atomics, so
+results do not depend on scheduling.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .mech import StateProbe
+
+__all__ = ["Population", "Connection", "NetworkRunResult", "Network", "delay_steps"]
+
+SPIKE_RECORDINGS = ("population", "none")
+
+
+def _name(value, what):
+    if not isinstance(value, str) or not value:
+        raise ValueError(f"{what} must be a non-empty string.")
+    return value
+
+
+def _index_array(value, name):
+    arr = np.asarray(value)
+    if arr.ndim == 0:
+        arr = arr.reshape(1)
+    if arr.ndim != 1:
+        raise ValueError(f"{name} must be one-dimensional.")
+    if arr.size and not np.issubdtype(arr.dtype, np.integer):
+        raise TypeError(f"{name} must hold integers.")
+    return arr.astype(np.int64)
+
+
+def delay_steps(delay_ms, dt_ms: float, quantization: str = "ceil") -> np.ndarray:
+    """Fixed-step delays (``lowering.py:185-228``): raw = delay / dt snapped to
+    the nearest integer within 1e-7, then ceil / floor / strict, at least 1."""
+    if quantization not in ("ceil", "floor", "strict"):
+        raise ValueError(f"Network delay_quantization must be 'ceil', 'floor', or 'strict', got {quantization!r}.")
+    d = np.asarray(delay_ms, dtype=float)
+    if np.any(d < 0.0):
+        raise ValueError("Connection delay must be >= 0.")
+    raw = d / float(dt_ms)
+    r = np.rint(raw)
+    raw = np.where(np.isclose(raw, r, rtol=1e-7, atol=1e-7), r, raw)
+    if quantization == "strict":
+        steps = np.rint(raw)
+        if not np.allclose(raw, steps, rtol=1e-9, atol=1e-9):
+            raise ValueError("Connection delay must be an integer multiple of dt when delay_quantization='strict'.")
+    elif quantization == "ceil":
+        steps = np.ceil(raw - 1e-12)
+    else:
+        steps = np.floor(raw + 1e-12)
+    return np.maximum(steps.astype(np.int64), 1)
+
+
+@dataclass
+class Population:
+    """A named population of identical cells (``core.py:28-80``)."""
+
+    name: str
+    cell: object
+
+    def __post_init__(self):
+        _name(self.name, "Population name")
+        ps = tuple(getattr(self.cell, "pop_size", ()))
+        if len(ps) != 1 or int(ps[0]) <= 0:
+            raise ValueError(f"A population needs a cell with a one-dimensional positive pop_size, got {ps!r}.")
+
+    @property
+    def size(self) -> int:
+        return int(self.cell.pop_size[0])
+
+
+@dataclass
+class Connection:
+    """Contact table between two populations (``core.py:82-165``): contact c
+    sends the spikes of ``pre_index[c]`` to location ``synapse_index[c]`` of
+    the synapse pool ``synapse`` on cell ``post_index[c]``.  ``weight`` (uS,
+    scalar or per contact; None: the pool's default weight) and ``delay``
+    (ms, scalar or per contact)."""
+
+    pre_population: str
+    post_population: str
+    pre_index: object
+    post_index: object
+    synapse: str
+    weight: object = None
+    delay: object = 0.0
+    synapse_index: object = None
+
+    def __post_init__(self):
+        _name(self.pre_population, "pre_population")
+        _name(self.post_population, "post_population")
+        _name(self.synapse, "synapse")
+        self.pre_index = _index_array(self.pre_index, "pre_index")
+        self.post_index = _index_array(self.post_index, "post_index")
+        if self.pre_index.shape != self.post_index.shape:
+            raise ValueError("Connection pre_index and post_index must have the same shape; "
+                             f"got {self.pre_index.shape!r} and {self.post_index.shape!r}.")
+        if self.synapse_index is None:
+            self.synapse_index = np.zeros(self.pre_index.shape, dtype=np.int64)
+        else:
+            self.synapse_index = _index_array(self.synapse_index, "synapse_index")
+            if self.synapse_index.shape != self.pre_index.shape:
+                raise ValueError("Connection synapse_index must have the same shape as pre_index.")
+
+    @property
+    def n_contact(self) -> int:
+        return int(self.pre_index.shape[0])
+
+    def per_contact(self, value, what):
+        arr = np.asarray(value, dtype=float)
+        if arr.ndim == 0:
+            return np.full(self.n_contact, float(arr))
+        if arr.shape != (self.n_contact,):
+            raise ValueError(f"Connection {what} must be scalar or shape {(self.n_contact,)!r}, got {arr.shape!r}.")
+        return arr
+
+
+@dataclass(frozen=True)
+class NetworkRunResult:
+    """``core.py:167-182``: step-start times (ms), traces[population][probe]
+    [n_steps, size (, n_locations)] and spikes[population] [n_steps, size]."""
+
+    time: np.ndarray
+    traces: dict = field(default_factory=dict)
+    spikes: dict = field(default_factory=dict)
+
+
+class Network:
+    """Populations and connections integrated together (``engine.py:67-548``)."""
+
+    def __init__(self, name: str = None):
+        self.name = name
+        self.populations = {}
+        self.connections = []
+
+    def __repr__(self):
+        return (f"Network(name={self.name!r}, populations={list(self.populations)}, "
+                f"connections={len(self.connections)})")
+
+    def add_population(self, name: str, cell) -> Population:
+        if name in self.populations:
+            raise ValueError(f"Population {name!r} already exists.")
+        pop = Population(name, cell)
+        self.populations[name] = pop
+        return pop
+
+    def add_connection(self, connection: Connection) -> Connection:
+        if not isinstance(connection, Connection):
+            raise TypeError(f"add_connection expects a Connection, got {type(connection).__name__}.")
+        for p in (connection.pre_population, connection.post_population):
+            if p not in self.populations:
+                raise ValueError(f"Unknown population {p!r}.")
+        pre, post = self.populations[connection.pre_population], self.populations[connection.post_population]
+        for idx, size, what in ((connection.pre_index, pre.size, "pre_index"),
+                                (connection.post_index, post.size, "post_index")):
+            if idx.size and (idx.min() < 0 or idx.max() >= size):
+                raise ValueError(f"Connection {what} out of range for a population of {size}.")
+        self.connections.append(connection)
+        return connection
+
+    # ------------------------------------------------------------------ build
+    def build(self, dt: float, *, delay_quantization: str = "ceil", chunk: int = 200, threads: int = 256):
+        """Lower every population into one packed layout with its synapse and
+        contact tables; returns a :class:`NetworkRuntimeTables`."""
+        if not self.populations:
+            raise ValueError("Network.run(...) requires at least one population.")
+        from .runtime import StaggerLayout
+
+        names = list(self.populations)
+        cells = [self.populations[n].cell for n in names]
+        precision = {c.precision for c in cells}
+        if len(precision) != 1:
+            raise ValueError("every population must use the same precision")
+        v_th = {float(c.V_th) for c in cells}
+        if len(v_th) != 1:
+            raise ValueError("every population must use the same V_th")
+        models, probes, probe_points = [], [], {}
+        pop_cell0, n_cells = {}, 0
+        for n, c in zip(names, cells):
+            m = c.lower()
+            pts = c._probe_points() if c._probes else []
+            for p, _ in pts:
+                if not isinstance(p, StateProbe):
+                    raise NotImplementedError(f"network runs record StateProbe voltages; {p!r} is not one")
+            cvs = sorted({cv for _, cvl in pts for cv in cvl})
+            probe_points[n] = (pts, cvs)
+            size = self.populations[n].size
+            pop_cell0[n] = n_cells
+            n_cells += size
+            models += [m] * size
+            probes += [np.asarray(cvs, dtype=np.int64)] * size
+        layout = StaggerLayout(models, threads=threads, probes=probes)
+        if layout.cv_per_thread > 8:
+            raise NotImplementedError("network kernels hold at most 8 CVs per thread (cells up to 2048 CVs)")
+        return _NetworkTables(self, names, layout, pop_cell0, n_cells, probe_points, float(dt),
+                              delay_quantization, int(chunk), precision.pop(), v_th.pop())
+
+    # -------------------------------------------------------------------- run
+    def run(self, *, dt: float, duration: float, delay_quantization: str = "ceil",
+            spike_recording: str = "population", chunk: int = 200) -> NetworkRunResult:
+        """Run for ``duration`` ms at step ``dt`` ms (``engine.py:294-366``)."""
+        if not (float(dt) > 0.0) or not (float(duration) > 0.0):
+            raise ValueError("Network.run(...) needs dt > 0 and duration > 0.")
+        if spike_recording not in SPIKE_RECORDINGS:
+            raise ValueError(f"spike_recording must be one of {SPIKE_RECORDINGS}, got {spike_recording!r} "
+                             "(per-CV 'full' recording is not kept by the kernel)")
+        n_steps = int(np.arange(0.0, float(duration), float(dt)).shape[0])
+        if n_steps == 0:
+            raise ValueError("Network.run(...) produced no timesteps; ensure duration > 0 and dt > 0.")
+        tabs = self.build(dt, delay_quantization=delay_quantization, chunk=chunk)
+        rt = tabs.runtime(n_steps)
+        trace = rt.new_trace(n_steps)
+        rt.run(n_steps, trace, chunk=tabs.launch_steps)
+        rt.torch.cuda.synchronize()
+        rt.check_status()
+        return tabs.result(rt, trace, n_steps, spike_recording)
+
+
+class _NetworkTables:
+    """Synapse, contact and spike-history tables of a network layout."""
+
+    def __init__(self, net, names, layout, pop_cell0, n_cells, probe_points, dt, quantization, chunk, precision,
+                 v_th):
+        self.net, self.names, self.L = net, names, layout
+        self.pop_cell0, self.n_cells, self.probe_points = pop_cell0, n_cells, probe_points
+        self.dt, self.precision, self.v_th = dt, precision, v_th
+        L = layout
+        # synapses: one per (cell, pool, location), ordered by (pack, row)
+        recs = []  # (pack, row, cell, pool, site, par)
+        for gid, c in enumerate(L.cells):
+            for (name, syn, loc) in c.synapses or []:
+                e, fa, fb, ka, kb = syn.kernel_terms(dt)
+                for j, (cv, area, cm) in enumerate(loc):
+                    row = int(L.cv_info[L.cell_cv0[gid] + cv] & 0xFFFF)
+                    k = dt / cm * 1e-3 / area  # dt/Cm times uS -> mS/cm^2
+                    recs.append((int(L.cell_pack[gid]), row, gid, name, j, (k, e, fa, fb, ka, kb),
+                                 syn.resolved()["weight"]))
+        recs.sort(key=lambda r: (r[0], r[1]))
+        self.syn_key = {(r[2], r[3], r[4]): s for s, r in enumerate(recs)}
+        self.syn_weight = np.asarray([r[6] for r in recs], dtype=np.float64)
+        self.syn_par = np.asarray([r[5] for r in recs], dtype=np.float64).reshape(-1, 6)
+        pack = np.asarray([r[0] for r in recs], dtype=np.int64)
+        row = np.asarray([r[1] for r in recs], dtype=np.int64)
+        new_row = np.ones(len(recs), dtype=bool)
+        if len(recs) > 1:
+            new_row[1:] = (pack[1:] != pack[:-1]) | (row[1:] != row[:-1])
+        starts = np.flatnonzero(new_row)
+        self.syn_row = row[starts].astype(np.int32)
+        self.syn_row_off = np.concatenate((starts, [len(recs)])).astype(np.int32)
+        row_pack = pack[starts]
+        self.syn_pack_off = np.searchsorted(row_pack, np.arange(L.n_packs + 1), side="left").astype(np.int32)
+        self.n_syn = len(recs)
+        # contacts, grouped by target synapse in connection order
+        tgt, pre, dly, w = [], [], [], []
+        for con in net.connections:
+            pool_of = self._pool_cells(con.post_population, con.synapse)
+            n = con.n_contact
+            post_gid = pop_cell0[con.post_population] + con.post_index
+            pre_gid = pop_cell0[con.pre_population] + con.pre_index
+            s = np.empty(n, dtype=np.int64)
+            for c_ in range(n):
+                key = (int(post_gid[c_]), con.synapse, int(con.synapse_index[c_]))
+                if key not in self.syn_key:
+                    raise ValueError(f"synapse_index {int(con.synapse_index[c_])} out of range for pool "
+                                     f"{con.synapse!r} ({pool_of} locations)")
+                s[c_] = self.syn_key[key]
+            weight = self.syn_weight[s] if con.weight is None else con.per_contact(con.weight, "weight")
+            tgt.append(s)
+            pre.append(pre_gid)
+            dly.append(delay_steps(con.per_contact(con.delay, "delay"), dt, quantization))
+            w.append(weight)
+        if tgt:
+            tgt, pre, dly, w = (np.concatenate(x) for x in (tgt, pre, dly, w))
+        else:
+            tgt = pre = dly = np.zeros(0, dtype=np.int64)
+            w = np.zeros(0)
+        order = np.argsort(tgt, kind="stable")
+        self.con_pre = pre[order].astype(np.int32)
+        self.con_delay = dly[order].astype(np.int32)
+        self.con_w = w[order]
+        self.con_off = np.searchsorted(tgt[order], np.arange(self.n_syn + 1), side="left").astype(np.int32)
+        self.min_delay = int(dly.min()) if dly.size else int(chunk)
+        self.max_delay = int(dly.max()) if dly.size else int(chunk)
+        self.launch_steps = max(1, min(int(chunk), self.min_delay))
+        cv_cell = np.empty(L.n_cv_total, dtype=np.int32)
+        for gid in range(len(L.cells)):
+            cv_cell[L.cell_cv0[gid]: L.cell_cv0[gid] + L.cell_ncv[gid]] = gid
+        self.cv_cell = cv_cell
+
+    def _pool_cells(self, population, pool):
+        g0 = self.pop_cell0[population]
+        for (name, _, loc) in self.L.cells[g0].synapses or []:
+            if name == pool:
+                return len(loc)
+        raise ValueError(f"Population {population!r} has no synapse pool {pool!r}.")
+
+    def runtime(self, n_steps: int):
+        """Device runtime with the synapse tables attached; the spike history
+        keeps the whole run (at least max_delay + one launch)."""
+        import torch
+
+        from ._native import SynapseDesc
+        from .runtime import StaggerRuntime
+
+        rt = StaggerRuntime(self.L, self.dt, precision=self.precision)
+        dev, dtype = rt.device, rt.dtype
+        t = lambda a, ty: torch.as_tensor(np.ascontiguousarray(a)).to(device=dev, dtype=ty)
+        nz = lambda a, ty, fill=0: t(a if a.size else np.full(1, fill), ty)
+        self.spk_len = max(self.max_delay + self.launch_steps, int(n_steps))
+        buf = {
+            "pack_off": t(self.syn_pack_off, torch.int32), "row": nz(self.syn_row, torch.int32),
+            "row_off": t(self.syn_row_off, torch.int32), "par": nz(self.syn_par.reshape(-1), dtype),
+            "state": torch.zeros(max(1, 2 * self.n_syn), dtype=dtype, device=dev),
+            "con_off": t(self.con_off, torch.int32), "con_pre": nz(self.con_pre, torch.int32),
+            "con_delay": nz(self.con_delay, torch.int32, 1), "con_w": nz(self.con_w, dtype),
+            "cv_cell": t(self.cv_cell, torch.int32),
+            "spk": torch.zeros(self.spk_len * self.n_cells, dtype=torch.uint8, device=dev),
+        }
+        d = SynapseDesc()
+        d.n_cells, d.spk_len = self.n_cells, self.spk_len
+        d.min_delay, d.max_delay = self.launch_steps if not self.con_pre.size else self.min_delay, self.max_delay
+        d.v_th = self.v_th
+        for k, v in buf.items():
+            setattr(d, k, v.data_ptr())
+        rt.synapse_desc = d
+        rt.synapse_buffers = buf
+        return rt
+
+    def result(self, rt, trace, n_steps, spike_recording) -> NetworkRunResult:
+        tr = trace.double().cpu().numpy()
+        L = self.L
+        traces, spikes = {}, {}
+        spk = rt.synapse_buffers["spk"].view(self.spk_len, self.n_cells)[:n_steps].cpu().numpy().astype(bool)
+        for n in self.names:
+            g0, size = self.pop_cell0[n], self.net.populations[n].size
+            pts, cvs = self.probe_points[n]
+            col = {cv: k for k, cv in enumerate(cvs)}
+            traces[n] = {}
+            for p, cvl in sorted(pts, key=lambda x: x[0].name):
+                cols = np.asarray([[L.cell_probe0[g0 + i] + col[cv] for cv in cvl] for i in range(size)])
+                v = tr[:, cols]  # [n_steps, size, n_loc]
+                traces[n][p.name] = v[:, :, 0] if len(cvl) == 1 else v
+            if spike_recording == "population":
+                spikes[n] = spk[:, g0:g0 + size]
+        return NetworkRunResult(time=np.arange(n_steps) * self.dt, traces=traces, spikes=spikes)

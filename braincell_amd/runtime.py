@@ -96,6 +96,7 @@ class _CellHost:
     cv_cls: np.ndarray  # [C]
     clamps: list  # (kind, local index (cv or row), t0, t1, amp)
     probes: np.ndarray  # CV indices
+    synapses: list = None  # (instance name, mech.Synapse, [(cv, area cm^2, Cm uF/cm^2) per site])
 
 
 class StaggerLayout:
@@ -119,7 +120,12 @@ class StaggerLayout:
         self.slots = state_slots(self.chset)
         self.cc = self._channel_constants()
         self._geom_cache = {}
-        self.cells = [self._lower_cell(m, probes) for m in self.models]
+        if isinstance(probes, list):  # one probe spec per model
+            if len(probes) != len(self.models):
+                raise ValueError("a list of probe specs needs one entry per model")
+            self.cells = [self._lower_cell(m, pr) for m, pr in zip(self.models, probes)]
+        else:
+            self.cells = [self._lower_cell(m, probes) for m in self.models]
         del self._geom_cache
         self._configure(threads, cv_per_thread, rows_per_pack)
 
@@ -258,7 +264,18 @@ class StaggerLayout:
                 raise ValueError(f"unknown probe spec {probes!r}")
         else:
             pr = np.asarray(probes, dtype=np.int64)
-        return _CellHost(geom, chains, uniq, cv_cls, clamps, pr)
+        syns = []
+        cmb = m.per_branch(m.cm)
+        for (name, syn, sites) in getattr(m, "synapses", []) or []:
+            loc = []
+            for (b, x) in sites:
+                kind, idx = geom.locate(int(b), float(x), ncvb)
+                if kind != "mid":
+                    raise ValueError(f"synapse {name!r} at branch {b}, x = {x} sits on a CV boundary node; "
+                                     "place synapses inside a CV")
+                loc.append((int(idx), float(geom.cv_area_um2[idx]) * 1e-8, float(cmb[geom.cv_branch[idx]])))
+            syns.append((name, syn, loc))
+        return _CellHost(geom, chains, uniq, cv_cls, clamps, pr, syns)
 
     def _plan_packs(self, cap_cv):
         """First-fit decreasing by row count, under the row and CV budgets."""
@@ -617,6 +634,8 @@ class StaggerRuntime:
         d.n_steps = int(n_steps)
         d.dt = self.dt
         d.flags = self.flags
+        syn = getattr(self, "synapse_desc", None)
+        d.syn = ctypes.cast(ctypes.pointer(syn), ctypes.c_void_p) if syn is not None else None
         return d
 
     def stream_handle(self):

@@ -91,6 +91,39 @@ typedef struct bc_pack {
  *                 when the chain tree of a pack does not resolve (an
  *                 inconsistent layout) (out)
  * real = float (precision 32) or double (precision 64). */
+/* Synapses and delayed spike delivery for a network run (the synaptic step
+ * either side of the cell update: braincell/network/engine.py:474-548,
+ * delivery.py:317-410; ExpSyn / Exp2Syn, braincell/synapse/markov.py:65-200).
+ * Every synapse has two exponential states A, B (uS) and conductance
+ * g = B - A; ExpSyn uses B alone (fa = 0).  At the start of step n a synapse
+ * receives drive = sum over its incoming contacts c of w_c * spike(pre_c,
+ * n - delay_c) and sets A += fa drive, B += fb drive; the voltage solve then
+ * sees the current g (V - e) at its CV (conductance k g in the row, k =
+ * dt / (Cm area) in the kernel's units), and afterwards A and B decay by
+ * exponential Euler (A -= ka A, ka = dt exprel(-dt / tau_a) / tau_a).  A
+ * cell spikes at step n when any of its CVs crosses v_th upwards (V_n <
+ * v_th <= V_{n+1}); the kernel writes that into the history ring spk.
+ * Every launch must satisfy n_steps <= min_delay (each step reads spikes of
+ * earlier launches only) and spk_len >= max_delay + n_steps. */
+typedef struct bc_synapse_desc {
+  int32_t n_cells;       /* cells of the layout (columns of spk) */
+  int32_t spk_len;       /* steps kept in the spike history ring */
+  int32_t min_delay;     /* smallest contact delay in steps (>= 1) */
+  int32_t max_delay;     /* largest contact delay in steps */
+  double v_th;           /* spike threshold (mV), every cell */
+  const int32_t* pack_off;  /* [n_packs + 1] synapse-row range of each pack */
+  const int32_t* row;       /* [n_synrow] pack-local DHS row of a synapse row */
+  const int32_t* row_off;   /* [n_synrow + 1] synapses of each synapse row */
+  const void* par;          /* [n_syn * 6] k, e, fa, fb, ka, kb */
+  void* state;              /* [2 * n_syn] A, B */
+  const int32_t* con_off;   /* [n_syn + 1] incoming contacts of each synapse */
+  const int32_t* con_pre;   /* [n_con] presynaptic cell */
+  const int32_t* con_delay; /* [n_con] delay in steps */
+  const void* con_w;        /* [n_con] weight (uS) */
+  const int32_t* cv_cell;   /* [n_cv_total] cell of each CV */
+  uint8_t* spk;             /* [spk_len * n_cells] spike history ring */
+} bc_synapse_desc;
+
 typedef struct bc_stagger_desc {
   int32_t precision;    /* 32 or 64 */
   uint32_t channels;    /* BC_CH_* set compiled into the kernel */
@@ -129,6 +162,7 @@ typedef struct bc_stagger_desc {
   int32_t n_steps;
   double dt;       /* ms */
   uint32_t flags;  /* BC_STEP_* */
+  const bc_synapse_desc* syn; /* NULL: no synapses */
 } bc_stagger_desc;
 
 /* bc_stagger_desc.flags */

@@ -1,0 +1,206 @@
+"""Oracle restatement (FP64) of a network run: populations of cells,
+ExpSyn / Exp2Syn synapses and delayed spike delivery through per-delay ring
+buffers.  Test infrastructure only.
+
+Per step n (``network/engine.py:474-548``):
+
+1. ``write_arrivals`` (``delivery.py:317-359``): every delivery block (the
+   contacts of one connection with one delay d) reads its ring slot at the
+   cursor, sums it into the target pool's ``pre_spike`` and clears it;
+2. ``_begin_step`` -> ``apply_discrete_events`` (``synapse/markov.py:
+   100-102, 180-184``): ExpSyn g += drive, Exp2Syn A, B += drive * factor;
+3. the cell step (``StaggeredOracle.step``) with each synapse's current
+   g (V - e) / area at its CV (``_multi_compartment/currents.py:138-163``),
+   then the synapse states by ind_exp_euler (``cell.py:2506-2527``):
+   y + dt exprel(-dt / tau) (-y / tau);
+4. ``get_spike`` (``_base.py:421-431``) per CV, reduced per cell with any
+   (``delivery.py:103-123``): V_n < V_th <= V_{n+1};
+5. ``enqueue_future_events`` (``delivery.py:362-397``): each block adds
+   w * spike(pre) at ring slot (cursor + d) mod (d + 1); cursors advance.
+
+Delays are ``max(ceil(delay / dt - 1e-12), 1)`` steps after snapping to the
+nearest integer within 1e-7 (``lowering.py:185-228``).  ``solver`` (a callable
+or a dict population -> callable) replaces step 3 with a toy voltage map, as
+the reference's network tests do; their synapses then never decay.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+from scipy.special import exprel
+
+from .staggered import StaggeredOracle
+
+
+def _steps(delay_ms, dt, quantization="ceil"):
+    raw = np.asarray(delay_ms, dtype=float) / dt
+    r = np.rint(raw)
+    raw = np.where(np.isclose(raw, r, rtol=1e-7, atol=1e-7), r, raw)
+    if quantization == "ceil":
+        s = np.ceil(raw - 1e-12)
+    elif quantization == "floor":
+        s = np.floor(raw + 1e-12)
+    else:
+        s = np.rint(raw)
+    return np.maximum(s.astype(np.int64), 1)
+
+
+def _exp2_factor(tau1, tau2):
+    """markov.py:152-162."""
+    ratio = tau1 / tau2
+    t1 = tau2 * 0.9999 if ratio > 0.9999 else (tau2 * 1e-9 if ratio < 1e-9 else tau1)
+    tp = (t1 * tau2) / (tau2 - t1) * math.log(tau2 / t1)
+    return t1, 1.0 / (-math.exp(-(tp / t1)) + math.exp(-(tp / tau2)))
+
+
+class _Pool:
+    """One synapse pool of one cell: states per location."""
+
+    def __init__(self, syn, cvs):
+        p = syn.resolved()
+        self.kind = syn.synapse_type
+        self.e = float(p["e"])
+        self.weight = float(p["weight"])
+        self.cvs = np.asarray(cvs, dtype=np.int64)
+        n = len(cvs)
+        self.A = np.zeros(n)
+        self.B = np.zeros(n)
+        if self.kind == "ExpSyn":
+            self.tau_a, self.tau_b, self.fa, self.fb = 1.0, float(p["tau"]), 0.0, 1.0
+        else:
+            t1, f = _exp2_factor(float(p["tau1"]), float(p["tau2"]))
+            self.tau_a, self.tau_b, self.fa, self.fb = t1, float(p["tau2"]), f, f
+
+    @property
+    def g(self):
+        return self.B - self.A
+
+    def apply(self, drive):
+        self.A = self.A + drive * self.fa
+        self.B = self.B + drive * self.fb
+
+    def decay(self, dt):
+        self.A = self.A + dt * exprel(-dt / self.tau_a) * (-self.A / self.tau_a)
+        self.B = self.B + dt * exprel(-dt / self.tau_b) * (-self.B / self.tau_b)
+
+
+class NetworkOracle:
+    """``populations``: list of (name, CellModel, size, V_th); ``connections``:
+    list of dicts with pre, post, pre_index, post_index, synapse,
+    synapse_index, weight (uS or None) and delay (ms)."""
+
+    def __init__(self, populations, connections, dt, *, quantization="ceil", solver=None, V_init=None):
+        self.dt = float(dt)
+        self.solver = solver
+        self.pops = {}
+        self.order = []
+        for name, model, size, v_th in populations:
+            cells, pools = [], []
+            for _ in range(size):
+                c = StaggeredOracle(model)
+                c.reset_state(model.V0 if V_init is None else V_init)
+                cp = {}
+                for (pname, syn, sites) in model.synapses:
+                    cvs = []
+                    for (b, x) in sites:
+                        kind, idx = c._locate(b, x)
+                        if kind != "mid":
+                            raise ValueError("synapses sit inside a CV")
+                        cvs.append(idx)
+                    cp[pname] = _Pool(syn, cvs)
+                cells.append(c)
+                pools.append(cp)
+            self.pops[name] = {"cells": cells, "pools": pools, "v_th": float(v_th), "size": size}
+            self.order.append(name)
+        # delivery blocks: (connection, delay) groups with a ring [d + 1, post, n_active]
+        self.blocks = []
+        for con in connections:
+            post = self.pops[con["post"]]
+            n_active = len(post["pools"][0][con["synapse"]].cvs)
+            n = len(con["pre_index"])
+            w = con.get("weight")
+            if w is None:
+                w = np.full(n, post["pools"][0][con["synapse"]].weight)
+            w = np.broadcast_to(np.asarray(w, dtype=float), (n,))
+            d = _steps(np.broadcast_to(np.asarray(con.get("delay", 0.0), dtype=float), (n,)), self.dt, quantization)
+            sidx = np.asarray(con.get("synapse_index", np.zeros(n, dtype=np.int64)), dtype=np.int64)
+            for dd in np.unique(d):
+                m = d == dd
+                self.blocks.append({
+                    "pre": con["pre"], "post": con["post"], "synapse": con["synapse"], "d": int(dd),
+                    "pre_index": np.asarray(con["pre_index"])[m], "post_index": np.asarray(con["post_index"])[m],
+                    "syn_index": sidx[m], "w": w[m],
+                    "ring": np.zeros((int(dd) + 1, post["size"], n_active)), "cursor": 0})
+        self.pre_spike = {}
+        self.spikes = {name: np.zeros(self.pops[name]["size"], dtype=bool) for name in self.order}
+
+    def step(self, n):
+        t = n * self.dt
+        # 1. due arrivals
+        grouped = {}
+        for b in self.blocks:
+            key = (b["post"], b["synapse"])
+            arr = b["ring"][b["cursor"]].copy()
+            grouped[key] = arr if key not in grouped else grouped[key] + arr
+            b["ring"][b["cursor"]] = 0.0
+        for key, arr in grouped.items():
+            self.pre_spike[key] = arr
+        for name in self.order:
+            P = self.pops[name]
+            for i in range(P["size"]):
+                c, pools = P["cells"][i], P["pools"][i]
+                # 2. discrete events
+                for pname, pool in pools.items():
+                    drive = self.pre_spike.get((name, pname))
+                    if drive is not None:
+                        pool.apply(drive[i])
+                V_old = c.V.copy()
+                # 3. cell step with the synapses' conductance, then their decay
+                toy = self.solver.get(name) if isinstance(self.solver, dict) else self.solver
+                if toy is not None:
+                    c.V = toy(c.V)
+                else:
+                    Gs = np.zeros(c.C)
+                    GEs = np.zeros(c.C)
+                    for pool in pools.values():
+                        gk = 1e-3 * pool.g / c.area_cm2[pool.cvs]  # uS -> mS/cm^2
+                        np.add.at(Gs, pool.cvs, gk)
+                        np.add.at(GEs, pool.cvs, gk * pool.e)
+                    c.point_conductance = (Gs, GEs)
+                    c.step(t, self.dt)
+                    for pool in pools.values():
+                        pool.decay(self.dt)
+                # 4. spike: any CV crossing V_th upwards
+                self.spikes[name][i] = bool(np.any((V_old < P["v_th"]) & (c.V >= P["v_th"])))
+        # 5. enqueue and advance
+        for b in self.blocks:
+            spk = self.spikes[b["pre"]][b["pre_index"]].astype(float)
+            ev = np.zeros(b["ring"].shape[1:])
+            np.add.at(ev, (b["post_index"], b["syn_index"]), b["w"] * spk)
+            tgt = (b["cursor"] + b["d"]) % b["ring"].shape[0]
+            b["ring"][tgt] += ev
+        for b in self.blocks:
+            b["cursor"] = (b["cursor"] + 1) % b["ring"].shape[0]
+
+    def run(self, n_steps, probe_cvs=None, pool_probes=None):
+        """Returns (V traces {pop: [n_steps, size, n_cv_probe]}, pool g traces
+        {(pop, pool): [n_steps, size, n_active]}, spikes {pop: [n_steps, size]})."""
+        probe_cvs = probe_cvs or {}
+        pool_probes = pool_probes or []
+        V = {k: np.zeros((n_steps, self.pops[k]["size"], len(v))) for k, v in probe_cvs.items()}
+        G = {k: np.zeros((n_steps, self.pops[k[0]]["size"], len(self.pops[k[0]]["pools"][0][k[1]].cvs)))
+             for k in pool_probes}
+        S = {k: np.zeros((n_steps, self.pops[k]["size"]), dtype=bool) for k in self.order}
+        for n in range(n_steps):
+            self.step(n)
+            for k, cvs in probe_cvs.items():
+                for i, c in enumerate(self.pops[k]["cells"]):
+                    V[k][n, i] = c.V[cvs]
+            for (pk, pn) in pool_probes:
+                for i, pools in enumerate(self.pops[pk]["pools"]):
+                    G[(pk, pn)][n, i] = pools[pn].g
+            for k in self.order:
+                S[k][n] = self.spikes[k]
+        return V, G, S

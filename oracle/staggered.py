@@ -81,6 +81,9 @@ class StaggeredOracle:
             kind, idx = self._locate(b, x)
             (self.mid_clamps if kind == "mid" else self.edge_clamps).append((idx, delay, dur, amp))
         self.V = None
+        # per-CV synaptic conductance (mS/cm^2) and conductance x reversal
+        # (uA/cm^2) for the next step, set by a network driver
+        self.point_conductance = None
 
     # ----------------------------------------------------------------- layout
     def _locate(self, b, x):
@@ -251,6 +254,10 @@ class StaggeredOracle:
         I_ca = self.ca_current(V, Ci) if self.ca is not None else None
         I, G = self.membrane_current(V, Ci)
         I = I + self.clamp_density(t)
+        if self.point_conductance is not None:  # synapses (currents.py:138-163): g (e - V) / area
+            Gs, GEs = self.point_conductance
+            I = I + GEs - Gs * V
+            G = G + Gs
         f = I / self.cm
         lin = -G / self.cm
         const = f - V * lin
