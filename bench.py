@@ -48,7 +48,8 @@ def cpu_baseline(models, soma_cvs, n_steps, gpu_soma, dt):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("cells", "point", "adjoint", "closure", "n144"), default="cells",
+    ap.add_argument("--workload", choices=("cells", "point", "adjoint", "closure", "n144", "network"),
+                    default="cells",
                     help="cells: 4096 uneven trees (default, the headline); point: 100k HTC point neurons; "
                          "adjoint: 512 region trees, forward + checkpointed adjoint; "
                          "closure: learned gating closure at dt 0.1 vs plain dt 0.025 on the 4096 trees")
@@ -80,6 +81,8 @@ def main():
         return bench_adjoint(args)
     if args.workload == "closure":
         return bench_closure(args)
+    if args.workload == "network":
+        return bench_network(args)
     if args.workload == "n144":
         return bench_n144(args)
     return bench_cells(args)
@@ -459,6 +462,140 @@ def bench_n144(args):
                      "gate": "RMSE <= 2 mV (simulator_compare/common.py:41-45)"},
     }
     print(json.dumps(out), flush=True)
+
+
+def _network_cell(size, amp, *, syn=None, syn_sites=((0, 0.5),)):
+    from braincell_amd.cell import Cell
+    from braincell_amd.filter import AllRegion, RootLocation
+    from braincell_amd.mech import CableProperty, Channel, CurrentClamp, StateProbe
+    from braincell_amd.morph import Branch, Morphology
+    from braincell_amd.policy import CVPerBranchList
+
+    m = Morphology.from_root(Branch.from_lengths(lengths=[20.0], radii=[10.0, 10.0], type="soma"))
+    m.attach(0, Branch.from_lengths(lengths=[200.0], radii=[1.5, 0.8], type="basal_dendrite"))
+    m.attach(0, Branch.from_lengths(lengths=[150.0], radii=[1.2, 0.6], type="basal_dendrite"))
+    c = Cell(m, cv_policy=CVPerBranchList((3, 8, 6)), pop_size=(size,), precision=32)
+    c.paint(AllRegion(), CableProperty(resting_potential=-65.0, axial_resistivity=150.0),
+            Channel("IL", g_max=3e-4, E=-54.4), Channel("Na_HH1952", g_max=0.12), Channel("K_HH1952", g_max=0.036))
+    if amp:
+        c.place(RootLocation(0.5), CurrentClamp(delay=1.0, durations=[1e4], amplitudes=[amp]))
+    if syn is not None:
+        class _Sites:
+            def locations(self, morpho):
+                return list(syn_sites)
+        c.place(_Sites(), syn)
+    c.place(RootLocation(0.5), StateProbe("v"))
+    return c
+
+
+def _network(scale, seed=11):
+    """E: four populations of 1000 * scale clamp-driven HH cells (17 CVs:
+    soma + two dendrites) carrying an inhibitory ExpSyn pool; I: 1000 * scale
+    cells with a two-site Exp2Syn pool.  Every I cell receives 100 * scale
+    contacts from random E cells (delays 1-3 ms), every E cell 20 * scale
+    from random I cells (delays 1-2 ms)."""
+    from braincell_amd.mech import Synapse
+    from braincell_amd.network import Connection, Network
+
+    rng = np.random.default_rng(seed)
+    n_e, n_i = max(1, int(1000 * scale)), max(1, int(1000 * scale))
+    fan_ei, fan_ie = max(1, int(100 * scale)), max(1, int(20 * scale))
+    net = Network()
+    gaba = Synapse("ExpSyn", tau=5.0, e=-80.0, weight=0.002, name="gaba")
+    for k, amp in enumerate((0.2, 0.3, 0.4, 0.5)):
+        net.add_population(f"E{k}", _network_cell(n_e, amp, syn=gaba))
+    ampa = Synapse("Exp2Syn", tau1=0.5, tau2=3.0, e=0.0, weight=0.004, name="ampa")
+    net.add_population("I", _network_cell(n_i, 0.0, syn=ampa, syn_sites=((1, 0.3), (2, 0.3))))
+    for k in range(4):
+        post = np.repeat(np.arange(n_i), fan_ei // 4)
+        pre = rng.integers(0, n_e, post.size)
+        net.add_connection(Connection(f"E{k}", "I", pre, post, "ampa", delay=rng.uniform(1.0, 3.0, post.size),
+                                      synapse_index=rng.integers(0, 2, post.size)))
+        post = np.repeat(np.arange(n_e), fan_ie)
+        net.add_connection(Connection("I", f"E{k}", rng.integers(0, n_i, post.size), post, "gaba",
+                                      delay=rng.uniform(1.0, 2.0, post.size)))
+    return net
+
+
+def bench_network(args):
+    """Synaptically coupled HH populations (5000 cells, 85 000 CVs, 180 000
+    contacts): synapse conductances in the voltage solve, in-kernel spike
+    detection and delayed delivery; launches of min_delay = 40 steps in a
+    hipGraph, dt 0.025 ms."""
+    import torch
+
+    dt = 0.025
+    net = _network(1.0)
+    t0 = time.perf_counter()
+    tabs = net.build(dt, chunk=200, autotune=not args.no_autotune)
+    L = tabs.L
+    log(f"network layout {L.summary()}, {tabs.n_syn} synapses, {tabs.con_pre.size} contacts, "
+        f"launch {tabs.launch_steps} steps, built in {time.perf_counter() - t0:.1f}s")
+    ls = tabs.launch_steps
+    K = max(ls, (args.steps // ls) * ls)
+    rt = tabs.runtime(K)
+    trace = rt.new_trace(K)
+    graph = rt.make_graph(K // ls, ls, trace)
+    stream = rt.stream_handle()
+    rt.save_state()
+    buf = rt.synapse_buffers
+    samples = []
+    for rep_ in range(1 + 3):
+        rt.restore_state()
+        buf["state"].zero_()
+        buf["spk"].zero_()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        graph.launch(stream)
+        torch.cuda.synchronize()
+        if rep_:
+            samples.append(time.perf_counter() - t1)
+    rt.check_status()
+    el = float(np.median(samples))
+    spk = buf["spk"].view(rt.synapse_desc.spk_len, tabs.n_cells)[:K].sum().item()
+    events = float(np.sum(np.bincount(tabs.con_pre, minlength=tabs.n_cells) *
+                          buf["spk"].view(rt.synapse_desc.spk_len, tabs.n_cells)[:K].sum(0).cpu().numpy()))
+    n_cells, n_cv = tabs.n_cells, int(L.n_cv_total)
+    val = n_cells * K / el
+    launch_s = el / (K // ls)
+    byts = ls * (76.0 * n_cv + 13.0 * tabs.con_pre.size + 40.0 * tabs.n_syn)
+    ach = byts / launch_s / 1e9
+    log(f"{n_cells} cells x {K} steps: {el * 1e3:.1f} ms, {val:.4g} cell-steps/s, {spk} spikes, "
+        f"{events:.4g} synaptic events")
+    cpu = None
+    if not args.no_cpu:
+        from oracle.network import NetworkOracle
+
+        sub = _network(0.01)
+        pops = [(n, p.cell.lower(), p.size, p.cell.V_th) for n, p in sub.populations.items()]
+        cons = [dict(pre=c.pre_population, post=c.post_population, pre_index=c.pre_index,
+                     post_index=c.post_index, synapse=c.synapse, synapse_index=c.synapse_index, weight=c.weight,
+                     delay=c.delay) for c in sub.connections]
+        o = NetworkOracle(pops, cons, dt)
+        ns = 200
+        t2 = time.perf_counter()
+        o.run(ns)
+        cel = time.perf_counter() - t2
+        nsub = sum(p.size for p in sub.populations.values())
+        cpu = {"value": float(f"{nsub * ns / cel:.4g}"), "unit": "cell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"NumPy FP64 network oracle, the same network at 1/100 scale ({nsub} cells) x {ns} steps, "
+                         f"{cel:.1f}s"}
+    print(json.dumps({
+        "metric": "cell_steps_per_second", "value": float(f"{val:.4g}"), "unit": "cell-steps/s", "n_gpus": 1,
+        "steps": K, "warmup": K, "ms_per_step": round(el / K * 1e3, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"network: 4 x {n_cells // 5} clamp-driven HH E cells + {n_cells // 5} I cells "
+                               f"(17 CVs each), {tabs.con_pre.size} contacts (Exp2Syn E->I, ExpSyn I->E, delays "
+                               f"1-3 ms), dt 0.025 ms", "cells": n_cells, "compartments": n_cv,
+                   "synapses": tabs.n_syn, "contacts": int(tabs.con_pre.size), "launch_steps": ls,
+                   "spikes": int(spk), "synaptic_events": events,
+                   "compartment_updates_per_s": float(f"{n_cv * K / el:.4g}")},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "stagger_kernel<float, HH1952, 1, synapses>", "launch_ms": round(launch_s * 1e3, 4),
+                     "bytes_per_launch": byts,
+                     "bytes_model": "76 B per CV-step + 13 B per contact-step + 40 B per synapse-step"},
+        "cpu_baseline": cpu}), flush=True)
 
 
 if __name__ == "__main__":

@@ -59,6 +59,19 @@ def candidate_plans(layout: StaggerLayout, precision: int = 32):
             if cpt * threads < ncv_max:
                 continue
             out.append((threads, cpt, rows))
+    # batches of many small cells: several cells per pack, a pack count
+    # that fills the device several times over
+    total_rows = sum(c.geom.n_point for c in layout.cells)
+    for target in (1024, 2048, 4096):
+        rows = max(nrow_max, total_rows // target)
+        if rows >= 3 * nrow_max or total_rows // target < nrow_max // 2:
+            continue
+        for threads in (64, 128):
+            cpt = 1
+            while cpt * threads < max(ncv_max, int(rows * 0.8)) and cpt < 16:
+                cpt *= 2
+            if cpt * threads >= ncv_max:
+                out.append((threads, cpt, rows))
     # dedupe, keep order
     seen, plans = set(), []
     for p in out:

@@ -509,32 +509,41 @@ __host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int ma
 // Synapse pass (network runs), after phase A has written every row: each
 // synapse takes its due arrivals (a gather over its incoming contacts from
 // the spike history of earlier launches), adds its conductance and current
-// at V_n to its row and decays its states.  A thread owns whole synapse
-// rows, so the per-row sums are formed in a fixed order.
+// at V_n to its row and decays its states.  One wavefront owns a synapse
+// row: its lanes gather the contacts of a synapse in parallel and a fixed
+// butterfly sums them, so the result does not depend on scheduling.
 template <typename real>
 __device__ __forceinline__ void synapse_pass(const StaggerArgs<real>& a, HRow<real>* R, const real* xv, int gstep,
                                              int tid, int NT) {
+  const int lane = tid & 63, wave = tid >> 6, n_waves = NT >> 6;
   const int j1 = a.syn_pack_off[blockIdx.x + 1];
-  for (int j = a.syn_pack_off[blockIdx.x] + tid; j < j1; j += NT) {
+  for (int j = a.syn_pack_off[blockIdx.x] + wave; j < j1; j += n_waves) {
     const int row = a.syn_row[j];
     real gs = real(0), gse = real(0);
     for (int s = a.syn_row_off[j]; s < a.syn_row_off[j + 1]; ++s) {
       real drive = real(0);
-      for (int c = a.con_off[s]; c < a.con_off[s + 1]; ++c) {
+      const int c1 = a.con_off[s + 1];
+      for (int c = a.con_off[s] + lane; c < c1; c += 64) {
         const int src = gstep - a.con_delay[c];
         if (src >= 0 && a.spk[(int64_t)(src % a.spk_len) * a.n_cells + a.con_pre[c]]) drive += a.con_w[c];
       }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) drive += __shfl_xor(drive, off);
       const real* q = a.syn_par + (int64_t)s * 6;
       const real A = a.syn_state[2 * (int64_t)s] + drive * q[2];
       const real B = a.syn_state[2 * (int64_t)s + 1] + drive * q[3];
       const real kg = q[0] * (B - A);
       gs += kg;
       gse += kg * q[1];
-      a.syn_state[2 * (int64_t)s] = A - q[4] * A;
-      a.syn_state[2 * (int64_t)s + 1] = B - q[5] * B;
+      if (lane == 0) {
+        a.syn_state[2 * (int64_t)s] = A - q[4] * A;
+        a.syn_state[2 * (int64_t)s + 1] = B - q[5] * B;
+      }
     }
-    R[row].d += gs;
-    R[row].r += gse - gs * xv[row];
+    if (lane == 0) {
+      R[row].d += gs;
+      R[row].r += gse - gs * xv[row];
+    }
   }
 }
 

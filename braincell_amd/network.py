@@ -184,9 +184,12 @@ class Network:
         return connection
 
     # ------------------------------------------------------------------ build
-    def build(self, dt: float, *, delay_quantization: str = "ceil", chunk: int = 200, threads: int = 256):
+    def build(self, dt: float, *, delay_quantization: str = "ceil", chunk: int = 200, threads: int = None,
+              rows_per_pack: int = None, autotune: bool = False):
         """Lower every population into one packed layout with its synapse and
-        contact tables; returns a :class:`NetworkRuntimeTables`."""
+        contact tables.  Pack plan: ``autotune`` times the candidate plans on
+        the device; otherwise ``threads`` / ``rows_per_pack`` or, by default,
+        about 2048 packs of 64 threads (networks are many small cells)."""
         if not self.populations:
             raise ValueError("Network.run(...) requires at least one population.")
         from .runtime import StaggerLayout
@@ -214,7 +217,21 @@ class Network:
             n_cells += size
             models += [m] * size
             probes += [np.asarray(cvs, dtype=np.int64)] * size
-        layout = StaggerLayout(models, threads=threads, probes=probes)
+        layout = StaggerLayout(models, threads=256, probes=probes)
+        if autotune:
+            from .autotune import autotune as _autotune
+
+            layout, _ = _autotune(layout, float(dt), precision=precision.copy().pop())
+        elif threads is not None or rows_per_pack is not None:
+            layout = layout.repacked(threads=threads or 256, rows_per_pack=rows_per_pack)
+        else:
+            nrow_max = max(c.geom.n_point for c in layout.cells)
+            total = sum(c.geom.n_point for c in layout.cells)
+            rows = max(nrow_max, total // 2048)
+            t = 64 if max(c.geom.n_cv for c in layout.cells) <= 512 and rows <= 512 else 256
+            layout = layout.repacked(threads=t, rows_per_pack=rows)
+        if layout.cv_per_thread > 8:
+            layout = layout.repacked(threads=256, cv_per_thread=8, rows_per_pack=layout.rows_per_pack)
         if layout.cv_per_thread > 8:
             raise NotImplementedError("network kernels hold at most 8 CVs per thread (cells up to 2048 CVs)")
         return _NetworkTables(self, names, layout, pop_cell0, n_cells, probe_points, float(dt),
