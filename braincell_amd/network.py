@@ -217,7 +217,7 @@ class Network:
             n_cells += size
             models += [m] * size
             probes += [np.asarray(cvs, dtype=np.int64)] * size
-        layout = StaggerLayout(models, threads=256, probes=probes)
+        layout = StaggerLayout(models, threads=256, probes_per_model=probes)
         if autotune:
             from .autotune import autotune as _autotune
 

@@ -103,7 +103,7 @@ class StaggerLayout:
     """Host-side packed layout of a batch of cells."""
 
     def __init__(self, models, *, threads: int = 256, cv_per_thread: int = None, rows_per_pack: int = None,
-                 probes="soma", region_classes: bool = False):
+                 probes="soma", region_classes: bool = False, probes_per_model=None):
         if len(models) == 0:
             raise ValueError("need at least one cell")
         self.models = list(models)
@@ -120,10 +120,10 @@ class StaggerLayout:
         self.slots = state_slots(self.chset)
         self.cc = self._channel_constants()
         self._geom_cache = {}
-        if isinstance(probes, list):  # one probe spec per model
-            if len(probes) != len(self.models):
-                raise ValueError("a list of probe specs needs one entry per model")
-            self.cells = [self._lower_cell(m, pr) for m, pr in zip(self.models, probes)]
+        if probes_per_model is not None:  # one probe spec per model
+            if len(probes_per_model) != len(self.models):
+                raise ValueError("probes_per_model needs one entry per model")
+            self.cells = [self._lower_cell(m, pr) for m, pr in zip(self.models, probes_per_model)]
         else:
             self.cells = [self._lower_cell(m, probes) for m in self.models]
         del self._geom_cache

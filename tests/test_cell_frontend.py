@@ -152,6 +152,8 @@ def test_probe_placement_and_locations():
     # branch 2, x = 1: a boundary node, sampled at the branch's last CV
     first = int(np.sum(m.n_cv_branch()[:2]))
     assert pts["v_tip"] == [first + m.n_cv_branch()[2] - 1]
+    cell.init_state()  # host layout only: the probe CVs become trace columns
+    assert sorted(int(c) for c in cell._layout.cells[0].probes) == sorted({c for v in pts.values() for c in v})
     with pytest.raises(ValueError, match="unique"):
         cell.place(RootLocation(0.5), StateProbe("v_soma"))
     with pytest.raises(ValueError):
