@@ -48,12 +48,13 @@ __device__ __forceinline__ bool operator>=(bc_dual<T> a, bc_dual<T> b) { return 
 template <typename T>
 __device__ __forceinline__ bc_dual<T> fabs(bc_dual<T> a) { return a.v < T(0) ? -a : a; }
 
-// Math policy for the rate templates: the library exponential of the value,
-// chained into the derivative.
-template <typename T>
+// Math policy for the rate templates: the exponential of the value (the
+// library one by default, or that of the policy B), chained into the
+// derivative.
+template <typename T, typename B = bc_math<T>>
 struct bc_dmath {
   static __device__ __forceinline__ bc_dual<T> ex(bc_dual<T> x) {
-    const T e = bc_math<T>::ex(x.v);
+    const T e = B::ex(x.v);
     return bc_dual<T>::of(e, e * x.d);
   }
 };

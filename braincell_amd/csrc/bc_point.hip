@@ -49,9 +49,34 @@ __device__ __forceinline__ void acc2(real& hi, real& lo, real d) {
   hi = h;
 }
 
+// Math policy of the point kernels.  FP64: the IEEE
+// functions and quotients.  FP32: the hardware exponential and logarithm
+// (bc_fastmath), v_rcp_f32 products for quotients, and exprel from its
+// Taylor series for |x| <= 1/2 (terms to x^7 / 8!, truncation below 1.2e-8
+// relative) or (2^(x log2 e) - 1) / x beyond, where e^x - 1 loses at most a
+// factor 2.6 in relative accuracy.  The accurate library forms cost about
+// twice the instructions, and the kernels are bound by instruction issue.
+template <typename real>
+__device__ __forceinline__ real pdiv(real a, real b) {
+  if constexpr (std::is_same<real, double>::value) return a / b;
+  else return a * __builtin_amdgcn_rcpf(b);
+}
+
+template <typename real>
+__device__ __forceinline__ real pexprel(real x) {
+  if constexpr (std::is_same<real, double>::value) {
+    return bc_exprel<double>(x);
+  } else {
+    if (fabsf(x) <= 0.5f)
+      return 1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f + x * (1.0f / 120.0f +
+             x * (1.0f / 720.0f + x * (1.0f / 5040.0f + x * (1.0f / 40320.0f)))))));
+    return (__builtin_amdgcn_exp2f(x * 1.4426950408889634f) - 1.0f) * __builtin_amdgcn_rcpf(x);
+  }
+}
+
 template <typename real>
 __device__ __forceinline__ real incr(real y, real f, real lam, real dt) {
-  return dt * bc_exprel<real>(dt * lam) * f;
+  return dt * pexprel<real>(dt * lam) * f;
 }
 
 template <typename real>
@@ -61,13 +86,13 @@ __device__ __forceinline__ real incr_ab(real x, real a, real b, real phi, real d
 
 template <typename real>
 __device__ __forceinline__ real incr_it(real x, real inf, real tau, real phi, real dt) {
-  const real it = real(1) / tau;
+  const real it = pdiv<real>(real(1), tau);
   return incr<real>(x, phi * (inf - x) * it, -phi * it, dt);
 }
 
 template <typename real>
 __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
-  typedef bc_math<real> M;
+  typedef bc_fastmath<real> M;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const int64_t n = a.n;
@@ -88,7 +113,7 @@ __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
   uint32_t nspk = 0, bad = 0;
   for (int step = 0; step < a.n_steps; ++step) {
     const real ci = s[9];
-    const real eca = cc.ca_rtzf * M::lg(cc.ca_Co / ci);
+    const real eca = cc.ca_rtzf * M::lg(pdiv<real>(cc.ca_Co, ci));
     // conductances at the old state
     const real g_na = g[0] * (s[0] * s[0] * s[0]) * s[1];
     const real g_kl = g[1];
@@ -111,29 +136,29 @@ __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
       const real drive = raw > real(0) ? raw : real(0);
       const real f = drive + (cc.ca_rest - ci) * cc.ca_tau_inv;
       real lam = -cc.ca_tau_inv;
-      if (raw > real(0)) lam -= cc.ca_drive * gca * cc.ca_rtzf / ci;
+      if (raw > real(0)) lam -= pdiv<real>(cc.ca_drive * gca * cc.ca_rtzf, ci);
       d[9] = incr<real>(ci, f, lam, dt);
     }
     {
       real ap, bp, aq, bq;
-      bc_na_rates<real>(V, cc.na_vsh, ap, bp, aq, bq);
+      bc_na_rates<real, M>(V, cc.na_vsh, ap, bp, aq, bq);
       d[0] = incr_ab<real>(s[0], ap, bp, cc.na_phi, dt);
       d[1] = incr_ab<real>(s[1], aq, bq, cc.na_phi, dt);
     }
     {
       real ap, bp;
-      bc_kdr_rates<real>(V, cc.kdr_vsh, ap, bp);
+      bc_kdr_rates<real, M>(V, cc.kdr_vsh, ap, bp);
       d[2] = incr_ab<real>(s[2], ap, bp, cc.kdr_phi, dt);
     }
     {
       real pi, pt, qi, qt;
-      bc_cal_rates<real>(V, cc.cal_vsh, pi, pt, qi, qt);
+      bc_cal_rates<real, M>(V, cc.cal_vsh, pi, pt, qi, qt);
       d[3] = incr_it<real>(s[3], pi, pt, cc.cal_phi_p, dt);
       d[4] = incr_it<real>(s[4], qi, qt, cc.cal_phi_q, dt);
     }
     {
       real pi, pt, qi, qt;
-      bc_cat_rates<real>(V, cc.cat_vsh, pi, pt, qi, qt);
+      bc_cat_rates<real, M>(V, cc.cat_vsh, pi, pt, qi, qt);
       d[5] = incr_it<real>(s[5], pi, pt, cc.cat_phi_p, dt);
       d[6] = incr_it<real>(s[6], qi, qt, cc.cat_phi_q, dt);
     }
@@ -143,7 +168,7 @@ __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
     }
     {
       real pi, pt;
-      bc_hcn_rates<real>(V, pi, pt);
+      bc_hcn_rates<real, M>(V, pi, pt);
       d[8] = incr_it<real>(s[8], pi, pt, cc.hcn_phi, dt);
     }
 #pragma unroll
@@ -199,9 +224,9 @@ __device__ __forceinline__ void it_terms(real x, bc_dual<real> inf, bc_dual<real
 // Cramer's rule; gate pivots 1 + theta dt phi (a + b) are >= 1.
 template <typename real>
 __global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, real theta) {
-  typedef bc_math<real> M;
+  typedef bc_fastmath<real> M;
   typedef bc_dual<real> D;
-  typedef bc_dmath<real> DM;
+  typedef bc_dmath<real, bc_fastmath<real>> DM;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const int64_t n = a.n;
@@ -223,8 +248,8 @@ __global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, 
   uint32_t nspk = 0, bad = 0;
   for (int step = 0; step < a.n_steps; ++step) {
     const real ci = s[9];
-    const real eca = cc.ca_rtzf * M::lg(cc.ca_Co / ci);
-    const real deca = -cc.ca_rtzf / ci;  // dE_Ca / dCi
+    const real eca = cc.ca_rtzf * M::lg(pdiv<real>(cc.ca_Co, ci));
+    const real deca = pdiv<real>(-cc.ca_rtzf, ci);  // dE_Ca / dCi
     // conductances and their partial derivatives in the gates
     const real p3 = s[0] * s[0] * s[0];
     const real g_na = g[0] * p3 * s[1];
@@ -282,7 +307,7 @@ __global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, 
     const real al = cc.ahp_alpha * M::pw(ci, cc.ahp_n);
     f[7] = cc.ahp_phi * (al * (real(1) - s[7]) - cc.ahp_beta * s[7]);
     lam[7] = -cc.ahp_phi * (al + cc.ahp_beta);
-    fv[7] = cc.ahp_phi * (al * cc.ahp_n / ci) * (real(1) - s[7]);  // d f / dCi
+    fv[7] = cc.ahp_phi * pdiv<real>(al * cc.ahp_n, ci) * (real(1) - s[7]);  // d f / dCi
     {
       D pi, pt;
       bc_hcn_rates<D, DM>(Vd, pi, pt);
@@ -303,7 +328,7 @@ __global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, 
     real A[9], B[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const real piv = real(1) / (real(1) - h * lam[k]);
+      const real piv = pdiv<real>(real(1), real(1) - h * lam[k]);
       A[k] = dt * f[k] * piv;
       B[k] = h * fv[k] * piv;
       if (k == 7) {
@@ -318,7 +343,7 @@ __global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, 
       m21 -= h * JC[k] * B[3 + k];
       r2 += h * JC[k] * A[3 + k];
     }
-    const real idet = real(1) / (m11 * m22 - m12 * m21);
+    const real idet = pdiv<real>(real(1), m11 * m22 - m12 * m21);
     const real dV = (r1 * m22 - m12 * r2) * idet;
     const real dC = (m11 * r2 - m21 * r1) * idet;
 #pragma unroll
