@@ -47,9 +47,21 @@ struct AdjArgs {
   real* cls_grad;
   int32_t n_steps;
   real dt;
-  int32_t max_rows, max_chains, max_cls;
+  int32_t max_rows, max_chains, max_cls, max_levels;
   bc_chan_const<real> cc;
 };
+
+// LDS of one workgroup: per-chain records {j0 | j1 << 16, top slot (-1 for a
+// root), first kid | kid count << 16, 0} first (16-byte aligned), then the
+// rows' diagonal, rhs, lowers and uppers, the chains' kid contributions, the
+// mechanism classes and their gradient accumulators, then the pack's kid
+// list and level offsets (int16).  The transposed solve then walks LDS only.
+__host__ __device__ inline size_t adj_lds_bytes(int max_rows, int max_chains, int max_cls, int max_levels,
+                                                size_t real_size) {
+  const size_t mc = (size_t)((max_chains + 3) & ~3);
+  return mc * 16 + (size_t)(4 * max_rows + 2 * max_chains + 2 * max_cls * BC_NCLS) * real_size +
+         (mc + (size_t)((max_levels + 4) & ~3)) * sizeof(int16_t);
+}
 
 constexpr uint32_t kRowMask = 0xFFFFu;
 constexpr uint32_t kProbeBit = 1u << 30;
@@ -60,7 +72,9 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   constexpr int NS = bc_nstates(CH) > 0 ? bc_nstates(CH) : 1;
   constexpr int NG = 4;  // gradient columns: Na, KDR, IL, CaL
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  real* bd = reinterpret_cast<real*>(smem_raw);
+  const int mc = (a.max_chains + 3) & ~3;
+  int4* CR = reinterpret_cast<int4*>(smem_raw);
+  real* bd = reinterpret_cast<real*>(smem_raw + (size_t)mc * 16);
   real* br = bd + a.max_rows;
   real* LO = br + a.max_rows;
   real* UP = LO + a.max_rows;
@@ -68,6 +82,8 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   real* kr = kd + a.max_chains;
   real* cls = kr + a.max_chains;
   real* gacc = cls + a.max_cls * BC_NCLS;
+  int16_t* m_kid = reinterpret_cast<int16_t*>(gacc + a.max_cls * BC_NCLS);
+  int16_t* m_lv = m_kid + mc;
 
   const bc_pack P = a.packs[blockIdx.x];
   const int tid = threadIdx.x;
@@ -80,6 +96,15 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
     LO[i] = a.lo[(int64_t)P.row0 + i];
     UP[i] = a.up[(int64_t)P.row0 + i];
   }
+  const int kbase = a.kid_off[P.ch0];
+  for (int k = tid; k < P.n_ch; k += NT) {
+    const int gk = P.ch0 + k;
+    const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
+    const int q0 = a.kid_off[gk] - kbase, nk = a.kid_off[gk + 1] - kbase - q0;
+    CR[k] = make_int4(j0 | (j1 << 16), a.chain_top[gk], q0 | (nk << 16), 0);
+  }
+  for (int i = tid; i < a.kid_off[P.ch0 + P.n_ch] - kbase; i += NT) m_kid[i] = (int16_t)a.kids[kbase + i];
+  for (int i = tid; i <= P.n_lv; i += NT) m_lv[i] = (int16_t)a.level_off[P.lv0 + i];
   const bc_chan_const<real>& cc = a.cc;
   const real dt = a.dt;
   const int64_t ncv = a.n_cv_total;
@@ -134,7 +159,7 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
       real Xbar = vb[i];
       if constexpr ((CH & BC_CH_NA) != 0) {
         real ap, bp, aq, bq, dap, dbp, daq, dbq, dxx, dxv;
-        bc_na_rates_d<real>(X, cc.na_vsh, ap, bp, aq, bq, dap, dbp, daq, dbq);
+        bc_na_rates_d<real, bc_fastmath<real>>(X, cc.na_vsh, ap, bp, aq, bq, dap, dbp, daq, dbq);
         bc_gate_ab_grad<real>(s0[bc_slot_na(CH)], ap, bp, dap, dbp, cc.na_phi, dt, dxx, dxv);
         Xbar += sb[i][bc_slot_na(CH)] * dxv;
         sb[i][bc_slot_na(CH)] *= dxx;
@@ -144,14 +169,14 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
       }
       if constexpr ((CH & BC_CH_KDR) != 0) {
         real ap, bp, dap, dbp, dxx, dxv;
-        bc_kdr_rates_d<real>(X, cc.kdr_vsh, ap, bp, dap, dbp);
+        bc_kdr_rates_d<real, bc_fastmath<real>>(X, cc.kdr_vsh, ap, bp, dap, dbp);
         bc_gate_ab_grad<real>(s0[bc_slot_kdr(CH)], ap, bp, dap, dbp, cc.kdr_phi, dt, dxx, dxv);
         Xbar += sb[i][bc_slot_kdr(CH)] * dxv;
         sb[i][bc_slot_kdr(CH)] *= dxx;
       }
       if constexpr ((CH & BC_CH_CAL) != 0) {
         real pi, pt, qi, qt, dpi, dpt, dqi, dqt, dxx, dxv;
-        bc_cal_rates_d<real>(X, cc.cal_vsh, pi, pt, qi, qt, dpi, dpt, dqi, dqt);
+        bc_cal_rates_d<real, bc_fastmath<real>>(X, cc.cal_vsh, pi, pt, qi, qt, dpi, dpt, dqi, dqt);
         bc_gate_it_grad<real>(s0[bc_slot_cal(CH)], pi, pt, dpi, dpt, cc.cal_phi_p, dt, dxx, dxv);
         Xbar += sb[i][bc_slot_cal(CH)] * dxv;
         sb[i][bc_slot_cal(CH)] *= dxx;
@@ -165,9 +190,9 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
         const real ci = s0[sc];
         const real cib = sb[i][sc];  // adjoint of Ci_{n+1}
         const real u = -dt * cc.ca_tau_inv;
-        const real ec = real(1) + u * bc_exprel<real>(u);
-        const real bcf = dt * bc_exprel<real>(u);
-        const real eca = cc.ca_rtzf * bc_math<real>::lg(cc.ca_Co / ci);
+        const real ec = real(1) + u * bc_qexprel<real>(u);
+        const real bcf = dt * bc_qexprel<real>(u);
+        const real eca = cc.ca_rtzf * bc_fastmath<real>::lg(bc_qdiv<real>(cc.ca_Co, ci));
         real ica = real(0), gcal = real(0), pp = real(0), qq = real(0), gmax = real(0);
         if constexpr ((CH & BC_CH_CAL) != 0) {
           pp = s0[bc_slot_cal(CH)];
@@ -182,7 +207,7 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
         real cin = cib * ec;
         if constexpr ((CH & BC_CH_CAL) != 0) {
           vb[i] = -Ib * gcal;  // start of Vbar_n (the solve adds Rbar below)
-          cin += Ib * gcal * (-cc.ca_rtzf / ci);
+          cin += Ib * gcal * bc_qdiv<real>(-cc.ca_rtzf, ci);
           const real drv = Ib * (eca - V0);  // d/d g_eff
           sb[i][bc_slot_cal(CH)] += drv * gmax * real(2) * pp * qq;
           sb[i][bc_slot_cal(CH) + 1] += drv * gmax * pp * pp;
@@ -223,15 +248,17 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
     // e - lo, as in the forward solve); the off-diagonals swap, so the
     // child's rhs contribution uses lowers and the back substitution uppers.
     for (int l = 0; l < P.n_lv; ++l) {
-      const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
+      const int k0 = m_lv[l], k1 = m_lv[l + 1];
       for (int k = k0 + tid; k < k1; k += NT) {
-        const int gk = P.ch0 + k;
-        const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
+        const int4 c = CR[k];
+        const int j0 = c.x & 0xFFFF, j1 = (int)((unsigned)c.x >> 16);
+        const int q0 = c.z & 0xFFFF, q1 = q0 + (int)((unsigned)c.z >> 16);
         real e = bd[j0];
         real r = br[j0];
-        for (int q = a.kid_off[gk]; q < a.kid_off[gk + 1]; ++q) {
-          e -= kd[a.kids[q]];
-          r -= kr[a.kids[q]];
+        for (int q = q0; q < q1; ++q) {
+          const int kq = m_kid[q];
+          e -= kd[kq];
+          r -= kr[kq];
         }
         real kdc, krc;
         for (int j = j0;;) {
@@ -251,11 +278,11 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
       __syncthreads();
     }
     for (int l = P.n_lv - 1; l >= 0; --l) {
-      const int k0 = a.level_off[P.lv0 + l], k1 = a.level_off[P.lv0 + l + 1];
+      const int k0 = m_lv[l], k1 = m_lv[l + 1];
       for (int k = k0 + tid; k < k1; k += NT) {
-        const int gk = P.ch0 + k;
-        const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
-        const int tp = a.chain_top[gk];
+        const int4 c = CR[k];
+        const int j0 = c.x & 0xFFFF, j1 = (int)((unsigned)c.x >> 16);
+        const int tp = c.y;
         real xp = tp >= 0 ? br[tp] : real(0);
         for (int j = j1 - 1; j >= j0; --j) {
           const real x = (br[j] - UP[j] * xp) * bd[j];
@@ -297,9 +324,9 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
         real eca = cc.E_Ca_fixed;
         if constexpr ((CH & BC_CH_POOL) != 0) {
           const real ci = r0[(int64_t)(bc_slot_ca(CH) + 1) * ncv + g];
-          eca = cc.ca_rtzf * bc_math<real>::lg(cc.ca_Co / ci);
+          eca = cc.ca_rtzf * bc_fastmath<real>::lg(bc_qdiv<real>(cc.ca_Co, ci));
           const real geff = cp[BC_CLS_G_CAL] * (p * p) * q;
-          sb[i][bc_slot_ca(CH)] += w * geff * (-cc.ca_rtzf / ci);
+          sb[i][bc_slot_ca(CH)] += w * geff * bc_qdiv<real>(-cc.ca_rtzf, ci);
         }
         const real gbar = w * (eca - X);
         const real gm = cp[BC_CLS_G_CAL];
@@ -388,8 +415,12 @@ int adj_launch(const bc_adjoint_desc* d, hipStream_t s) {
   a.max_rows = f->max_rows;
   a.max_chains = f->max_chains;
   a.max_cls = f->max_cls;
+  a.max_levels = f->max_levels;
   a.cc = *static_cast<const bc_chan_const<real>*>(f->chan_const);
-  const size_t lds = (size_t)(4 * f->max_rows + 2 * f->max_chains + 2 * f->max_cls * BC_NCLS) * sizeof(real);
+  if (f->max_rows > 0xFFFF || f->max_chains > 0x7FFF)
+    return bc::fail(BC_INVALID_LAYOUT, "adjoint pack too large (max_rows %d, max_chains %d)", f->max_rows,
+                    f->max_chains);
+  const size_t lds = adj_lds_bytes(f->max_rows, f->max_chains, f->max_cls, f->max_levels, sizeof(real));
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "adjoint pack needs %zu bytes of LDS", lds);
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (f->n_packs == 0 || d->n_steps == 0) return BC_OK;

@@ -243,23 +243,59 @@ __device__ __forceinline__ void bc_hcn_rates(real V, real& pi, real& pt) {
 // ----------------------------------------------------------------------------
 // Derivatives used by the hand-written adjoint (reverse of one staggered step).
 
-// d/dx exprel(x) = sum_k k x^(k-1) / (k+1)!  =  (x e^x - (e^x - 1)) / x^2
+// Quotients and exprel at the precision policy of the hand-written point
+// and adjoint kernels.  FP64: the IEEE quotient and bc_exprel.  FP32:
+// v_rcp_f32 products, and exprel from its Taylor series for |x| <= 1/2
+// (terms to x^7 / 8!, truncation below 1.2e-8 relative) or
+// (2^(x log2 e) - 1) / x beyond, where e^x - 1 loses at most a factor 2.6
+// in relative accuracy.  The library forms take about twice the
+// instructions, and these kernels are bound by instruction issue.
 template <typename real>
-__host__ __device__ __forceinline__ real bc_exprel_d(real x) {
-  const real eps = sizeof(real) == 4 ? real(0.05) : real(1e-3);
-  if (fabs(x) < eps)
-    return real(0.5) + x * (real(1.0 / 3.0) + x * (real(0.125) + x * (real(1.0 / 30.0) + x * real(1.0 / 144.0))));
-  const real em = bc_math<real>::em1(x);
-  return (x * (em + real(1)) - em) / (x * x);
+__device__ __forceinline__ real bc_qdiv(real a, real b) {
+  if constexpr (std::is_same<real, double>::value) return a / b;
+  else return a * __builtin_amdgcn_rcpf(b);
+}
+
+template <typename real>
+__device__ __forceinline__ real bc_qexprel(real x) {
+  if constexpr (std::is_same<real, double>::value) {
+    return bc_exprel<double>(x);
+  } else {
+    if (fabsf(x) <= 0.5f)
+      return 1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f + x * (1.0f / 120.0f +
+             x * (1.0f / 720.0f + x * (1.0f / 5040.0f + x * (1.0f / 40320.0f)))))));
+    return (__builtin_amdgcn_exp2f(x * 1.4426950408889634f) - 1.0f) * __builtin_amdgcn_rcpf(x);
+  }
+}
+
+// d/dx exprel(x) = sum_k k x^(k-1) / (k+1)!  =  (x e^x - (e^x - 1)) / x^2.
+// FP64: the series for |x| < 1e-3, the closed form beyond (expm1).  FP32:
+// the series to k = 8 for |x| <= 1/2 (truncation below 2e-8 relative; the
+// closed form cancels there), the closed form with the hardware exponential
+// beyond (numerator >= 0.17, error ~1e-6 relative).
+template <typename real>
+__device__ __forceinline__ real bc_qexprel_d(real x) {
+  if constexpr (std::is_same<real, double>::value) {
+    if (fabs(x) < 1e-3) return 0.5 + x * (1.0 / 3.0 + x * (0.125 + x * (1.0 / 30.0 + x * (1.0 / 144.0))));
+    const double em = expm1(x);
+    return (x * (em + 1.0) - em) / (x * x);
+  } else {
+    if (fabsf(x) <= 0.5f)
+      return 0.5f + x * (1.0f / 3.0f + x * (0.125f + x * (1.0f / 30.0f + x * (1.0f / 144.0f +
+             x * (1.0f / 840.0f + x * (1.0f / 5760.0f + x * (1.0f / 45360.0f)))))));
+    const float ex = __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
+    const float ix = __builtin_amdgcn_rcpf(x);
+    return (x * ex - (ex - 1.0f)) * (ix * ix);
+  }
 }
 
 // Alpha/beta gate update x' = x + dt exprel(u) f, u = -dt phi (a + b):
 // returns dx'/dx = e^u and dx'/dV given da/dV, db/dV.
 template <typename real>
-__host__ __device__ __forceinline__ void bc_gate_ab_grad(real x, real a, real b, real da, real db, real phi, real dt,
-                                                         real& dxx, real& dxv) {
+__device__ __forceinline__ void bc_gate_ab_grad(real x, real a, real b, real da, real db, real phi, real dt,
+                                                real& dxx, real& dxv) {
   const real u = -dt * phi * (a + b);
-  const real er = bc_exprel<real>(u), erd = bc_exprel_d<real>(u);
+  const real er = bc_qexprel<real>(u), erd = bc_qexprel_d<real>(u);
   const real f = phi * (a * (real(1) - x) - b * x);
   const real common = dt * erd * (-dt * phi) * f;
   const real dA = common + dt * er * phi * (real(1) - x);
@@ -270,11 +306,11 @@ __host__ __device__ __forceinline__ void bc_gate_ab_grad(real x, real a, real b,
 
 // Inf/tau gate update: x' = x + dt exprel(u) phi (inf - x) w, w = 1/tau, u = -dt phi w.
 template <typename real>
-__host__ __device__ __forceinline__ void bc_gate_it_grad(real x, real inf, real tau, real dinf, real dtau, real phi,
-                                                         real dt, real& dxx, real& dxv) {
-  const real w = real(1) / tau;
+__device__ __forceinline__ void bc_gate_it_grad(real x, real inf, real tau, real dinf, real dtau, real phi, real dt,
+                                                real& dxx, real& dxv) {
+  const real w = bc_qdiv<real>(real(1), tau);
   const real u = -dt * phi * w;
-  const real er = bc_exprel<real>(u), erd = bc_exprel_d<real>(u);
+  const real er = bc_qexprel<real>(u), erd = bc_qexprel_d<real>(u);
   const real f = phi * (inf - x) * w;
   const real dI = dt * er * phi * w;
   const real dW = dt * erd * (-dt * phi) * f + dt * er * phi * (inf - x);
@@ -282,51 +318,52 @@ __host__ __device__ __forceinline__ void bc_gate_it_grad(real x, real inf, real 
   dxv = dI * dinf + dW * (-dtau * w * w);
 }
 
-template <typename real>
-__host__ __device__ __forceinline__ void bc_na_rates_d(real V, real vsh, real& ap, real& bp, real& aq, real& bq,
-                                                       real& dap, real& dbp, real& daq, real& dbq) {
-  typedef bc_math<real> M;
+// Rates and their V derivatives (the adjoint's rate evaluations); M is the
+// exponential policy (bc_math, or bc_fastmath for the hardware exp in FP32).
+template <typename real, typename M = bc_math<real>>
+__device__ __forceinline__ void bc_na_rates_d(real V, real vsh, real& ap, real& bp, real& aq, real& bq, real& dap,
+                                              real& dbp, real& daq, real& dbq) {
   const real Vp = V - vsh;
   const real xa = -(Vp - real(13)) * real(0.25), xb = (Vp - real(40)) * real(0.2);
-  const real ea = bc_exprel<real>(xa), eb = bc_exprel<real>(xb);
-  ap = real(1.28) / ea;
-  bp = real(1.4) / eb;
-  dap = real(0.32) * bc_exprel_d<real>(xa) / (ea * ea);
-  dbp = real(-0.28) * bc_exprel_d<real>(xb) / (eb * eb);
-  aq = real(0.128) * M::ex(-(Vp - real(17)) / real(18));
-  daq = -aq / real(18);
+  const real ea = bc_qexprel<real>(xa), eb = bc_qexprel<real>(xb);
+  ap = bc_qdiv<real>(real(1.28), ea);
+  bp = bc_qdiv<real>(real(1.4), eb);
+  dap = bc_qdiv<real>(real(0.32) * bc_qexprel_d<real>(xa), ea * ea);
+  dbp = bc_qdiv<real>(real(-0.28) * bc_qexprel_d<real>(xb), eb * eb);
+  aq = real(0.128) * M::ex(bc_qdiv<real>(-(Vp - real(17)), real(18)));
+  daq = bc_qdiv<real>(-aq, real(18));
   const real e = M::ex(-(Vp - real(40)) * real(0.2));
-  bq = real(4) / (real(1) + e);
-  dbq = bq * (e / (real(1) + e)) * real(0.2);
+  bq = bc_qdiv<real>(real(4), real(1) + e);
+  dbq = bq * bc_qdiv<real>(e, real(1) + e) * real(0.2);
 }
 
-template <typename real>
-__host__ __device__ __forceinline__ void bc_kdr_rates_d(real V, real vsh, real& ap, real& bp, real& dap, real& dbp) {
-  typedef bc_math<real> M;
+template <typename real, typename M = bc_math<real>>
+__device__ __forceinline__ void bc_kdr_rates_d(real V, real vsh, real& ap, real& bp, real& dap, real& dbp) {
   const real Vp = V - vsh;
   const real x = -(Vp - real(15)) * real(0.2);
-  const real e = bc_exprel<real>(x);
-  ap = real(0.16) / e;
-  dap = real(0.032) * bc_exprel_d<real>(x) / (e * e);
-  bp = real(0.5) * M::ex(-(Vp - real(10)) / real(40));
-  dbp = -bp / real(40);
+  const real e = bc_qexprel<real>(x);
+  ap = bc_qdiv<real>(real(0.16), e);
+  dap = bc_qdiv<real>(real(0.032) * bc_qexprel_d<real>(x), e * e);
+  bp = real(0.5) * M::ex(bc_qdiv<real>(-(Vp - real(10)), real(40)));
+  dbp = bc_qdiv<real>(-bp, real(40));
 }
 
-template <typename real>
-__host__ __device__ __forceinline__ void bc_cal_rates_d(real V, real vsh, real& pi, real& pt, real& qi, real& qt,
-                                                        real& dpi, real& dpt, real& dqi, real& dqt) {
-  typedef bc_math<real> M;
+template <typename real, typename M = bc_math<real>>
+__device__ __forceinline__ void bc_cal_rates_d(real V, real vsh, real& pi, real& pt, real& qi, real& qt, real& dpi,
+                                               real& dpt, real& dqi, real& dqt) {
   const real Vp = V - vsh;
-  pi = real(1) / (real(1) + M::ex(-(Vp + real(10)) * real(0.25)));
+  pi = bc_qdiv<real>(real(1), real(1) + M::ex(-(Vp + real(10)) * real(0.25)));
   dpi = pi * (real(1) - pi) * real(0.25);
-  const real e1 = M::ex((Vp + real(5)) / real(15));
-  const real s1 = real(1) / e1 + e1;
-  pt = real(0.4) + real(0.7) / s1;
-  dpt = real(-0.7) * ((e1 - real(1) / e1) / real(15)) / (s1 * s1);
-  qi = real(1) / (real(1) + M::ex((Vp + real(25)) * real(0.5)));
+  const real e1 = M::ex(bc_qdiv<real>(Vp + real(5), real(15)));
+  const real r1 = bc_qdiv<real>(real(1), e1);
+  const real s1 = r1 + e1;
+  pt = real(0.4) + bc_qdiv<real>(real(0.7), s1);
+  dpt = bc_qdiv<real>(real(-0.7) * bc_qdiv<real>(e1 - r1, real(15)), s1 * s1);
+  qi = bc_qdiv<real>(real(1), real(1) + M::ex((Vp + real(25)) * real(0.5)));
   dqi = -qi * (real(1) - qi) * real(0.5);
-  const real e2 = M::ex((Vp + real(40)) / real(9.5));
-  const real s2 = e2 + real(1) / e2;
-  qt = real(300) + real(100) / s2;
-  dqt = real(-100) * ((e2 - real(1) / e2) / real(9.5)) / (s2 * s2);
+  const real e2 = M::ex(bc_qdiv<real>(Vp + real(40), real(9.5)));
+  const real r2 = bc_qdiv<real>(real(1), e2);
+  const real s2 = e2 + r2;
+  qt = real(300) + bc_qdiv<real>(real(100), s2);
+  dqt = bc_qdiv<real>(real(-100) * bc_qdiv<real>(e2 - r2, real(9.5)), s2 * s2);
 }

@@ -49,34 +49,11 @@ __device__ __forceinline__ void acc2(real& hi, real& lo, real d) {
   hi = h;
 }
 
-// Math policy of the point kernels.  FP64: the IEEE
-// functions and quotients.  FP32: the hardware exponential and logarithm
-// (bc_fastmath), v_rcp_f32 products for quotients, and exprel from its
-// Taylor series for |x| <= 1/2 (terms to x^7 / 8!, truncation below 1.2e-8
-// relative) or (2^(x log2 e) - 1) / x beyond, where e^x - 1 loses at most a
-// factor 2.6 in relative accuracy.  The accurate library forms cost about
-// twice the instructions, and the kernels are bound by instruction issue.
-template <typename real>
-__device__ __forceinline__ real pdiv(real a, real b) {
-  if constexpr (std::is_same<real, double>::value) return a / b;
-  else return a * __builtin_amdgcn_rcpf(b);
-}
-
-template <typename real>
-__device__ __forceinline__ real pexprel(real x) {
-  if constexpr (std::is_same<real, double>::value) {
-    return bc_exprel<double>(x);
-  } else {
-    if (fabsf(x) <= 0.5f)
-      return 1.0f + x * (0.5f + x * (1.0f / 6.0f + x * (1.0f / 24.0f + x * (1.0f / 120.0f +
-             x * (1.0f / 720.0f + x * (1.0f / 5040.0f + x * (1.0f / 40320.0f)))))));
-    return (__builtin_amdgcn_exp2f(x * 1.4426950408889634f) - 1.0f) * __builtin_amdgcn_rcpf(x);
-  }
-}
-
+// FP32 math policy (hardware exp / log, v_rcp_f32 quotients, series exprel:
+// bc_fastmath, bc_qdiv, bc_qexprel); FP64 keeps the IEEE functions.
 template <typename real>
 __device__ __forceinline__ real incr(real y, real f, real lam, real dt) {
-  return dt * pexprel<real>(dt * lam) * f;
+  return dt * bc_qexprel<real>(dt * lam) * f;
 }
 
 template <typename real>
@@ -86,7 +63,7 @@ __device__ __forceinline__ real incr_ab(real x, real a, real b, real phi, real d
 
 template <typename real>
 __device__ __forceinline__ real incr_it(real x, real inf, real tau, real phi, real dt) {
-  const real it = pdiv<real>(real(1), tau);
+  const real it = bc_qdiv<real>(real(1), tau);
   return incr<real>(x, phi * (inf - x) * it, -phi * it, dt);
 }
 
@@ -113,7 +90,7 @@ __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
   uint32_t nspk = 0, bad = 0;
   for (int step = 0; step < a.n_steps; ++step) {
     const real ci = s[9];
-    const real eca = cc.ca_rtzf * M::lg(pdiv<real>(cc.ca_Co, ci));
+    const real eca = cc.ca_rtzf * M::lg(bc_qdiv<real>(cc.ca_Co, ci));
     // conductances at the old state
     const real g_na = g[0] * (s[0] * s[0] * s[0]) * s[1];
     const real g_kl = g[1];
@@ -136,7 +113,7 @@ __global__ void __launch_bounds__(256) point_kernel(PointArgs<real> a) {
       const real drive = raw > real(0) ? raw : real(0);
       const real f = drive + (cc.ca_rest - ci) * cc.ca_tau_inv;
       real lam = -cc.ca_tau_inv;
-      if (raw > real(0)) lam -= pdiv<real>(cc.ca_drive * gca * cc.ca_rtzf, ci);
+      if (raw > real(0)) lam -= bc_qdiv<real>(cc.ca_drive * gca * cc.ca_rtzf, ci);
       d[9] = incr<real>(ci, f, lam, dt);
     }
     {
@@ -248,8 +225,8 @@ __global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, 
   uint32_t nspk = 0, bad = 0;
   for (int step = 0; step < a.n_steps; ++step) {
     const real ci = s[9];
-    const real eca = cc.ca_rtzf * M::lg(pdiv<real>(cc.ca_Co, ci));
-    const real deca = pdiv<real>(-cc.ca_rtzf, ci);  // dE_Ca / dCi
+    const real eca = cc.ca_rtzf * M::lg(bc_qdiv<real>(cc.ca_Co, ci));
+    const real deca = bc_qdiv<real>(-cc.ca_rtzf, ci);  // dE_Ca / dCi
     // conductances and their partial derivatives in the gates
     const real p3 = s[0] * s[0] * s[0];
     const real g_na = g[0] * p3 * s[1];
@@ -307,7 +284,7 @@ __global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, 
     const real al = cc.ahp_alpha * M::pw(ci, cc.ahp_n);
     f[7] = cc.ahp_phi * (al * (real(1) - s[7]) - cc.ahp_beta * s[7]);
     lam[7] = -cc.ahp_phi * (al + cc.ahp_beta);
-    fv[7] = cc.ahp_phi * pdiv<real>(al * cc.ahp_n, ci) * (real(1) - s[7]);  // d f / dCi
+    fv[7] = cc.ahp_phi * bc_qdiv<real>(al * cc.ahp_n, ci) * (real(1) - s[7]);  // d f / dCi
     {
       D pi, pt;
       bc_hcn_rates<D, DM>(Vd, pi, pt);
@@ -328,7 +305,7 @@ __global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, 
     real A[9], B[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-      const real piv = pdiv<real>(real(1), real(1) - h * lam[k]);
+      const real piv = bc_qdiv<real>(real(1), real(1) - h * lam[k]);
       A[k] = dt * f[k] * piv;
       B[k] = h * fv[k] * piv;
       if (k == 7) {
@@ -343,7 +320,7 @@ __global__ void __launch_bounds__(256) point_implicit_kernel(PointArgs<real> a, 
       m21 -= h * JC[k] * B[3 + k];
       r2 += h * JC[k] * A[3 + k];
     }
-    const real idet = pdiv<real>(real(1), m11 * m22 - m12 * m21);
+    const real idet = bc_qdiv<real>(real(1), m11 * m22 - m12 * m21);
     const real dV = (r1 * m22 - m12 * r2) * idet;
     const real dC = (m11 * r2 - m21 * r1) * idet;
 #pragma unroll
