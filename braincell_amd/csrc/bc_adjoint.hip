@@ -141,20 +141,53 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   }
   __syncthreads();
 
+  // Recorded states.  Up to 4 FP32 CVs per thread (kCarry) they live in
+  // registers: cur = the start of step n (V, then the states), xn = V after
+  // step n; the next step's record is loaded while this step's transposed
+  // solve runs, and V after step n - 1 is this step's starting V, so every
+  // record is read from HBM once.  Larger thread shares read the record
+  // where it is used (a second record set would not fit the registers).
+  constexpr int NR = bc_nstates(CH) + 1;
+  constexpr bool kCarry = MAXC * NR * sizeof(real) <= 4 * 8 * sizeof(float);
+  constexpr int CM = kCarry ? MAXC : 1, CR_ = kCarry ? NR : 1;
+  real cur[CM][CR_], xn[CM];
+  if constexpr (kCarry) {
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      xn[i] = real(0);
+#pragma unroll
+      for (int k = 0; k < NR; ++k) cur[i][k] = real(0);
+      if (info[i] != kNone && a.n_steps > 0) {
+        const int64_t g = (int64_t)P.cv0 + tid + i * NT;
+        const real* r0 = a.rec + (int64_t)(a.n_steps - 1) * slab;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) cur[i][k] = r0[(int64_t)k * ncv + g];
+        xn[i] = a.rec[(int64_t)a.n_steps * slab + g];
+      }
+    }
+  }
+  // record value k (0: V, 1 + s: state s) at the start of step n, and V
+  // after step n, of the thread's CV i
+  auto rec0 = [&](int n, int i, int k) -> real {
+    if constexpr (kCarry) return cur[kCarry ? i : 0][kCarry ? k : 0];
+    else return a.rec[(int64_t)n * slab + (int64_t)k * ncv + (int64_t)P.cv0 + tid + i * NT];
+  };
+  auto rec1 = [&](int n, int i) -> real {
+    if constexpr (kCarry) return xn[kCarry ? i : 0];
+    else return a.rec[(int64_t)(n + 1) * slab + (int64_t)P.cv0 + tid + i * NT];
+  };
+
   for (int n = a.n_steps - 1; n >= 0; --n) {
-    const real* r0 = a.rec + (int64_t)n * slab;        // state at the start of step n
-    const real* r1 = a.rec + (int64_t)(n + 1) * slab;  // state after step n
     // ---- phase 1: gates and Ca pool backward; transposed-solve inputs
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       if (info[i] == kNone) continue;
-      const int64_t g = (int64_t)P.cv0 + tid + i * NT;
       const int row = (int)(info[i] & kRowMask);
       const real* cp = cls + ((info[i] >> 16) & 0x3FFFu) * BC_NCLS;
-      const real X = r1[g];
+      const real X = rec1(n, i);
       real s0[NS];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) s0[s] = (s < bc_nstates(CH)) ? r0[(int64_t)(s + 1) * ncv + g] : real(0);
+      for (int s = 0; s < NS; ++s) s0[s] = (s < bc_nstates(CH)) ? rec0(n, i, s + 1) : real(0);
       if (pcol[i] >= 0) vb[i] += a.trace_bar[(int64_t)n * a.trace_ld + pcol[i]];
       real Xbar = vb[i];
       if constexpr ((CH & BC_CH_NA) != 0) {
@@ -184,7 +217,7 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
         Xbar += sb[i][bc_slot_cal(CH) + 1] * dxv;
         sb[i][bc_slot_cal(CH) + 1] *= dxx;
       }
-      const real V0 = r0[g];
+      const real V0 = rec0(n, i, 0);
       if constexpr ((CH & BC_CH_POOL) != 0) {
         const int sc = bc_slot_ca(CH);
         const real ci = s0[sc];
@@ -242,6 +275,19 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
       bd[row] = real(0);
       br[row] = real(0);
     }
+    // the record of step n - 1, in flight during the solve
+    real nxt[CM][CR_];
+#pragma unroll
+    for (int i = 0; i < CM; ++i) {
+#pragma unroll
+      for (int k = 0; k < CR_; ++k) nxt[i][k] = real(0);
+      if (kCarry && info[i] != kNone && n > 0) {
+        const int64_t g = (int64_t)P.cv0 + tid + i * NT;
+        const real* rp = a.rec + (int64_t)(n - 1) * slab;
+#pragma unroll
+        for (int k = 0; k < CR_; ++k) nxt[i][k] = rp[(int64_t)k * ncv + g];
+      }
+    }
     __syncthreads();
 
     // ---- phase 2: Rbar = M^-T Xbar.  M^T has the same diagonal (rebuilt as
@@ -297,15 +343,14 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
 #pragma unroll
     for (int i = 0; i < MAXC; ++i) {
       if (info[i] == kNone) continue;
-      const int64_t g = (int64_t)P.cv0 + tid + i * NT;
       const int row = (int)(info[i] & kRowMask);
       const real* cp = cls + ((info[i] >> 16) & 0x3FFFu) * BC_NCLS;
-      const real X = r1[g];
+      const real X = rec1(n, i);
       const real Rb = br[row];
       const real w = cp[BC_CLS_DTCM] * Rb;  // d loss / d (dt/Cm * g_eff * E) per unit
       vb[i] += Rb;
       if constexpr ((CH & BC_CH_NA) != 0) {
-        const real p = r0[(int64_t)(bc_slot_na(CH) + 1) * ncv + g], q = r0[(int64_t)(bc_slot_na(CH) + 2) * ncv + g];
+        const real p = rec0(n, i, bc_slot_na(CH) + 1), q = rec0(n, i, bc_slot_na(CH) + 2);
         const real gbar = w * (cc.E_Na - X);
         const real gm = cp[BC_CLS_G_NA];
         sb[i][bc_slot_na(CH)] += gbar * gm * real(3) * p * p * q;
@@ -313,17 +358,17 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
         gg[i][0] += gbar * p * p * p * q;
       }
       if constexpr ((CH & BC_CH_KDR) != 0) {
-        const real p = r0[(int64_t)(bc_slot_kdr(CH) + 1) * ncv + g];
+        const real p = rec0(n, i, bc_slot_kdr(CH) + 1);
         const real gbar = w * (cc.E_K - X);
         sb[i][bc_slot_kdr(CH)] += gbar * cp[BC_CLS_G_KDR] * real(4) * p * p * p;
         gg[i][1] += gbar * (p * p) * (p * p);
       }
       if constexpr ((CH & BC_CH_IL) != 0) gg[i][2] += w * (cc.E_IL - X);
       if constexpr ((CH & BC_CH_CAL) != 0) {
-        const real p = r0[(int64_t)(bc_slot_cal(CH) + 1) * ncv + g], q = r0[(int64_t)(bc_slot_cal(CH) + 2) * ncv + g];
+        const real p = rec0(n, i, bc_slot_cal(CH) + 1), q = rec0(n, i, bc_slot_cal(CH) + 2);
         real eca = cc.E_Ca_fixed;
         if constexpr ((CH & BC_CH_POOL) != 0) {
-          const real ci = r0[(int64_t)(bc_slot_ca(CH) + 1) * ncv + g];
+          const real ci = rec0(n, i, bc_slot_ca(CH) + 1);
           eca = cc.ca_rtzf * bc_fastmath<real>::lg(bc_qdiv<real>(cc.ca_Co, ci));
           const real geff = cp[BC_CLS_G_CAL] * (p * p) * q;
           sb[i][bc_slot_ca(CH)] += w * geff * bc_qdiv<real>(-cc.ca_rtzf, ci);
@@ -333,6 +378,14 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
         sb[i][bc_slot_cal(CH)] += gbar * gm * real(2) * p * q;
         sb[i][bc_slot_cal(CH) + 1] += gbar * gm * p * p;
         gg[i][3] += gbar * (p * p) * q;
+      }
+    }
+    if constexpr (kCarry) {
+#pragma unroll
+      for (int i = 0; i < CM; ++i) {
+        xn[i] = cur[i][0];
+#pragma unroll
+        for (int k = 0; k < CR_; ++k) cur[i][k] = nxt[i][k];
       }
     }
   }
