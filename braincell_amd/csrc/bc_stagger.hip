@@ -70,7 +70,8 @@ struct StaggerArgs {
   int32_t n_steps;
   real dt;
   int32_t max_rows, max_chains, max_cls, max_levels;
-  uint32_t flags;
+  uint32_t flags;  // BC_STEP_VOLTAGE_ONLY arrives as BC_STEP_SKIP_GATES with ca_b = 0
+  real ca_b;       // Ca pool increment factor dt exprel(-dt / tau); 0 freezes the pool
   bc_chan_const<real> cc;
   // synapses and spike delivery (network runs; unused by the plain kernels)
   const int32_t* syn_pack_off;
@@ -173,7 +174,6 @@ template <typename real, uint32_t CH>
 __device__ __forceinline__ void advance_states(const bc_chan_const<real>& cc, real* s, real V, real ica, real dt,
                                                real ca_b, uint32_t flags) {
   typedef bc_fastmath<real> M;
-  if (flags & BC_STEP_VOLTAGE_ONLY) return;
   real ci = real(0), ci_gates = real(0);
   if constexpr ((CH & BC_CH_POOL) != 0) {
     real c = s[bc_slot_ca(CH)];
@@ -660,7 +660,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
 
   const int dummy = (a.max_rows + 3) & ~3;
   const real dt = a.dt;
-  const real ca_b = dt * bc_exprel<real>(-dt * a.cc.ca_tau_inv);
+  const real ca_b = a.ca_b;
   uint32_t bad = 0;
   for (int step = 0; step < a.n_steps; ++step) {
     const int gstep = a.step0 + step;
@@ -944,13 +944,15 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->step0 = d->step0;
   a->n_steps = d->n_steps;
   a->dt = static_cast<real>(d->dt);
-  a->flags = d->flags;
+  // voltage-only = gates skipped and the pool frozen (ca_b = 0 below)
+  a->flags = d->flags | ((d->flags & BC_STEP_VOLTAGE_ONLY) ? BC_STEP_SKIP_GATES : 0u);
   a->max_rows = d->max_rows;
   a->max_chains = d->max_chains;
   a->max_cls = d->max_cls;
   a->max_levels = d->max_levels;
   if (d->chan_const == nullptr) return bc::fail(BC_INVALID_ARG, "chan_const is NULL");
   a->cc = *static_cast<const bc_chan_const<real>*>(d->chan_const);
+  a->ca_b = (d->flags & BC_STEP_VOLTAGE_ONLY) ? real(0) : a->dt * bc_exprel<real>(-a->dt * a->cc.ca_tau_inv);
   const bc_synapse_desc* y = d->syn;
   a->syn_pack_off = y ? y->pack_off : nullptr;
   a->syn_row = y ? y->row : nullptr;
