@@ -25,6 +25,11 @@ HIPCC_FLAGS = [
 ]
 
 
+# Per-source flags: the closure kernels take the VGPR form of MFMA (the AGPR
+# form costs one v_accvgpr_read per accumulator element read back).
+EXTRA_FLAGS = {"bc_closure.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
@@ -39,7 +44,8 @@ def build(verbose: bool = True) -> str:
     objs = []
     for src in sources():
         obj = os.path.join(LIBDIR, os.path.basename(src).replace(".hip", ".o"))
-        cmd = [hipcc, *[f for f in HIPCC_FLAGS if f != "-shared"], "-c", "-I", CSRC, "-I", INCLUDE, src, "-o", obj]
+        extra = EXTRA_FLAGS.get(os.path.basename(src), [])
+        cmd = [hipcc, *[f for f in HIPCC_FLAGS if f != "-shared"], *extra, "-c", "-I", CSRC, "-I", INCLUDE, src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -44,10 +44,27 @@ static_assert(NPARAM == BC_CLOSURE_NPARAM, "parameter count");
 // Gate slots of the Na/KDR/IL/CaL/pool state layout.
 constexpr int S_NAP = 0, S_NAQ = 1, S_KDP = 2, S_CLP = 3, S_CLQ = 4, S_CI = 5, NSTATE = 6;
 
-__device__ __forceinline__ float fast_tanh(float x) {
-  // tanh x = 1 - 2 / (e^{2x} + 1); saturates correctly for large |x|
-  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2 log2(e)
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+// The MFMA inference evaluates tanh z = 1 - 2 r with r = 1 / (1 + 2^{s z}),
+// s = 2 log2(e), and folds both affine steps into the weights when they are
+// loaded: a layer feeding a tanh has its weights and bias scaled by s (the
+// MFMA returns s z), and the layer after it takes r instead of tanh z, with
+// weights -2 W and bias b + sum_k W[k][.].  One hidden unit then costs an
+// add, v_exp_f32 and v_rcp_f32; r saturates correctly at 0 and 1.
+constexpr float kTanhScale = 2.8853900817779268f;  // 2 log2(e)
+__device__ __forceinline__ float tanh_r(float sz) { return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(sz)); }
+
+// Column sums of the layer-2 and layer-3 weights (the folded biases), into
+// cs[0 .. NH + NOUT) of LDS, by the threads of a workgroup.
+__device__ __forceinline__ void column_sums(const float* P, float* cs) {
+  for (int t = threadIdx.x; t < NH + NOUT; t += blockDim.x) {
+    float a = 0.0f;
+    if (t < NH) {
+      for (int k = 0; k < NH; ++k) a += P[OFF_W2 + k * NH + t];
+    } else {
+      for (int k = 0; k < NH; ++k) a += P[OFF_W3 + k * NOUT + (t - NH)];
+    }
+    cs[t] = a;
+  }
 }
 
 // Plain exponential-Euler update of the 5 gates at voltage V (the staggered
@@ -277,7 +294,7 @@ struct MlpRegs {
   floatx4 b3;
 };
 
-__device__ __forceinline__ void load_mlp(MlpRegs& m, const float* P, int lane) {
+__device__ __forceinline__ void load_mlp(MlpRegs& m, const float* P, const float* cs, int lane) {
   const int g = lane >> 4, n = lane & 15;
 #pragma unroll
   for (int T = 0; T < 4; ++T) {
@@ -288,21 +305,26 @@ __device__ __forceinline__ void load_mlp(MlpRegs& m, const float* P, int lane) {
       float w = 0.0f;
       if (k < NIN) w = P[OFF_W1 + k * NH + o];
       else if (k == NIN) w = P[OFF_B1 + o];
-      m.a1[T][j] = (__bf16)w;
+      m.a1[T][j] = (__bf16)(kTanhScale * w);
     }
 #pragma unroll
     for (int c = 0; c < 2; ++c)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) m.a2[T][c][j] = (__bf16)P[OFF_W2 + hid_feature(c, g, j) * NH + o];
+      for (int j = 0; j < 8; ++j)
+        m.a2[T][c][j] = (__bf16)(-2.0f * kTanhScale * P[OFF_W2 + hid_feature(c, g, j) * NH + o]);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) m.b2[T][r] = P[OFF_B2 + 16 * T + 4 * g + r];
+    for (int r = 0; r < 4; ++r) {
+      const int o2 = 16 * T + 4 * g + r;
+      m.b2[T][r] = kTanhScale * (P[OFF_B2 + o2] + cs[o2]);
+    }
   }
 #pragma unroll
   for (int c = 0; c < 2; ++c)
 #pragma unroll
-    for (int j = 0; j < 8; ++j) m.a3[c][j] = n < NOUT ? (__bf16)P[OFF_W3 + hid_feature(c, g, j) * NOUT + n] : (__bf16)0.0f;
+    for (int j = 0; j < 8; ++j)
+      m.a3[c][j] = n < NOUT ? (__bf16)(-2.0f * P[OFF_W3 + hid_feature(c, g, j) * NOUT + n]) : (__bf16)0.0f;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) m.b3[r] = (4 * g + r < NOUT) ? P[OFF_B3 + 4 * g + r] : 0.0f;
+  for (int r = 0; r < 4; ++r) m.b3[r] = (4 * g + r < NOUT) ? P[OFF_B3 + 4 * g + r] + cs[NH + 4 * g + r] : 0.0f;
 }
 
 // One tile of 16 samples.  xin: in lane group 0, the normalised features of
@@ -318,7 +340,7 @@ __device__ __forceinline__ floatx4 mlp_tile(const MlpRegs& m, int lane, const fl
     floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
     z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m.a1[T], b, z, 0, 0, 0);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) h[T][r] = fast_tanh(z[r]);
+    for (int r = 0; r < 4; ++r) h[T][r] = tanh_r(z[r]);
   }
   bf16x8 hb[2];
 #pragma unroll
@@ -331,7 +353,7 @@ __device__ __forceinline__ floatx4 mlp_tile(const MlpRegs& m, int lane, const fl
     z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m.a2[T][0], hb[0], z, 0, 0, 0);
     z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m.a2[T][1], hb[1], z, 0, 0, 0);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) h[T][r] = fast_tanh(z[r]);
+    for (int r = 0; r < 4; ++r) h[T][r] = tanh_r(z[r]);
   }
 #pragma unroll
   for (int c = 0; c < 2; ++c)
@@ -350,11 +372,14 @@ __global__ void __launch_bounds__(256) closure_fwd_mfma(int64_t n, const float* 
                                                         const float* __restrict__ P, const float* __restrict__ norm,
                                                         float* __restrict__ out) {
   __shared__ float Ps[NPARAM];
+  __shared__ float cs[NH + NOUT];
   for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) Ps[p] = P[p];
+  __syncthreads();
+  column_sums(Ps, cs);
   __syncthreads();
   MlpRegs m;
   const int lane = threadIdx.x & 63;
-  load_mlp(m, Ps, lane);
+  load_mlp(m, Ps, cs, lane);
   const int g = lane >> 4, cn = lane & 15;
   const int64_t n_tiles = (n + 15) / 16;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -387,11 +412,14 @@ __global__ void __launch_bounds__(256) closure_apply(int64_t n, float dt, bc_cha
                                                      const float* __restrict__ V1, float* __restrict__ S, int64_t ld,
                                                      const float* __restrict__ P, const float* __restrict__ norm) {
   __shared__ float Ps[NPARAM];
+  __shared__ float cs[NH + NOUT];
   for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) Ps[p] = P[p];
+  __syncthreads();
+  column_sums(Ps, cs);
   __syncthreads();
   MlpRegs m;
   const int lane = threadIdx.x & 63;
-  load_mlp(m, Ps, lane);
+  load_mlp(m, Ps, cs, lane);
   const int g = lane >> 4, cn = lane & 15;
   float mu[8], isd[8], sc[NOUT];
 #pragma unroll
