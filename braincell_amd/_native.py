@@ -45,6 +45,8 @@ class StaggerDesc(ctypes.Structure):
         ("state", ctypes.c_void_p), ("trace", ctypes.c_void_p), ("trace_ld", ctypes.c_int64),
         ("rec", ctypes.c_void_p), ("status", ctypes.c_void_p), ("step0", ctypes.c_int32), ("n_steps", ctypes.c_int32),
         ("dt", ctypes.c_double), ("flags", ctypes.c_uint32), ("syn", ctypes.c_void_p),
+        ("chain_rec", ctypes.c_void_p), ("chain_par", ctypes.c_void_p), ("kid_pad", ctypes.c_void_p),
+        ("slot_link", ctypes.c_void_p), ("slot_chain", ctypes.c_void_p),
     ]
 
 

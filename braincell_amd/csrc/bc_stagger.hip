@@ -87,6 +87,12 @@ struct StaggerArgs {
   uint8_t* spk;
   int32_t spk_len, n_cells;
   real v_th;
+  // chain tables in LDS form (bc_stagger_desc.chain_rec ..)
+  const int4* chain_rec;
+  const int16_t* chain_par;
+  const int16_t* kid_pad;
+  const int16_t* slot_link;
+  const int16_t* slot_chain;
 };
 
 constexpr uint32_t kRowMask = 0xFFFFu;
@@ -573,28 +579,21 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   const int tid = threadIdx.x;
   const int NT = blockDim.x;
   for (int i = tid; i < P.n_cls * BC_NCLS; i += NT) cls[i] = a.cls_tab[(int64_t)P.cls0 * BC_NCLS + i];
+  // the pack's rows and chain tables, copied with independent coalesced
+  // loads (the host precomputed their LDS form)
   for (int i = tid; i < P.n_rows; i += NT) {
-    R[i].lo = a.lo[(int64_t)P.row0 + i];
-    R[i].up = a.up[(int64_t)P.row0 + i];
+    const int64_t g = (int64_t)P.row0 + i;
+    R[i].lo = a.lo[g];
+    R[i].up = a.up[g];
+    m_link[i] = a.slot_link[g];
+    m_rowch[i] = a.slot_chain[g];
   }
-  const int kbase = a.kid_off[P.ch0];
   for (int i = tid; i < P.n_ch; i += NT) {
     const int g = P.ch0 + i;
-    const int j0 = a.chain_off[g] - P.row0, j1 = a.chain_off[g + 1] - P.row0;
-    const int q0 = a.kid_off[g] - kbase, nk = a.kid_off[g + 1] - kbase - q0;
-    const int k0 = nk > 0 ? a.kids[kbase + q0] : 0;
-    const int k1 = nk > 1 ? a.kids[kbase + q0 + 1] : 0;
-    const int tp = a.chain_top[g];
-    CR[i] = make_int4(j0 | (j1 << 16), (tp & 0xFFFF) | (nk << 16), k0 | (k1 << 16), q0);
-    for (int j = j0; j < j1; ++j) {
-      const int par = (j + 1 < j1) ? j + 1 : (tp >= 0 ? tp : j);
-      m_link[j] = (int16_t)(par | (j == j0 ? 0x8000 : 0));
-      m_rowch[j] = (int16_t)i;
-    }
-    for (int q = 0; q < nk; ++q) m_par[a.kids[kbase + q0 + q]] = (int16_t)i;
-    if (tp < 0) m_par[i] = -1;
+    CR[i] = a.chain_rec[g];
+    m_par[i] = a.chain_par[g];
+    m_kid[i] = a.kid_pad[g];
   }
-  for (int i = tid; i < a.kid_off[P.ch0 + P.n_ch] - kbase; i += NT) m_kid[i] = (int16_t)a.kids[kbase + i];
   for (int i = tid; i <= P.n_lv; i += NT) m_lv[i] = (int16_t)a.level_off[P.lv0 + i];
 
   real st[MAXC][NS];
@@ -968,6 +967,11 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->spk_len = y ? y->spk_len : 1;
   a->n_cells = y ? y->n_cells : 0;
   a->v_th = y ? static_cast<real>(y->v_th) : real(0);
+  a->chain_rec = reinterpret_cast<const int4*>(d->chain_rec);
+  a->chain_par = d->chain_par;
+  a->kid_pad = d->kid_pad;
+  a->slot_link = d->slot_link;
+  a->slot_chain = d->slot_chain;
   return BC_OK;
 }
 
@@ -987,7 +991,8 @@ int validate(const bc_stagger_desc* d) {
   if ((d->flags & ~(BC_STEP_SKIP_GATES | BC_STEP_INTEGRATION_ORDER | BC_STEP_VOLTAGE_ONLY)) != 0)
     return bc::fail(BC_INVALID_ARG, "unknown flags 0x%x", d->flags);
   if (d->n_packs > 0 && (d->packs == nullptr || d->cv_info == nullptr || d->V == nullptr || d->Vbnd == nullptr ||
-                         d->status == nullptr))
+                         d->status == nullptr || d->chain_rec == nullptr || d->chain_par == nullptr ||
+                         d->kid_pad == nullptr || d->slot_link == nullptr || d->slot_chain == nullptr))
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
   if (const bc_synapse_desc* y = d->syn) {
     if (y->n_cells <= 0 || y->min_delay < 1 || y->max_delay < y->min_delay)

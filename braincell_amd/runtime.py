@@ -307,6 +307,7 @@ class StaggerLayout:
         cv_info, bnd_info, bnd_cell = [], [], []
         chain_off, chain_top, kid_cnt, kids, level_off = [0], [], [], [], []
         slot_lo, slot_up, slot_rows_all = [], [], []
+        chain_rec_all, chain_par_all, kid_pad_all, slot_link_all, slot_chain_all = [], [], [], [], []
         cls_tab, clamp_rows, clamp_t, clamp_amp_raw, clamp_kind, clamp_area = [], [], [], [], [], []
         probe_row, probe_col = [], []
         self.cell_region_cls = np.full((n_cells, len(REGION_CODES)), -1, dtype=np.int64)
@@ -357,7 +358,26 @@ class StaggerLayout:
             parent = np.where(tp_slot >= 0, chain_at_bottom[np.maximum(tp_slot, 0)], -1)
             cnt = np.bincount(parent[parent >= 0], minlength=n_ch)
             kid_cnt.extend(cnt.tolist())
-            kids.extend(np.argsort(np.where(parent >= 0, parent, n_ch), kind="stable")[: int(cnt.sum())].tolist())
+            pk = np.argsort(np.where(parent >= 0, parent, n_ch), kind="stable")[: int(cnt.sum())]
+            kids.extend(pk.tolist())
+            # the pack's chain metadata in the kernel's LDS form (see bc_stagger_desc)
+            q0 = np.concatenate(([0], np.cumsum(cnt)))[:-1]
+            pkx = np.concatenate((pk, [0, 0]))
+            k0 = np.where(cnt > 0, pkx[q0], 0)
+            k1 = np.where(cnt > 1, pkx[np.minimum(q0 + 1, len(pkx) - 1)], 0)
+            j0, j1 = new_off[:-1], new_off[1:]
+            rec = np.stack([j0 | (j1 << 16), (tp_slot & 0xFFFF) | (cnt << 16), k0 | (k1 << 16), q0], axis=1)
+            chain_rec_all.append(rec.astype(np.int32))
+            chain_par_all.append(parent.astype(np.int16))
+            kp = np.zeros(n_ch, dtype=np.int16)
+            kp[: len(pk)] = pk
+            kid_pad_all.append(kp)
+            ch_of = np.repeat(np.arange(n_ch), lens)
+            link = np.arange(n_rows, dtype=np.int64) + 1
+            link[j1 - 1] = np.where(tp_slot >= 0, tp_slot, j1 - 1)
+            link[j0] |= 0x8000
+            slot_link_all.append(link.astype(np.uint16).view(np.int16))
+            slot_chain_all.append(ch_of.astype(np.int16))
             lv_sorted = lev[perm]
             nlev = int(lv_sorted.max()) + 1
             level_off.extend(np.searchsorted(lv_sorted, np.arange(nlev + 1), side="left").tolist())
@@ -434,6 +454,11 @@ class StaggerLayout:
         self.kid_off = np.concatenate(([0], np.cumsum(kid_cnt))).astype(np.int32)
         self.kids = np.asarray(kids, dtype=np.int32)
         self.level_off = np.asarray(level_off, dtype=np.int32)
+        self.chain_rec = np.concatenate(chain_rec_all, axis=0)
+        self.chain_par = np.concatenate(chain_par_all)
+        self.kid_pad = np.concatenate(kid_pad_all)
+        self.slot_link = np.concatenate(slot_link_all)
+        self.slot_chain = np.concatenate(slot_chain_all)
         self.slot_lo = np.concatenate(slot_lo)
         self.slot_up = np.concatenate(slot_up)
         self.slot_rows = slot_rows_all
@@ -545,6 +570,11 @@ class StaggerRuntime:
         self.kid_off = t(layout.kid_off, torch.int32)
         self.kids = t(layout.kids if layout.kids.size else np.zeros(1, np.int32), torch.int32)
         self.level_off = t(layout.level_off, torch.int32)
+        self.chain_rec = t(layout.chain_rec.reshape(-1), torch.int32)
+        self.chain_par = t(layout.chain_par, torch.int16)
+        self.kid_pad = t(layout.kid_pad, torch.int16)
+        self.slot_link = t(layout.slot_link, torch.int16)
+        self.slot_chain = t(layout.slot_chain, torch.int16)
         self.probe_row = t(layout.probe_row if layout.probe_row.size else np.zeros(1, np.int32), torch.int32)
         self.probe_col = t(layout.probe_col if layout.probe_col.size else np.zeros(1, np.int32), torch.int32)
         self.clamp_row = t(layout.clamp_row if layout.clamp_row.size else np.zeros(1, np.int32), torch.int32)
@@ -614,6 +644,8 @@ class StaggerRuntime:
         d.packs, d.cv_info, d.bnd_info = p(self.packs), p(self.cv_info), p(self.bnd_info)
         d.chain_off, d.chain_top, d.kid_off, d.kids = p(self.chain_off), p(self.chain_top), p(self.kid_off), p(self.kids)
         d.level_off = p(self.level_off)
+        d.chain_rec, d.chain_par, d.kid_pad = p(self.chain_rec), p(self.chain_par), p(self.kid_pad)
+        d.slot_link, d.slot_chain = p(self.slot_link), p(self.slot_chain)
         d.lo, d.up, d.cls_tab = p(self.lo), p(self.up), p(self.cls_tab)
         d.clamp_row, d.clamp_on, d.clamp_off, d.clamp_amp = p(self.clamp_row), p(self.clamp_on), p(self.clamp_off), p(self.clamp_amp)
         d.probe_row, d.probe_col = p(self.probe_row), p(self.probe_col)

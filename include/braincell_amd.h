@@ -163,6 +163,22 @@ typedef struct bc_stagger_desc {
   double dt;       /* ms */
   uint32_t flags;  /* BC_STEP_* */
   const bc_synapse_desc* syn; /* NULL: no synapses */
+  /* The chain tables above in the kernel's LDS form, precomputed per layout
+   * so that a workgroup copies them with independent loads (all pack-local):
+   *  chain_rec[4k..4k+3] per global chain k: j0 | j1 << 16 (slot range),
+   *                 top & 0xFFFF | n_kids << 16, kid0 | kid1 << 16 (first
+   *                 two kid chains), first kid entry of the pack's kid list
+   *  chain_par[k]   parent chain, -1 for a root
+   *  kid_pad[ch0 + i] the pack's kid list (entry i; kid_off order), padded
+   *                 with zeros to the pack's n_ch entries
+   *  slot_link[row] parent slot (a root's top slot: itself) | 0x8000 at
+   *                 a chain's bottom slot
+   *  slot_chain[row] chain of the slot */
+  const int32_t* chain_rec;
+  const int16_t* chain_par;
+  const int16_t* kid_pad;
+  const int16_t* slot_link;
+  const int16_t* slot_chain;
 } bc_stagger_desc;
 
 /* bc_stagger_desc.flags */

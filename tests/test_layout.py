@@ -203,3 +203,42 @@ def test_interior_attachment_packed_hines_matches_oracle_dhs():
         sl = L.cell_slice(i)
         Vn = dhs_solve(st, V[sl][None], (-G[sl])[None], (I[sl] + G[sl] * V[sl])[None], dt)[0]
         np.testing.assert_allclose(V[sl] + dV[rows[sl]], Vn, rtol=0, atol=1e-9)
+
+
+def test_precomputed_lds_tables_equal_the_kernels_derivation():
+    """chain_rec / chain_par / kid_pad / slot_link / slot_chain (copied into
+    LDS by the kernel) equal what a workgroup would derive from chain_off,
+    chain_top, kid_off and kids, pack by pack."""
+    from braincell_amd import workloads as W
+    from braincell_amd.runtime import StaggerLayout
+
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=900) for i in range(9)] + [W.single_tree_cell(0)]
+    L = StaggerLayout(models, rows_per_pack=2600)
+    assert L.n_packs > 1
+    rec = L.chain_rec.reshape(-1, 4)
+    for P in L.pack_array:
+        _, _, row0, n_rows, ch0, n_ch = P[:6]
+        kbase = L.kid_off[ch0]
+        par = np.full(n_ch, -1)
+        link = np.zeros(n_rows, dtype=np.int64)
+        chain = np.zeros(n_rows, dtype=np.int64)
+        for i in range(n_ch):
+            g = ch0 + i
+            j0, j1 = L.chain_off[g] - row0, L.chain_off[g + 1] - row0
+            q0, nk = L.kid_off[g] - kbase, L.kid_off[g + 1] - L.kid_off[g]
+            k0 = L.kids[kbase + q0] if nk > 0 else 0
+            k1 = L.kids[kbase + q0 + 1] if nk > 1 else 0
+            tp = L.chain_top[g]
+            assert tuple(rec[g]) == (j0 | (j1 << 16), (tp & 0xFFFF) | (nk << 16), k0 | (k1 << 16), q0)
+            for j in range(j0, j1):
+                p = j + 1 if j + 1 < j1 else (tp if tp >= 0 else j)
+                link[j] = p | (0x8000 if j == j0 else 0)
+                chain[j] = i
+            for q in range(nk):
+                par[L.kids[kbase + q0 + q]] = i
+        nkid = L.kid_off[ch0 + n_ch] - kbase
+        assert np.array_equal(L.chain_par[ch0:ch0 + n_ch], par)
+        assert np.array_equal(L.kid_pad[ch0:ch0 + nkid], L.kids[kbase:kbase + nkid])
+        assert not L.kid_pad[ch0 + nkid:ch0 + n_ch].any()
+        assert np.array_equal(L.slot_link[row0:row0 + n_rows].view(np.uint16), link)
+        assert np.array_equal(L.slot_chain[row0:row0 + n_rows], chain)
