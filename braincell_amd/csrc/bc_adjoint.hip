@@ -33,6 +33,9 @@ struct AdjArgs {
   const int32_t* kid_off;
   const int32_t* kids;
   const int32_t* level_off;
+  const int4* chain_rec;
+  const int16_t* chain_par;
+  const int16_t* kid_pad;
   const real* lo;
   const real* up;
   const real* cls_tab;
@@ -51,16 +54,18 @@ struct AdjArgs {
   bc_chan_const<real> cc;
 };
 
-// LDS of one workgroup: per-chain records {j0 | j1 << 16, top slot (-1 for a
-// root), first kid | kid count << 16, 0} first (16-byte aligned), then the
-// rows' diagonal, rhs, lowers and uppers, the chains' kid contributions, the
-// mechanism classes and their gradient accumulators, then the pack's kid
-// list and level offsets (int16).  The transposed solve then walks LDS only.
+// LDS of one workgroup: the pack's chain records in the forward kernel's
+// form {j0 | j1 << 16, top & 0xFFFF | n_kids << 16, kid0 | kid1 << 16,
+// first kid entry} (bc_stagger_desc.chain_rec) and the chains' arrival
+// counters first (16-byte aligned), then the rows' diagonal, rhs, lowers and
+// uppers, the chains' kid contributions, the mechanism classes and their
+// gradient accumulators, then the pack's kid list, parent chains and level
+// offsets (int16).  The transposed solve walks LDS only.
 __host__ __device__ inline size_t adj_lds_bytes(int max_rows, int max_chains, int max_cls, int max_levels,
                                                 size_t real_size) {
   const size_t mc = (size_t)((max_chains + 3) & ~3);
-  return mc * 16 + (size_t)(4 * max_rows + 2 * max_chains + 2 * max_cls * BC_NCLS) * real_size +
-         (mc + (size_t)((max_levels + 4) & ~3)) * sizeof(int16_t);
+  return mc * 16 + mc * 4 + (size_t)(4 * max_rows + 2 * max_chains + 2 * max_cls * BC_NCLS) * real_size +
+         (2 * mc + (size_t)((max_levels + 4) & ~3)) * sizeof(int16_t);
 }
 
 constexpr uint32_t kRowMask = 0xFFFFu;
@@ -74,7 +79,8 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int mc = (a.max_chains + 3) & ~3;
   int4* CR = reinterpret_cast<int4*>(smem_raw);
-  real* bd = reinterpret_cast<real*>(smem_raw + (size_t)mc * 16);
+  int* m_cnt = reinterpret_cast<int*>(smem_raw + (size_t)mc * 16);  // kids eliminated so far
+  real* bd = reinterpret_cast<real*>(smem_raw + (size_t)mc * 20);
   real* br = bd + a.max_rows;
   real* LO = br + a.max_rows;
   real* UP = LO + a.max_rows;
@@ -83,7 +89,8 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   real* cls = kr + a.max_chains;
   real* gacc = cls + a.max_cls * BC_NCLS;
   int16_t* m_kid = reinterpret_cast<int16_t*>(gacc + a.max_cls * BC_NCLS);
-  int16_t* m_lv = m_kid + mc;
+  int16_t* m_par = m_kid + mc;  // parent chain, -1 for a root
+  int16_t* m_lv = m_par + mc;
 
   const bc_pack P = a.packs[blockIdx.x];
   const int tid = threadIdx.x;
@@ -96,14 +103,11 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
     LO[i] = a.lo[(int64_t)P.row0 + i];
     UP[i] = a.up[(int64_t)P.row0 + i];
   }
-  const int kbase = a.kid_off[P.ch0];
   for (int k = tid; k < P.n_ch; k += NT) {
-    const int gk = P.ch0 + k;
-    const int j0 = a.chain_off[gk] - P.row0, j1 = a.chain_off[gk + 1] - P.row0;
-    const int q0 = a.kid_off[gk] - kbase, nk = a.kid_off[gk + 1] - kbase - q0;
-    CR[k] = make_int4(j0 | (j1 << 16), a.chain_top[gk], q0 | (nk << 16), 0);
+    CR[k] = a.chain_rec[P.ch0 + k];
+    m_kid[k] = a.kid_pad[P.ch0 + k];
+    m_par[k] = a.chain_par[P.ch0 + k];
   }
-  for (int i = tid; i < a.kid_off[P.ch0 + P.n_ch] - kbase; i += NT) m_kid[i] = (int16_t)a.kids[kbase + i];
   for (int i = tid; i <= P.n_lv; i += NT) m_lv[i] = (int16_t)a.level_off[P.lv0 + i];
   const bc_chan_const<real>& cc = a.cc;
   const real dt = a.dt;
@@ -275,6 +279,7 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
       bd[row] = real(0);
       br[row] = real(0);
     }
+    for (int k = tid; k < P.n_ch; k += NT) m_cnt[k] = 0;
     // the record of step n - 1, in flight during the solve
     real nxt[CM][CR_];
 #pragma unroll
@@ -293,47 +298,105 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
     // ---- phase 2: Rbar = M^-T Xbar.  M^T has the same diagonal (rebuilt as
     // e - lo, as in the forward solve); the off-diagonals swap, so the
     // child's rhs contribution uses lowers and the back substitution uppers.
-    for (int l = 0; l < P.n_lv; ++l) {
-      const int k0 = m_lv[l], k1 = m_lv[l + 1];
-      for (int k = k0 + tid; k < k1; k += NT) {
+    // Elimination as a dataflow over chains (no barrier): every thread
+    // starts at leaf chains, bumps the parent's arrival counter after a
+    // chain, and the last kid to arrive goes on with the parent.  A chain's
+    // arithmetic does not depend on which thread takes it.  Rows are read
+    // four at a time before any of them is written.
+    {
+      const int lv0 = m_lv[0], n_leaf = m_lv[1] - lv0;
+      int leaf = tid;
+      int k = leaf < n_leaf ? lv0 + leaf : -1;
+      while (k >= 0) {
         const int4 c = CR[k];
         const int j0 = c.x & 0xFFFF, j1 = (int)((unsigned)c.x >> 16);
-        const int q0 = c.z & 0xFFFF, q1 = q0 + (int)((unsigned)c.z >> 16);
+        const int nk = (int)((unsigned)c.y >> 16);
         real e = bd[j0];
         real r = br[j0];
-        for (int q = q0; q < q1; ++q) {
-          const int kq = m_kid[q];
+        if (nk > 0) {
+          const int q = c.z & 0xFFFF;
+          e -= kd[q];
+          r -= kr[q];
+        }
+        if (nk > 1) {
+          const int q = (int)((unsigned)c.z >> 16);
+          e -= kd[q];
+          r -= kr[q];
+        }
+        for (int q = 2; q < nk; ++q) {
+          const int kq = m_kid[c.w + q];
           e -= kd[kq];
           r -= kr[kq];
         }
-        real kdc, krc;
-        for (int j = j0;;) {
-          const real lo = LO[j], up = UP[j];
-          const real inv = bc_rcp<real>(e - lo);
-          bd[j] = inv;
-          br[j] = r;
-          kdc = up * e * inv;
-          krc = lo * r * inv;
-          if (++j >= j1) break;
-          e = bd[j] - kdc;
-          r = br[j] - krc;
+        real kdc = real(0), krc = real(0);
+        for (int j = j0; j < j1; j += 4) {
+          real glo[4], gup[4], gd[4], gr[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int jj = j + i < j1 ? j + i : j1 - 1;
+            glo[i] = LO[jj];
+            gup[i] = UP[jj];
+            gd[i] = bd[jj];
+            gr[i] = br[jj];
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (j + i < j1) {
+              if (j + i > j0) {
+                e = gd[i] - kdc;
+                r = gr[i] - krc;
+              }
+              const real inv = bc_rcp<real>(e - glo[i]);
+              bd[j + i] = inv;
+              br[j + i] = r;
+              kdc = gup[i] * e * inv;
+              krc = glo[i] * r * inv;
+            }
+          }
         }
         kd[k] = kdc;
         kr[k] = krc;
+        const int p = m_par[k];
+        bool climb = false;
+        if (p >= 0) {
+          const int pk = (int)((unsigned)CR[p].y >> 16);
+          climb = __hip_atomic_fetch_add(&m_cnt[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == pk - 1;
+        }
+        if (climb) {
+          k = p;
+        } else {
+          leaf += NT;
+          k = leaf < n_leaf ? lv0 + leaf : -1;
+        }
       }
-      __syncthreads();
     }
+    __syncthreads();
+    // back substitution, top level first (a chain's top row hangs from a row
+    // of a higher level), rows read four at a time
     for (int l = P.n_lv - 1; l >= 0; --l) {
       const int k0 = m_lv[l], k1 = m_lv[l + 1];
       for (int k = k0 + tid; k < k1; k += NT) {
         const int4 c = CR[k];
         const int j0 = c.x & 0xFFFF, j1 = (int)((unsigned)c.x >> 16);
-        const int tp = c.y;
+        const int tp = (c.y << 16) >> 16;
         real xp = tp >= 0 ? br[tp] : real(0);
-        for (int j = j1 - 1; j >= j0; --j) {
-          const real x = (br[j] - UP[j] * xp) * bd[j];
-          br[j] = x;
-          xp = x;
+        for (int j = j1 - 1; j >= j0; j -= 4) {
+          real gb[4], gu[4], gd[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int jj = j - i >= j0 ? j - i : j0;
+            gb[i] = br[jj];
+            gu[i] = UP[jj];
+            gd[i] = bd[jj];
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (j - i >= j0) {
+              const real x = (gb[i] - gu[i] * xp) * gd[i];
+              br[j - i] = x;
+              xp = x;
+            }
+          }
         }
       }
       __syncthreads();
@@ -451,6 +514,9 @@ int adj_launch(const bc_adjoint_desc* d, hipStream_t s) {
   a.kid_off = f->kid_off;
   a.kids = f->kids;
   a.level_off = f->level_off;
+  a.chain_rec = reinterpret_cast<const int4*>(f->chain_rec);
+  a.chain_par = f->chain_par;
+  a.kid_pad = f->kid_pad;
   a.lo = static_cast<const real*>(f->lo);
   a.up = static_cast<const real*>(f->up);
   a.cls_tab = static_cast<const real*>(f->cls_tab);
@@ -499,6 +565,9 @@ int bc_stagger_adjoint(const bc_adjoint_desc* d, void* stream) {
     return bc::fail(BC_UNSUPPORTED, "the adjoint differentiates the full step: BC_STEP_SKIP_GATES and "
                                     "BC_STEP_VOLTAGE_ONLY are not supported");
   if (f->syn != nullptr) return bc::fail(BC_UNSUPPORTED, "the adjoint does not differentiate synapses");
+  if (f->n_packs > 0 && (f->chain_rec == nullptr || f->chain_par == nullptr || f->kid_pad == nullptr ||
+                         f->level_off == nullptr))
+    return bc::fail(BC_INVALID_ARG, "layout chain tables (chain_rec, chain_par, kid_pad, level_off) are NULL");
   if (f->n_packs > 0 && (d->rec == nullptr || d->Vbar == nullptr || d->Sbar == nullptr || d->cls_grad == nullptr ||
                          (f->trace_ld > 0 && d->trace_bar == nullptr)))
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
