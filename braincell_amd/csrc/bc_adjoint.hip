@@ -60,12 +60,14 @@ struct AdjArgs {
 // counters first (16-byte aligned), then the rows' diagonal, rhs, lowers and
 // uppers, the chains' kid contributions, the mechanism classes and their
 // gradient accumulators, then the pack's kid list, parent chains and level
-// offsets (int16).  The transposed solve walks LDS only.
+// offsets (int16), and two more per-chain (real, real) buffers and two
+// int16 pointer arrays for the back substitution's pointer jumping.  The
+// transposed solve walks LDS only.
 __host__ __device__ inline size_t adj_lds_bytes(int max_rows, int max_chains, int max_cls, int max_levels,
                                                 size_t real_size) {
   const size_t mc = (size_t)((max_chains + 3) & ~3);
-  return mc * 16 + mc * 4 + (size_t)(4 * max_rows + 2 * max_chains + 2 * max_cls * BC_NCLS) * real_size +
-         (2 * mc + (size_t)((max_levels + 4) & ~3)) * sizeof(int16_t);
+  return mc * 16 + mc * 4 + (size_t)(4 * max_rows + 4 * max_chains + 2 * max_cls * BC_NCLS) * real_size +
+         (4 * mc + (size_t)((max_levels + 4) & ~3)) * sizeof(int16_t);
 }
 
 constexpr uint32_t kRowMask = 0xFFFFu;
@@ -86,11 +88,15 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   real* UP = LO + a.max_rows;
   real* kd = UP + a.max_rows;
   real* kr = kd + a.max_chains;
-  real* cls = kr + a.max_chains;
+  real* kd2 = kr + a.max_chains;  // pointer jumping: second (A, B) buffer
+  real* kr2 = kd2 + a.max_chains;
+  real* cls = kr2 + a.max_chains;
   real* gacc = cls + a.max_cls * BC_NCLS;
   int16_t* m_kid = reinterpret_cast<int16_t*>(gacc + a.max_cls * BC_NCLS);
   int16_t* m_par = m_kid + mc;  // parent chain, -1 for a root
-  int16_t* m_lv = m_par + mc;
+  int16_t* m_ptr = m_par + mc;  // pointer jumping (two buffers)
+  int16_t* m_ptr2 = m_ptr + mc;
+  int16_t* m_lv = m_ptr2 + mc;
 
   const bc_pack P = a.packs[blockIdx.x];
   const int tid = threadIdx.x;
@@ -371,36 +377,82 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
       }
     }
     __syncthreads();
-    // back substitution, top level first (a chain's top row hangs from a row
-    // of a higher level), rows read four at a time
-    for (int l = P.n_lv - 1; l >= 0; --l) {
-      const int k0 = m_lv[l], k1 = m_lv[l + 1];
-      for (int k = k0 + tid; k < k1; k += NT) {
-        const int4 c = CR[k];
-        const int j0 = c.x & 0xFFFF, j1 = (int)((unsigned)c.x >> 16);
-        const int tp = (c.y << 16) >> 16;
-        real xp = tp >= 0 ? br[tp] : real(0);
-        for (int j = j1 - 1; j >= j0; j -= 4) {
-          real gb[4], gu[4], gd[4];
+    // Back substitution in affine form (the transposed system's x_j =
+    // (r_j - up_j x_{j+1}) d_j inside a chain, d_j the stored inverse
+    // pivot): every chain at once computes x_j = alpha_j + beta_j X, X the
+    // solution of the row it hangs from (its parent chain's bottom row),
+    // alpha and beta overwriting br and bd; the chains' bottom solutions then
+    // follow by pointer jumping over the chain tree, and every row takes its
+    // solution.  Rows are read four at a time.
+    for (int k = tid; k < P.n_ch; k += NT) {
+      const int4 c = CR[k];
+      const int j0 = c.x & 0xFFFF, j1 = (int)((unsigned)c.x >> 16);
+      const int tp = (c.y << 16) >> 16;
+      real al = real(0), be = tp >= 0 ? real(1) : real(0);
+      for (int j = j1 - 1; j >= j0; j -= 4) {
+        real gb[4], gu[4], gd[4];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int jj = j - i >= j0 ? j - i : j0;
-            gb[i] = br[jj];
-            gu[i] = UP[jj];
-            gd[i] = bd[jj];
-          }
+        for (int i = 0; i < 4; ++i) {
+          const int jj = j - i >= j0 ? j - i : j0;
+          gb[i] = br[jj];
+          gu[i] = UP[jj];
+          gd[i] = bd[jj];
+        }
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            if (j - i >= j0) {
-              const real x = (gb[i] - gu[i] * xp) * gd[i];
-              br[j - i] = x;
-              xp = x;
-            }
+        for (int i = 0; i < 4; ++i) {
+          if (j - i >= j0) {
+            const real a2 = (gb[i] - gu[i] * al) * gd[i];
+            const real b2 = -gu[i] * be * gd[i];
+            br[j - i] = a2;
+            bd[j - i] = b2;
+            al = a2;
+            be = b2;
           }
         }
       }
-      __syncthreads();
+      kd[k] = al;
+      kr[k] = be;
+      m_ptr[k] = m_par[k];
     }
+    __syncthreads();
+    const real* Xb;
+    {
+      // (A, B, ptr): x_bottom(k) = A + B x_bottom(ptr); ptr < 0 once resolved.
+      // A forest of at most 32767 chains resolves in 16 rounds.
+      real *A0 = kd, *B0 = kr, *A1 = kd2, *B1 = kr2;
+      int16_t *p0 = m_ptr, *p1 = m_ptr2;
+      for (int round = 0; round <= 16; ++round) {
+        int open = 0;
+        for (int k = tid; k < P.n_ch; k += NT) {
+          const int q = p0[k];
+          real va = A0[k], vb = B0[k];
+          int qn = q;
+          if (q >= 0) {
+            va = va + vb * A0[q];
+            vb = vb * B0[q];
+            qn = p0[q];
+            open |= qn >= 0;
+          }
+          A1[k] = va;
+          B1[k] = vb;
+          p1[k] = (int16_t)qn;
+        }
+        const int more = __syncthreads_or(open);
+        real* t = A0; A0 = A1; A1 = t;
+        t = B0; B0 = B1; B1 = t;
+        int16_t* tq = p0; p0 = p1; p1 = tq;
+        if (!more) break;
+      }
+      Xb = A0;
+    }
+    for (int k = tid; k < P.n_ch; k += NT) {
+      const int4 c = CR[k];
+      const int j0 = c.x & 0xFFFF, j1 = (int)((unsigned)c.x >> 16);
+      const int par = m_par[k];
+      const real X = par >= 0 ? Xb[par] : real(0);
+      for (int j = j0; j < j1; ++j) br[j] = br[j] + bd[j] * X;
+    }
+    __syncthreads();
 
     // ---- phase 3: linearisation backward
 #pragma unroll
