@@ -93,3 +93,40 @@ def test_gpu_adjoint_fp64_matches_fd_and_fp32_is_close(cuda):
     rel = np.abs(g32 - g).max() / np.abs(g).max()
     print("fp32 vs fp64 gradient, max err / max |g|:", rel)
     assert rel < 1e-2
+
+
+@pytest.mark.parametrize("threads,cvpt", [(256, 4), (128, 8)])
+def test_gpu_adjoint_fp64_large_trees_match_oracle_adjoint(cuda, threads, cvpt):
+    """Two ~1000-CV trees (chain trees of depth ~20: several pointer-jumping
+    rounds in the reverse solve), at 4 and 8 CVs per thread in FP64 (the
+    reverse kernel then reads the recorded states where it uses them instead
+    of carrying them in registers), over 240 steps in checkpoint segments of
+    100, 100 and 40."""
+    import torch
+    from braincell_amd.adjoint import AdjointRuntime
+    from oracle.adjoint import adjoint, forward_states
+    from oracle.staggered import StaggeredOracle
+
+    steps = 240
+    models = [W.region_tree_cell(i, n_branches=250) for i in range(2)]
+    for m in models:
+        m.clamps = [(0, 0.5, 1.0, 1e9, 0.3)]
+    adj = AdjointRuntime(models, 0.025, precision=64, checkpoint_every=100, threads=threads)
+    assert adj.L.cv_per_thread == cvpt and adj.L.max_levels > 10
+    wts = np.random.default_rng(3).standard_normal((steps, 2))
+    adj.set_params(torch.as_tensor(adj.params(), dtype=torch.float64))
+    adj.forward(steps)
+    g = adj.theta_grad(adj.backward(torch.as_tensor(wts, device="cuda"))).cpu().numpy()
+    for i, m in enumerate(models):
+        o = StaggeredOracle(m)
+        o.reset_state(m.V0)
+        states = forward_states(o, 0.025, steps)
+        o2 = StaggeredOracle(m)
+        o2.reset_state(m.V0)
+        gr = adjoint(o2, states, 0.025, wts[:, i], soma_cv(m))
+        types = [b.type for b in m.morph.branches]
+        for r, reg in enumerate(W.REGIONS):
+            cvm = np.asarray([t == reg for t in types])[o.branch_of]
+            for k, nm in enumerate(("Na_Ba2002", "KDR_Ba2002", "IL", "CaL_IS2008")):
+                ref = gr[nm][cvm].sum() * 1e3  # per S/cm^2
+                assert abs(g[i, r, k] - ref) <= 1e-6 * max(1.0, abs(ref)) + 1e-5 * abs(ref), (i, reg, nm, g[i, r, k], ref)
