@@ -71,7 +71,8 @@ def main():
                     help="finish with a 2 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "braincell_amd benchmarks one GPU; multi-GPU runs are not implemented "
+                                   "(cells would shard by packs with no per-step communication)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
