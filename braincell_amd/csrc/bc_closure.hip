@@ -67,21 +67,21 @@ __device__ __forceinline__ void column_sums(const float* P, float* cs) {
   }
 }
 
-// Plain exponential-Euler update of the 5 gates at voltage V (the staggered
-// step's phase-C formulas).
+// Plain exponential-Euler update of the 5 gates at voltage V: the staggered
+// step's phase-C formulas under its FP32 math policy (hardware exponential).
 __device__ __forceinline__ void plain_gates(const bc_chan_const<float>& cc, float V, const float* x, float dt,
                                             float* out) {
   float ap, bp, aq, bq;
-  bc_na_rates<float>(V, cc.na_vsh, ap, bp, aq, bq);
-  out[0] = bc_gate_ab_cf<float>(x[0], ap, bp, cc.na_phi, dt);
-  out[1] = bc_gate_ab_cf<float>(x[1], aq, bq, cc.na_phi, dt);
+  bc_na_rates<float, bc_fastmath<float>>(V, cc.na_vsh, ap, bp, aq, bq);
+  out[0] = bc_gate_ab_cf<float, bc_fastmath<float>>(x[0], ap, bp, cc.na_phi, dt);
+  out[1] = bc_gate_ab_cf<float, bc_fastmath<float>>(x[1], aq, bq, cc.na_phi, dt);
   float kp, kb;
-  bc_kdr_rates<float>(V, cc.kdr_vsh, kp, kb);
-  out[2] = bc_gate_ab_cf<float>(x[2], kp, kb, cc.kdr_phi, dt);
+  bc_kdr_rates<float, bc_fastmath<float>>(V, cc.kdr_vsh, kp, kb);
+  out[2] = bc_gate_ab_cf<float, bc_fastmath<float>>(x[2], kp, kb, cc.kdr_phi, dt);
   float pi, pt, qi, qt;
-  bc_cal_rates<float>(V, cc.cal_vsh, pi, pt, qi, qt);
-  out[3] = bc_gate_it_cf<float>(x[3], pi, pt, cc.cal_phi_p, dt);
-  out[4] = bc_gate_it_cf<float>(x[4], qi, qt, cc.cal_phi_q, dt);
+  bc_cal_rates<float, bc_fastmath<float>>(V, cc.cal_vsh, pi, pt, qi, qt);
+  out[3] = bc_gate_it_cf<float, bc_fastmath<float>>(x[3], pi, pt, cc.cal_phi_p, dt);
+  out[4] = bc_gate_it_cf<float, bc_fastmath<float>>(x[4], qi, qt, cc.cal_phi_q, dt);
 }
 
 // ------------------------------------------------------------ training data
