@@ -847,14 +847,19 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   }
 
   if (a.rec) record_state<real, MAXC, NS>(a, P, tid, NT, a.n_steps, vv, st, info);
+  // with the gates skipped only V and the Ca pool changed: the gates' HBM
+  // copies already hold their values
+  const bool gates_out = !(a.flags & BC_STEP_SKIP_GATES);
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
     if (info[i] != 0xFFFFFFFFu) {
       int64_t g = (int64_t)P.cv0 + tid + i * NT;
       a.V[g] = vv[i];
 #pragma unroll
-      for (int s = 0; s < NS; ++s)
-        if (s < bc_nstates(CH)) a.state[(int64_t)s * a.n_cv_total + g] = st[i][s];
+      for (int s = 0; s < NS; ++s) {
+        const bool pool = (CH & BC_CH_POOL) != 0 && s == bc_slot_ca(CH);
+        if (s < bc_nstates(CH) && (gates_out || pool)) a.state[(int64_t)s * a.n_cv_total + g] = st[i][s];
+      }
     }
   }
   for (int i = tid; i < P.n_bnd; i += NT) a.Vbnd[P.bnd0 + i] = xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu];
