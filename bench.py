@@ -89,6 +89,22 @@ def main():
     return bench_cells(args)
 
 
+def step_plan(steps: int, warmup: int, chunk: int):
+    """Launch plan of the timed region: exactly ``steps`` steps as ``n_full``
+    launches of ``chunk`` steps (at most the requested chunk) plus one launch
+    of ``tail`` steps; ``warmup`` untimed steps before it.
+    Returns (chunk, n_full, tail, warmup)."""
+    if steps < 1:
+        raise ValueError("--steps must be >= 1")
+    if warmup < 0:
+        raise ValueError("--warmup must be >= 0")
+    if chunk < 1:
+        raise ValueError("--chunk must be >= 1")
+    chunk = min(chunk, steps)
+    n_full = steps // chunk
+    return chunk, n_full, steps - n_full * chunk, warmup
+
+
 def bench_cells(args):
     import torch
     from braincell_amd import workloads as W
@@ -106,37 +122,42 @@ def bench_cells(args):
         L, _ = autotune(L, dt, precision=32, log=log)
         log(f"autotuned layout {L.summary()} in {time.perf_counter() - t0:.1f}s")
     rt = StaggerRuntime(L, dt, precision=32)
-    chunk = args.chunk
-    K = (args.steps // chunk) * chunk
-    Wm = max(chunk, (args.warmup // chunk) * chunk)
-    trace = rt.new_trace(max(K, Wm))
-    # warmup (also used for the CPU-baseline accuracy sample)
-    rt.run(Wm, trace=trace[:Wm], chunk=chunk)
+    chunk, n_full, tail, Wm = step_plan(args.steps, args.warmup, args.chunk)
+    K = n_full * chunk + tail
+    if Wm:
+        rt.run(Wm, trace=rt.new_trace(Wm), chunk=chunk)
     torch.cuda.synchronize()
     rt.check_status()
-    # timed: hipGraph of K / chunk persistent launches
+    # timed: exactly K steps from the initial state, a hipGraph of n_full
+    # persistent launches of `chunk` steps plus one launch of the tail
     rt.reset_state()
-    graph = rt.make_graph(K // chunk, chunk, trace)
+    trace = rt.new_trace(K)
+    graph = rt.make_graph(n_full, chunk, trace) if n_full else None
     stream = rt.stream_handle()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    graph.launch(stream)
+    if graph is not None:
+        graph.launch(stream)
+    if tail:
+        rt.step_index = n_full * chunk
+        rt.run(tail, trace=trace[n_full * chunk:], chunk=tail)
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
     rt.check_status()
     cu = float(L.n_cv_total) * K / el
-    log(f"timed {K} steps in {el:.3f}s -> {cu:.3e} compartment-updates/s")
+    log(f"timed {K} steps ({n_full} x {chunk} in a graph + {tail}) in {el:.4f}s -> {cu:.3e} compartment-updates/s")
 
-    # per-launch duration of the dominant kernel with HIP events on its stream
-    # (own trace buffer: the timed run's trace is the accuracy sample below)
+    # per-launch duration of the dominant kernel with HIP events on its stream:
+    # launches of the roofline chunk (the profiled launch shape), whatever --steps is
+    rc_chunk = args.chunk
     rt.reset_state()
     n_ev = 5
-    ev_trace = rt.new_trace(chunk)
+    ev_trace = rt.new_trace(rc_chunk)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(n_ev + 1)]
     torch.cuda.synchronize()
     evs[0].record()
     for i in range(n_ev):
-        rt.run(chunk, trace=ev_trace, chunk=chunk)
+        rt.run(rc_chunk, trace=ev_trace, chunk=rc_chunk)
         evs[i + 1].record()
     torch.cuda.synchronize()
     durs = [evs[i].elapsed_time(evs[i + 1]) * 1e-3 for i in range(n_ev)]
@@ -144,12 +165,12 @@ def bench_cells(args):
     ca_cv = 0
     for i, m in enumerate(models):
         ca_cv += int(np.sum(np.asarray(m.ca["mask_branch"])[L.cells[i].geom.cv_branch])) if m.ca else 0
-    bytes_per_launch = (ca_cv * BYTES_CA_CV + (L.n_cv_total - ca_cv) * BYTES_PLAIN_CV) * chunk
+    bytes_per_launch = (ca_cv * BYTES_CA_CV + (L.n_cv_total - ca_cv) * BYTES_PLAIN_CV) * rc_chunk
     achieved = bytes_per_launch / launch_s / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": f"stagger_kernel<float, NA|KDR|IL|CaL|pool, {L.cv_per_thread}>", "launch_ms": round(launch_s * 1e3, 3),
-            "steps_per_launch": chunk}
+            "steps_per_launch": rc_chunk, "bytes_per_launch": bytes_per_launch}
     tf = os.path.join(ROOT, "profiles", "stagger_c2_traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
@@ -158,11 +179,15 @@ def bench_cells(args):
     cpu = None
     acc = None
     if not args.no_cpu:
-        # the timed run started from the initial state, so its trace is the
-        # GPU side of the accuracy comparison
-        ns = min(args.cpu_sample_steps, K)
+        # accuracy sample: a fixed window of cpu_sample_steps steps from the
+        # initial state, run in launches of the roofline chunk (independent of --steps)
+        ns = args.cpu_sample_steps
+        rt.reset_state()
+        acc_trace = rt.run(ns, chunk=rc_chunk)
+        torch.cuda.synchronize()
+        rt.check_status()
         sample = list(range(min(args.cpu_sample_cells, args.cells)))
-        gpu_soma = trace[:ns, [int(L.cell_probe0[i]) for i in sample]].double().cpu().numpy()
+        gpu_soma = acc_trace[:, [int(L.cell_probe0[i]) for i in sample]].double().cpu().numpy()
         t2 = time.perf_counter()
         cvals, cel, cth, rms = cpu_baseline([models[i] for i in sample], [L.cells[i].probes[0] for i in sample], ns,
                                             gpu_soma, dt)
@@ -189,7 +214,8 @@ def bench_cells(args):
                    "cells": args.cells, "compartments": int(L.n_cv_total), "dhs_rows": int(L.n_rows_total),
                    "packs": L.n_packs, "threads": L.threads, "cv_per_thread": L.cv_per_thread,
                    "rows_per_pack": L.rows_per_pack, "autotuned": not args.no_autotune,
-                   "steps_per_launch": chunk, "parallelism": "1 GPU, one workgroup per pack"},
+                   "steps_per_launch": chunk, "tail_steps": tail,
+                   "parallelism": "1 GPU, one workgroup per pack"},
         "roofline": roof, "cpu_baseline": cpu, "accuracy": acc,
     }
     print(json.dumps(out), flush=True)
