@@ -34,7 +34,7 @@ def cpu_baseline(models, soma_cvs, n_steps, gpu_soma, dt):
     """FP64 C port of the reference algorithm (OpenMP over cells, every host
     core this process may use) on a bounded sample of the same workload.
     Returns (compartment-updates/s, seconds, threads, per-cell soma RMSE of
-    the GPU FP32 trace against the port's FP64 trace)."""
+    the GPU FP32 trace against the port's FP64 trace, the port's soma trace)."""
     from oracle.cport_run import PortBatch
 
     batch = PortBatch(models, dt, soma_cvs)
@@ -43,7 +43,7 @@ def cpu_baseline(models, soma_cvs, n_steps, gpu_soma, dt):
     tr = batch.run(n_steps, sequential=True, threads=threads)
     el = time.perf_counter() - t0
     rm = [float(np.sqrt(np.mean((tr[:, i] - gpu_soma[:, i]) ** 2))) for i in range(len(models))]
-    return batch.n_cv * n_steps / el, el, threads, rm
+    return batch.n_cv * n_steps / el, el, threads, rm, tr
 
 
 def main():
@@ -124,6 +124,10 @@ def bench_cells(args):
     rt = StaggerRuntime(L, dt, precision=32)
     chunk, n_full, tail, Wm = step_plan(args.steps, args.warmup, args.chunk)
     K = n_full * chunk + tail
+    # spike output (V_n < 0 mV <= V_{n+1} at every soma) is part of the step
+    from braincell_amd.cell import spike_cap
+
+    rt.enable_spikes(0.0, cap=spike_cap(max(K, args.cpu_sample_steps), dt))
     if Wm:
         rt.run(Wm, trace=rt.new_trace(Wm), chunk=chunk)
     torch.cuda.synchronize()
@@ -145,7 +149,9 @@ def bench_cells(args):
     el = time.perf_counter() - t1
     rt.check_status()
     cu = float(L.n_cv_total) * K / el
-    log(f"timed {K} steps ({n_full} x {chunk} in a graph + {tail}) in {el:.4f}s -> {cu:.3e} compartment-updates/s")
+    n_spikes = int(rt._spk_buf["count"].sum().item())
+    log(f"timed {K} steps ({n_full} x {chunk} in a graph + {tail}) in {el:.4f}s -> {cu:.3e} compartment-updates/s, "
+        f"{n_spikes} soma spikes")
 
     # per-launch duration of the dominant kernel with HIP events on its stream:
     # launches of the roofline chunk (the profiled launch shape), whatever --steps is
@@ -188,9 +194,22 @@ def bench_cells(args):
         rt.check_status()
         sample = list(range(min(args.cpu_sample_cells, args.cells)))
         gpu_soma = acc_trace[:, [int(L.cell_probe0[i]) for i in sample]].double().cpu().numpy()
+        gpu_spk = rt.spike_steps()
         t2 = time.perf_counter()
-        cvals, cel, cth, rms = cpu_baseline([models[i] for i in sample], [L.cells[i].probes[0] for i in sample], ns,
-                                            gpu_soma, dt)
+        cvals, cel, cth, rms, port_soma = cpu_baseline([models[i] for i in sample],
+                                                       [L.cells[i].probes[0] for i in sample], ns, gpu_soma, dt)
+        # spike steps: the kernel's record against the port's crossings
+        n_mis, dmax, n_ref = 0, 0, 0
+        for k, i in enumerate(sample):
+            v = port_soma[:, k]
+            prev = np.concatenate([[models[i].V0], v[:-1]])
+            ref_st = np.nonzero((prev < 0.0) & (v >= 0.0))[0]
+            got_st = gpu_spk[int(L.cell_probe0[i])]
+            n_ref += ref_st.size
+            if got_st.size != ref_st.size:
+                n_mis += 1
+            elif ref_st.size:
+                dmax = max(dmax, int(np.abs(got_st - ref_st).max()))
         log(f"cpu baseline {cvals:.3e} compartment-updates/s on {cth} threads ({cel:.1f}s run, "
             f"{time.perf_counter() - t2:.1f}s with setup)")
         ncv = sum(L.cells[i].geom.n_cv for i in sample)
@@ -198,6 +217,8 @@ def bench_cells(args):
                "sample": f"FP64 C port (OpenMP, {cth} threads), cells 0-{len(sample) - 1} of the same batch "
                          f"({ncv} CVs) x {ns} steps, {cel:.1f}s"}
         acc = {"soma_rmse_mV_max": float(f"{max(rms):.3g}"), "soma_rmse_mV_mean": float(f"{np.mean(rms):.3g}"),
+               "spikes_reference": n_ref, "cells_with_spike_count_mismatch": n_mis,
+               "spike_time_err_max_ms": round(dmax * dt, 6),
                "cells": len(sample), "steps": ns, "reference": "FP64 C port"}
     if args.calibrate_copy:
         a = torch.empty(512 * 1024 * 1024, dtype=torch.float32, device="cuda")

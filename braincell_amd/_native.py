@@ -46,8 +46,13 @@ class StaggerDesc(ctypes.Structure):
         ("rec", ctypes.c_void_p), ("status", ctypes.c_void_p), ("step0", ctypes.c_int32), ("n_steps", ctypes.c_int32),
         ("dt", ctypes.c_double), ("flags", ctypes.c_uint32), ("syn", ctypes.c_void_p),
         ("chain_rec", ctypes.c_void_p), ("chain_par", ctypes.c_void_p), ("kid_pad", ctypes.c_void_p),
-        ("slot_link", ctypes.c_void_p), ("slot_chain", ctypes.c_void_p),
+        ("slot_link", ctypes.c_void_p), ("slot_chain", ctypes.c_void_p), ("spk", ctypes.c_void_p),
     ]
+
+
+class SpikeDesc(ctypes.Structure):
+    _fields_ = [("cap", ctypes.c_int32), ("v_th", ctypes.c_double), ("v_last", ctypes.c_void_p),
+                ("count", ctypes.c_void_p), ("steps", ctypes.c_void_p)]
 
 
 class SynapseDesc(ctypes.Structure):

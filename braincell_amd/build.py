@@ -41,15 +41,21 @@ def library_path() -> str:
 def build(verbose: bool = True) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
+
+    cmds, objs = [], []
     for src in sources():
         obj = os.path.join(LIBDIR, os.path.basename(src).replace(".hip", ".o"))
         extra = EXTRA_FLAGS.get(os.path.basename(src), [])
-        cmd = [hipcc, *[f for f in HIPCC_FLAGS if f != "-shared"], *extra, "-c", "-I", CSRC, "-I", INCLUDE, src, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([hipcc, *[f for f in HIPCC_FLAGS if f != "-shared"], *extra, "-c", "-I", CSRC, "-I", INCLUDE, src,
+                     "-o", obj])
         objs.append(obj)
+        if verbose:
+            print(" ".join(cmds[-1]), flush=True)
+    # one hipcc per source, concurrently (each is single-threaded)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as ex:
+        for r in list(ex.map(lambda c: subprocess.run(c, check=True), cmds)):
+            pass
     out = library_path()
     tmp = out + ".tmp"
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]

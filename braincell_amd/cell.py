@@ -14,7 +14,7 @@ runs many cells (uneven trees) in one packed launch.
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -34,6 +34,21 @@ VOLTAGE_ONLY = 4  # BC_STEP_VOLTAGE_ONLY
 class RunResult:
     time: np.ndarray  # ms, step start times (run.py:64-71)
     traces: dict
+    # voltage trace name -> per column, the times (entries of ``time``) of
+    # the steps at which that CV crossed V_th upwards, V_n < V_th <= V_{n+1}
+    # (the steps at which the reference's ``Cell.spike`` is 1)
+    spikes: dict = field(default_factory=dict)
+
+
+def spike_cap(n_steps: int, dt: float) -> int:
+    """Spike record per column for a run: 500 Hz over its duration, and no
+    more than the ceil(n / 2) upward crossings that n steps can hold."""
+    return int(max(1, min((n_steps + 1) // 2, np.ceil(n_steps * float(dt) * 0.5) + 8)))
+
+
+def _spikes_by_column(rt, step0: int, times: np.ndarray, cols) -> list:
+    st = rt.spike_steps()
+    return [times[st[c] - step0] for c in cols]
 
 
 class Cell:
@@ -246,13 +261,32 @@ class Cell:
         n = int(times.shape[0])
         if n == 0:
             raise ValueError("Cell.run(...) produced no timesteps; ensure duration > 0 and dt > 0.")
-        start = rt.step_index * float(dt)
+        step0 = rt.step_index
+        start = step0 * float(dt)
+        if rt._spk is None or rt._spk.cap < spike_cap(n, dt) or rt._spk.v_th != self.V_th:
+            rt.enable_spikes(self.V_th, spike_cap(n, dt))
+        else:
+            rt.clear_spikes()
         if not self._probes:  # no probes placed: the layout's voltage trace
             tr = rt.run(n, chunk=chunk)
             torch.cuda.synchronize()
             rt.check_status()
-            return RunResult(time=start + times, traces={"V": tr.double().cpu().numpy()})
-        return RunResult(time=start + times, traces=self._run_probes(rt, n, chunk))
+            V = tr.double().cpu().numpy()
+            return RunResult(time=start + times, traces={"V": V},
+                             spikes={"V": _spikes_by_column(rt, step0, start + times, range(V.shape[1]))})
+        traces, spike_cols = self._run_probes(rt, n, chunk)
+        st = {name: _spikes_by_column(rt, step0, start + times, cols) for name, cols in spike_cols.items()}
+        return RunResult(time=start + times, traces=traces, spikes=st)
+
+    @property
+    def spike(self):
+        """Per voltage-trace column, whether the last step crossed V_th
+        upwards (the reference's ``Cell.spike`` after ``update``)."""
+        rt = self._runtime
+        if rt is None or rt._spk is None:
+            raise RuntimeError("no spike record yet: run the cell first")
+        last = rt.step_index - 1
+        return np.asarray([s.size > 0 and int(s[-1]) == last for s in rt.spike_steps()])
 
     # ---------------------------------------------------------------- probes
     def _probe_points(self):
@@ -299,13 +333,15 @@ class Cell:
         m = self._model
         cc = L.cc
         out = {}
+        spike_cols = {}
         for p, cvs in points:
             cvs = np.asarray(cvs, dtype=np.int64)
             if isinstance(p, StateProbe):
                 if any(int(c) not in col for c in cvs):
                     raise ValueError(f"probe {p.name!r}: its CVs are not in the layout's probe set; "
                                      "call init_state() after placing probes")
-                out[p.name] = trace[:, [col[int(c)] for c in cvs]]
+                spike_cols[p.name] = [col[int(c)] for c in cvs]
+                out[p.name] = trace[:, spike_cols[p.name]]
             elif isinstance(p, MechanismProbe):
                 out[p.name] = snaps[:, 1 + gate_slot(L.slots, p.mechanism, p.field)][:, cvs]
             else:
@@ -322,16 +358,17 @@ class Cell:
                 for nm in names:
                     I = I + channel_current(m, nm, cvs, V, st, Ci, cc["E_Ca_fixed"], cc["ca_rtzf"], cc["ca_Co"])
                 out[p.name] = I
-        return {k: out[k] for k in sorted(out)}
+        return {k: out[k] for k in sorted(out)}, {k: spike_cols[k] for k in sorted(spike_cols)}
 
 
 class CellBatch:
     """Many cells (uneven morphologies) integrated in one packed launch."""
 
     def __init__(self, cells_or_models, *, precision: int = 32, threads: int = 256, rows_per_pack=None,
-                 probes="soma", device="cuda"):
+                 probes="soma", device="cuda", V_th: float = 0.0):
         from .runtime import StaggerLayout
 
+        self.V_th = float(V_th)
         models = [c.lower() if isinstance(c, Cell) else c for c in cells_or_models]
         self.layout = StaggerLayout(models, threads=threads, rows_per_pack=rows_per_pack, probes=probes)
         self.precision = precision
@@ -350,6 +387,12 @@ class CellBatch:
 
         rt = self.runtime(dt)
         n = int(np.arange(0.0, float(duration), float(dt)).shape[0])
+        step0 = rt.step_index
+        times = (step0 + np.arange(n)) * float(dt)
+        if rt._spk is None or rt._spk.cap < spike_cap(n, dt) or rt._spk.v_th != self.V_th:
+            rt.enable_spikes(self.V_th, spike_cap(n, dt))
+        else:
+            rt.clear_spikes()
         trace = rt.new_trace(n)
         if use_graph and n % chunk == 0:
             g = rt.make_graph(n // chunk, chunk, trace)
@@ -359,5 +402,7 @@ class CellBatch:
             rt.run(n, trace=trace, chunk=chunk)
         torch.cuda.synchronize()
         rt.check_status()
-        return RunResult(time=np.arange(n) * float(dt), traces={"V_soma": trace.double().cpu().numpy()})
+        V = trace.double().cpu().numpy()
+        return RunResult(time=times, traces={"V_soma": V},
+                         spikes={"V_soma": _spikes_by_column(rt, step0, times, range(V.shape[1]))})
 

@@ -918,6 +918,30 @@ __global__ void spike_clear(uint8_t* spk, int spk_len, int n_cells, int step0, i
   spk[(int64_t)slot * n_cells + i % n_cells] = 0;
 }
 
+// Threshold crossings on the launch's trace rows: one thread per trace
+// column, rows read in order (coalesced across the columns of a row).
+template <typename real>
+__global__ void __launch_bounds__(256) spike_scan(const real* __restrict__ trace, int64_t ld, int32_t n_steps,
+                                                  int32_t step0, real v_th, real* __restrict__ v_last,
+                                                  int32_t* __restrict__ count, int32_t* __restrict__ steps,
+                                                  int32_t cap) {
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ld) return;
+  real v = v_last[col];
+  int32_t c = count[col];
+  int32_t* out = steps + col * cap;
+  for (int32_t n = 0; n < n_steps; ++n) {
+    const real x = trace[(int64_t)n * ld + col];
+    if (v < v_th && x >= v_th) {
+      if (c < cap) out[c] = step0 + n;
+      ++c;
+    }
+    v = x;
+  }
+  v_last[col] = v;
+  count[col] = c;
+}
+
 template <typename real>
 int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->packs = d->packs;
@@ -999,6 +1023,12 @@ int validate(const bc_stagger_desc* d) {
                          d->status == nullptr || d->chain_rec == nullptr || d->chain_par == nullptr ||
                          d->kid_pad == nullptr || d->slot_link == nullptr || d->slot_chain == nullptr))
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
+  if (const bc_spike_desc* k = d->spk) {
+    if (k->cap < 0) return bc::fail(BC_INVALID_ARG, "spike desc: cap < 0");
+    if (d->trace_ld <= 0) return bc::fail(BC_INVALID_ARG, "spike desc: detectors are trace columns (trace_ld > 0)");
+    if (k->v_last == nullptr || k->count == nullptr || (k->cap > 0 && k->steps == nullptr))
+      return bc::fail(BC_INVALID_ARG, "spike desc: required pointer is NULL");
+  }
   if (const bc_synapse_desc* y = d->syn) {
     if (y->n_cells <= 0 || y->min_delay < 1 || y->max_delay < y->min_delay)
       return bc::fail(BC_INVALID_ARG, "synapse desc: n_cells > 0 and 1 <= min_delay <= max_delay required");
@@ -1026,6 +1056,7 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "pack needs %zu bytes of LDS (> 160 KiB)", lds);
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (d->n_packs == 0 || n_steps == 0) return BC_OK;
+  if (d->spk != nullptr && trace == nullptr) return bc::fail(BC_INVALID_ARG, "spike output needs a trace buffer");
   if (const bc_synapse_desc* y = d->syn) {
     if (n_steps > y->min_delay || y->spk_len < y->max_delay + n_steps || step0 < 0)
       return bc::fail(BC_INVALID_ARG,
@@ -1038,6 +1069,11 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
   }
   void* args[] = {&a};
   BC_HIP(hipLaunchKernel(fn, dim3(d->n_packs), dim3(d->threads), args, lds, s));
+  if (const bc_spike_desc* k = d->spk) {
+    hipLaunchKernelGGL(spike_scan<real>, dim3((unsigned)((d->trace_ld + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const real*>(trace), d->trace_ld, n_steps, step0, static_cast<real>(k->v_th),
+                       static_cast<real*>(k->v_last), k->count, k->steps, k->cap);
+  }
   return BC_OK;
 }
 

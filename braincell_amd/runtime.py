@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import rates
-from ._native import BcPack, StaggerDesc, chan_const_type, check, lib
+from ._native import BcPack, SpikeDesc, StaggerDesc, chan_const_type, check, lib
 from .channels import CA_POOL_BIT, CHANNELS, FARADAY, GAS_CONSTANT
 from .layout import build_chains, discretize_cell
 
@@ -536,6 +536,14 @@ class StaggerLayout:
     def cell_slice(self, i: int) -> slice:
         return slice(int(self.cell_cv0[i]), int(self.cell_cv0[i] + self.cell_ncv[i]))
 
+    def probe_cvs_global(self) -> np.ndarray:
+        """Global CV index of every trace column."""
+        out = np.empty(self.n_probes, dtype=np.int64)
+        for i, c in enumerate(self.cells):
+            for k, cv in enumerate(c.probes):
+                out[int(self.cell_probe0[i]) + k] = int(self.cell_cv0[i]) + int(cv)
+        return out
+
     def summary(self) -> dict:
         p = self.pack_array
         return {"cells": len(self.cells), "packs": self.n_packs, "cv_total": int(self.n_cv_total),
@@ -589,6 +597,7 @@ class StaggerRuntime:
         self.flags = 0  # BC_STEP_* bits for every launch
         self.step_index = 0
         self._graphs = {}
+        self._spk = None
 
     def _t(self, arr, dtype):
         return self.torch.as_tensor(np.ascontiguousarray(arr)).to(device=self.device, dtype=dtype)
@@ -613,6 +622,50 @@ class StaggerRuntime:
             self.Vbnd.copy_(self._t(Vb, self.dtype))
         self.step_index = 0
         self.status.zero_()
+        if self._spk is not None:
+            self.clear_spikes()
+
+    # ---------------------------------------------------------------- spikes
+    def enable_spikes(self, v_th: float = 0.0, cap: int = 256):
+        """Record threshold crossings (V_n < v_th <= V_{n+1}) of every trace
+        column in every later launch: up to ``cap`` step indices per column
+        and the count of all of them (``get_spike``, ``braincell/_base.py:
+        421-431``).  Counting starts now, from the current voltages."""
+        if self.L.n_probes == 0:
+            raise ValueError("spike detection reads the probe trace: the layout has no probes")
+        if cap < 0:
+            raise ValueError("cap must be >= 0")
+        torch = self.torch
+        n = self.L.n_probes
+        self._spk_cv = torch.as_tensor(self.L.probe_cvs_global(), device=self.device)
+        self._spk_buf = {"v_last": torch.empty(n, dtype=self.dtype, device=self.device),
+                         "count": torch.zeros(n, dtype=torch.int32, device=self.device),
+                         "steps": torch.full((n, max(cap, 1)), -1, dtype=torch.int32, device=self.device)}
+        b = self._spk_buf
+        self._spk = SpikeDesc(int(cap), float(v_th), b["v_last"].data_ptr(), b["count"].data_ptr(),
+                              b["steps"].data_ptr())
+        self.clear_spikes()
+        return self
+
+    def clear_spikes(self):
+        """Restart the spike record at the current voltages."""
+        b = self._spk_buf
+        b["v_last"].copy_(self.V[self._spk_cv])
+        b["count"].zero_()
+        b["steps"].fill_(-1)
+
+    def spike_steps(self):
+        """Per trace column, the recorded spike step indices (global step n
+        of a crossing V_n < v_th <= V_{n+1}); raises if a column spiked more
+        often than the record holds."""
+        if self._spk is None:
+            raise RuntimeError("spike output is off: call enable_spikes() first")
+        cnt = self._spk_buf["count"].cpu().numpy()
+        if (cnt > self._spk.cap).any():
+            raise RuntimeError(f"{int(cnt.max())} spikes on one column exceed the record of {self._spk.cap} "
+                               "per column: enable_spikes(cap=...) with a larger cap")
+        st = self._spk_buf["steps"].cpu().numpy()
+        return [st[c, :cnt[c]].astype(np.int64) for c in range(cnt.size)]
 
     def save_state(self):
         """Device-side snapshot of V, the states and the boundary voltages."""
@@ -626,6 +679,8 @@ class StaggerRuntime:
         self.Vbnd.copy_(Vb)
         self.step_index = n
         self.status.zero_()
+        if self._spk is not None:
+            self.clear_spikes()
 
     def desc(self, n_steps: int, trace=None, step0: int = None, rec=None) -> StaggerDesc:
         L = self.L
@@ -668,6 +723,7 @@ class StaggerRuntime:
         d.flags = self.flags
         syn = getattr(self, "synapse_desc", None)
         d.syn = ctypes.cast(ctypes.pointer(syn), ctypes.c_void_p) if syn is not None else None
+        d.spk = ctypes.cast(ctypes.pointer(self._spk), ctypes.c_void_p) if self._spk is not None else None
         return d
 
     def stream_handle(self):

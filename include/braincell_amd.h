@@ -124,6 +124,22 @@ typedef struct bc_synapse_desc {
   uint8_t* spk;             /* [spk_len * n_cells] spike history ring */
 } bc_synapse_desc;
 
+/* Spike output of a plain run (get_spike, braincell/_base.py:421-431, set
+ * every step by Cell._update_dynamics, _multi_compartment/cell.py:2384-2386):
+ * every trace column is a detector; step n spikes at a column when its
+ * voltage crosses v_th upwards, V_n < v_th <= V_{n+1}.  After each launch a
+ * scan over the launch's trace rows appends the global step index n of every
+ * crossing: steps[col * cap + count[col]] while count[col] < cap; count[col]
+ * always counts.  v_last[col] holds the column's voltage before the launch's
+ * first step and is updated to its last row.  Needs a trace (trace_ld > 0). */
+typedef struct bc_spike_desc {
+  int32_t cap;     /* step slots per column in steps (>= 0) */
+  double v_th;     /* threshold (mV) */
+  void* v_last;    /* [trace_ld] real, in/out */
+  int32_t* count;  /* [trace_ld] in/out */
+  int32_t* steps;  /* [trace_ld * cap] out (NULL when cap = 0) */
+} bc_spike_desc;
+
 typedef struct bc_stagger_desc {
   int32_t precision;    /* 32 or 64 */
   uint32_t channels;    /* BC_CH_* set compiled into the kernel */
@@ -179,6 +195,7 @@ typedef struct bc_stagger_desc {
   const int16_t* kid_pad;
   const int16_t* slot_link;
   const int16_t* slot_chain;
+  const bc_spike_desc* spk; /* NULL: no spike output */
 } bc_stagger_desc;
 
 /* bc_stagger_desc.flags */

@@ -69,3 +69,38 @@ def test_point_descriptor_method_is_checked_without_gpu():
     assert b"method" in L.bc_last_error()
     d.method = 2
     assert L.bc_point_run(ctypes.byref(d), None) == 0  # n = 0: nothing to launch
+
+
+def test_spike_descriptor_is_checked_without_gpu():
+    L = _native.lib()
+    d = _native.StaggerDesc()
+    d.precision, d.threads, d.max_rows, d.max_chains, d.max_cls, d.max_levels, d.dt = 32, 256, 64, 8, 1, 1, 0.025
+    k = _native.SpikeDesc()
+    k.cap = -1
+    d.spk = ctypes.cast(ctypes.pointer(k), ctypes.c_void_p)
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    assert b"cap" in L.bc_last_error()
+    k.cap = 4
+    d.trace_ld = 0
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    assert b"trace" in L.bc_last_error()
+    d.trace_ld = 3
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    assert b"NULL" in L.bc_last_error()
+    buf = (ctypes.c_int32 * 64)()
+    k.v_last = k.count = k.steps = ctypes.cast(buf, ctypes.c_void_p)
+    d.channels, d.cv_per_thread = 0x117, 8
+    cc = _native.chan_const_type(32)()
+    d.chan_const = ctypes.cast(ctypes.pointer(cc), ctypes.c_void_p)
+    # a complete descriptor passes validation (without a GPU the call then
+    # fails at the runtime's error check, not with INVALID_ARG)
+    assert L.bc_stagger_run(ctypes.byref(d), None) in (0, 3)
+
+
+def test_spike_cap_bounds():
+    from braincell_amd.cell import RunResult, spike_cap
+
+    assert spike_cap(1, 0.025) == 1
+    assert spike_cap(40000, 0.025) == 508
+    assert spike_cap(10, 1.0) == 5
+    assert RunResult(time=None, traces={}).spikes == {}
