@@ -30,6 +30,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "bc_channels.h"
 #include "bc_common.h"
 #include "braincell_amd.h"
@@ -483,6 +485,48 @@ int grid_for(int64_t n, int per) {
   return (int)(b < 1 ? 1 : (b > 65535 ? 65535 : b));
 }
 
+// Scratch of bc_closure_grad_noalloc: the per-block partial gradients and
+// losses of closure_grad in a file-scope device array, so that the entry
+// point needs no workspace from its caller.  Each GPU holds its own copy for
+// the life of the process; every call on a GPU reuses it.
+constexpr int kScratchBlocks = BC_CLOSURE_NOALLOC_MAX_BATCH / TB;
+static_assert(kScratchBlocks * TB == BC_CLOSURE_NOALLOC_MAX_BATCH, "batch bound is a whole number of blocks");
+__device__ __align__(256) float scratch[(size_t)kScratchBlocks * NPARAM + kScratchBlocks];
+
+// Device address of `scratch` on the current device, resolved once per device
+// with hipGetSymbolAddress (the array's name is a host-side stand-in in host
+// code and must never be passed to a kernel or a copy).
+constexpr int kMaxDevices = 64;
+float* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_address(float** out) {
+  int dev = -1;
+  BC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return bc::fail(BC_UNSUPPORTED, "device ordinal %d >= %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    BC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) return bc::fail(BC_HIP_ERROR, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_dev[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return BC_OK;
+}
+
+int closure_grad_launch(const bc_closure_train_desc* d, int nblk, float* partial, float* loss_part,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(closure_grad, dim3(nblk), dim3(TB), 0, st, d->batch, static_cast<const float*>(d->X),
+                     static_cast<const float*>(d->Dn), static_cast<const float*>(d->Yn), d->idx,
+                     static_cast<const float*>(d->P), partial, loss_part);
+  BC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(closure_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, st, nblk, partial, loss_part,
+                     static_cast<float*>(d->grad), static_cast<float*>(d->loss));
+  BC_HIP(hipGetLastError());
+  return bc::ok();
+}
+
 }  // namespace
 
 extern "C" {
@@ -511,16 +555,25 @@ int bc_closure_grad(const bc_closure_train_desc* d, void* stream) {
   const int nblk = (int)((d->batch + TB - 1) / TB);
   if (nblk > d->max_blocks) return bc::fail(BC_INVALID_ARG, "batch needs %d partial blocks > max_blocks %d", nblk,
                                             d->max_blocks);
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(closure_grad, dim3(nblk), dim3(TB), 0, st, d->batch, static_cast<const float*>(d->X),
-                     static_cast<const float*>(d->Dn), static_cast<const float*>(d->Yn), d->idx, static_cast<const float*>(d->P),
-                     static_cast<float*>(d->partial), static_cast<float*>(d->loss_part));
-  BC_HIP(hipGetLastError());
-  hipLaunchKernelGGL(closure_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, st, nblk,
-                     static_cast<const float*>(d->partial), static_cast<const float*>(d->loss_part),
-                     static_cast<float*>(d->grad), static_cast<float*>(d->loss));
-  BC_HIP(hipGetLastError());
-  return bc::ok();
+  return closure_grad_launch(d, nblk, static_cast<float*>(d->partial), static_cast<float*>(d->loss_part),
+                             static_cast<hipStream_t>(stream));
+}
+
+int bc_closure_grad_noalloc(const bc_closure_train_desc* d, void* stream) {
+  if (d == nullptr) return bc::fail(BC_INVALID_ARG, "desc is NULL");
+  if (d->batch <= 0) return bc::fail(BC_INVALID_ARG, "batch must be > 0");
+  if (d->batch > BC_CLOSURE_NOALLOC_MAX_BATCH)
+    return bc::fail(BC_INVALID_ARG, "batch %lld exceeds the %d samples the scratch array holds",
+                    (long long)d->batch, BC_CLOSURE_NOALLOC_MAX_BATCH);
+  if (!d->X || !d->Dn || !d->Yn || !d->idx || !d->P || !d->grad || !d->loss)
+    return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
+  float* base = nullptr;
+  int rc = scratch_address(&base);
+  if (rc != BC_OK) return rc;
+  const int nblk = (int)((d->batch + TB - 1) / TB);
+  // closure_grad writes every partial of its nblk blocks before closure_reduce
+  // reads them, so nothing in the scratch needs clearing between calls
+  return closure_grad_launch(d, nblk, base, base + (size_t)kScratchBlocks * NPARAM, static_cast<hipStream_t>(stream));
 }
 
 int bc_closure_adam(const bc_closure_adam_desc* d, void* stream) {

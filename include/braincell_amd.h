@@ -358,6 +358,20 @@ typedef struct bc_closure_train_desc {
 } bc_closure_train_desc;
 int bc_closure_grad(const bc_closure_train_desc* desc, void* stream);
 
+/* bc_closure_grad without caller workspace: the same kernels and result
+ * (bitwise), with the per-block partial sums in a fixed device array of the
+ * library instead of desc->partial / desc->loss_part (ignored, like
+ * max_blocks).  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under hipGraph capture, serving loops,
+ * harnesses that forbid hipMalloc); it allocates nothing.  Batches are
+ * limited to BC_CLOSURE_NOALLOC_MAX_BATCH samples (the array holds
+ * BC_CLOSURE_NOALLOC_MAX_BATCH / 128 blocks x (BC_CLOSURE_NPARAM + 1) floats,
+ * 5.2 MB, on every GPU the library runs on, until the process ends); larger
+ * batches return BC_INVALID_ARG.  All calls on one GPU share the array:
+ * calls that overlap on one GPU (different streams or threads) race. */
+#define BC_CLOSURE_NOALLOC_MAX_BATCH 32768
+int bc_closure_grad_noalloc(const bc_closure_train_desc* desc, void* stream);
+
 /* One Adam step (bias-corrected, step counts from 1). */
 typedef struct bc_closure_adam_desc {
   int64_t step;
