@@ -93,6 +93,7 @@ struct StaggerArgs {
   const int16_t* kid_pad;
   const int16_t* slot_link;
   const int16_t* slot_chain;
+  const int16_t* chain_work;  // optional work order of the per-chain passes
 };
 
 constexpr uint32_t kRowMask = 0xFFFFu;
@@ -324,7 +325,7 @@ constexpr int kGroup = 4;
 
 // Boundary rows per thread whose metadata is kept in registers (more are
 // handled by a loop that reads it from global memory).
-constexpr int kBndRegs = 1;
+constexpr int kBndRegs = 2;
 
 // Forward elimination in the delta-V form.  (sd, sr) is what the rows below
 // subtract from the current row: the kids' contributions at the bottom,
@@ -364,38 +365,138 @@ __device__ __forceinline__ void eliminate_head(const HRow<real>* g, int rem, HRo
   if (any3) eliminate_row<real>(g[2], (rem > 2 ? Rj : Rd) + 2, rem > 2, sd, sr);
 }
 
+// Elimination through chain composites.  Eliminating row j maps what the
+// rows below subtract, (sd, sr), to what row j passes up:
+//   sd' = up (d - sd) / (d - lo - sd),   sr' = up (r - sr) / (d - lo - sd),
+// a linear-fractional map.  In homogeneous coordinates (x, y, z), sd = x/z,
+// sr = y/z, it is linear:
+//   x' = -up x + up d z,   y' = -up y + up r z,   z' = -x + (d - lo) z,
+// so a chain's rows compose into one map (x, y, z) = M (sd, sr, 1) with
+//   x = a sd + b,   y = c sd + f sr + g,   z = h sd + k.
+// With up, lo < 0 and d >= 0 the entries keep the signs a, k > 0, b, h <= 0
+// row after row, so every product term of a, b, h, k and of x, z (sd <= 0)
+// adds values of one sign: no cancellation.  M is rescaled to k = 1 after
+// every group of rows (homogeneous coordinates).  The top edge's axial
+// term, added to what the chain passes up, is folded into c and g.
+//   pass 1 (every chain at once): the chain's composite;
+//   pass 2 (dataflow over the chain tree, last-arriving kid climbs): one
+//          composite application per chain, so the critical path costs
+//          O(1) per chain instead of O(rows);
+//   pass 3 (every chain at once): the rows' pivots and reduced right-hand
+//          sides from the chain's input, then the chain's affine back
+//          substitution.
 template <typename real>
-__device__ __forceinline__ void eliminate_chain(int k, int4 rec, int dummy, const int16_t* m_kid, HRow<real>* R,
-                                                const real* xv, KPair<real>* KK) {
-  const ChainRec c = unpack_chain(rec);
+struct alignas(4 * sizeof(real)) Comp {
+  real a, b, h, c, f, g, pad0, pad1;  // 32 bytes (FP32): two 16-byte loads
+};
+
+template <typename real>
+__device__ __forceinline__ void comp_row(const HRow<real>& w, bool valid, real& a, real& b, real& h, real& k, real& c,
+                                         real& f, real& g) {
+  const real dl = w.d - w.lo;
+  const real a2 = w.up * (w.d * h - a);
+  const real b2 = w.up * (w.d * k - b);
+  const real c2 = w.up * (w.r * h - c);
+  const real g2 = w.up * (w.r * k - g);
+  const real h2 = dl * h - a;
+  const real k2 = dl * k - b;
+  const real f2 = -w.up * f;
+  a = valid ? a2 : a;
+  b = valid ? b2 : b;
+  c = valid ? c2 : c;
+  g = valid ? g2 : g;
+  h = valid ? h2 : h;
+  k = valid ? k2 : k;
+  f = valid ? f2 : f;
+}
+
+template <typename real>
+__device__ __forceinline__ void chain_composite(int k, const int4* CR, const HRow<real>* R, const real* xv,
+                                                Comp<real>* CM, KPair<real>* KK) {
+  const ChainRec c = unpack_chain(CR[k]);
+  real a = real(1), b = real(0), h = real(0), kk = real(1), cc = real(0), f = real(1), g = real(0);
+  HRow<real> w[kGroup], nx[kGroup];
+#pragma unroll
+  for (int i = 0; i < kGroup; ++i) w[i] = R[c.j0 + i];
+  const int tq = c.tp >= 0 ? c.tp : c.j1 - 1;
+  const real td = R[c.j1 - 1].up * (xv[c.j1 - 1] - xv[tq]);
+  for (int j = c.j0; j < c.j1; j += kGroup) {
+    // the next group is loaded before this one is used (rows past the chain
+    // land in other chains or padding and are discarded)
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) nx[i] = R[j + kGroup + i];
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) comp_row<real>(w[i], j + i < c.j1, a, b, h, kk, cc, f, g);
+    const real ik = bc_rcp<real>(kk);
+    a *= ik;
+    b *= ik;
+    h *= ik;
+    cc *= ik;
+    f *= ik;
+    g *= ik;
+    kk = real(1);
+#pragma unroll
+    for (int i = 0; i < kGroup; ++i) w[i] = nx[i];
+  }
+  Comp<real> m;
+  m.a = a;
+  m.b = b;
+  m.h = h;
+  m.c = cc + td * h;
+  m.f = f;
+  m.g = g + td;
+  CM[k] = m;
+  if (c.nk == 0) {  // a leaf chain's input is zero: its contribution is (b, g)
+    KPair<real> o;
+    o.d = m.b;
+    o.r = m.g;
+    KK[k] = o;
+  }
+}
+
+// Sum of the kid chains' contributions (fixed order: first kid, second kid,
+// then the rest of the kid list), the input of chain c's bottom row.
+template <typename real>
+__device__ __forceinline__ KPair<real> kid_sum(const ChainRec& c, const int16_t* m_kid, const KPair<real>* KK) {
+  // both first kids are loaded unconditionally (an absent kid's index is 0
+  // and its value is dropped), so the two loads share one round trip
+  const KPair<real> t0 = KK[c.k0];
+  const KPair<real> t1 = KK[c.k1];
+  KPair<real> s;
+  s.d = c.nk > 1 ? t0.d + t1.d : (c.nk > 0 ? t0.d : real(0));
+  s.r = c.nk > 1 ? t0.r + t1.r : (c.nk > 0 ? t0.r : real(0));
+  for (int q = 2; q < c.nk; ++q) {
+    const KPair<real> t = KK[m_kid[c.q0 + q]];
+    s.d += t.d;
+    s.r += t.r;
+  }
+  return s;
+}
+
+template <typename real>
+__device__ __forceinline__ KPair<real> apply_composite(const Comp<real>& m, const KPair<real>& s) {
+  const real iz = bc_rcp<real>(m.h * s.d + real(1));
+  KPair<real> o;
+  o.d = (m.a * s.d + m.b) * iz;
+  o.r = (m.c * s.d + m.f * s.r + m.g) * iz;
+  return o;
+}
+
+// Pass 3 of the elimination: chain k's rows from its kids' contributions
+// (R[j].d becomes 1 / pivot, R[j].r the reduced right-hand side).
+template <typename real>
+__device__ __forceinline__ void eliminate_rows(int k, const int4* CR, int dummy, const int16_t* m_kid, HRow<real>* R,
+                                               const KPair<real>* KK) {
+  const ChainRec c = unpack_chain(CR[k]);
   const int rem = (c.j1 - c.j0) & (kGroup - 1);
   int j = c.j0 + rem;
-  // the head's rows and the first full group are loaded together; two
-  // buffers for the full groups are used in turn
   HRow<real> gh[kGroup - 1], ga[kGroup], gb[kGroup];
 #pragma unroll
   for (int i = 0; i < kGroup - 1; ++i) gh[i] = R[c.j0 + i];
 #pragma unroll
   for (int i = 0; i < kGroup; ++i) ga[i] = R[j + i];
-  // the top edge's axial term for the row the chain hangs from (zero for a root)
-  const int tq = c.tp >= 0 ? c.tp : c.j1 - 1;
-  const real top_defect = R[c.j1 - 1].up * (xv[c.j1 - 1] - xv[tq]);
-  real sd = real(0), sr = real(0);
-  if (c.nk > 0) {
-    const KPair<real> t = KK[c.k0];
-    sd = t.d;
-    sr = t.r;
-  }
-  if (c.nk > 1) {
-    const KPair<real> t = KK[c.k1];
-    sd += t.d;
-    sr += t.r;
-  }
-  for (int q = 2; q < c.nk; ++q) {
-    const KPair<real> t = KK[m_kid[c.q0 + q]];
-    sd += t.d;
-    sr += t.r;
-  }
+  const KPair<real> s = kid_sum<real>(c, m_kid, KK);
+  real sd = s.d, sr = s.r;
   eliminate_head<real>(gh, rem, R + c.j0, R + dummy, sd, sr);
   while (j < c.j1) {
 #pragma unroll
@@ -408,8 +509,6 @@ __device__ __forceinline__ void eliminate_chain(int k, int4 rec, int dummy, cons
     eliminate_full<real>(gb, R + j, sd, sr);
     j += kGroup;
   }
-  KK[k].d = sd;
-  KK[k].r = sr + top_defect;
 }
 
 // Back substitution in affine form.  Inside chain k the solution of row j
@@ -434,7 +533,7 @@ __device__ __forceinline__ void affine_row(const HRow<real>& w, HRow<real>* dst,
 }
 
 template <typename real>
-__device__ __forceinline__ void chain_affine(int k, int dummy, const int4* CR, HRow<real>* R, KPair<real>* KK) {
+__device__ __forceinline__ void chain_affine(int k, int dummy, const int4* CR, HRow<real>* R) {
   const ChainRec c = unpack_chain(CR[k]);
   real al = real(0), be = c.tp >= 0 ? real(1) : real(0);
   const int rem = (c.j1 - c.j0) & (kGroup - 1);
@@ -461,8 +560,6 @@ __device__ __forceinline__ void chain_affine(int k, int dummy, const int4* CR, H
     for (int i = 0; i < kGroup; ++i) affine_row<real>(gb[i], R + (j - i), true, al, be);
     j -= kGroup;
   }
-  KK[k].d = al;
-  KK[k].r = be;
 }
 
 // Solution of one row after the affine pass and the pointer jumping.
@@ -486,9 +583,12 @@ constexpr int kDummySlots = 8;
 constexpr int kPadBefore = 8;
 
 // LDS carve-up (bytes), shared with the launch code: rows and chains are
-// rounded up to multiples of 4 so every array stays naturally aligned.
+// rounded up to multiples of 4 so every array stays naturally aligned.  The
+// chain composites of the elimination and the pointer-jumping buffers of
+// the back substitution (kk2, ptr, ptr2) are live in different passes and
+// share one region.
 struct LdsPlan {
-  size_t rows, recs, kk, kk2, xv, cls, cnt, kid, lv, link, rowch, par, ptr, ptr2, total;
+  size_t rows, recs, kk, cmp, kk2, ptr, ptr2, xv, cls, kid, work, lv, dfw, total;
 };
 template <typename real>
 __host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int max_cls, int max_levels) {
@@ -497,18 +597,17 @@ __host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int ma
   p.rows = kPadBefore * sizeof(HRow<real>);  // row 0; kPadBefore padding rows precede it
   p.recs = p.rows + (mr + kDummySlots) * sizeof(HRow<real>);
   p.kk = p.recs + mc * sizeof(int4);
-  p.kk2 = p.kk + mc * sizeof(KPair<real>);
-  p.xv = p.kk2 + mc * sizeof(KPair<real>);
-  p.cls = p.xv + mr * sizeof(real);
-  p.cnt = p.cls + (size_t)max_cls * BC_NCLS * sizeof(real);
-  p.kid = p.cnt + mc * sizeof(int);
-  p.lv = p.kid + mc * sizeof(int16_t);
-  p.link = p.lv + (size_t)((max_levels + 4) & ~3) * sizeof(int16_t);
-  p.rowch = p.link + mr * sizeof(int16_t);
-  p.par = p.rowch + mr * sizeof(int16_t);
-  p.ptr = p.par + mc * sizeof(int16_t);
+  p.cmp = p.kk + mc * sizeof(KPair<real>);
+  p.kk2 = p.cmp;
+  p.ptr = p.kk2 + mc * sizeof(KPair<real>);
   p.ptr2 = p.ptr + mc * sizeof(int16_t);
-  p.total = p.ptr2 + mc * sizeof(int16_t);
+  p.xv = p.cmp + mc * sizeof(Comp<real>);
+  p.cls = p.xv + mr * sizeof(real);
+  p.kid = p.cls + (size_t)max_cls * BC_NCLS * sizeof(real);
+  p.work = p.kid + mc * sizeof(int16_t);
+  p.lv = p.work + mc * sizeof(int16_t);
+  p.dfw = p.lv + (size_t)((max_levels + 4) & ~3) * sizeof(int16_t);
+  p.total = p.dfw + mc * sizeof(uint32_t);
   return p;
 }
 
@@ -564,14 +663,12 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   real* xv = reinterpret_cast<real*>(smem_raw + lp.xv);  // V_n at every slot
   real* cls = reinterpret_cast<real*>(smem_raw + lp.cls);
   int16_t* m_kid = reinterpret_cast<int16_t*>(smem_raw + lp.kid);
+  int16_t* m_work = reinterpret_cast<int16_t*>(smem_raw + lp.work);  // chain order of the per-chain passes
   int16_t* m_lv = reinterpret_cast<int16_t*>(smem_raw + lp.lv);
   KPair<real>* KK2 = reinterpret_cast<KPair<real>*>(smem_raw + lp.kk2);
-  int* m_cnt = reinterpret_cast<int*>(smem_raw + lp.cnt);  // kids eliminated so far
-  // parent slot of every slot (15 bits; a root's top points at itself) | 1 << 15
-  // for a chain's bottom slot (its children are the kid chains, not slot - 1)
-  int16_t* m_link = reinterpret_cast<int16_t*>(smem_raw + lp.link);
-  int16_t* m_rowch = reinterpret_cast<int16_t*>(smem_raw + lp.rowch);  // chain of every slot
-  int16_t* m_par = reinterpret_cast<int16_t*>(smem_raw + lp.par);    // parent chain, -1 for a root
+  Comp<real>* CM = reinterpret_cast<Comp<real>*>(smem_raw + lp.cmp);  // chain composites (aliases KK2, ptr)
+  // per chain: parent chain (low half, 0xFFFF for a root) | the parent's kid count << 16
+  uint32_t* m_dfw = reinterpret_cast<uint32_t*>(smem_raw + lp.dfw);
   int16_t* m_ptr = reinterpret_cast<int16_t*>(smem_raw + lp.ptr);  // pointer jumping (two buffers)
   int16_t* m_ptr2 = reinterpret_cast<int16_t*>(smem_raw + lp.ptr2);
 
@@ -585,14 +682,15 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     const int64_t g = (int64_t)P.row0 + i;
     R[i].lo = a.lo[g];
     R[i].up = a.up[g];
-    m_link[i] = a.slot_link[g];
-    m_rowch[i] = a.slot_chain[g];
   }
   for (int i = tid; i < P.n_ch; i += NT) {
     const int g = P.ch0 + i;
     CR[i] = a.chain_rec[g];
-    m_par[i] = a.chain_par[g];
+    const int par = a.chain_par[g];
+    const uint32_t nkp = par >= 0 ? (uint32_t)a.chain_rec[P.ch0 + par].y >> 16 : 0u;
+    m_dfw[i] = (uint32_t)(uint16_t)par | (nkp << 16);
     m_kid[i] = a.kid_pad[g];
+    m_work[i] = a.chain_work ? a.chain_work[g] : (int16_t)i;
   }
   for (int i = tid; i <= P.n_lv; i += NT) m_lv[i] = (int16_t)a.level_off[P.lv0 + i];
 
@@ -629,18 +727,22 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   __syncthreads();
 
   // Static per-row metadata the steps need, kept in registers: for every CV
-  // its slot link (low half) and the chain its row's solution hangs from
+  // its slot link (low half: the parent slot, a root's top pointing at
+  // itself, | 0x8000 at a chain's bottom slot, whose children are the kid
+  // chains rather than slot - 1) and the chain its row's solution hangs from
   // (high half, 0xFFFF for a root chain); for the first kBndRegs boundary
   // rows of the thread the link (bl) and the row (bits 0-14), the clamp bit
-  // (bit 15) and the chain its solution hangs from (high half) (bx).  The steps then read each operand in one LDS round trip for
-  // all of the thread's rows.
+  // (bit 15) and the chain its solution hangs from (high half) (bx).  The
+  // steps then read each operand in one LDS round trip for all of the
+  // thread's rows.  Links and slot chains come from global memory (further
+  // boundary rows read them there every step).
   uint32_t aux[MAXC];
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
     aux[i] = 0;
     if (info[i] != 0xFFFFFFFFu) {
-      const int row = (int)(info[i] & kRowMask);
-      aux[i] = (uint32_t)(uint16_t)m_link[row] | ((uint32_t)(uint16_t)m_par[m_rowch[row]] << 16);
+      const int64_t g = (int64_t)P.row0 + (int)(info[i] & kRowMask);
+      aux[i] = (uint32_t)(uint16_t)a.slot_link[g] | ((m_dfw[a.slot_chain[g]] & 0xFFFFu) << 16);
     }
   }
   uint32_t bx[kBndRegs], bl[kBndRegs];
@@ -652,8 +754,9 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     if (i < P.n_bnd) {
       const uint32_t u = a.bnd_info[P.bnd0 + i];
       const int row = (int)(u & 0x7FFFFFFFu);
-      bx[b] = (uint32_t)row | ((u & kClampBit) ? 0x8000u : 0u) | ((uint32_t)(uint16_t)m_par[m_rowch[row]] << 16);
-      bl[b] = (uint32_t)(uint16_t)m_link[row];
+      const int64_t g = (int64_t)P.row0 + row;
+      bx[b] = (uint32_t)row | ((u & kClampBit) ? 0x8000u : 0u) | ((m_dfw[a.slot_chain[g]] & 0xFFFFu) << 16);
+      bl[b] = (uint32_t)(uint16_t)a.slot_link[g];
     }
   }
 
@@ -700,71 +803,74 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     for (int i = tid + kBndRegs * NT; i < P.n_bnd; i += NT) {
       uint32_t b = a.bnd_info[P.bnd0 + i];
       int row = (int)(b & 0x7FFFFFFFu);
-      const real D = axial_defect<real>(row, xv[row], R, xv, (uint32_t)(uint16_t)m_link[row]);
+      const real D = axial_defect<real>(row, xv[row], R, xv, (uint32_t)(uint16_t)a.slot_link[(int64_t)P.row0 + row]);
       R[row].d = real(0);
       R[row].r = ((b & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0)) + D;
     }
-    for (int k = tid; k < P.n_ch; k += NT) m_cnt[k] = 0;
     __syncthreads();
     if constexpr (SYN) {
       synapse_pass<real>(a, R, xv, gstep, tid, NT);
       __syncthreads();
     }
 
-    // ---- phase B: Hines elimination as a dataflow over chains, no barrier.
-    // Diagonal rebuilt from the zero row sums of the axial operator:
-    // d_j = e_j - lo_j with e_j = diag_j - sum over children c of
-    // up_c e_c / d_c >= 0, so no step subtracts nearly equal numbers.
-    // Every thread starts at leaf chains; after eliminating a chain it bumps
-    // the parent's arrival counter and the last kid to arrive goes on with
-    // the parent, so a chain starts as soon as its kids are done and the
-    // critical path is the longest leaf-to-root path.  Chain values do not
-    // depend on which thread takes a chain: results are deterministic.
-    {
-      const int lv0 = m_lv[0];
-      const int n_leaf = m_lv[1] - lv0;
-      int leaf = tid;
-      int k = leaf < n_leaf ? lv0 + leaf : -1;
-      int4 rec = CR[k >= 0 ? k : 0];
-      int p = m_par[k >= 0 ? k : 0];
-      while (k >= 0) {
-        // the record and parent of both possible next chains (the parent,
-        // or this thread's next leaf) are read before the chain is
-        // eliminated, so only the arrival counter's round trip separates two
-        // chains of a climb; chain records and parents are not written in
-        // this phase
-        const int nl = leaf + NT;
-        const int kl = nl < n_leaf ? lv0 + nl : -1;
-        const int pq = p >= 0 ? p : k;
-        const int lq = kl >= 0 ? kl : k;
-        const int4 rec_p = CR[pq];
-        const int pp = m_par[pq];
-        const int4 rec_l = CR[lq];
-        const int pl = m_par[lq];
-        eliminate_chain<real>(k, rec, dummy, m_kid, R, xv, KK);
-        bool climb = false;
-        if (p >= 0) {
-          const int nk = (int)((unsigned)rec_p.y >> 16);
-          climb = __hip_atomic_fetch_add(&m_cnt[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == nk - 1;
+    // ---- phase B: Hines elimination through chain composites.  Diagonal
+    // rebuilt from the zero row sums of the axial operator: d_j = e_j - lo_j
+    // with e_j = diag_j - sum over children c of up_c e_c / d_c >= 0, so no
+    // step subtracts nearly equal numbers.
+    // Pass 1: every chain's composite map (and a leaf chain's contribution).
+    for (int i = tid; i < P.n_ch; i += NT) chain_composite<real>(m_work[i], CR, R, xv, CM, KK);
+    __syncthreads();
+    // Pass 2 in wavefront 0: the chain levels in order (chains are sorted by
+    // level, level 0 being the leaves), each chain applying its composite to
+    // the sum of its kids' contributions; a wavefront's LDS operations
+    // complete in order, so a level only needs the wavefront's own sync
+    // before the next one reads it.  Chain values do not depend on the
+    // schedule: results are deterministic.  Meanwhile the other wavefronts
+    // run pass 3 and the affine back substitution of the leaf chains (whose
+    // input is zero); they write rows only, never what pass 2 reads.
+    const int n_leaf = m_lv[1] - m_lv[0];
+    const int lw = NT > kWave ? kWave : 0;  // first thread of the leaf crew
+    if (tid < kWave) {
+      // the sweep is the pack's critical path: its wavefront gets issue
+      // priority over the other wavefronts of its SIMD while it runs
+      __builtin_amdgcn_s_setprio(2);
+      for (int lev = 1; lev < P.n_lv; ++lev) {
+        const int c1 = m_lv[lev + 1];
+        for (int k = m_lv[lev] + tid; k < c1; k += kWave) {
+          const int4 rec = CR[k];
+          const Comp<real> m = CM[k];
+          KK[k] = apply_composite<real>(m, kid_sum<real>(unpack_chain(rec), m_kid, KK));
         }
-        if (climb) {
-          k = p;
-          rec = rec_p;
-          p = pp;
-        } else {
-          leaf = nl;
-          k = kl;
-          rec = rec_l;
-          p = pl;
-        }
+        wave_sync();
+      }
+      __builtin_amdgcn_s_setprio(0);
+    }
+    if (lw > 0 && tid >= lw) {
+      for (int i = tid - lw; i < n_leaf; i += NT - lw) {
+        const int k = m_work[i];
+        eliminate_rows<real>(k, CR, dummy, m_kid, R, KK);
+        chain_affine<real>(k, dummy, CR, R);
       }
     }
     __syncthreads();
-    // ---- back substitution: per-chain affine maps, pointer jumping over the
-    // chain tree for the chains' bottom solutions, then every row at once
+    // Pass 3 and back substitution of the other chains: every chain's rows
+    // from its input, then its affine map x_j = alpha_j + beta_j X into the
+    // rows (X = the solution of the row the chain hangs from).
+    for (int i = (lw > 0 ? n_leaf : 0) + tid; i < P.n_ch; i += NT) {
+      const int k = m_work[i];
+      eliminate_rows<real>(k, CR, dummy, m_kid, R, KK);
+      chain_affine<real>(k, dummy, CR, R);
+    }
+    __syncthreads();
+    // pointer jumping over the chain tree for the chains' bottom solutions:
+    // start from each chain's bottom-row (alpha, beta) and its parent
     for (int k = tid; k < P.n_ch; k += NT) {
-      chain_affine<real>(k, dummy, CR, R, KK);
-      m_ptr[k] = m_par[k];
+      const HRow<real> w = R[CR[k].x & 0xFFFF];
+      KPair<real> v;
+      v.d = w.r;
+      v.r = w.d;
+      KK2[k] = v;
+      m_ptr[k] = (int16_t)(m_dfw[k] & 0xFFFFu);
     }
     __syncthreads();
     // solution of row j of chain k: x_j = alpha_j + beta_j X, X = the solution
@@ -772,13 +878,17 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     const KPair<real>* Xb;
     {
       // (A, B, ptr): x_bottom(k) = A + B x_bottom(ptr); ptr < 0 once resolved
-      KPair<real>* A0 = KK;
-      KPair<real>* A1 = KK2;
+      KPair<real>* A0 = KK2;
+      KPair<real>* A1 = KK;
       int16_t* p0 = m_ptr;
       int16_t* p1 = m_ptr2;
-      // a forest of at most 32767 chains resolves in 16 rounds; the bound
-      // only turns an inconsistent layout into an error instead of a hang
-      for (int round = 0;; ++round) {
+      // a chain's depth below its root chain is at most n_lv - 1 (levels are
+      // heights), so ceil(log2(n_lv)) rounds resolve every pointer; one still
+      // open afterwards means an inconsistent layout and raises the error
+      // flag instead of hanging
+      int rounds = 0;
+      while ((1 << rounds) < P.n_lv) ++rounds;
+      for (int round = 0; round < rounds; ++round) {
         int open = 0;
         for (int k = tid; k < P.n_ch; k += NT) {
           const int q = p0[k];
@@ -794,18 +904,14 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
           A1[k] = v;
           p1[k] = (int16_t)qn;
         }
-        const int more = __syncthreads_or(open);
+        __syncthreads();
         KPair<real>* ta = A0;
         A0 = A1;
         A1 = ta;
         int16_t* tq = p0;
         p0 = p1;
         p1 = tq;
-        if (!more) break;
-        if (round >= 16) {
-          bad |= 2u;
-          break;
-        }
+        if (round == rounds - 1 && open) bad |= 2u;
       }
       Xb = A0;
     }
@@ -823,7 +929,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
     for (int i = tid + kBndRegs * NT; i < P.n_bnd; i += NT) {
       const int row = (int)(a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu);
-      xv[row] += row_solution<real>(row, R, (uint32_t)(uint16_t)m_par[m_rowch[row]] << 16, Xb);
+      xv[row] += row_solution<real>(row, R, (m_dfw[a.slot_chain[(int64_t)P.row0 + row]] & 0xFFFFu) << 16, Xb);
     }
 #pragma unroll
     for (int b = 0; b < kBndRegs; ++b)
@@ -1001,6 +1107,7 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->kid_pad = d->kid_pad;
   a->slot_link = d->slot_link;
   a->slot_chain = d->slot_chain;
+  a->chain_work = d->chain_work;
   return BC_OK;
 }
 

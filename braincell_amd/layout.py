@@ -368,8 +368,11 @@ class Chains:
         return int(self.level_off.shape[0] - 1)
 
 
-def build_chains(parent_rows: np.ndarray) -> Chains:
-    """Split a row forest (parents given per row; -1 for roots) into chains."""
+def build_chains(parent_rows: np.ndarray, max_len: int = None) -> Chains:
+    """Split a row forest (parents given per row; -1 for roots) into chains:
+    maximal runs in which every row is the only child of the next, cut from
+    the bottom into pieces of at most ``max_len`` rows (a piece's top then
+    hangs from the bottom row of the piece above)."""
     parent_rows = np.asarray(parent_rows, dtype=np.int64)
     P = parent_rows.shape[0]
     has_p = parent_rows >= 0
@@ -378,19 +381,25 @@ def build_chains(parent_rows: np.ndarray) -> Chains:
     chain_of_row = np.full(P, -1, dtype=np.int64)
     rows_list = []
     tops_parent = []
-    for k, b in enumerate(bottoms.tolist()):
+    cap = int(max_len) if max_len else P + 1
+    if cap < 1:
+        raise ValueError("max_len must be >= 1")
+    for b in bottoms.tolist():
         seq = [b]
-        chain_of_row[b] = k
         r = b
         while True:
             p = parent_rows[r]
             if p < 0 or nchild[p] != 1:
                 break
             seq.append(int(p))
-            chain_of_row[p] = k
             r = p
-        rows_list.append(seq)
-        tops_parent.append(int(parent_rows[r]))
+        top = int(parent_rows[r])
+        for q in range(0, len(seq), cap):
+            piece = seq[q:q + cap]
+            chain_of_row[piece] = len(rows_list)
+            rows_list.append(piece)
+            tops_parent.append(seq[q + cap] if q + cap < len(seq) else top)
+    bottoms = np.asarray([rl[0] for rl in rows_list], dtype=np.int64)
     nch = len(rows_list)
     top_parent = np.asarray(tops_parent, dtype=np.int64)
     parent_chain = np.where(top_parent >= 0, chain_of_row[np.maximum(top_parent, 0)], -1)

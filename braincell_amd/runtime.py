@@ -70,6 +70,40 @@ def state_slots(chset: int) -> list:
     return out
 
 
+def balanced_work_order(lens: np.ndarray, threads: int, n_leaf: int = None) -> np.ndarray:
+    """Order in which the threads of a pack take chains in the per-chain
+    passes, balanced so that the threads walk similar numbers of rows.
+    With ``n_leaf`` (the pack's leaf chains come first, chains 0 ..
+    n_leaf - 1) and more than one wavefront, positions 0 .. n_leaf - 1 hold
+    the leaf chains for the threads after the first wavefront (which runs
+    the level sweep meanwhile) and the other chains follow for all threads;
+    otherwise every chain is ordered for all threads."""
+    lens = np.asarray(lens, dtype=np.int64)
+    if n_leaf is not None and threads > 64:
+        leaves = _snake_order(lens[:n_leaf], threads - 64)
+        rest = _snake_order(lens[n_leaf:], threads) + n_leaf
+        return np.concatenate([leaves, rest]).astype(np.int16)
+    return _snake_order(lens, threads).astype(np.int16)
+
+
+def _snake_order(lens: np.ndarray, threads: int) -> np.ndarray:
+    """Thread t takes positions t, t + threads, ...: the longest chains go
+    one per thread, and every further round runs in the opposite direction
+    (snake), so the threads with the longest first chains get the shortest
+    later ones."""
+    order = np.argsort(-lens, kind="stable")
+    out = np.empty(lens.size, dtype=np.int64)
+    n = lens.size
+    for r0 in range(0, n, threads):
+        m = min(threads, n - r0)
+        chunk = order[r0:r0 + m]
+        if (r0 // threads) % 2 == 1:
+            # odd rounds: the last (shortest) chain to thread 0
+            chunk = chunk[::-1]
+        out[r0:r0 + m] = chunk
+    return out
+
+
 def first_step_at(t: float, dt: float) -> int:
     """Smallest n >= 0 with n * dt >= t in float64 (t_n = n dt, ``run.py:64-131``)."""
     if t <= 0.0:
@@ -103,10 +137,13 @@ class StaggerLayout:
     """Host-side packed layout of a batch of cells."""
 
     def __init__(self, models, *, threads: int = 256, cv_per_thread: int = None, rows_per_pack: int = None,
-                 probes="soma", region_classes: bool = False, probes_per_model=None):
+                 probes="soma", region_classes: bool = False, probes_per_model=None, max_chain_rows: int = 16):
         if len(models) == 0:
             raise ValueError("need at least one cell")
         self.models = list(models)
+        # chains longer than this are cut into pieces (bounds the per-chain
+        # passes of the elimination and back substitution)
+        self.max_chain_rows = int(max_chain_rows) if max_chain_rows else None
         self.region_classes = bool(region_classes)
         mask = 0
         for m in self.models:
@@ -165,8 +202,12 @@ class StaggerLayout:
         mr = (self.max_rows + 3) & ~3
         mc = (self.max_chains + 3) & ~3
         rows = (8 + mr + 8) * 4 * rs
-        return int(rows + mc * 16 + 2 * mc * 2 * rs + mr * rs + self.max_cls * 12 * rs + mc * 4 + mc * 2
-                   + ((self.max_levels + 4) & ~3) * 2 + 2 * mr * 2 + 3 * mc * 2)
+        # rows, chain records, chain contributions, chain composites (shared
+        # with the pointer-jumping buffers), V per slot, class table, kid
+        # list, work order, chain levels, level offsets, dataflow words,
+        # queue words
+        return int(rows + mc * 16 + mc * 2 * rs + mc * 8 * rs + mr * rs + self.max_cls * 12 * rs
+                   + mc * 2 + mc * 2 + mc * 2 + ((self.max_levels + 4) & ~3) * 2 + mc * 4 + 16)
 
     # ------------------------------------------------------------ constants
     def _channel_constants(self):
@@ -235,7 +276,7 @@ class StaggerLayout:
             geom, chains = hit[1], hit[2]
         else:
             geom = discretize_cell(m.morph, ncvb, cmb, rab)
-            chains = build_chains(geom.parent_rows)
+            chains = build_chains(geom.parent_rows, self.max_chain_rows)
             self._geom_cache[key] = (m.morph, geom, chains)
         nb = m.n_branches
         table = np.zeros((nb, NCLS))
@@ -308,6 +349,7 @@ class StaggerLayout:
         chain_off, chain_top, kid_cnt, kids, level_off = [0], [], [], [], []
         slot_lo, slot_up, slot_rows_all = [], [], []
         chain_rec_all, chain_par_all, kid_pad_all, slot_link_all, slot_chain_all = [], [], [], [], []
+        chain_work_all = []
         cls_tab, clamp_rows, clamp_t, clamp_amp_raw, clamp_kind, clamp_area = [], [], [], [], [], []
         probe_row, probe_col = [], []
         self.cell_region_cls = np.full((n_cells, len(REGION_CODES)), -1, dtype=np.int64)
@@ -372,6 +414,7 @@ class StaggerLayout:
             kp = np.zeros(n_ch, dtype=np.int16)
             kp[: len(pk)] = pk
             kid_pad_all.append(kp)
+            chain_work_all.append(balanced_work_order(j1 - j0, self.threads, int(np.sum(lev[perm] == 0))))
             ch_of = np.repeat(np.arange(n_ch), lens)
             link = np.arange(n_rows, dtype=np.int64) + 1
             link[j1 - 1] = np.where(tp_slot >= 0, tp_slot, j1 - 1)
@@ -457,6 +500,7 @@ class StaggerLayout:
         self.chain_rec = np.concatenate(chain_rec_all, axis=0)
         self.chain_par = np.concatenate(chain_par_all)
         self.kid_pad = np.concatenate(kid_pad_all)
+        self.chain_work = np.concatenate(chain_work_all)
         self.slot_link = np.concatenate(slot_link_all)
         self.slot_chain = np.concatenate(slot_chain_all)
         self.slot_lo = np.concatenate(slot_lo)
@@ -583,6 +627,7 @@ class StaggerRuntime:
         self.kid_pad = t(layout.kid_pad, torch.int16)
         self.slot_link = t(layout.slot_link, torch.int16)
         self.slot_chain = t(layout.slot_chain, torch.int16)
+        self.chain_work = t(layout.chain_work, torch.int16)
         self.probe_row = t(layout.probe_row if layout.probe_row.size else np.zeros(1, np.int32), torch.int32)
         self.probe_col = t(layout.probe_col if layout.probe_col.size else np.zeros(1, np.int32), torch.int32)
         self.clamp_row = t(layout.clamp_row if layout.clamp_row.size else np.zeros(1, np.int32), torch.int32)
@@ -701,6 +746,7 @@ class StaggerRuntime:
         d.level_off = p(self.level_off)
         d.chain_rec, d.chain_par, d.kid_pad = p(self.chain_rec), p(self.chain_par), p(self.kid_pad)
         d.slot_link, d.slot_chain = p(self.slot_link), p(self.slot_chain)
+        d.chain_work = p(self.chain_work)
         d.lo, d.up, d.cls_tab = p(self.lo), p(self.up), p(self.cls_tab)
         d.clamp_row, d.clamp_on, d.clamp_off, d.clamp_amp = p(self.clamp_row), p(self.clamp_on), p(self.clamp_off), p(self.clamp_amp)
         d.probe_row, d.probe_col = p(self.probe_row), p(self.probe_col)

@@ -196,6 +196,11 @@ typedef struct bc_stagger_desc {
   const int16_t* slot_link;
   const int16_t* slot_chain;
   const bc_spike_desc* spk; /* NULL: no spike output */
+  /* Optional (NULL: chain order): per global chain position, the pack-local
+   * chain that the elimination's per-chain passes take at that position;
+   * thread t of a pack takes positions t, t + threads, ...  A permutation of
+   * each pack's chains that balances the rows the threads walk. */
+  const int16_t* chain_work;
 } bc_stagger_desc;
 
 /* bc_stagger_desc.flags */

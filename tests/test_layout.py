@@ -43,12 +43,20 @@ def test_multisegment_branch_geometry_matches_oracle():
     np.testing.assert_allclose(g.diag_axial, st.diag, rtol=1e-12)
 
 
+@pytest.mark.parametrize("max_len", [None, 16, 3, 1])
 @pytest.mark.parametrize("maker", CELLS)
-def test_chain_invariants(maker):
+def test_chain_invariants(maker, max_len):
     m = maker()
     g = discretize_cell(m.morph, m.n_cv_branch(), m.per_branch(m.cm), m.per_branch(m.ra))
-    ch = build_chains(g.parent_rows)
+    ch = build_chains(g.parent_rows, max_len)
     assert sorted(ch.rows.tolist()) == list(range(g.n_point))  # each row in exactly one chain
+    full = build_chains(g.parent_rows)
+    assert ch.n_chains >= full.n_chains
+    if max_len:
+        assert np.diff(ch.off).max() <= max_len
+    else:  # maximal chains: a chain's top parent has several children (or none)
+        nchild = np.bincount(g.parent_rows[g.parent_rows >= 0], minlength=g.n_point)
+        assert all(tp < 0 or nchild[tp] != 1 for tp in ch.top_parent.tolist())
     for k in range(ch.n_chains):
         rows = ch.rows[ch.off[k]:ch.off[k + 1]]
         for a, b in zip(rows[:-1], rows[1:]):
@@ -73,9 +81,10 @@ def test_pack_level_widths_never_grow():
         assert np.all(w > 0) and np.all(np.diff(w) <= 0)
 
 
-def test_packed_hines_emulation_matches_oracle_dhs():
+@pytest.mark.parametrize("max_chain_rows", [16, 2, None])
+def test_packed_hines_emulation_matches_oracle_dhs(max_chain_rows):
     models = [W.uneven_tree_cell(i, cmin=200, cmax=400) for i in range(5)] + [W.single_tree_cell(0)]
-    L = StaggerLayout(models, rows_per_pack=1200, cv_per_thread=4)
+    L = StaggerLayout(models, rows_per_pack=1200, cv_per_thread=4, max_chain_rows=max_chain_rows)
     assert L.n_packs < len(models)  # several cells share a pack
     dt = 0.025
     rng = np.random.default_rng(0)
@@ -124,11 +133,12 @@ def test_probes_and_soma_location():
     assert L.clamp_kind == ["mid"]
 
 
-def test_packed_transposed_solve_matches_dense():
+@pytest.mark.parametrize("max_chain_rows", [16, 2, None])
+def test_packed_transposed_solve_matches_dense(max_chain_rows):
     from oracle.dhs import assemble_dense, numeric_state
 
     models = [W.uneven_tree_cell(i, cmin=200, cmax=300) for i in range(3)]
-    L = StaggerLayout(models, rows_per_pack=1200, cv_per_thread=4)
+    L = StaggerLayout(models, rows_per_pack=1200, cv_per_thread=4, max_chain_rows=max_chain_rows)
     dt = 0.025
     rng = np.random.default_rng(4)
     G = rng.uniform(0, 1, L.n_cv_total)
