@@ -30,6 +30,37 @@ HIPCC_FLAGS = [
 EXTRA_FLAGS = {"bc_closure.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 
 
+def asan_library_path() -> str:
+    return os.path.join(LIBDIR, "asan", "libbraincell_amd_asan.so")
+
+
+def build_asan(verbose: bool = False) -> str:
+    """Test build of the same sources with AddressSanitizer on the host code
+    (descriptor validation, launch set-up, graph handling): -O0 -g, device
+    code compiled normally.  Loaded under the clang ASan runtime by the CPU
+    sanitizer test; never used for device work."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    out_dir = os.path.dirname(asan_library_path())
+    os.makedirs(out_dir, exist_ok=True)
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    flags = ["-O0", "-g", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops", "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+    cmds, objs = [], []
+    for src in sources():
+        obj = os.path.join(out_dir, os.path.basename(src).replace(".hip", ".o"))
+        cmds.append([hipcc, *flags, "-c", "-I", CSRC, "-I", INCLUDE, src, "-o", obj])
+        objs.append(obj)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1)) as ex:
+        for r in list(ex.map(lambda c: subprocess.run(c, check=True, capture_output=not verbose), cmds)):
+            pass
+    out = asan_library_path()
+    subprocess.run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address", *objs,
+                    "-o", out + ".tmp"], check=True, capture_output=not verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
@@ -67,4 +98,7 @@ def build(verbose: bool = True) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose="-q" not in sys.argv)
+    if "--asan" in sys.argv:
+        print(build_asan(verbose="-q" not in sys.argv))
+    else:
+        build(verbose="-q" not in sys.argv)

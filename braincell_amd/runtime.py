@@ -577,6 +577,131 @@ class StaggerLayout:
                 S[s] = ci0
         return V, S, Vbnd
 
+    def validate(self):
+        """Checks the packed tables a launch reads before anything touches the
+        device, so that a corrupted layout (a parent index out of its pack, a
+        cycle in the chain tree, a slot used twice, a link that points
+        outside its pack ...) raises ``NativeError`` with status
+        ``INVALID_LAYOUT`` instead of reaching the kernel.  The kernel keeps
+        its own guard (an unresolved chain tree sets status bit 2)."""
+        import hashlib
+
+        from ._native import NativeError
+
+        def bad(msg):
+            raise NativeError(2, msg)
+
+        # tables already checked and unchanged since: nothing to do
+        h = hashlib.blake2b(digest_size=16)
+        for arr in (self.pack_array, self.chain_rec, self.chain_off, self.chain_par, self.chain_top, self.slot_link,
+                    self.slot_chain, self.cv_info, self.bnd_info, self.level_off, self.kid_pad, self.chain_work,
+                    self.probe_row, self.probe_col, self.clamp_row):
+            h.update(np.ascontiguousarray(arr).view(np.uint8))
+        h.update(np.asarray([self.n_cv_total, self.n_rows_total, self.max_rows, self.max_chains, self.n_probes],
+                            dtype=np.int64).view(np.uint8))
+        digest = h.digest()
+        if getattr(self, "_valid_digest", None) == digest:
+            return self
+
+        P = self.pack_array
+        if P.ndim != 2 or P.shape[1] != 16 or P.shape[0] != self.n_packs:
+            bad("pack table has the wrong shape")
+        cv0, ncv, row0, nrow, ch0, nch, lv0, nlv = (P[:, i].astype(np.int64) for i in range(8))
+        bnd0, nbnd, cls0, ncls, cl0, ncl, pr0, npr = (P[:, i].astype(np.int64) for i in range(8, 16))
+        for name, off, cnt, total in (("CV", cv0, ncv, self.n_cv_total), ("row", row0, nrow, self.n_rows_total),
+                                      ("chain", ch0, nch, len(self.chain_rec)), ("boundary", bnd0, nbnd, len(self.bnd_info)),
+                                      ("class", cls0, ncls, len(self.cls_tab)), ("clamp", cl0, ncl, len(self.clamp_row)),
+                                      ("probe", pr0, npr, len(self.probe_row))):
+            if np.any(cnt < 0) or np.any(off != np.concatenate(([0], np.cumsum(cnt)[:-1]))) or int(cnt.sum()) != total:
+                bad(f"pack {name} ranges are not contiguous over the {name} arrays")
+        if np.any(lv0 != np.concatenate(([0], np.cumsum(nlv + 1)[:-1]))) or int((nlv + 1).sum()) != len(self.level_off):
+            bad("pack level ranges are not contiguous")
+        if np.any(nch < 1) or np.any(nlv < 1) or nrow.max() > self.max_rows or nch.max() > self.max_chains:
+            bad("pack sizes exceed the layout maxima")
+        pack_of_ch = np.repeat(np.arange(self.n_packs), nch)
+        lrow0 = row0[pack_of_ch]
+        co = self.chain_off.astype(np.int64)
+        j0, j1 = co[:-1] - lrow0, co[1:] - lrow0
+        if len(co) != len(self.chain_rec) + 1 or np.any(j1 <= j0) or np.any(j0 < 0) or np.any(j1 > nrow[pack_of_ch]):
+            bad("a chain's slot range lies outside its pack")
+        if np.any(j0[ch0] != 0) or np.any(j1[ch0 + nch - 1] != nrow):
+            bad("chains do not cover their pack's rows")
+        rec = self.chain_rec.astype(np.int64)
+        if np.any((rec[:, 0] & 0xFFFF) != j0) or np.any((rec[:, 0] >> 16) & 0xFFFF != j1):
+            bad("chain records disagree with the chain offsets")
+        local = np.arange(len(rec)) - ch0[pack_of_ch]
+        par = self.chain_par.astype(np.int64)
+        if np.any(par < -1) or np.any(par >= nch[pack_of_ch]) or np.any(par == local):
+            bad("a parent chain index lies outside its pack")
+        # levels (heights): a parent sits strictly higher than each kid, so
+        # the chain tree is acyclic
+        lvo = self.level_off.astype(np.int64)
+        last = lv0 + nlv
+        is_last = np.zeros(len(lvo), dtype=bool)
+        is_last[last] = True
+        pack_of_lv = np.repeat(np.arange(self.n_packs), nlv + 1)
+        inner = np.nonzero(~is_last)[0]
+        if np.any(lvo[lv0] != 0) or np.any(lvo[last] != nch) or np.any(lvo[inner + 1] <= lvo[inner]):
+            bad("level offsets do not partition their pack's chains")
+        flag = np.bincount(lvo[inner] + ch0[pack_of_lv[inner]], minlength=len(rec))
+        level = np.cumsum(flag) - (lv0 - np.arange(self.n_packs))[pack_of_ch] - 1
+        has_p = par >= 0
+        gpar = np.where(has_p, par + ch0[pack_of_ch], 0)
+        if np.any(level[gpar][has_p] <= level[has_p]):
+            bad("a parent chain is not above its kid (cycle or corrupted parent index)")
+        tp = ((rec[:, 1] & 0xFFFF) ^ 0x8000) - 0x8000
+        if np.any(np.where(has_p, tp != j0[gpar], tp != -1)):
+            bad("a chain's top does not hang from its parent's bottom row")
+        nk = (rec[:, 1] >> 16) & 0xFFFF
+        cnt = np.bincount(gpar[has_p], minlength=len(rec))
+        if np.any(cnt != nk):
+            bad("kid counts disagree with the parent indices")
+        if np.any(level[nk == 0] != 0) or np.any(level[nk > 0] == 0):
+            bad("leaf chains are not exactly level 0")
+        # slots: each slot in exactly one chain; links inside the pack
+        slot_pack = np.repeat(np.arange(self.n_packs), nrow)
+        n_slot = len(self.slot_link)
+        if n_slot != self.n_rows_total or len(self.slot_chain) != n_slot:
+            bad("slot tables have the wrong length")
+        ch_of_slot = np.repeat(np.arange(len(rec)), j1 - j0)
+        if np.any(self.slot_chain.astype(np.int64) != local[ch_of_slot]):
+            bad("slot chains disagree with the chain ranges")
+        link = self.slot_link.astype(np.int64) & 0xFFFF
+        lpar = link & 0x7FFF
+        pos = np.arange(n_slot) - row0[slot_pack]
+        top = pos == (j1[ch_of_slot] - 1)
+        exp_par = np.where(top, np.where(has_p[ch_of_slot], tp[ch_of_slot], pos), pos + 1)
+        if np.any(lpar != exp_par) or np.any(((link & 0x8000) != 0) != (pos == j0[ch_of_slot])):
+            bad("a slot link points outside its chain's parent structure")
+        # CV and boundary rows: every slot of a pack is one or the other, once
+        cv_pack = np.repeat(np.arange(self.n_packs), ncv)
+        cs = self.cv_info.astype(np.int64) & 0xFFFF
+        if np.any(cs >= nrow[cv_pack]):
+            bad("a CV's slot lies outside its pack")
+        if np.any(((self.cv_info.astype(np.int64) >> 16) & 0x3FFF) >= ncls[cv_pack]):
+            bad("a CV's mechanism class lies outside its pack")
+        bnd_pack = np.repeat(np.arange(self.n_packs), nbnd)
+        bs = self.bnd_info.astype(np.int64) & 0x7FFFFFFF
+        if np.any(bs >= nrow[bnd_pack]):
+            bad("a boundary row's slot lies outside its pack")
+        used = np.bincount(np.concatenate([cs + row0[cv_pack], bs + row0[bnd_pack]]), minlength=n_slot)
+        if used.size != n_slot or np.any(used != 1):
+            bad("a slot is not exactly one CV or boundary row")
+        pr_pack = np.repeat(np.arange(self.n_packs), npr)
+        if np.any(self.probe_row >= nrow[pr_pack]) or np.any(self.probe_col < 0) or np.any(self.probe_col >= max(self.n_probes, 1)):
+            bad("a probe lies outside its pack or the trace")
+        cl_pack = np.repeat(np.arange(self.n_packs), ncl)
+        if np.any(self.clamp_row < 0) or np.any(self.clamp_row >= nrow[cl_pack]):
+            bad("a clamp row lies outside its pack")
+        kid_ok = self.kid_pad.astype(np.int64)
+        if np.any(kid_ok < 0) or np.any(kid_ok >= nch[pack_of_ch]):
+            bad("a kid list entry lies outside its pack")
+        w = self.chain_work.astype(np.int64)
+        if len(w) != len(rec) or np.any(np.sort(w + ch0[pack_of_ch]) != np.arange(len(rec))):
+            bad("the chain work order is not a permutation of each pack's chains")
+        self._valid_digest = digest
+        return self
+
     def cell_slice(self, i: int) -> slice:
         return slice(int(self.cell_cv0[i]), int(self.cell_cv0[i] + self.cell_ncv[i]))
 
@@ -609,6 +734,7 @@ class StaggerRuntime:
         self.dtype = torch.float32 if precision == 32 else torch.float64
         self.device = torch.device(device)
         self.dt = float(dt)
+        layout.validate()  # INVALID_LAYOUT before any device work
         lib()  # fail loudly when the native library is missing
         if not lib().bc_stagger_supported(precision, layout.chset, layout.cv_per_thread):
             raise RuntimeError(f"no kernel for channel set {hex(layout.chset)} / cv_per_thread "

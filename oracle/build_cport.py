@@ -18,5 +18,24 @@ def build(verbose=True):
     return OUT
 
 
+ASAN_OUT = os.path.join(HERE, "cport", "asan", "libbcport_asan.so")
+
+
+def build_asan(verbose=False):
+    """The port with AddressSanitizer and UndefinedBehaviorSanitizer (test
+    build, loaded under gcc's ASan runtime by the CPU sanitizer test)."""
+    os.makedirs(os.path.dirname(ASAN_OUT), exist_ok=True)
+    cmd = ["gcc", "-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+           "-fopenmp", "-fPIC", "-shared", SRC, "-o", ASAN_OUT + ".tmp", "-lm"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(ASAN_OUT + ".tmp", ASAN_OUT)
+    return ASAN_OUT
+
+
 if __name__ == "__main__":
-    build("-q" not in sys.argv)
+    if "--asan" in sys.argv:
+        build_asan("-q" not in sys.argv)
+    else:
+        build("-q" not in sys.argv)
