@@ -234,7 +234,8 @@ def bench_cells(args):
                                f"dendritic CaL/CalciumDetailed, dt 0.025 ms, staggered plain step",
                    "cells": args.cells, "compartments": int(L.n_cv_total), "dhs_rows": int(L.n_rows_total),
                    "packs": L.n_packs, "threads": L.threads, "cv_per_thread": L.cv_per_thread,
-                   "rows_per_pack": L.rows_per_pack, "autotuned": not args.no_autotune,
+                   "rows_per_pack": L.rows_per_pack, "chain_rows": L.max_chain_rows,
+                   "autotuned": not args.no_autotune,
                    "steps_per_launch": chunk, "tail_steps": tail,
                    "parallelism": "1 GPU, one workgroup per pack"},
         "roofline": roof, "cpu_baseline": cpu, "accuracy": acc,

@@ -47,7 +47,7 @@ class StaggerDesc(ctypes.Structure):
         ("dt", ctypes.c_double), ("flags", ctypes.c_uint32), ("syn", ctypes.c_void_p),
         ("chain_rec", ctypes.c_void_p), ("chain_par", ctypes.c_void_p), ("kid_pad", ctypes.c_void_p),
         ("slot_link", ctypes.c_void_p), ("slot_chain", ctypes.c_void_p), ("spk", ctypes.c_void_p),
-        ("chain_work", ctypes.c_void_p),
+        ("chain_work", ctypes.c_void_p), ("cv_aux", ctypes.c_void_p), ("bnd_aux", ctypes.c_void_p),
     ]
 
 

@@ -8,9 +8,13 @@ of a pack (how many cells share a workgroup and its LDS).  Bigger packs
 share the latency of the tree solve between more cells but need more LDS
 per workgroup, which lowers the number of resident workgroups per compute
 unit.  The best trade-off depends on the cell-size mix, so it is measured:
-every candidate plan that fits is timed for a few steps on the device and
-the fastest one is kept, in memory and in a small JSON cache keyed by the
-batch's size signature.
+every candidate plan that fits is timed on the device with the launch
+shape of a production run (hipGraph-captured launches of 200 steps) and the
+fastest one is kept, in memory and in a small JSON cache keyed by the
+batch's size signature.  A fourth knob shapes the tree solve: the length at
+which chains are cut (longer chains mean fewer chain levels for the level
+sweep of the elimination, shorter ones less work per thread in the
+per-chain passes); the best pack plan is tried with a few cut lengths.
 """
 
 from __future__ import annotations
@@ -82,17 +86,21 @@ def candidate_plans(layout: StaggerLayout, precision: int = 32):
 
 
 def _time_plan(L: StaggerLayout, dt: float, precision: int, steps: int, reps: int) -> float:
+    """Seconds per step of a hipGraph-captured launch of ``steps`` steps
+    (the best of ``reps`` replays after one warm-up replay)."""
     import torch
 
     rt = StaggerRuntime(L, dt, precision=precision)
     tr = rt.new_trace(steps)
-    rt.run(steps, trace=tr)
+    g = rt.make_graph(1, steps, tr)
+    stream = rt.stream_handle()
+    g.launch(stream)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     best = float("inf")
     for _ in range(reps):
         e0.record()
-        rt.run(steps, trace=tr)
+        g.launch(stream)
         e1.record()
         torch.cuda.synchronize()
         best = min(best, e0.elapsed_time(e1) * 1e-3)
@@ -100,8 +108,11 @@ def _time_plan(L: StaggerLayout, dt: float, precision: int, steps: int, reps: in
     return best / steps
 
 
-def autotune(layout: StaggerLayout, dt: float, *, precision: int = 32, steps: int = 40, reps: int = 2,
-             use_cache: bool = True, log=None):
+CHAIN_ROWS = (16, 12, 24)
+
+
+def autotune(layout: StaggerLayout, dt: float, *, precision: int = 32, steps: int = 200, reps: int = 2,
+             chain_rows=CHAIN_ROWS, use_cache: bool = True, log=None):
     """Times every candidate plan of ``layout``'s cells and returns
     (best layout, table of (plan, seconds per step)).  Results are cached by
     the batch's size signature."""
@@ -114,10 +125,12 @@ def autotune(layout: StaggerLayout, dt: float, *, precision: int = 32, steps: in
                 cache = json.load(f)
         except (OSError, ValueError):
             cache = {}
-    if use_cache and sig in cache:
-        t, c, r = cache[sig]["plan"]
+    if use_cache and sig in cache and len(cache[sig]["plan"]) == 4:
+        t, c, r, S = cache[sig]["plan"]
         if log:
-            log(f"autotune: cached plan threads={t} cv_per_thread={c} rows_per_pack={r}")
+            log(f"autotune: cached plan threads={t} cv_per_thread={c} rows_per_pack={r} chain_rows={S}")
+        if S != layout.max_chain_rows:
+            return layout.rechained(S, threads=t, cv_per_thread=c, rows_per_pack=r), cache[sig]["table"]
         return layout.repacked(threads=t, cv_per_thread=c, rows_per_pack=r), cache[sig]["table"]
     table = []
     best = None
@@ -138,6 +151,24 @@ def autotune(layout: StaggerLayout, dt: float, *, precision: int = 32, steps: in
             best = (L, sec, (t, c, r))
     if best is None:
         raise RuntimeError("no candidate plan fits the device")
+    # the best pack plan with other chain cut lengths
+    S0 = layout.max_chain_rows
+    best = (best[0], best[1], (*best[2], S0))
+    t, c, r, _ = best[2]
+    for S in chain_rows:
+        if S == S0:
+            continue
+        t0 = time.perf_counter()
+        L = layout.rechained(S, threads=t, cv_per_thread=c, rows_per_pack=r)
+        if L.lds_bytes(precision) > LDS_LIMIT:
+            continue
+        sec = _time_plan(L, dt, precision, steps, reps)
+        table.append(((t, c, r, S), sec))
+        if log:
+            log(f"autotune: threads={t} cv_per_thread={c} rows_per_pack={r} chain_rows={S} packs={L.n_packs} "
+                f"lds={L.lds_bytes(precision)} -> {sec * 1e3:.4f} ms/step ({time.perf_counter() - t0:.1f}s)")
+        if sec < best[1]:
+            best = (L, sec, (t, c, r, S))
     if use_cache:
         cache[sig] = {"plan": list(best[2]), "table": [[list(p), s] for p, s in table]}
         try:

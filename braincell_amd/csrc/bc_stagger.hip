@@ -94,6 +94,8 @@ struct StaggerArgs {
   const int16_t* slot_link;
   const int16_t* slot_chain;
   const int16_t* chain_work;  // optional work order of the per-chain passes
+  const uint32_t* cv_aux;     // per CV: slot link | parent chain of its chain << 16
+  const uint32_t* bnd_aux;    // per boundary row: the same
 };
 
 constexpr uint32_t kRowMask = 0xFFFFu;
@@ -667,7 +669,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   int16_t* m_lv = reinterpret_cast<int16_t*>(smem_raw + lp.lv);
   KPair<real>* KK2 = reinterpret_cast<KPair<real>*>(smem_raw + lp.kk2);
   Comp<real>* CM = reinterpret_cast<Comp<real>*>(smem_raw + lp.cmp);  // chain composites (aliases KK2, ptr)
-  // per chain: parent chain (low half, 0xFFFF for a root) | the parent's kid count << 16
+  // per chain: its parent chain (low half; 0xFFFF for a root)
   uint32_t* m_dfw = reinterpret_cast<uint32_t*>(smem_raw + lp.dfw);
   int16_t* m_ptr = reinterpret_cast<int16_t*>(smem_raw + lp.ptr);  // pointer jumping (two buffers)
   int16_t* m_ptr2 = reinterpret_cast<int16_t*>(smem_raw + lp.ptr2);
@@ -686,9 +688,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   for (int i = tid; i < P.n_ch; i += NT) {
     const int g = P.ch0 + i;
     CR[i] = a.chain_rec[g];
-    const int par = a.chain_par[g];
-    const uint32_t nkp = par >= 0 ? (uint32_t)a.chain_rec[P.ch0 + par].y >> 16 : 0u;
-    m_dfw[i] = (uint32_t)(uint16_t)par | (nkp << 16);
+    m_dfw[i] = (uint32_t)(uint16_t)a.chain_par[g];
     m_kid[i] = a.kid_pad[g];
     m_work[i] = a.chain_work ? a.chain_work[g] : (int16_t)i;
   }
@@ -741,8 +741,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   for (int i = 0; i < MAXC; ++i) {
     aux[i] = 0;
     if (info[i] != 0xFFFFFFFFu) {
-      const int64_t g = (int64_t)P.row0 + (int)(info[i] & kRowMask);
-      aux[i] = (uint32_t)(uint16_t)a.slot_link[g] | ((m_dfw[a.slot_chain[g]] & 0xFFFFu) << 16);
+      aux[i] = a.cv_aux[(int64_t)P.cv0 + tid + i * NT];
     }
   }
   uint32_t bx[kBndRegs], bl[kBndRegs];
@@ -754,9 +753,9 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     if (i < P.n_bnd) {
       const uint32_t u = a.bnd_info[P.bnd0 + i];
       const int row = (int)(u & 0x7FFFFFFFu);
-      const int64_t g = (int64_t)P.row0 + row;
-      bx[b] = (uint32_t)row | ((u & kClampBit) ? 0x8000u : 0u) | ((m_dfw[a.slot_chain[g]] & 0xFFFFu) << 16);
-      bl[b] = (uint32_t)(uint16_t)a.slot_link[g];
+      const uint32_t x = a.bnd_aux[P.bnd0 + i];
+      bx[b] = (uint32_t)row | ((u & kClampBit) ? 0x8000u : 0u) | (x & 0xFFFF0000u);
+      bl[b] = x & 0xFFFFu;
     }
   }
 
@@ -803,7 +802,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     for (int i = tid + kBndRegs * NT; i < P.n_bnd; i += NT) {
       uint32_t b = a.bnd_info[P.bnd0 + i];
       int row = (int)(b & 0x7FFFFFFFu);
-      const real D = axial_defect<real>(row, xv[row], R, xv, (uint32_t)(uint16_t)a.slot_link[(int64_t)P.row0 + row]);
+      const real D = axial_defect<real>(row, xv[row], R, xv, a.bnd_aux[P.bnd0 + i] & 0xFFFFu);
       R[row].d = real(0);
       R[row].r = ((b & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0)) + D;
     }
@@ -929,7 +928,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
     for (int i = tid + kBndRegs * NT; i < P.n_bnd; i += NT) {
       const int row = (int)(a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu);
-      xv[row] += row_solution<real>(row, R, (m_dfw[a.slot_chain[(int64_t)P.row0 + row]] & 0xFFFFu) << 16, Xb);
+      xv[row] += row_solution<real>(row, R, a.bnd_aux[P.bnd0 + i] & 0xFFFF0000u, Xb);
     }
 #pragma unroll
     for (int b = 0; b < kBndRegs; ++b)
@@ -1108,6 +1107,8 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->slot_link = d->slot_link;
   a->slot_chain = d->slot_chain;
   a->chain_work = d->chain_work;
+  a->cv_aux = d->cv_aux;
+  a->bnd_aux = d->bnd_aux;
   return BC_OK;
 }
 
@@ -1128,7 +1129,8 @@ int validate(const bc_stagger_desc* d) {
     return bc::fail(BC_INVALID_ARG, "unknown flags 0x%x", d->flags);
   if (d->n_packs > 0 && (d->packs == nullptr || d->cv_info == nullptr || d->V == nullptr || d->Vbnd == nullptr ||
                          d->status == nullptr || d->chain_rec == nullptr || d->chain_par == nullptr ||
-                         d->kid_pad == nullptr || d->slot_link == nullptr || d->slot_chain == nullptr))
+                         d->kid_pad == nullptr || d->slot_link == nullptr || d->slot_chain == nullptr ||
+                         d->cv_aux == nullptr || d->bnd_aux == nullptr))
     return bc::fail(BC_INVALID_ARG, "required pointer is NULL");
   if (const bc_spike_desc* k = d->spk) {
     if (k->cap < 0) return bc::fail(BC_INVALID_ARG, "spike desc: cap < 0");
@@ -1250,6 +1252,14 @@ int bc_stagger_graph_create(const bc_stagger_desc* desc, int32_t n_chunks, int32
     (void)hipGraphDestroy(h->graph);
     delete h;
     return bc::fail(BC_HIP_ERROR, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  // move the executable graph to the device now, not at its first launch
+  e = hipGraphUpload(h->exec, s);
+  if (e != hipSuccess) {
+    (void)hipGraphExecDestroy(h->exec);
+    (void)hipGraphDestroy(h->graph);
+    delete h;
+    return bc::fail(BC_HIP_ERROR, "hipGraphUpload: %s", hipGetErrorString(e));
   }
   *graph_out = h;
   return bc::ok();

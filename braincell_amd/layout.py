@@ -31,7 +31,7 @@ import numpy as np
 EPS_PARAM = 1e-9
 EPS_LEN_UM = 1e-6
 
-__all__ = ["CellGeometry", "discretize_cell", "build_chains", "Chains"]
+__all__ = ["CellGeometry", "discretize_cell", "build_chains", "build_chains_batch", "Chains"]
 
 
 @dataclass
@@ -373,62 +373,107 @@ def build_chains(parent_rows: np.ndarray, max_len: int = None) -> Chains:
     maximal runs in which every row is the only child of the next, cut from
     the bottom into pieces of at most ``max_len`` rows (a piece's top then
     hangs from the bottom row of the piece above)."""
-    parent_rows = np.asarray(parent_rows, dtype=np.int64)
-    P = parent_rows.shape[0]
-    has_p = parent_rows >= 0
-    nchild = np.bincount(parent_rows[has_p], minlength=P)
-    bottoms = np.nonzero(nchild != 1)[0]
-    chain_of_row = np.full(P, -1, dtype=np.int64)
-    rows_list = []
-    tops_parent = []
-    cap = int(max_len) if max_len else P + 1
-    if cap < 1:
+    return build_chains_batch([parent_rows], max_len)[0]
+
+
+def build_chains_batch(parent_rows_list, max_len: int = None) -> list:
+    """``build_chains`` for many row forests at once (vectorised over all of
+    their rows).  Chains of each forest are sorted by (level, bottom row),
+    level = height in the chain tree (leaves 0); kids are listed in
+    increasing chain index."""
+    cap = int(max_len) if max_len else None
+    if cap is not None and cap < 1:
         raise ValueError("max_len must be >= 1")
-    for b in bottoms.tolist():
-        seq = [b]
-        r = b
+    sizes = np.asarray([len(p) for p in parent_rows_list], dtype=np.int64)
+    n_f = len(sizes)
+    base = np.concatenate(([0], np.cumsum(sizes)))
+    N = int(base[-1])
+    if N == 0:
+        raise ValueError("empty row forest")
+    loc = np.concatenate([np.asarray(p, dtype=np.int64) for p in parent_rows_list])
+    forest = np.repeat(np.arange(n_f), sizes)
+    par = np.where(loc >= 0, loc + base[forest], -1)
+    has_p = par >= 0
+    nchild = np.bincount(par[has_p], minlength=N)
+    # a row continues its parent's chain when it is that parent's only child
+    up = np.where(has_p, par, 0)
+    cont = has_p & (nchild[up] == 1)
+    prev = np.full(N, -1, dtype=np.int64)
+    prev[par[cont]] = np.nonzero(cont)[0]
+    # list ranking: distance of every row to the bottom of its maximal chain
+    low = np.where(prev >= 0, prev, np.arange(N))
+    dist = (prev >= 0).astype(np.int64)
+    while True:
+        nxt = low[low]
+        if np.array_equal(nxt, low):
+            break
+        dist = dist + dist[low] * (low != np.arange(N))
+        low = nxt
+    # (the loop adds dist[low] while low has not reached the bottom)
+    bottom = low
+    piece = dist // cap if cap else np.zeros(N, dtype=np.int64)
+    # rows in chain order: by (bottom, distance); pieces are contiguous runs
+    order = np.lexsort((dist, bottom))
+    b_o, p_o = bottom[order], piece[order]
+    start = np.ones(N, dtype=bool)
+    start[1:] = (b_o[1:] != b_o[:-1]) | (p_o[1:] != p_o[:-1])
+    chain_of_pos = np.cumsum(start) - 1
+    n_ch = int(chain_of_pos[-1]) + 1
+    chain_of_row = np.empty(N, dtype=np.int64)
+    chain_of_row[order] = chain_of_pos
+    first = np.nonzero(start)[0]
+    last = np.concatenate((first[1:], [N])) - 1
+    bot_row = order[first]
+    top_row = order[last]
+    top_parent = par[top_row]
+    tp_ok = top_parent >= 0
+    parent_chain = np.where(tp_ok, chain_of_row[np.maximum(top_parent, 0)], -1)
+    # levels: height in the chain tree, by relaxation over the kid edges
+    kid_idx = np.nonzero(parent_chain >= 0)[0]
+    kpar = parent_chain[kid_idx]
+    srt = np.argsort(kpar, kind="stable")
+    kid_idx, kpar = kid_idx[srt], kpar[srt]
+    level = np.zeros(n_ch, dtype=np.int64)
+    if kid_idx.size:
+        grp = np.nonzero(np.concatenate(([True], kpar[1:] != kpar[:-1])))[0]
+        gpar = kpar[grp]
         while True:
-            p = parent_rows[r]
-            if p < 0 or nchild[p] != 1:
+            cand = np.maximum.reduceat(level[kid_idx] + 1, grp)
+            new = level.copy()
+            new[gpar] = np.maximum(new[gpar], cand)
+            if np.array_equal(new, level):
                 break
-            seq.append(int(p))
-            r = p
-        top = int(parent_rows[r])
-        for q in range(0, len(seq), cap):
-            piece = seq[q:q + cap]
-            chain_of_row[piece] = len(rows_list)
-            rows_list.append(piece)
-            tops_parent.append(seq[q + cap] if q + cap < len(seq) else top)
-    bottoms = np.asarray([rl[0] for rl in rows_list], dtype=np.int64)
-    nch = len(rows_list)
-    top_parent = np.asarray(tops_parent, dtype=np.int64)
-    parent_chain = np.where(top_parent >= 0, chain_of_row[np.maximum(top_parent, 0)], -1)
-    # level: leaves 0; parent = 1 + max(kids).  Process chains from deepest rows
-    # upward: a chain's bottom row has a larger peel than every row below it, so
-    # ordering by the bottom row index descending (rows are peel-descending in
-    # the reference order) visits kids first.
-    level = np.zeros(nch, dtype=np.int64)
-    ordk = np.argsort(-bottoms, kind="stable")
-    for k in ordk.tolist():
-        pc = parent_chain[k]
-        if pc >= 0:
-            level[pc] = max(level[pc], level[k] + 1)
-    # sort chains by (level, bottom row)
-    perm = np.lexsort((bottoms, level))
-    inv = np.empty(nch, dtype=np.int64)
-    inv[perm] = np.arange(nch)
-    lens = np.asarray([len(rows_list[k]) for k in perm.tolist()], dtype=np.int64)
-    off = np.concatenate(([0], np.cumsum(lens)))
-    rows = np.concatenate([np.asarray(rows_list[k], dtype=np.int64) for k in perm.tolist()]) if nch else np.zeros(0, np.int64)
-    tp = top_parent[perm]
-    pch = parent_chain[perm]
-    pch_new = np.where(pch >= 0, inv[np.maximum(pch, 0)], -1)
-    # kids CSR (kids listed in increasing new chain index)
-    kid_cnt = np.bincount(pch_new[pch_new >= 0], minlength=nch)
-    kid_off = np.concatenate(([0], np.cumsum(kid_cnt)))
-    kids = np.argsort(np.where(pch_new >= 0, pch_new, nch), kind="stable")[: int(kid_cnt.sum())]
-    lv = level[perm]
-    nlev = int(lv.max()) + 1 if nch else 0
-    level_off = np.searchsorted(lv, np.arange(nlev + 1), side="left")
-    return Chains(off=off, rows=rows, top_parent=tp, kid_off=kid_off, kids=kids,
-                  level_off=level_off.astype(np.int64), level=lv)
+            level = new
+    ch_forest = forest[bot_row]
+    out = []
+    ch_base = np.concatenate(([0], np.cumsum(np.bincount(ch_forest, minlength=n_f))))
+    lens_all = last - first + 1
+    for f in range(n_f):
+        c0, c1 = int(ch_base[f]), int(ch_base[f + 1])
+        b0 = base[f]
+        bottoms = bot_row[c0:c1] - b0
+        lv = level[c0:c1]
+        perm = np.lexsort((bottoms, lv))
+        nch = c1 - c0
+        inv = np.empty(nch, dtype=np.int64)
+        inv[perm] = np.arange(nch)
+        lens = lens_all[c0:c1][perm]
+        off = np.concatenate(([0], np.cumsum(lens)))
+        # rows of the forest's chains in the new chain order
+        pos0 = first[c0:c1][perm]
+        idx = np.repeat(pos0 - off[:-1], lens) + np.arange(int(off[-1]))
+        rows = order[idx] - b0
+        tp = np.where(tp_ok[c0:c1], top_parent[c0:c1] - b0, -1)[perm]
+        pch = parent_chain[c0:c1][perm]
+        pch_new = np.where(pch >= 0, inv[np.maximum(pch - c0, 0)], -1)
+        kid_cnt = np.bincount(pch_new[pch_new >= 0], minlength=nch)
+        kid_off = np.concatenate(([0], np.cumsum(kid_cnt)))
+        kids = np.argsort(np.where(pch_new >= 0, pch_new, nch), kind="stable")[: int(kid_cnt.sum())]
+        lvs = lv[perm]
+        nlev = int(lvs.max()) + 1
+        level_off = np.searchsorted(lvs, np.arange(nlev + 1), side="left")
+        out.append(Chains(off=off, rows=rows, top_parent=tp, kid_off=kid_off, kids=kids,
+                          level_off=level_off.astype(np.int64), level=lvs))
+    return out
+
+

@@ -16,7 +16,7 @@ import numpy as np
 from . import rates
 from ._native import BcPack, SpikeDesc, StaggerDesc, chan_const_type, check, lib
 from .channels import CA_POOL_BIT, CHANNELS, FARADAY, GAS_CONSTANT
-from .layout import build_chains, discretize_cell
+from .layout import build_chains_batch, discretize_cell
 
 __all__ = ["StaggerLayout", "StaggerRuntime", "KERNEL_CHANNEL_SETS", "choose_channel_set", "first_step_at"]
 
@@ -164,7 +164,33 @@ class StaggerLayout:
         else:
             self.cells = [self._lower_cell(m, probes) for m in self.models]
         del self._geom_cache
+        self._chain_cells()
         self._configure(threads, cv_per_thread, rows_per_pack)
+
+    def _chain_cells(self):
+        """Chains of every cell (cells sharing a discretisation share them)."""
+        uniq = {}
+        for c in self.cells:
+            uniq.setdefault(id(c.geom), c.geom)
+        geoms = list(uniq.values())
+        chains = build_chains_batch([g.parent_rows for g in geoms], self.max_chain_rows)
+        by_geom = {id(g): ch for g, ch in zip(geoms, chains)}
+        for c in self.cells:
+            c.chains = by_geom[id(c.geom)]
+
+    def rechained(self, max_chain_rows: int = None, **plan):
+        """The same lowered cells with chains cut at ``max_chain_rows`` rows
+        (and optionally another pack plan: threads, cv_per_thread,
+        rows_per_pack)."""
+        import copy
+
+        L = copy.copy(self)
+        L.max_chain_rows = int(max_chain_rows) if max_chain_rows else None
+        L.cells = [copy.copy(c) for c in self.cells]
+        L._chain_cells()
+        L._configure(plan.get("threads", self.threads), plan.get("cv_per_thread", self.cv_per_thread),
+                     plan.get("rows_per_pack", self.rows_per_pack))
+        return L
 
     def _configure(self, threads, cv_per_thread, rows_per_pack):
         """Workgroup size, register slots and pack plan over the lowered cells."""
@@ -273,11 +299,11 @@ class StaggerLayout:
         key = (id(m.morph), ncvb.tobytes(), np.asarray(cmb, np.float64).tobytes(), np.asarray(rab, np.float64).tobytes())
         hit = self._geom_cache.get(key)
         if hit is not None and hit[0] is m.morph:
-            geom, chains = hit[1], hit[2]
+            geom = hit[1]
         else:
             geom = discretize_cell(m.morph, ncvb, cmb, rab)
-            chains = build_chains(geom.parent_rows, self.max_chain_rows)
-            self._geom_cache[key] = (m.morph, geom, chains)
+            self._geom_cache[key] = (m.morph, geom)
+        chains = None  # built for all cells at once (_chain_cells)
         nb = m.n_branches
         table = np.zeros((nb, NCLS))
         table[:, 0] = m.per_branch(m.cm)
@@ -350,6 +376,7 @@ class StaggerLayout:
         slot_lo, slot_up, slot_rows_all = [], [], []
         chain_rec_all, chain_par_all, kid_pad_all, slot_link_all, slot_chain_all = [], [], [], [], []
         chain_work_all = []
+        cv_aux, bnd_aux = [], []
         cls_tab, clamp_rows, clamp_t, clamp_amp_raw, clamp_kind, clamp_area = [], [], [], [], [], []
         probe_row, probe_col = [], []
         self.cell_region_cls = np.full((n_cells, len(REGION_CODES)), -1, dtype=np.int64)
@@ -421,6 +448,8 @@ class StaggerLayout:
             link[j0] |= 0x8000
             slot_link_all.append(link.astype(np.uint16).view(np.int16))
             slot_chain_all.append(ch_of.astype(np.int16))
+            # per slot: its link | the chain its chain hangs from << 16 (0xFFFF: none)
+            slot_aux = (link & 0xFFFF) | ((parent[ch_of] & 0xFFFF) << 16)
             lv_sorted = lev[perm]
             nlev = int(lv_sorted.max()) + 1
             level_off.extend(np.searchsorted(lv_sorted, np.arange(nlev + 1), side="left").tolist())
@@ -456,12 +485,14 @@ class StaggerLayout:
                 pflag[c.probes] = True
                 info |= (pflag.astype(np.uint64) << np.uint64(30)) | (clamp_cv.astype(np.uint64) << np.uint64(31))
                 cv_info.append(info.astype(np.uint32))
+                cv_aux.append(slot_aux[cv_slots].astype(np.uint32))
                 for k, cvi in enumerate(c.probes.tolist()):
                     probe_row.append(int(cv_slots[cvi]))
                     probe_col.append(int(self.cell_probe0[i] + k))
                 bslots = slot_of[np.nonzero(~g.row_is_mid)[0] + ro]
                 bflag = np.asarray([int(b) in clamp_bnd for b in bslots.tolist()], dtype=np.uint64)
                 bnd_info.append((bslots.astype(np.uint64) | (bflag << np.uint64(31))).astype(np.uint32))
+                bnd_aux.append(slot_aux[bslots].astype(np.uint32))
                 bnd_cell.append(np.full(len(bslots), i, dtype=np.int64))
                 cls_tab.append(c.cls_rows)
                 if self.region_classes:
@@ -491,6 +522,8 @@ class StaggerLayout:
         self.n_rows_total = grow
         self.cv_info = np.concatenate(cv_info)
         self.bnd_info = np.concatenate(bnd_info) if bnd_info else np.zeros(0, np.uint32)
+        self.cv_aux = np.concatenate(cv_aux)
+        self.bnd_aux = np.concatenate(bnd_aux) if bnd_aux else np.zeros(0, np.uint32)
         self.bnd_cell = np.concatenate(bnd_cell) if bnd_cell else np.zeros(0, np.int64)
         self.chain_off = np.asarray(chain_off, dtype=np.int32)
         self.chain_top = np.asarray(chain_top, dtype=np.int32)
@@ -595,7 +628,7 @@ class StaggerLayout:
         h = hashlib.blake2b(digest_size=16)
         for arr in (self.pack_array, self.chain_rec, self.chain_off, self.chain_par, self.chain_top, self.slot_link,
                     self.slot_chain, self.cv_info, self.bnd_info, self.level_off, self.kid_pad, self.chain_work,
-                    self.probe_row, self.probe_col, self.clamp_row):
+                    self.probe_row, self.probe_col, self.clamp_row, self.cv_aux, self.bnd_aux):
             h.update(np.ascontiguousarray(arr).view(np.uint8))
         h.update(np.asarray([self.n_cv_total, self.n_rows_total, self.max_rows, self.max_chains, self.n_probes],
                             dtype=np.int64).view(np.uint8))
@@ -696,6 +729,11 @@ class StaggerLayout:
         kid_ok = self.kid_pad.astype(np.int64)
         if np.any(kid_ok < 0) or np.any(kid_ok >= nch[pack_of_ch]):
             bad("a kid list entry lies outside its pack")
+        slot_aux = (link & 0xFFFF) | ((par[ch_of_slot] & 0xFFFF) << 16)
+        if len(self.cv_aux) != self.n_cv_total or np.any(self.cv_aux.astype(np.int64) != slot_aux[cs + row0[cv_pack]]):
+            bad("per-CV slot metadata disagrees with the slot links and parent chains")
+        if len(self.bnd_aux) != len(self.bnd_info) or np.any(self.bnd_aux.astype(np.int64) != slot_aux[bs + row0[bnd_pack]]):
+            bad("per-boundary-row slot metadata disagrees with the slot links and parent chains")
         w = self.chain_work.astype(np.int64)
         if len(w) != len(rec) or np.any(np.sort(w + ch0[pack_of_ch]) != np.arange(len(rec))):
             bad("the chain work order is not a permutation of each pack's chains")
@@ -754,6 +792,8 @@ class StaggerRuntime:
         self.slot_link = t(layout.slot_link, torch.int16)
         self.slot_chain = t(layout.slot_chain, torch.int16)
         self.chain_work = t(layout.chain_work, torch.int16)
+        self.cv_aux = t(layout.cv_aux.view(np.int32), torch.int32)
+        self.bnd_aux = t(layout.bnd_aux.view(np.int32) if layout.bnd_aux.size else np.zeros(1, np.int32), torch.int32)
         self.probe_row = t(layout.probe_row if layout.probe_row.size else np.zeros(1, np.int32), torch.int32)
         self.probe_col = t(layout.probe_col if layout.probe_col.size else np.zeros(1, np.int32), torch.int32)
         self.clamp_row = t(layout.clamp_row if layout.clamp_row.size else np.zeros(1, np.int32), torch.int32)
@@ -873,6 +913,7 @@ class StaggerRuntime:
         d.chain_rec, d.chain_par, d.kid_pad = p(self.chain_rec), p(self.chain_par), p(self.kid_pad)
         d.slot_link, d.slot_chain = p(self.slot_link), p(self.slot_chain)
         d.chain_work = p(self.chain_work)
+        d.cv_aux, d.bnd_aux = p(self.cv_aux), p(self.bnd_aux)
         d.lo, d.up, d.cls_tab = p(self.lo), p(self.up), p(self.cls_tab)
         d.clamp_row, d.clamp_on, d.clamp_off, d.clamp_amp = p(self.clamp_row), p(self.clamp_on), p(self.clamp_off), p(self.clamp_amp)
         d.probe_row, d.probe_col = p(self.probe_row), p(self.probe_col)

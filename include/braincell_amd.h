@@ -201,6 +201,12 @@ typedef struct bc_stagger_desc {
    * thread t of a pack takes positions t, t + threads, ...  A permutation of
    * each pack's chains that balances the rows the threads walk. */
   const int16_t* chain_work;
+  /* Per CV (cv_info order) and per boundary row (bnd_info order): the row's
+   * slot_link | the pack-local chain its chain hangs from << 16 (0xFFFF for
+   * a root chain); precomputed so a launch reads them with independent
+   * loads. */
+  const uint32_t* cv_aux;
+  const uint32_t* bnd_aux;
 } bc_stagger_desc;
 
 /* bc_stagger_desc.flags */

@@ -62,12 +62,32 @@ def test_every_plan_gives_the_same_trajectories(cuda, tmp_path, monkeypatch):
         return tr.cpu().numpy()[:, cols]
 
     ref = soma(base)
-    for (t, c, r), _ in table:
-        got = soma(base.repacked(threads=t, cv_per_thread=c, rows_per_pack=r))
-        # per-cell arithmetic does not depend on the plan (kernels compiled for
-        # other register-slot counts may contract differently: 1e-4 mV)
-        assert np.max(np.abs(got - ref)) < 1e-4, (t, c, r)
+    assert any(len(p) == 4 for p, _ in table)  # chain cut lengths were searched
+    for plan, _ in table:
+        t, c, r = plan[:3]
+        if len(plan) == 4:
+            # another chain cut groups the elimination's rounding differently
+            got = soma(base.rechained(plan[3], threads=t, cv_per_thread=c, rows_per_pack=r))
+            assert np.max(np.abs(got - ref)) < 1e-3, plan
+        else:
+            got = soma(base.repacked(threads=t, cv_per_thread=c, rows_per_pack=r))
+            # per-cell arithmetic does not depend on the plan (kernels compiled for
+            # other register-slot counts may contract differently: 1e-4 mV)
+            assert np.max(np.abs(got - ref)) < 1e-4, plan
     # the cached plan comes back without timing again
     again, _ = autotune(base, 0.025, steps=20, reps=1)
-    assert (again.threads, again.cv_per_thread, again.rows_per_pack) == (best.threads, best.cv_per_thread,
-                                                                          best.rows_per_pack)
+    assert (again.threads, again.cv_per_thread, again.rows_per_pack, again.max_chain_rows) == (
+        best.threads, best.cv_per_thread, best.rows_per_pack, best.max_chain_rows)
+
+
+def test_rechained_layout_equals_a_fresh_one():
+    models = _batch(12)
+    base = StaggerLayout(models, threads=256)
+    for S in (12, 4, None):
+        a = base.rechained(S, rows_per_pack=1500)
+        b = StaggerLayout(models, threads=256, rows_per_pack=1500, max_chain_rows=S)
+        assert a.max_chain_rows == b.max_chain_rows and a.pack_cells == b.pack_cells
+        for k in ("cv_info", "bnd_info", "chain_off", "chain_top", "kid_off", "kids", "level_off", "chain_rec",
+                  "chain_work", "slot_link"):
+            assert np.array_equal(getattr(a, k), getattr(b, k)), (S, k)
+    assert base.max_chain_rows == 16 and base.cells[0].chains is not a.cells[0].chains

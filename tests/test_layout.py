@@ -252,3 +252,86 @@ def test_precomputed_lds_tables_equal_the_kernels_derivation():
         assert not L.kid_pad[ch0 + nkid:ch0 + n_ch].any()
         assert np.array_equal(L.slot_link[row0:row0 + n_rows].view(np.uint16), link)
         assert np.array_equal(L.slot_chain[row0:row0 + n_rows], chain)
+
+
+# The chain definition written as a walk over the rows (each bottom row climbs
+# while its parent has exactly one child), to check the vectorised builder.
+def walk_chains(parent_rows: np.ndarray, max_len: int = None) :
+    """Split a row forest (parents given per row; -1 for roots) into chains:
+    maximal runs in which every row is the only child of the next, cut from
+    the bottom into pieces of at most ``max_len`` rows (a piece's top then
+    hangs from the bottom row of the piece above)."""
+    parent_rows = np.asarray(parent_rows, dtype=np.int64)
+    P = parent_rows.shape[0]
+    has_p = parent_rows >= 0
+    nchild = np.bincount(parent_rows[has_p], minlength=P)
+    bottoms = np.nonzero(nchild != 1)[0]
+    chain_of_row = np.full(P, -1, dtype=np.int64)
+    rows_list = []
+    tops_parent = []
+    cap = int(max_len) if max_len else P + 1
+    if cap < 1:
+        raise ValueError("max_len must be >= 1")
+    for b in bottoms.tolist():
+        seq = [b]
+        r = b
+        while True:
+            p = parent_rows[r]
+            if p < 0 or nchild[p] != 1:
+                break
+            seq.append(int(p))
+            r = p
+        top = int(parent_rows[r])
+        for q in range(0, len(seq), cap):
+            piece = seq[q:q + cap]
+            chain_of_row[piece] = len(rows_list)
+            rows_list.append(piece)
+            tops_parent.append(seq[q + cap] if q + cap < len(seq) else top)
+    bottoms = np.asarray([rl[0] for rl in rows_list], dtype=np.int64)
+    nch = len(rows_list)
+    top_parent = np.asarray(tops_parent, dtype=np.int64)
+    parent_chain = np.where(top_parent >= 0, chain_of_row[np.maximum(top_parent, 0)], -1)
+    # level: leaves 0; parent = 1 + max(kids).  Process chains from deepest rows
+    # upward: a chain's bottom row has a larger peel than every row below it, so
+    # ordering by the bottom row index descending (rows are peel-descending in
+    # the reference order) visits kids first.
+    level = np.zeros(nch, dtype=np.int64)
+    ordk = np.argsort(-bottoms, kind="stable")
+    for k in ordk.tolist():
+        pc = parent_chain[k]
+        if pc >= 0:
+            level[pc] = max(level[pc], level[k] + 1)
+    # sort chains by (level, bottom row)
+    perm = np.lexsort((bottoms, level))
+    inv = np.empty(nch, dtype=np.int64)
+    inv[perm] = np.arange(nch)
+    lens = np.asarray([len(rows_list[k]) for k in perm.tolist()], dtype=np.int64)
+    off = np.concatenate(([0], np.cumsum(lens)))
+    rows = np.concatenate([np.asarray(rows_list[k], dtype=np.int64) for k in perm.tolist()]) if nch else np.zeros(0, np.int64)
+    tp = top_parent[perm]
+    pch = parent_chain[perm]
+    pch_new = np.where(pch >= 0, inv[np.maximum(pch, 0)], -1)
+    # kids CSR (kids listed in increasing new chain index)
+    kid_cnt = np.bincount(pch_new[pch_new >= 0], minlength=nch)
+    kid_off = np.concatenate(([0], np.cumsum(kid_cnt)))
+    kids = np.argsort(np.where(pch_new >= 0, pch_new, nch), kind="stable")[: int(kid_cnt.sum())]
+    lv = level[perm]
+    nlev = int(lv.max()) + 1 if nch else 0
+    level_off = np.searchsorted(lv, np.arange(nlev + 1), side="left")
+    return dict(off=off, rows=rows, top_parent=tp, kid_off=kid_off, kids=kids,
+                  level_off=level_off.astype(np.int64), level=lv)
+
+
+@pytest.mark.parametrize("max_len", [None, 16, 3, 1])
+def test_batch_chains_match_the_row_walk(max_len):
+    from braincell_amd.layout import build_chains_batch
+    from tests.helpers import interior_cell
+
+    ms = [W.uneven_tree_cell(i, cmin=200, cmax=900) for i in range(8)] + [W.single_tree_cell(0), interior_cell(0),
+                                                                          interior_cell(1, 3)]
+    gs = [discretize_cell(m.morph, m.n_cv_branch(), m.per_branch(m.cm), m.per_branch(m.ra)) for m in ms]
+    got = build_chains_batch([g.parent_rows for g in gs], max_len)
+    for g, ch in zip(gs, got):
+        ref = walk_chains(g.parent_rows, max_len)
+        for k, v in ref.items():
+            assert np.array_equal(getattr(ch, k), v), k
