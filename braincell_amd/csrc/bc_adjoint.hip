@@ -58,15 +58,16 @@ struct AdjArgs {
 // form {j0 | j1 << 16, top & 0xFFFF | n_kids << 16, kid0 | kid1 << 16,
 // first kid entry} (bc_stagger_desc.chain_rec) and the chains' arrival
 // counters first (16-byte aligned), then the rows' diagonal, rhs, lowers and
-// uppers, the chains' kid contributions, the mechanism classes and their
-// gradient accumulators, then the pack's kid list, parent chains and level
-// offsets (int16), and two more per-chain (real, real) buffers and two
-// int16 pointer arrays for the back substitution's pointer jumping.  The
-// transposed solve walks LDS only.
+// uppers, the chains' kid contributions, the mechanism classes and the
+// per-wavefront partial sums of the gradient reduction, then the pack's kid
+// list, parent chains and level offsets (int16), and two more per-chain
+// (real, real) buffers and two int16 pointer arrays for the back
+// substitution's pointer jumping.  The transposed solve walks LDS only.
+constexpr int kRedSlots = 16;  // 4 wavefronts x 4 gradient columns
 __host__ __device__ inline size_t adj_lds_bytes(int max_rows, int max_chains, int max_cls, int max_levels,
                                                 size_t real_size) {
   const size_t mc = (size_t)((max_chains + 3) & ~3);
-  return mc * 16 + mc * 4 + (size_t)(4 * max_rows + 4 * max_chains + 2 * max_cls * BC_NCLS) * real_size +
+  return mc * 16 + mc * 4 + (size_t)(4 * max_rows + 4 * max_chains + max_cls * BC_NCLS + kRedSlots) * real_size +
          (4 * mc + (size_t)((max_levels + 4) & ~3)) * sizeof(int16_t);
 }
 
@@ -91,8 +92,8 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   real* kd2 = kr + a.max_chains;  // pointer jumping: second (A, B) buffer
   real* kr2 = kd2 + a.max_chains;
   real* cls = kr2 + a.max_chains;
-  real* gacc = cls + a.max_cls * BC_NCLS;
-  int16_t* m_kid = reinterpret_cast<int16_t*>(gacc + a.max_cls * BC_NCLS);
+  real* red = cls + a.max_cls * BC_NCLS;  // [wavefront][gradient column]
+  int16_t* m_kid = reinterpret_cast<int16_t*>(red + kRedSlots);
   int16_t* m_par = m_kid + mc;  // parent chain, -1 for a root
   int16_t* m_ptr = m_par + mc;  // pointer jumping (two buffers)
   int16_t* m_ptr2 = m_ptr + mc;
@@ -101,10 +102,7 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   const bc_pack P = a.packs[blockIdx.x];
   const int tid = threadIdx.x;
   const int NT = blockDim.x;
-  for (int i = tid; i < P.n_cls * BC_NCLS; i += NT) {
-    cls[i] = a.cls_tab[(int64_t)P.cls0 * BC_NCLS + i];
-    gacc[i] = real(0);
-  }
+  for (int i = tid; i < P.n_cls * BC_NCLS; i += NT) cls[i] = a.cls_tab[(int64_t)P.cls0 * BC_NCLS + i];
   for (int i = tid; i < P.n_rows; i += NT) {
     LO[i] = a.lo[(int64_t)P.row0 + i];
     UP[i] = a.up[(int64_t)P.row0 + i];
@@ -505,7 +503,7 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
     }
   }
 
-  // carried adjoints out; parameter gradients reduced per class
+  // carried adjoints out
 #pragma unroll
   for (int i = 0; i < MAXC; ++i) {
     if (info[i] == kNone) continue;
@@ -514,15 +512,44 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
 #pragma unroll
     for (int s = 0; s < NS; ++s)
       if (s < bc_nstates(CH)) a.Sbar[(int64_t)s * ncv + g] = sb[i][s];
-    const int c = (int)((info[i] >> 16) & 0x3FFFu);
-    atomicAdd(&gacc[c * BC_NCLS + BC_CLS_G_NA], gg[i][0]);
-    atomicAdd(&gacc[c * BC_NCLS + BC_CLS_G_KDR], gg[i][1]);
-    atomicAdd(&gacc[c * BC_NCLS + BC_CLS_G_IL], gg[i][2]);
-    atomicAdd(&gacc[c * BC_NCLS + BC_CLS_G_CAL], gg[i][3]);
   }
-  __syncthreads();
-  for (int i = tid; i < P.n_cls * BC_NCLS; i += NT)
-    if (gacc[i] != real(0)) atomicAdd(&a.cls_grad[(int64_t)P.cls0 * BC_NCLS + i], gacc[i]);
+  // parameter gradients per mechanism class, reduced in a fixed order (the
+  // result does not depend on scheduling): every thread sums its own CVs in
+  // register order, a butterfly sums the lanes of a wavefront, thread 0 adds
+  // the wavefronts in order.  A class belongs to one pack and segment
+  // launches run in order, so thread 0's update of its class is the only one.
+  const int lane = tid & 63, wave = tid >> 6, n_waves = NT >> 6;
+  constexpr int gcol[NG] = {BC_CLS_G_NA, BC_CLS_G_KDR, BC_CLS_G_IL, BC_CLS_G_CAL};
+  for (int c = 0; c < P.n_cls; ++c) {
+    real sm[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) sm[k] = real(0);
+#pragma unroll
+    for (int i = 0; i < MAXC; ++i) {
+      if (info[i] != kNone && (int)((info[i] >> 16) & 0x3FFFu) == c) {
+#pragma unroll
+        for (int k = 0; k < NG; ++k) sm[k] += gg[i][k];
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+      for (int k = 0; k < NG; ++k) sm[k] += __shfl_xor(sm[k], off);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < NG; ++k) red[wave * NG + k] = sm[k];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int k = 0; k < NG; ++k) {
+        real t = red[k];
+        for (int w = 1; w < n_waves; ++w) t += red[w * NG + k];
+        a.cls_grad[(int64_t)(P.cls0 + c) * BC_NCLS + gcol[k]] += t;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 constexpr uint32_t kSetPassive = BC_CH_IL;
