@@ -312,17 +312,67 @@ def bench_point(args):
         "cpu_baseline": cpu}), flush=True)
 
 
+def _oracle_adjoint_cell(job):
+    """One region tree through the FP64 oracle: forward storing every step's
+    state, then the reverse pass for a soma-trace loss (CPU baseline worker)."""
+    index, n_steps, dt = job
+    from threadpoolctl import threadpool_limits
+
+    from braincell_amd import workloads as W
+    from braincell_amd.layout import discretize_cell
+    from oracle.adjoint import adjoint, forward_states
+    from oracle.staggered import StaggeredOracle
+
+    with threadpool_limits(1):
+        m = W.region_tree_cell(index)
+        geom = discretize_cell(m.morph, m.n_cv_branch(), m.per_branch(m.cm), m.per_branch(m.ra))
+        soma = int(geom.locate(0, 0.5, m.n_cv_branch())[1])
+        o = StaggeredOracle(m)
+        o.reset_state(m.V0)
+        o2 = StaggeredOracle(m)
+        o2.reset_state(m.V0)
+        t0 = time.perf_counter()
+        states = forward_states(o, dt, n_steps)
+        adjoint(o2, states, dt, np.random.default_rng(index).standard_normal(n_steps), soma)
+        return len(o.V), time.perf_counter() - t0
+
+
+def adjoint_cpu_baseline(n_cells, n_steps, dt):
+    """FP64 oracle forward + reverse pass, one region tree per process over
+    the host cores this process may use (at most 16).  Runs before the GPU is
+    touched (the worker processes are forked)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+
+    workers = max(1, min(16, len(os.sched_getaffinity(0)), n_cells))
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
+        res = list(ex.map(_oracle_adjoint_cell, [(i, n_steps, dt) for i in range(n_cells)]))
+    el = time.perf_counter() - t0
+    n_cv = sum(r[0] for r in res)
+    return {"value": float(f"{n_cv * n_steps / el:.4g}"), "unit": "compartment-steps/s", "cores": workers,
+            "kind": "port",
+            "sample": f"FP64 NumPy oracle forward (every state stored) + reverse pass, {n_cells} region trees "
+                      f"({n_cv} CVs) x {n_steps} steps, one tree per process on {workers} processes, {el:.1f}s"}
+
+
 def bench_adjoint(args):
     """512 region trees (~1000 CVs), FP32, 200 ms: MSE loss against a target
     trace from hidden per-region densities; forward with checkpoints every
-    200 steps plus the hand-written adjoint."""
+    200 steps plus the hand-written adjoint.  The reverse launch is the
+    dominant kernel: its roofline counts the recorded state it reads (one
+    FP32 value per state variable and V per compartment-step)."""
+    cpu = None
+    if not args.no_cpu:
+        cpu = adjoint_cpu_baseline(64, 1000, 0.025)
+        log(f"adjoint CPU baseline {cpu}")
     import torch
     from braincell_amd import workloads as W
-    from braincell_amd.adjoint import AdjointRuntime
+    from braincell_amd.adjoint import GRAD_CHANNELS, AdjointRuntime
 
     n_cells, dt, T = 512, 0.025, 8000
     rng = np.random.default_rng(7)
-    base = np.asarray([W.BASE_DENSITIES[k] for k in ("Na_Ba2002", "KDR_Ba2002", "IL", "CaL_IS2008")])
+    base = np.asarray([W.BASE_DENSITIES[k] for k in GRAD_CHANNELS])
     hidden = base[None, None, :] * np.exp(0.3 * rng.standard_normal((n_cells, 3, 4)))
     models = [W.region_tree_cell(i) for i in range(n_cells)]
     t0 = time.perf_counter()
@@ -336,11 +386,24 @@ def bench_adjoint(args):
     t1 = time.perf_counter()
     tr = adj.forward(T)
     bar = 2.0 * (tr - target) / tr.numel()
-    g = adj.theta_grad(adj.backward(bar))
+    evs = []
+    g = adj.theta_grad(adj.backward(bar, events=evs))
     torch.cuda.synchronize()
     el = time.perf_counter() - t1
     ncv = adj.L.n_cv_total
     val = ncv * T / el
+    # dominant kernel: the reverse launches (adj.K steps each)
+    durs = [a.elapsed_time(b) * 1e-3 for a, b in evs]
+    launch_s = float(np.mean(durs))
+    n_rec = adj.ns + 1  # V and every state variable, FP32
+    bytes_cv_step = 4.0 * n_rec
+    gbs = bytes_cv_step * ncv * adj.K / launch_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "stagger_adjoint_kernel<float, 279u, 4>", "launch_ms": round(launch_s * 1e3, 3),
+            "steps_per_launch": adj.K, "bytes_per_cv_step": bytes_cv_step}
+    gn = g.double().cpu().numpy()
+    sens = gn * theta.numpy()  # d loss / d log(density)
     print(json.dumps({
         "metric": "compartment-steps/s (forward + adjoint)", "value": float(f"{val:.4g}"),
         "unit": "compartment-steps/s", "n_gpus": 1, "steps": T, "warmup": 0, "ms_per_step": round(el / T * 1e3, 5),
@@ -348,7 +411,12 @@ def bench_adjoint(args):
         "config": {"workload": "512 region trees (250 branches, ~1000 CVs), Na/KDR/IL/CaL + CalciumDetailed, dt 0.025 ms, "
                                "200 ms, MSE(soma V) vs hidden lognormal per-region densities, checkpoint every 200 steps",
                    "cells": n_cells, "compartments": int(ncv), "params_per_cell": 12},
-        "gradient_norm": float(g.norm().item()), "loss": float(((tr - target) ** 2).mean().item())}), flush=True)
+        "roofline": roof, "cpu_baseline": cpu,
+        "gradient_norm": float(np.linalg.norm(gn)),
+        "gradient_norm_by_channel": {nm: float(f"{np.linalg.norm(gn[..., k]):.4g}") for k, nm in enumerate(GRAD_CHANNELS)},
+        "log_density_gradient_norm": float(f"{np.linalg.norm(sens):.4g}"),
+        "log_density_gradient_max_cell_share": float(f"{(np.linalg.norm(sens, axis=(1, 2)) ** 2).max() / (sens ** 2).sum():.4g}"),
+        "loss": float(((tr - target) ** 2).mean().item())}), flush=True)
 
 
 def bench_closure(args):

@@ -80,7 +80,10 @@ class AdjointRuntime:
         return trace
 
     # ------------------------------------------------------------- backward
-    def backward(self, trace_bar):
+    def backward(self, trace_bar, events=None):
+        """Reverse pass for the loss gradient ``trace_bar`` (d loss / d trace);
+        returns the class-table gradient.  ``events``, a list, receives a
+        (start, end) pair of timing events around every reverse launch."""
         t = self.torch
         rt = self.rt
         ncv = self.L.n_cv_total
@@ -106,7 +109,13 @@ class AdjointRuntime:
             d.rec = rec.data_ptr()
             d.trace_bar = trace_bar[start:start + k].data_ptr()
             d.Vbar, d.Sbar, d.cls_grad = Vbar.data_ptr(), Sbar.data_ptr(), gcls.data_ptr()
+            if events is not None:
+                ev = (t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True))
+                ev[0].record()
             check(lib().bc_stagger_adjoint(ctypes.byref(d), rt.stream_handle()))
+            if events is not None:
+                ev[1].record()
+                events.append(ev)
         rt.V.copy_(V0)
         rt.S.copy_(S0)
         rt.Vbnd.copy_(B0)
