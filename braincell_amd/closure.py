@@ -34,7 +34,7 @@ from .runtime import StaggerLayout, StaggerRuntime
 from .workloads import CLAMP_FOREVER
 
 __all__ = ["NPARAM", "ClosureNet", "ClosureData", "generate_data", "train", "ClosureStepper", "ClosureBatch",
-           "with_clamp"]
+           "with_clamp", "soma_rmse", "spike_times_ms", "spike_error"]
 
 NPARAM = 5061
 NIN, NH, NOUT = 8, 64, 5
@@ -66,7 +66,8 @@ class _FwdDesc(ctypes.Structure):
 
 class _ApplyDesc(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("ld", ctypes.c_int64), ("dt", ctypes.c_double), ("chan_const", ctypes.c_void_p),
-                ("V1", ctypes.c_void_p), ("S", ctypes.c_void_p), ("P", ctypes.c_void_p), ("norm", ctypes.c_void_p)]
+                ("V1", ctypes.c_void_p), ("S", ctypes.c_void_p), ("P", ctypes.c_void_p), ("norm", ctypes.c_void_p),
+                ("flags", ctypes.c_void_p), ("flag_tol", ctypes.c_double)]
 
 
 SKIP_GATES = 1  # BC_STEP_SKIP_GATES
@@ -154,17 +155,31 @@ class ClosureData:
 
 
 def generate_data(models, *, dt_fine: float = 0.005, dt_coarse: float = 0.1, n_pairs: int = 100,
-                  warm_ms: float = 5.0, gap_ms: float = 0.5, threads: int = 256) -> ClosureData:
+                  warm_ms: float = 5.0, gap_ms: float = 0.5, threads: int = 256,
+                  features: str = "coarse") -> ClosureData:
     """Training pairs from plain ``dt_fine`` trajectories of ``models``:
     after ``warm_ms``, ``n_pairs`` times: state S0, ``dt_coarse`` of fine
     steps, state S1, then ``gap_ms`` more.  One pair per compartment each
-    time (n_pairs x total CVs samples)."""
+    time (n_pairs x total CVs samples).
+
+    The targets are the fine end gates S1.  With ``features="coarse"`` the
+    network's voltage and calcium inputs are those the closure sees when it
+    runs: one coarse step (gates skipped) from S0, the same state the coarse
+    step hands to ``bc_closure_apply``; with ``"fine"`` they are the fine
+    trajectory's V and Ca at the step's end."""
     import torch
 
+    if features not in ("coarse", "fine"):
+        raise ValueError("features must be 'coarse' or 'fine'")
     L = StaggerLayout(models, threads=threads, probes="soma")
     if L.chset != CHSET:
         raise ValueError("the closure is trained for the Na/KDR/IL/CaL/CalciumDetailed model")
     rt = StaggerRuntime(L, dt_fine, precision=32)
+    co = None
+    if features == "coarse":
+        co = StaggerRuntime(L, dt_coarse, precision=32)
+        co.flags = SKIP_GATES
+        tc = co.new_trace(1)
     k = int(round(dt_coarse / dt_fine))
     gap = int(round(gap_ms / dt_fine))
     tr = rt.new_trace(max(k, gap, int(round(warm_ms / dt_fine)), 1))
@@ -172,14 +187,26 @@ def generate_data(models, *, dt_fine: float = 0.005, dt_coarse: float = 0.1, n_p
     if w:
         rt.run(w, trace=tr[:w])
     ncv = L.n_cv_total
+    ns = len(L.slots)
     X = torch.empty((n_pairs * ncv, NIN), dtype=torch.float32, device=rt.device)
     D = torch.empty((n_pairs * ncv, NOUT), dtype=torch.float32, device=rt.device)
     Y = torch.empty((n_pairs * ncv, NOUT), dtype=torch.float32, device=rt.device)
     for i in range(n_pairs):
         S0 = rt.S.clone()
+        if co is not None:
+            co.V.copy_(rt.V)
+            co.S.copy_(rt.S)
+            co.Vbnd.copy_(rt.Vbnd)
+            co.step_index = int(round(rt.step_index * dt_fine / dt_coarse))
+            co.run(1, trace=tc)
         rt.run(k, trace=tr[:k])
-        d = _DataDesc(ncv, ncv, float(dt_coarse), ctypes.cast(ctypes.pointer(rt._cc), ctypes.c_void_p), _ptr(rt.V),
-                      _ptr(S0), _ptr(rt.S), ctypes.c_void_p(X.data_ptr() + i * ncv * NIN * 4),
+        V1, S1 = rt.V, rt.S
+        if co is not None:
+            V1 = co.V
+            S1 = rt.S.clone()
+            S1.view(ns, ncv)[NOUT].copy_(co.S.view(ns, ncv)[NOUT])  # Ca input from the coarse step
+        d = _DataDesc(ncv, ncv, float(dt_coarse), ctypes.cast(ctypes.pointer(rt._cc), ctypes.c_void_p), _ptr(V1),
+                      _ptr(S0), _ptr(S1), ctypes.c_void_p(X.data_ptr() + i * ncv * NIN * 4),
                       ctypes.c_void_p(D.data_ptr() + i * ncv * NOUT * 4),
                       ctypes.c_void_p(Y.data_ptr() + i * ncv * NOUT * 4))
         check(lib().bc_closure_data(ctypes.byref(d), _stream()))
@@ -187,6 +214,8 @@ def generate_data(models, *, dt_fine: float = 0.005, dt_coarse: float = 0.1, n_p
             rt.run(gap, trace=tr[:gap])
     torch.cuda.synchronize()
     rt.check_status()
+    if co is not None:
+        co.check_status()
     return ClosureData(X, D, Y)
 
 
@@ -258,16 +287,51 @@ class ClosureStepper:
     the gates (``BC_STEP_SKIP_GATES``), then ``bc_closure_apply`` performs
     the gate update (plain exp-Euler + network correction) in place."""
 
-    def __init__(self, models, dt: float, net: ClosureNet, *, threads: int = 256, layout=None):
+    def __init__(self, models, dt: float, net: ClosureNet, *, threads: int = 256, layout=None,
+                 flag_tol: float = 0.05):
         self.L = layout or StaggerLayout(models, threads=threads, probes="soma")
         if self.L.chset != CHSET:
             raise ValueError("the closure applies to the Na/KDR/IL/CaL/CalciumDetailed model")
         self.rt = StaggerRuntime(self.L, dt, precision=32)
         self.net = net
         self.dt = float(dt)
+        import torch
+
+        # per compartment: 1 once the network over-corrected a gate there
+        self.over = torch.zeros(self.L.n_cv_total, dtype=torch.uint8, device=self.rt.device)
+        self.flag_tol = float(flag_tol)
 
     def reset_state(self):
         self.rt.reset_state()
+        self.over.zero_()
+
+    def unhealthy(self, v_range=(-150.0, 100.0), gate_tol: float = 1e-3):
+        """Per cell: True when, since the last call, the network
+        over-corrected a gate of one of its compartments (a corrected gate
+        beyond ``flag_tol`` outside [0, 1] before the clip), or now a
+        compartment's V is outside ``v_range`` or not finite, a gate is
+        outside [-gate_tol, 1 + gate_tol] or the calcium concentration is not
+        positive.  One device reduction; clears the over-correction flags."""
+        import torch
+
+        L, rt = self.L, self.rt
+        if getattr(self, "_cell_of_cv", None) is None:
+            cell = np.zeros(L.n_cv_total, dtype=np.int64)
+            for i, (c0, c) in enumerate(zip(L.cell_cv0, L.cell_ncv)):
+                cell[int(c0):int(c0 + c)] = i
+            self._cell_of_cv = torch.as_tensor(cell, device=rt.device)
+        V = rt.V
+        S = rt.S.view(len(L.slots), L.n_cv_total)
+        bad = ~torch.isfinite(V) | (V < v_range[0]) | (V > v_range[1])
+        gates = S[:NOUT]
+        bad |= (~torch.isfinite(gates) | (gates < -gate_tol) | (gates > 1.0 + gate_tol)).any(0)
+        ca = S[NOUT]
+        bad |= ~torch.isfinite(ca) | (ca <= 0.0)
+        bad |= self.over.bool()
+        self.over.zero_()
+        per_cell = torch.zeros(len(L.cells), dtype=torch.int32, device=rt.device)
+        per_cell.index_add_(0, self._cell_of_cv, bad.to(torch.int32))
+        return (per_cell > 0).cpu().numpy()
 
     def run(self, n_steps: int, trace=None, closure: bool = True):
         rt = self.rt
@@ -278,7 +342,8 @@ class ClosureStepper:
             rt.flags = 0
             return rt.run(n_steps, trace=trace, chunk=200)
         cc = ctypes.cast(ctypes.pointer(rt._cc), ctypes.c_void_p)
-        d = _ApplyDesc(ncv, ncv, self.dt, cc, _ptr(rt.V), _ptr(rt.S), _ptr(self.net.P), _ptr(self.net.norm))
+        d = _ApplyDesc(ncv, ncv, self.dt, cc, _ptr(rt.V), _ptr(rt.S), _ptr(self.net.P), _ptr(self.net.norm),
+                       _ptr(self.over), self.flag_tol)
         rt.flags = SKIP_GATES
         try:
             for i in range(n_steps):
@@ -297,20 +362,54 @@ def soma_rmse(coarse, fine, ratio: int):
     return np.sqrt(np.mean((coarse[:n] - f[:n]) ** 2, axis=0))
 
 
+def spike_times_ms(col, dt: float, v_th: float = 0.0):
+    """Threshold crossings of one trace column (row i = V after step i):
+    V_{i-1} < v_th <= V_i is a spike at (i + 1) dt, as the reference's
+    ``get_spike`` marks the step (``braincell/_base.py:421-431``)."""
+    v = np.asarray(col)
+    i = np.nonzero((v[:-1] < v_th) & (v[1:] >= v_th))[0] + 1
+    return (i + 1) * dt
+
+
+def spike_error(coarse, fine, dt_coarse: float, dt_fine: float, v_th: float = 0.0):
+    """Per cell: (spike count of the fine trace, largest spike-time error in
+    ms; inf when the counts differ)."""
+    n_c = coarse.shape[0]
+    counts, err = [], []
+    for j in range(coarse.shape[1]):
+        tc = spike_times_ms(coarse[:, j], dt_coarse, v_th)
+        tf = spike_times_ms(fine[: int(round(n_c * dt_coarse / dt_fine)), j], dt_fine, v_th)
+        counts.append(len(tf))
+        err.append(float(np.max(np.abs(tc - tf))) if len(tc) == len(tf) and len(tf) else
+                   (0.0 if len(tc) == len(tf) else np.inf))
+    return np.asarray(counts), np.asarray(err)
+
+
 class ClosureBatch:
     """Closure at a coarse dt where the plain fine method accepts it, plain
-    ``fallback_dt`` elsewhere.  ``calibrate`` decides per cell: closure and
-    plain ``judge_dt`` run ``window_ms``; cells with soma RMSE above
-    ``threshold_mV`` fall back."""
+    ``fallback_dt`` elsewhere.
+
+    ``calibrate`` judges every cell: closure and plain ``judge_dt`` run
+    ``window_ms``; a cell is accepted when its soma RMSE is at most
+    ``threshold_mV`` and it has the fine run's spikes, each within
+    ``spike_tol_ms``.  ``run`` watches the accepted cells while they run:
+    every ``check_every`` coarse steps it reads their state, and a cell
+    whose voltage, gates or calcium leave the physical range (or are not
+    finite) moves to the plain fallback for the whole run.  A non-finite
+    value in a plain fallback run raises ``NativeError``."""
 
     def __init__(self, models, net: ClosureNet, *, dt: float = 0.1, fallback_dt: float = 0.025,
-                 judge_dt: float = 0.005, threshold_mV: float = 0.5):
+                 judge_dt: float = 0.005, threshold_mV: float = 0.5, spike_tol_ms: float = None,
+                 v_range=(-150.0, 100.0)):
         self.models = list(models)
         self.net = net
         self.dt, self.fallback_dt, self.judge_dt = float(dt), float(fallback_dt), float(judge_dt)
         self.threshold = float(threshold_mV)
+        self.spike_tol = float(self.dt if spike_tol_ms is None else spike_tol_ms)
+        self.v_range = (float(v_range[0]), float(v_range[1]))
         self.accept = np.ones(len(self.models), dtype=bool)
-        self.rmse = None
+        self.rmse = self.spike_err = self.spike_count = None
+        self.moved = []  # (cell index, simulated ms of the check that moved it)
 
     def calibrate(self, window_ms: float = 20.0):
         import torch
@@ -324,23 +423,48 @@ class ClosureBatch:
         rt = StaggerRuntime(L, self.judge_dt, precision=32)
         fine = rt.run(n * ratio, chunk=2000).double().cpu().numpy()
         torch.cuda.synchronize()
+        rt.check_status()
         cols = [int(L.cell_probe0[i]) for i in range(len(self.models))]
         self.rmse = soma_rmse(coarse[:, cols], fine[:, cols], ratio)
-        self.accept = self.rmse <= self.threshold
+        self.spike_count, self.spike_err = spike_error(coarse[:, cols], fine[:, cols], self.dt, self.judge_dt)
+        self.accept = (self.rmse <= self.threshold) & (self.spike_err <= self.spike_tol)
         return self.accept
 
-    def run(self, duration_ms: float):
+    def run(self, duration_ms: float, check_every: int = 50):
         """Returns (traces of accepted cells at dt, traces of fallback cells
-        at fallback_dt, accepted indices, fallback indices)."""
+        at fallback_dt, accepted indices, fallback indices); cells moved
+        while running are listed in ``self.moved`` and their traces come
+        from the fallback run."""
+        import torch
+
         acc = np.nonzero(self.accept)[0]
-        rej = np.nonzero(~self.accept)[0]
-        out_a = out_r = None
+        self.moved = []
+        out_a = None
         if acc.size:
             st = ClosureStepper([self.models[i] for i in acc], self.dt, self.net)
             st.reset_state()
-            out_a = st.run(int(round(duration_ms / self.dt)))
+            n = int(round(duration_ms / self.dt))
+            out_a = st.rt.new_trace(n)
+            alive = np.ones(acc.size, dtype=bool)
+            done = 0
+            while done < n:
+                k = min(int(check_every), n - done)
+                st.run(k, trace=out_a[done:done + k])
+                done += k
+                bad = st.unhealthy(self.v_range) & alive
+                for j in np.nonzero(bad)[0]:
+                    self.moved.append((int(acc[j]), round(done * self.dt, 6)))
+                alive &= ~bad
+                st.rt.status.zero_()  # cells that went non-finite have just been moved
+            cols = [int(st.L.cell_probe0[j]) for j in np.nonzero(alive)[0]]
+            out_a = out_a[:, cols] if cols else None
+            acc = acc[alive]
+        rej = np.setdiff1d(np.arange(len(self.models)), acc)
+        out_r = None
         if rej.size:
             L = StaggerLayout([self.models[i] for i in rej], probes="soma")
             rt = StaggerRuntime(L, self.fallback_dt, precision=32)
             out_r = rt.run(int(round(duration_ms / self.fallback_dt)), chunk=200)
+            torch.cuda.synchronize()
+            rt.check_status()
         return out_a, out_r, acc, rej

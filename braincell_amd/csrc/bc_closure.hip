@@ -412,7 +412,8 @@ __global__ void __launch_bounds__(256) closure_fwd_mfma(int64_t n, const float* 
 // 0, 1) with delta the plain increment.
 __global__ void __launch_bounds__(256) closure_apply(int64_t n, float dt, bc_chan_const<float> cc,
                                                      const float* __restrict__ V1, float* __restrict__ S, int64_t ld,
-                                                     const float* __restrict__ P, const float* __restrict__ norm) {
+                                                     const float* __restrict__ P, const float* __restrict__ norm,
+                                                     uint8_t* __restrict__ flags, float flag_tol) {
   __shared__ float Ps[NPARAM];
   __shared__ float cs[NH + NOUT];
   for (int p = threadIdx.x; p < NPARAM; p += blockDim.x) Ps[p] = P[p];
@@ -473,9 +474,14 @@ __global__ void __launch_bounds__(256) closure_apply(int64_t n, float dt, bc_cha
       }
     }
     if (ok) {
+      bool over = false;
 #pragma unroll
-      for (int k = 0; k < NOUT; ++k)
-        S[k * ld + s] = fminf(fmaxf(plain[k] + (plain[k] - x0[k]) * (c[k] * sc[k]), 0.0f), 1.0f);
+      for (int k = 0; k < NOUT; ++k) {
+        const float raw = plain[k] + (plain[k] - x0[k]) * (c[k] * sc[k]);
+        over |= !(raw >= -flag_tol && raw <= 1.0f + flag_tol);  // also true for NaN
+        S[k * ld + s] = fminf(fmaxf(raw, 0.0f), 1.0f);
+      }
+      if (over && flags != nullptr) flags[s] = 1;
     }
   }
 }
@@ -616,7 +622,7 @@ int bc_closure_apply(const bc_closure_apply_desc* d, void* stream) {
                      static_cast<hipStream_t>(stream), d->n, (float)d->dt,
                      *static_cast<const bc_chan_const<float>*>(d->chan_const), static_cast<const float*>(d->V1),
                      static_cast<float*>(d->S), d->ld, static_cast<const float*>(d->P),
-                     static_cast<const float*>(d->norm));
+                     static_cast<const float*>(d->norm), static_cast<uint8_t*>(d->flags), (float)d->flag_tol);
   BC_HIP(hipGetLastError());
   return bc::ok();
 }

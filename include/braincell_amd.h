@@ -411,7 +411,10 @@ int bc_closure_forward(const bc_closure_fwd_desc* desc, void* stream);
  * ([6][ld]; gates still at their step-start values, Ci updated).  Each gate
  * x becomes clip(x + delta (1 + c), 0, 1): delta = its plain exp-Euler
  * increment at V1, c = the BF16-MFMA network output * out_scale.
- * chan_const: host pointer to bc_chan_const<float>. */
+ * chan_const: host pointer to bc_chan_const<float>.  flags (optional, uint8
+ * [n]): set to 1 for a compartment whose corrected gate, before the clip,
+ * lies outside [-flag_tol, 1 + flag_tol] or is not finite (the network
+ * over-corrects there); never cleared by the kernel. */
 typedef struct bc_closure_apply_desc {
   int64_t n, ld;
   double dt;
@@ -420,6 +423,8 @@ typedef struct bc_closure_apply_desc {
   void* S;
   const void* P;
   const void* norm;
+  void* flags;
+  double flag_tol;
 } bc_closure_apply_desc;
 int bc_closure_apply(const bc_closure_apply_desc* desc, void* stream);
 

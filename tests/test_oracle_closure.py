@@ -56,3 +56,27 @@ def test_features_layout():
     f = oc.features(np.array([-60.0, -50.0]), np.zeros((5, 2)) + 0.5, np.array([1e-4, 2e-4]), 0.1)
     assert f.shape == (2, 8)
     assert np.allclose(f[:, 0], [-60, -50]) and np.allclose(f[:, 6], np.log([1e-4, 2e-4])) and np.all(f[:, 7] == 0.1)
+
+
+def test_closure_judge_spike_timing_and_rmse_helpers():
+    """The judge's helpers on hand-made traces: a spike at step i of a trace
+    with step dt is at (i + 1) dt; spike errors compare counts first."""
+    from braincell_amd.closure import soma_rmse, spike_error, spike_times_ms
+
+    v = np.full(10, -70.0)
+    v[3] = 10.0  # V_2 < 0 <= V_3: step 3 -> (3 + 1) dt
+    assert np.allclose(spike_times_ms(v, 0.1), [0.4])
+    fine = np.full((200, 3), -70.0)
+    fine[79:, 0] = 20.0  # crossing at row 79 -> 0.4 ms at dt 0.005
+    fine[59:, 1] = 20.0  # 0.3 ms
+    coarse = np.full((10, 3), -70.0)
+    coarse[3:, 0] = 20.0  # 0.4 ms: exact
+    coarse[4:, 1] = 20.0  # 0.5 ms: 0.2 ms late
+    coarse[2:, 2] = 20.0  # a spike the fine trace does not have
+    cnt, err = spike_error(coarse, fine, 0.1, 0.005)
+    assert cnt.tolist() == [1, 1, 0]
+    assert np.allclose(err[:2], [0.0, 0.2]) and np.isinf(err[2])
+    rm = soma_rmse(coarse, fine, 20)
+    ref = np.sqrt(np.mean((coarse - fine[19::20]) ** 2, axis=0))
+    assert np.allclose(rm, ref)
+    assert rm[0] == 0.0
