@@ -613,6 +613,8 @@ __host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int ma
   return p;
 }
 
+constexpr int kSynPar = BC_SYN_NPAR;  // parameters per synapse (bc_synapse_desc.par)
+
 // Synapse pass (network runs), after phase A has written every row: each
 // synapse takes its due arrivals (a gather over its incoming contacts from
 // the spike history of earlier launches), adds its conductance and current
@@ -636,15 +638,38 @@ __device__ __forceinline__ void synapse_pass(const StaggerArgs<real>& a, HRow<re
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) drive += __shfl_xor(drive, off);
-      const real* q = a.syn_par + (int64_t)s * 6;
-      const real A = a.syn_state[2 * (int64_t)s] + drive * q[2];
-      const real B = a.syn_state[2 * (int64_t)s + 1] + drive * q[3];
-      const real kg = q[0] * (B - A);
-      gs += kg;
-      gse += kg * q[1];
+      const real* q = a.syn_par + (int64_t)s * kSynPar;
+      const int kind = (int)q[2];  // uniform over the wavefront
+      real A = a.syn_state[2 * (int64_t)s];
+      real B = a.syn_state[2 * (int64_t)s + 1];
+      if (kind == BC_SYN_EXP) {
+        // events jump A and B; the solve sees g = B - A; both then decay
+        A += drive * q[3];
+        B += drive * q[4];
+        const real kg = q[0] * (B - A);
+        gs += kg;
+        gse += kg * q[1];
+        A -= q[5] * A;
+        B -= q[6] * B;
+      } else {
+        // the solve sees g = B at its step-start value; the events of the
+        // step drive the ODEs, integrated from the step-start states
+        const real kg = q[0] * B;
+        gs += kg;
+        gse += kg * q[1];
+        if (kind == BC_SYN_MARKOV1) {  // g' = aT D (1 - g) - beta g
+          const real u = q[3] * drive;
+          B = bc_expeuler<real>(B, u * (real(1) - B) - q[4] * B, -(u + q[4]), a.dt);
+        } else {  // NMDA: g' = a1 x (1 - g) - b1 g, x' = a2T D (1 - x) - b2 x (A = x, B = g)
+          const real u = q[5] * drive;
+          const real gn = bc_expeuler<real>(B, q[3] * A * (real(1) - B) - q[4] * B, -(q[3] * A + q[4]), a.dt);
+          A = bc_expeuler<real>(A, u * (real(1) - A) - q[6] * A, -(u + q[6]), a.dt);
+          B = gn;
+        }
+      }
       if (lane == 0) {
-        a.syn_state[2 * (int64_t)s] = A - q[4] * A;
-        a.syn_state[2 * (int64_t)s + 1] = B - q[5] * B;
+        a.syn_state[2 * (int64_t)s] = A;
+        a.syn_state[2 * (int64_t)s + 1] = B;
       }
     }
     if (lane == 0) {

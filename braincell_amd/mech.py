@@ -163,21 +163,36 @@ class CurrentProbe:
 _SYNAPSE_DEFAULTS = {
     "ExpSyn": {"tau": 0.1, "e": 0.0, "weight": 1.0},
     "Exp2Syn": {"tau1": 0.1, "tau2": 10.0, "e": 0.0, "weight": 1.0},
+    # synapse/markov.py:227-236, 271-280, 315-326 (rates 1/ms, g_max mS/cm^2, E_rev mV)
+    "AMPA": {"alpha": 0.98, "beta": 0.18, "T": 0.5, "g_max": 1.0, "E_rev": 0.0},
+    "GABAa": {"alpha": 0.53, "beta": 0.18, "T": 1.0, "g_max": 1.0, "E_rev": -70.0},
+    "NMDA": {"alpha1": 2.0, "beta1": 0.01, "alpha2": 1.0, "beta2": 0.5, "T": 1.0, "g_max": 1.0, "E_rev": 0.0},
 }
+# kinds of the kernel's synapse update (bc_synapse_desc.par[2])
+SYN_EXP, SYN_MARKOV1, SYN_MARKOV2 = 0, 1, 2
 
 
 class Synapse:
     """Registry-keyed synapse placed with ``Cell.place`` (``mech/_point.py:
     389-440``): ``Synapse("ExpSyn", tau=2.0, e=0.0, weight=1.0, name="exp")``.
 
-    Types (``synapse/markov.py:65-200``, NEURON's expsyn / exp2syn):
+    Types (``synapse/markov.py:65-360``):
 
-    - ``ExpSyn``: g' = -g / tau, an event adds its weight to g;
-    - ``Exp2Syn``: A' = -A / tau1, B' = -B / tau2, g = B - A, an event adds
-      weight * factor to A and B (factor normalises the peak of g to the
-      weight; tau1 is kept in [1e-9, 0.9999] * tau2).
+    - ``ExpSyn`` (NEURON's expsyn): g' = -g / tau, an event adds its weight
+      to g; current g (V - e) in nA (outward positive);
+    - ``Exp2Syn`` (exp2syn): A' = -A / tau1, B' = -B / tau2, g = B - A, an
+      event adds weight * factor to A and B (factor normalises the peak of g
+      to the weight; tau1 is kept in [1e-9, 0.9999] * tau2);
+    - ``AMPA`` / ``GABAa``: g' = alpha D T (1 - g) - beta g, with D the
+      summed weighted events of the step (a transmitter pulse one step
+      long), current density g_max g (E_rev - V) in mS/cm^2 * mV (inward
+      positive);
+    - ``NMDA``: g' = alpha1 x (1 - g) - beta1 g, x' = alpha2 D T (1 - x) -
+      beta2 x, same current density.
 
-    The current is g (V - e) (outward positive).  A placement over several
+    The Markov types integrate with ind_exp_euler after the voltage step
+    (every state from the step-start values) and have no default weight:
+    their connections give one (dimensionless).  A placement over several
     locations is one pool; a contact's ``synapse_index`` picks the location.
     """
 
@@ -219,6 +234,26 @@ class Synapse:
         tp = (t1 * t2) / (t2 - t1) * math.log(t2 / t1)
         return 1.0 / (-math.exp(-(tp / t1)) + math.exp(-(tp / t2)))
 
+    @property
+    def markov(self) -> bool:
+        return self.synapse_type in ("AMPA", "GABAa", "NMDA")
+
+    def kernel_par(self, dt: float, cm: float, area_cm2: float):
+        """The kernel's 8 parameters of one synapse at a CV of membrane
+        capacitance ``cm`` (uF/cm^2) and area ``area_cm2``: k (dt/Cm times
+        the state -> mS/cm^2 conductance factor), reversal potential, kind
+        (SYN_EXP, SYN_MARKOV1, SYN_MARKOV2) and five kind-specific terms:
+        exponential (fa, fb, ka, kb, 0); one-state Markov (alpha T, beta,
+        0, 0, 0); NMDA (alpha1, beta1, alpha2 T, beta2, 0)."""
+        p = self.resolved()
+        if not self.markov:
+            e, fa, fb, ka, kb = self.kernel_terms(dt)
+            return (dt / cm * 1e-3 / area_cm2, e, float(SYN_EXP), fa, fb, ka, kb, 0.0)
+        k = dt / cm * p["g_max"]
+        if self.synapse_type == "NMDA":
+            return (k, p["E_rev"], float(SYN_MARKOV2), p["alpha1"], p["beta1"], p["alpha2"] * p["T"], p["beta2"], 0.0)
+        return (k, p["E_rev"], float(SYN_MARKOV1), p["alpha"] * p["T"], p["beta"], 0.0, 0.0, 0.0)
+
     def kernel_terms(self, dt: float):
         """(e, fa, fb, ka, kb): event gains of the states A and B and their
         exponential-Euler decay fractions over ``dt`` (y -= k y with k =
@@ -231,6 +266,8 @@ class Synapse:
             return dt * ex / tau
 
         p = self.resolved()
+        if self.markov:
+            raise ValueError(f"{self.synapse_type} is not an exponential synapse")
         if self.synapse_type == "ExpSyn":
             return p["e"], 0.0, 1.0, 0.0, k(p["tau"])
         f = self.factor()

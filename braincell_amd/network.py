@@ -34,6 +34,8 @@ from .mech import StateProbe
 
 __all__ = ["Population", "Connection", "NetworkRunResult", "Network", "delay_steps"]
 
+SYN_NPAR = 8  # kernel parameters per synapse (bc_synapse_desc.par)
+
 SPIKE_RECORDINGS = ("population", "none")
 
 
@@ -98,9 +100,10 @@ class Population:
 class Connection:
     """Contact table between two populations (``core.py:82-165``): contact c
     sends the spikes of ``pre_index[c]`` to location ``synapse_index[c]`` of
-    the synapse pool ``synapse`` on cell ``post_index[c]``.  ``weight`` (uS,
-    scalar or per contact; None: the pool's default weight) and ``delay``
-    (ms, scalar or per contact)."""
+    the synapse pool ``synapse`` on cell ``post_index[c]``.  ``weight`` (uS
+    for ExpSyn / Exp2Syn, dimensionless for AMPA / GABAa / NMDA; scalar or
+    per contact; None: the pool's default weight, which the Markov types do
+    not have) and ``delay`` (ms, scalar or per contact)."""
 
     pre_population: str
     post_population: str
@@ -271,16 +274,14 @@ class _NetworkTables:
         recs = []  # (pack, row, cell, pool, site, par)
         for gid, c in enumerate(L.cells):
             for (name, syn, loc) in c.synapses or []:
-                e, fa, fb, ka, kb = syn.kernel_terms(dt)
+                w0 = syn.resolved().get("weight", np.nan)  # Markov types: no default weight
                 for j, (cv, area, cm) in enumerate(loc):
                     row = int(L.cv_info[L.cell_cv0[gid] + cv] & 0xFFFF)
-                    k = dt / cm * 1e-3 / area  # dt/Cm times uS -> mS/cm^2
-                    recs.append((int(L.cell_pack[gid]), row, gid, name, j, (k, e, fa, fb, ka, kb),
-                                 syn.resolved()["weight"]))
+                    recs.append((int(L.cell_pack[gid]), row, gid, name, j, syn.kernel_par(dt, cm, area), w0))
         recs.sort(key=lambda r: (r[0], r[1]))
         self.syn_key = {(r[2], r[3], r[4]): s for s, r in enumerate(recs)}
         self.syn_weight = np.asarray([r[6] for r in recs], dtype=np.float64)
-        self.syn_par = np.asarray([r[5] for r in recs], dtype=np.float64).reshape(-1, 6)
+        self.syn_par = np.asarray([r[5] for r in recs], dtype=np.float64).reshape(-1, SYN_NPAR)
         pack = np.asarray([r[0] for r in recs], dtype=np.int64)
         row = np.asarray([r[1] for r in recs], dtype=np.int64)
         new_row = np.ones(len(recs), dtype=bool)
@@ -306,6 +307,9 @@ class _NetworkTables:
                     raise ValueError(f"synapse_index {int(con.synapse_index[c_])} out of range for pool "
                                      f"{con.synapse!r} ({pool_of} locations)")
                 s[c_] = self.syn_key[key]
+            if con.weight is None and np.isnan(self.syn_weight[s]).any():
+                raise ValueError(f"Connection weight is None, but synapse pool {con.synapse!r} has no default "
+                                 "'weight' parameter.")
             weight = self.syn_weight[s] if con.weight is None else con.per_contact(con.weight, "weight")
             tgt.append(s)
             pre.append(pre_gid)

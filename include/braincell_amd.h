@@ -93,18 +93,31 @@ typedef struct bc_pack {
  * real = float (precision 32) or double (precision 64). */
 /* Synapses and delayed spike delivery for a network run (the synaptic step
  * either side of the cell update: braincell/network/engine.py:474-548,
- * delivery.py:317-410; ExpSyn / Exp2Syn, braincell/synapse/markov.py:65-200).
- * Every synapse has two exponential states A, B (uS) and conductance
- * g = B - A; ExpSyn uses B alone (fa = 0).  At the start of step n a synapse
- * receives drive = sum over its incoming contacts c of w_c * spike(pre_c,
- * n - delay_c) and sets A += fa drive, B += fb drive; the voltage solve then
- * sees the current g (V - e) at its CV (conductance k g in the row, k =
- * dt / (Cm area) in the kernel's units), and afterwards A and B decay by
- * exponential Euler (A -= ka A, ka = dt exprel(-dt / tau_a) / tau_a).  A
- * cell spikes at step n when any of its CVs crosses v_th upwards (V_n <
- * v_th <= V_{n+1}); the kernel writes that into the history ring spk.
- * Every launch must satisfy n_steps <= min_delay (each step reads spikes of
- * earlier launches only) and spk_len >= max_delay + n_steps. */
+ * delivery.py:317-410; braincell/synapse/markov.py:65-360).  At the start of
+ * step n a synapse receives drive D = sum over its incoming contacts c of
+ * w_c * spike(pre_c, n - delay_c).  Every synapse has two states A, B and a
+ * kind (par[2]):
+ *  BC_SYN_EXP (ExpSyn / Exp2Syn): A, B in uS, g = B - A; A += fa D,
+ *    B += fb D; the voltage solve sees g; then A -= ka A, B -= kb B
+ *    (ka = dt exprel(-dt / tau_a) / tau_a).  ExpSyn: fa = 0.
+ *  BC_SYN_MARKOV1 (AMPA / GABAa): B = g (dimensionless), the solve sees
+ *    g_n; then g' = aT D (1 - g) - beta g by exponential Euler.
+ *  BC_SYN_MARKOV2 (NMDA): A = x, B = g; the solve sees g_n; then
+ *    g' = a1 x (1 - g) - b1 g and x' = a2T D (1 - x) - b2 x, both from the
+ *    step-start states (ind_exp_euler, quad/_exp_euler.py:330-470).
+ * The current g (e - V) enters the CV's row with conductance k g in the
+ * kernel's units (k = dt / (Cm area) uS -> mS/cm^2 for BC_SYN_EXP,
+ * dt g_max / Cm for the Markov kinds).  par per synapse: k, e, kind, then
+ * BC_SYN_EXP: fa, fb, ka, kb, 0; BC_SYN_MARKOV1: aT, beta, 0, 0, 0;
+ * BC_SYN_MARKOV2: a1, b1, a2T, b2, 0.  A cell spikes at step n when any of
+ * its CVs crosses v_th upwards (V_n < v_th <= V_{n+1}); the kernel writes
+ * that into the history ring spk.  Every launch must satisfy n_steps <=
+ * min_delay (each step reads spikes of earlier launches only) and spk_len
+ * >= max_delay + n_steps. */
+#define BC_SYN_EXP 0
+#define BC_SYN_MARKOV1 1
+#define BC_SYN_MARKOV2 2
+#define BC_SYN_NPAR 8
 typedef struct bc_synapse_desc {
   int32_t n_cells;       /* cells of the layout (columns of spk) */
   int32_t spk_len;       /* steps kept in the spike history ring */
@@ -114,12 +127,12 @@ typedef struct bc_synapse_desc {
   const int32_t* pack_off;  /* [n_packs + 1] synapse-row range of each pack */
   const int32_t* row;       /* [n_synrow] pack-local DHS row of a synapse row */
   const int32_t* row_off;   /* [n_synrow + 1] synapses of each synapse row */
-  const void* par;          /* [n_syn * 6] k, e, fa, fb, ka, kb */
+  const void* par;          /* [n_syn * BC_SYN_NPAR] see above */
   void* state;              /* [2 * n_syn] A, B */
   const int32_t* con_off;   /* [n_syn + 1] incoming contacts of each synapse */
   const int32_t* con_pre;   /* [n_con] presynaptic cell */
   const int32_t* con_delay; /* [n_con] delay in steps */
-  const void* con_w;        /* [n_con] weight (uS) */
+  const void* con_w;        /* [n_con] weight (uS; dimensionless for the Markov kinds) */
   const int32_t* cv_cell;   /* [n_cv_total] cell of each CV */
   uint8_t* spk;             /* [spk_len * n_cells] spike history ring */
 } bc_synapse_desc;

@@ -1,6 +1,6 @@
 """Oracle restatement (FP64) of a network run: populations of cells,
-ExpSyn / Exp2Syn synapses and delayed spike delivery through per-delay ring
-buffers.  Test infrastructure only.
+ExpSyn / Exp2Syn / AMPA / GABAa / NMDA synapses and delayed spike delivery
+through per-delay ring buffers.  Test infrastructure only.
 
 Per step n (``network/engine.py:474-548``):
 
@@ -9,6 +9,7 @@ Per step n (``network/engine.py:474-548``):
    cursor, sums it into the target pool's ``pre_spike`` and clears it;
 2. ``_begin_step`` -> ``apply_discrete_events`` (``synapse/markov.py:
    100-102, 180-184``): ExpSyn g += drive, Exp2Syn A, B += drive * factor;
+   the Markov types keep the drive for their derivative (``pre_drive``);
 3. the cell step (``StaggeredOracle.step``) with each synapse's current
    g (V - e) / area at its CV (``_multi_compartment/currents.py:138-163``),
    then the synapse states by ind_exp_euler (``cell.py:2506-2527``):
@@ -55,35 +56,75 @@ def _exp2_factor(tau1, tau2):
     return t1, 1.0 / (-math.exp(-(tp / t1)) + math.exp(-(tp / tau2)))
 
 
+def _ind_exp_euler(y, f, lam, dt):
+    """``quad/_exp_euler.py:455-463``: y + dt exprel(dt lam) f."""
+    return y + dt * exprel(dt * lam) * f
+
+
 class _Pool:
-    """One synapse pool of one cell: states per location."""
+    """One synapse pool of one cell: states per location.
+
+    ExpSyn / Exp2Syn (``markov.py:65-200``): A, B in uS, g = B - A, events
+    jump A and B (``apply_discrete_events``), the states then decay.
+    AMPA / GABAa (``markov.py:203-307``): g' = alpha D T (1 - g) - beta g;
+    NMDA (``markov.py:310-360``): g' = alpha1 x (1 - g) - beta1 g,
+    x' = alpha2 D T (1 - x) - beta2 x; D is the step's summed weighted events
+    (``pre_drive``), used only by ``compute_derivative`` after the voltage
+    step; every state steps from the step-start values (ind_exp_euler:
+    each derivative and its own diagonal Jacobian at the old states,
+    ``_exp_euler.py:427-470``).  Their current is a density
+    g_max g (E_rev - V) (mS/cm^2 * mV; ``currents.py:138-163`` adds it at
+    the point without an area division)."""
 
     def __init__(self, syn, cvs):
         p = syn.resolved()
         self.kind = syn.synapse_type
-        self.e = float(p["e"])
-        self.weight = float(p["weight"])
+        self.markov = self.kind in ("AMPA", "GABAa", "NMDA")
+        self.p = {k: float(v) for k, v in p.items()}
+        self.e = float(p["E_rev"] if self.markov else p["e"])
+        self.weight = float(p["weight"]) if "weight" in p else None
         self.cvs = np.asarray(cvs, dtype=np.int64)
         n = len(cvs)
-        self.A = np.zeros(n)
-        self.B = np.zeros(n)
+        self.A = np.zeros(n)  # Exp2Syn A; NMDA x
+        self.B = np.zeros(n)  # ExpSyn / Exp2Syn B; Markov g
+        self.D = np.zeros(n)
         if self.kind == "ExpSyn":
             self.tau_a, self.tau_b, self.fa, self.fb = 1.0, float(p["tau"]), 0.0, 1.0
-        else:
+        elif self.kind == "Exp2Syn":
             t1, f = _exp2_factor(float(p["tau1"]), float(p["tau2"]))
             self.tau_a, self.tau_b, self.fa, self.fb = t1, float(p["tau2"]), f, f
 
     @property
     def g(self):
-        return self.B - self.A
+        return self.B if self.markov else self.B - self.A
+
+    def conductance_density(self, area_cm2):
+        """mS/cm^2 at each location: g_max g, or g (uS) / area."""
+        if self.markov:
+            return self.p["g_max"] * self.B
+        return 1e-3 * self.g / area_cm2
 
     def apply(self, drive):
+        if self.markov:
+            self.D = np.asarray(drive, dtype=float)
+            return
         self.A = self.A + drive * self.fa
         self.B = self.B + drive * self.fb
 
     def decay(self, dt):
-        self.A = self.A + dt * exprel(-dt / self.tau_a) * (-self.A / self.tau_a)
-        self.B = self.B + dt * exprel(-dt / self.tau_b) * (-self.B / self.tau_b)
+        if self.kind in ("AMPA", "GABAa"):
+            a, b, T = self.p["alpha"], self.p["beta"], self.p["T"]
+            g = self.B
+            self.B = _ind_exp_euler(g, a * self.D * T * (1 - g) - b * g, -(a * self.D * T + b), dt)
+        elif self.kind == "NMDA":
+            a1, b1, a2, b2, T = (self.p[k] for k in ("alpha1", "beta1", "alpha2", "beta2", "T"))
+            g, x = self.B, self.A
+            self.B = _ind_exp_euler(g, a1 * x * (1 - g) - b1 * g, -(a1 * x + b1), dt)
+            self.A = _ind_exp_euler(x, a2 * self.D * T * (1 - x) - b2 * x, -(a2 * self.D * T + b2), dt)
+        else:
+            self.A = self.A + dt * exprel(-dt / self.tau_a) * (-self.A / self.tau_a)
+            self.B = self.B + dt * exprel(-dt / self.tau_b) * (-self.B / self.tau_b)
+        self.D = np.zeros_like(self.D)
 
 
 class NetworkOracle:
@@ -122,7 +163,10 @@ class NetworkOracle:
             n = len(con["pre_index"])
             w = con.get("weight")
             if w is None:
-                w = np.full(n, post["pools"][0][con["synapse"]].weight)
+                w0 = post["pools"][0][con["synapse"]].weight
+                if w0 is None:  # lowering.py:130-133
+                    raise ValueError("Connection weight is None, but the target synapse has no default 'weight'.")
+                w = np.full(n, w0)
             w = np.broadcast_to(np.asarray(w, dtype=float), (n,))
             d = _steps(np.broadcast_to(np.asarray(con.get("delay", 0.0), dtype=float), (n,)), self.dt, quantization)
             sidx = np.asarray(con.get("synapse_index", np.zeros(n, dtype=np.int64)), dtype=np.int64)
@@ -165,7 +209,7 @@ class NetworkOracle:
                     Gs = np.zeros(c.C)
                     GEs = np.zeros(c.C)
                     for pool in pools.values():
-                        gk = 1e-3 * pool.g / c.area_cm2[pool.cvs]  # uS -> mS/cm^2
+                        gk = pool.conductance_density(c.area_cm2[pool.cvs])
                         np.add.at(Gs, pool.cvs, gk)
                         np.add.at(GEs, pool.cvs, gk * pool.e)
                     c.point_conductance = (Gs, GEs)

@@ -53,7 +53,7 @@ def _net(precision):
     return net
 
 
-def _oracle(net, dt, n_steps):
+def _oracle(net, dt, n_steps, pool_probes=None):
     from oracle.network import NetworkOracle
 
     pops = [(n, p.cell.lower(), p.size, p.cell.V_th) for n, p in net.populations.items()]
@@ -62,7 +62,7 @@ def _oracle(net, dt, n_steps):
             for c in net.connections]
     o = NetworkOracle(pops, cons, dt)
     cvs = {n: [o.pops[n]["cells"][0]._locate(0, 0.5)[1]] for n in net.populations}
-    return o.run(n_steps, probe_cvs=cvs)
+    return o.run(n_steps, probe_cvs=cvs, pool_probes=pool_probes)
 
 
 def test_network_fp64_matches_oracle(cuda):
@@ -116,3 +116,59 @@ def test_network_launches_are_bounded_by_the_shortest_delay(cuda):
         assert np.array_equal(a.traces[name]["v"], b.traces[name]["v"])
         assert np.array_equal(a.spikes[name], b.spikes[name])
     assert isinstance(net, Network)
+
+
+def _markov_net(precision):
+    """AMPA and NMDA pools on two sites of the I cells, GABAa on J
+    (``synapse/markov.py:203-360``); weights are dimensionless drives."""
+    from braincell_amd.mech import StateProbe, Synapse
+    from braincell_amd.network import Connection, Network
+
+    class _Sites:
+        def locations(self, morpho):
+            return [(0, 0.5), (1, 0.5)]
+
+    i_cell = _cell(4, 0.0, dend=True, precision=precision)
+    i_cell.place(_Sites(), Synapse("AMPA", g_max=3.0, name="ampa"))
+    i_cell.place(_Sites(), Synapse("NMDA", g_max=0.3, beta1=0.05, name="nmda"))
+    j_cell = _cell(2, 0.02, precision=precision, syn=Synapse("GABAa", g_max=2.0, name="gaba"))
+    net = Network()
+    net.add_population("E1", _cell(3, 0.25, precision=precision))
+    net.add_population("E2", _cell(2, 0.6, precision=precision))
+    net.add_population("I", i_cell)
+    net.add_population("J", j_cell)
+    net.add_connection(Connection("E1", "I", [0, 1, 2, 0, 2], [0, 1, 2, 3, 3], "ampa",
+                                  weight=[40.0, 60.0, 80.0, 50.0, 30.0], delay=[0.5, 1.0, 0.25, 2.0, 0.8],
+                                  synapse_index=[0, 1, 0, 1, 1]))
+    net.add_connection(Connection("E2", "I", [0, 1], [0, 3], "nmda", weight=20.0, delay=0.3,
+                                  synapse_index=[1, 0]))
+    net.add_connection(Connection("I", "J", [0, 1, 2, 3], [0, 0, 1, 1], "gaba", weight=10.0, delay=1.0))
+    net.add_connection(Connection("I", "I", [1], [2], "ampa", weight=5.0, delay=0.6, synapse_index=[1]))
+    return net
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_network_with_ampa_nmda_gabaa_matches_oracle(cuda, precision):
+    """FP64: every probe within 1e-7 mV of the oracle and identical spike
+    steps; FP32: soma RMSE <= 0.05 mV and spike steps within one step."""
+    dt, T = 0.025, 15.0
+    n = int(round(T / dt))
+    print(f"markov-synapse network fp{precision}, {n} steps", flush=True)
+    net = _markov_net(precision)
+    res = net.run(dt=dt, duration=T)
+    V, G, S = _oracle(net, dt, n, pool_probes=[("I", "ampa"), ("I", "nmda"), ("J", "gaba")])
+    for name in net.populations:
+        got, ref = res.traces[name]["v"], V[name][:, :, 0]
+        err = np.abs(got - ref).max()
+        worst = max(rmse(got[:, i], ref[:, i]) for i in range(ref.shape[1]))
+        print(name, "max |dV|", err, "worst RMSE", worst, "spikes", int(S[name].sum()))
+        if precision == 64:
+            assert err < 1e-7
+            assert np.array_equal(res.spikes[name], S[name])
+        else:
+            assert worst <= 0.05
+            for i in range(ref.shape[1]):
+                a, b = np.flatnonzero(res.spikes[name][:, i]), np.flatnonzero(S[name][:, i])
+                assert len(a) == len(b) and (len(a) == 0 or np.max(np.abs(a - b)) <= 1)
+    assert S["I"].any()  # AMPA / NMDA input makes the I cells fire, and every pool is driven
+    assert min(float(G[k].max()) for k in G) > 0.3

@@ -202,3 +202,112 @@ def test_product_tables_gather_the_oracle_contacts():
     for p in range(tabs.L.n_packs):
         rows = tabs.syn_row[tabs.syn_pack_off[p]:tabs.syn_pack_off[p + 1]]
         assert np.all(np.diff(rows) > 0)
+
+
+# ------------------------------------------------------- Markov synapses
+def _pool(syn):
+    from oracle.network import _Pool
+
+    return _Pool(syn, [0])
+
+
+def test_ampa_and_gabaa_steps_are_the_exact_solution_for_the_step_drive():
+    """g' = alpha D T (1 - g) - beta g is linear in g for the step's drive D,
+    so ind_exp_euler is its exact solution over the step:
+    g_inf + (g - g_inf) exp(-(alpha D T + beta) dt)."""
+    rng = np.random.default_rng(0)
+    dt = 0.025
+    for syn, (a, b, T) in ((Synapse("AMPA", name="a"), (0.98, 0.18, 0.5)),
+                           (Synapse("GABAa", alpha=0.4, T=2.0, name="b"), (0.4, 0.18, 2.0))):
+        p = _pool(syn)
+        g = 0.0
+        for D in rng.choice([0.0, 0.0, 0.0, 1.0, 7.5], 400):
+            p.apply(np.asarray([D]))
+            p.decay(dt)
+            k = a * D * T + b
+            ginf = a * D * T / k
+            g = ginf + (g - ginf) * math.exp(-k * dt)
+            assert abs(p.g[0] - g) <= 1e-14
+        assert 0.0 < g < 1.0
+
+
+def test_nmda_steps_every_state_from_the_step_start_values():
+    """ind_exp_euler takes each state's derivative and diagonal Jacobian at
+    the step-start values: g steps as the exact solution with x frozen at
+    its old value (not the new one), x as the exact solution for the
+    drive; and over many steps the scheme converges to the ODE at first
+    order in dt."""
+    from scipy.integrate import solve_ivp
+
+    syn = Synapse("NMDA", alpha1=2.0, beta1=0.3, alpha2=1.0, beta2=0.5, T=1.0, name="n")
+    p = _pool(syn)
+    p.A[:] = 0.6  # x
+    p.B[:] = 0.3  # g
+    D, dt = 2.0, 0.05
+    p.apply(np.asarray([D]))
+    p.decay(dt)
+    k = 2.0 * 0.6 + 0.3
+    g_old_x = 2.0 * 0.6 / k + (0.3 - 2.0 * 0.6 / k) * math.exp(-k * dt)
+    kx = 1.0 * D + 0.5
+    x1 = D / kx + (0.6 - D / kx) * math.exp(-kx * dt)
+    assert abs(p.g[0] - g_old_x) < 1e-14 and abs(p.A[0] - x1) < 1e-14
+    k1 = 2.0 * x1 + 0.3
+    assert abs(p.g[0] - (2.0 * x1 / k1 + (0.3 - 2.0 * x1 / k1) * math.exp(-k1 * dt))) > 1e-4
+
+    def rhs(t, y):
+        g, x = y
+        return [2.0 * x * (1 - g) - 0.3 * g, 1.0 * D * (1 - x) - 0.5 * x]
+
+    ref = solve_ivp(rhs, (0.0, 5.0), [0.0, 0.0], rtol=1e-12, atol=1e-14).y[:, -1]
+    errs = []
+    for dt in (0.05, 0.025, 0.0125):
+        q = _pool(syn)
+        for _ in range(int(round(5.0 / dt))):
+            q.apply(np.asarray([D]))
+            q.decay(dt)
+        errs.append(abs(q.g[0] - ref[0]))
+    assert errs[0] > errs[1] > errs[2] and 1.7 < errs[0] / errs[1] < 2.3 and 1.7 < errs[1] / errs[2] < 2.3
+
+
+def test_markov_synapse_current_is_a_density_at_its_cv():
+    """With beta = 0 an AMPA synapse keeps the conductance one event gives
+    it: g_max g (mS/cm^2, no area division) then acts as a second leak."""
+    EL, gL = -65.0, 1e-4
+    pre = soma_cell(1, V0=-10.0)
+    post = soma_cell(1, synapse=Synapse("AMPA", alpha=1.0, beta=0.0, T=1.0, g_max=0.05, E_rev=0.0, name="a"),
+                     g_leak=gL, e_leak=EL)
+    dt = 0.025
+    o = NetworkOracle([("E", pre.lower(), 1, 0.0), ("I", post.lower(), 1, 100.0)],
+                      [dict(pre="E", post="I", pre_index=[0], post_index=[0], synapse="a", weight=4.0)], dt,
+                      solver={"E": UP})
+    o.pops["E"]["cells"][0].V[:] = -10.0
+    V, G, _ = o.run(20000, probe_cvs={"I": [0]}, pool_probes=[("I", "a")])
+    g = G[("I", "a")][-1, 0, 0]
+    assert abs(g - (1.0 - math.exp(-4.0 * dt))) < 1e-14  # one event: D = 4 for one step
+    Gs = 0.05 * g
+    Vinf = (gL * 1e3 * EL + Gs * 0.0) / (gL * 1e3 + Gs)
+    assert abs(V["I"][-1, 0, 0] - Vinf) < 1e-6
+
+
+def test_markov_synapses_need_a_connection_weight_and_lower_to_kernel_terms():
+    from braincell_amd.mech import SYN_MARKOV1, SYN_MARKOV2
+
+    pre = soma_cell(1, V0=-10.0)
+    post = soma_cell(1, synapse=Synapse("AMPA", name="a"))
+    with pytest.raises(ValueError, match="weight"):
+        NetworkOracle([("E", pre.lower(), 1, 0.0), ("I", post.lower(), 1, 0.0)],
+                      [dict(pre="E", post="I", pre_index=[0], post_index=[0], synapse="a")], DT)
+    net = Network()
+    net.add_population("E", soma_cell(1))
+    net.add_population("I", soma_cell(1, synapse=Synapse("NMDA", name="n")))
+    net.add_connection(Connection("E", "I", [0], [0], "n"))
+    with pytest.raises(ValueError, match="weight"):
+        net.build(0.025)
+    with pytest.raises(ValueError):
+        Synapse("AMPA", weight=1.0)
+    a = Synapse("AMPA", g_max=0.4, T=2.0).kernel_par(0.025, 1.5, 1e-5)
+    assert a == pytest.approx((0.025 / 1.5 * 0.4, 0.0, SYN_MARKOV1, 0.98 * 2.0, 0.18, 0.0, 0.0, 0.0))
+    n = Synapse("NMDA", T=0.5).kernel_par(0.025, 1.0, 1e-5)
+    assert n == pytest.approx((0.025, 0.0, SYN_MARKOV2, 2.0, 0.01, 0.5, 0.5, 0.0))
+    gab = Synapse("GABAa").resolved()
+    assert gab["E_rev"] == -70.0 and gab["alpha"] == 0.53 and gab["beta"] == 0.18
