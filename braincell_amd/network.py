@@ -30,9 +30,16 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import edges as _edges
+from .edges import EdgeMethod, EdgeSet, all_pairs, dense, pairs, probability
 from .mech import StateProbe
+from .projections import (ContactMethod, ContactTable, Projection, ProjectionContactContext, ProjectionEdgeContext,
+                          by_post, explicit_contacts, per_edge)
 
-__all__ = ["Population", "Connection", "NetworkRunResult", "Network", "delay_steps"]
+__all__ = ["Population", "Connection", "NetworkRunResult", "Network", "delay_steps",
+           "EdgeSet", "EdgeMethod", "pairs", "dense", "all_pairs", "probability",
+           "Projection", "ContactTable", "ContactMethod", "ProjectionEdgeContext", "ProjectionContactContext",
+           "per_edge", "by_post", "explicit_contacts"]
 
 SYN_NPAR = 8  # kernel parameters per synapse (bc_synapse_desc.par)
 
@@ -154,15 +161,25 @@ class NetworkRunResult:
 
 
 class Network:
-    """Populations and connections integrated together (``engine.py:67-548``)."""
+    """Populations and connections integrated together (``engine.py:67-548``).
+
+    Contacts come from direct ``Connection`` declarations and from
+    projections of named edge sets; the run lowers both, direct connections
+    first (``engine.py:388``)."""
 
     def __init__(self, name: str = None):
+        if name is not None and (not isinstance(name, str) or not name):
+            raise ValueError("Network name must be a non-empty string or None.")
         self.name = name
         self.populations = {}
         self.connections = []
+        self.edge_sets = {}
+        self.proj = {}
+        self.projections = self.proj
 
     def __repr__(self):
-        return (f"Network(name={self.name!r}, populations={list(self.populations)}, "
+        return (f"Network(name={self.name!r}, populations={len(self.populations)}, "
+                f"edge_sets={len(self.edge_sets)}, projections={len(self.projections)}, "
                 f"connections={len(self.connections)})")
 
     def add_population(self, name: str, cell) -> Population:
@@ -175,6 +192,11 @@ class Network:
     def add_connection(self, connection: Connection) -> Connection:
         if not isinstance(connection, Connection):
             raise TypeError(f"add_connection expects a Connection, got {type(connection).__name__}.")
+        self._check_connection(connection)
+        self.connections.append(connection)
+        return connection
+
+    def _check_connection(self, connection: Connection) -> None:
         for p in (connection.pre_population, connection.post_population):
             if p not in self.populations:
                 raise ValueError(f"Unknown population {p!r}.")
@@ -183,8 +205,100 @@ class Network:
                                 (connection.post_index, post.size, "post_index")):
             if idx.size and (idx.min() < 0 or idx.max() >= size):
                 raise ValueError(f"Connection {what} out of range for a population of {size}.")
-        self.connections.append(connection)
-        return connection
+
+    # ------------------------------------------------- edge sets, projections
+    def add_edge_set(self, edge_set: EdgeSet) -> EdgeSet:
+        """Store a reusable cell-level edge set (``engine.py:155-160``)."""
+        if not isinstance(edge_set, EdgeSet):
+            raise TypeError(f"Network.add_edge_set(...) expects EdgeSet, got {type(edge_set).__name__}.")
+        if edge_set.name in self.edge_sets:
+            raise ValueError(f"Network already has an edge set named {edge_set.name!r}.")
+        self.edge_sets[edge_set.name] = edge_set
+        return edge_set
+
+    def add_edges(self, *, name: str, pre: str, post: str, method) -> EdgeSet:
+        """Build an edge set between two populations with an edge method or a
+        callable ``method(*, n_pre, n_post)`` and store it (``engine.py:163-178``)."""
+        if pre not in self.populations:
+            raise KeyError(f"Unknown pre population {pre!r}.")
+        if post not in self.populations:
+            raise KeyError(f"Unknown post population {post!r}.")
+        es = _edges.build(name=name, pre=pre, post=post, method=method, n_pre=self.populations[pre].size,
+                          n_post=self.populations[post].size)
+        return self.add_edge_set(es)
+
+    def make_edges(self, *, name: str, pre: str, post: str, connector: str, **kwargs) -> EdgeSet:
+        """``add_edges`` with the builder named by ``connector``: "pairs"
+        (``pairs=``), "all_pairs", "dense" (``adjacency=``) or "probability"
+        (``engine.py:212-237``)."""
+        extra = dict(kwargs)
+        if connector == "pairs":
+            method = pairs(extra.pop("pairs"))
+        elif connector == "dense":
+            method = dense(extra.pop("adjacency"))
+        elif connector == "all_pairs":
+            method, extra = all_pairs(**extra), {}
+        elif connector == "probability":
+            method, extra = probability(**extra), {}
+        else:
+            raise ValueError(f"Unknown edge connector {connector!r}.")
+        if extra:
+            raise TypeError(f"Unexpected connector arguments: {tuple(extra)!r}.")
+        return self.add_edges(name=name, pre=pre, post=post, method=method)
+
+    def add_projection(self, projection: Projection = None, *, name: str = None, edges=None, synapse=None,
+                       **kwargs) -> Projection:
+        """Store a projection, given as an object or as ``Projection`` keyword
+        arguments (``engine.py:181-209``)."""
+        if projection is None:
+            for value, what in ((name, "name"), (edges, "edges"), (synapse, "synapse")):
+                if value is None:
+                    raise ValueError(f"Network.add_projection(...) requires {what}.")
+            projection = Projection(name=name, edges=edges, synapse=synapse, **kwargs)
+        elif any(v is not None for v in (name, edges, synapse)) or kwargs:
+            raise TypeError("Network.add_projection(...) accepts either a Projection object "
+                            "or direct projection keyword arguments, not both.")
+        if not isinstance(projection, Projection):
+            raise TypeError(f"Network.add_projection(...) expects Projection, got {type(projection).__name__}.")
+        if projection.name in self.proj:
+            raise ValueError(f"Network already has a projection named {projection.name!r}.")
+        self.proj[projection.name] = projection
+        return projection
+
+    def project(self, *, name: str, edges, synapse, **kwargs) -> Projection:
+        """``add_projection`` by keywords (``engine.py:239-241``)."""
+        return self.add_projection(name=name, edges=edges, synapse=synapse, **kwargs)
+
+    def _pool_size(self, population: str, pool: str) -> int:
+        cell = self.populations[population].cell
+        for locset, syn in getattr(cell, "_synapses", []):
+            if syn.instance_name == pool:
+                return len(list(locset.locations(cell.morpho)))
+        raise ValueError(f"Population {population!r} has no synapse pool {pool!r}.")
+
+    def _projection_connections(self) -> tuple:
+        """Every projection expanded against its stored edge set, with the
+        population sizes and the target pool's location count
+        (``engine.py:561-581``)."""
+        out = []
+        for proj in self.proj.values():
+            if proj.edge_set_name not in self.edge_sets:
+                raise KeyError(f"Projection {proj.name!r} references unknown EdgeSet {proj.edge_set_name!r}.")
+            es = self.edge_sets[proj.edge_set_name]
+            for p in (es.pre_population, es.post_population):
+                if p not in self.populations:
+                    raise ValueError(f"Unknown population {p!r}.")
+            cons = proj.to_connections(es, pre_size=self.populations[es.pre_population].size,
+                                       post_size=self.populations[es.post_population].size,
+                                       pool_size=self._pool_size(es.post_population, proj.synapse))
+            for con in cons:
+                self._check_connection(con)
+            out.extend(cons)
+        return tuple(out)
+
+    def all_connections(self) -> tuple:
+        """Direct connections, then the projections' (the lowering order)."""
+        return tuple(self.connections) + self._projection_connections()
 
     # ------------------------------------------------------------------ build
     def build(self, dt: float, *, delay_quantization: str = "ceil", chunk: int = 200, threads: int = None,
@@ -295,7 +409,7 @@ class _NetworkTables:
         self.n_syn = len(recs)
         # contacts, grouped by target synapse in connection order
         tgt, pre, dly, w = [], [], [], []
-        for con in net.connections:
+        for con in net.all_connections():
             pool_of = self._pool_cells(con.post_population, con.synapse)
             n = con.n_contact
             post_gid = pop_cell0[con.post_population] + con.post_index

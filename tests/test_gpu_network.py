@@ -59,7 +59,7 @@ def _oracle(net, dt, n_steps, pool_probes=None):
     pops = [(n, p.cell.lower(), p.size, p.cell.V_th) for n, p in net.populations.items()]
     cons = [dict(pre=c.pre_population, post=c.post_population, pre_index=c.pre_index, post_index=c.post_index,
                  synapse=c.synapse, synapse_index=c.synapse_index, weight=c.weight, delay=c.delay)
-            for c in net.connections]
+            for c in net.all_connections()]
     o = NetworkOracle(pops, cons, dt)
     cvs = {n: [o.pops[n]["cells"][0]._locate(0, 0.5)[1]] for n in net.populations}
     return o.run(n_steps, probe_cvs=cvs, pool_probes=pool_probes)
@@ -172,3 +172,60 @@ def test_network_with_ampa_nmda_gabaa_matches_oracle(cuda, precision):
                 assert len(a) == len(b) and (len(a) == 0 or np.max(np.abs(a - b)) <= 1)
     assert S["I"].any()  # AMPA / NMDA input makes the I cells fire, and every pool is driven
     assert min(float(G[k].max()) for k in G) > 0.3
+
+
+def _projection_net(precision, explicit):
+    """Edge sets and projections (``network/edges.py``, ``projections.py``):
+    all_pairs, probability and pairs edge sets expanded by per_edge / by_post
+    / explicit contacts with per-edge weights and callable delays; with
+    ``explicit`` the same contacts are written as plain connections."""
+    from braincell_amd.mech import Synapse
+    from braincell_amd.network import (Connection, Network, all_pairs, by_post, explicit_contacts, pairs, per_edge,
+                                       probability)
+
+    net = Network()
+    net.add_population("E1", _cell(3, 0.25, precision=precision))
+    net.add_population("E2", _cell(2, 0.6, precision=precision))
+    net.add_population("I", _cell(4, 0.0, dend=True, precision=precision,
+                                  syn=Synapse("Exp2Syn", tau1=0.5, tau2=4.0, e=0.0, weight=0.01, name="ampa")))
+    net.add_population("J", _cell(2, 0.02, precision=precision,
+                                  syn=Synapse("ExpSyn", tau=3.0, e=-80.0, weight=0.02, name="gaba")))
+    net.add_edges(name="e1i", pre="E1", post="I", method=all_pairs(pre_indices=[0, 2], post_indices=[0, 1, 3]))
+    net.add_edges(name="e2i", pre="E2", post="I", method=probability(p=0.6, seed=3))
+    net.add_edges(name="ij", pre="I", post="J", method=pairs([(0, 0), (1, 0), (2, 1), (3, 1)]))
+    net.project(name="p1", edges="e1i", synapse="ampa", method=per_edge(number=2, replace=False, seed=5),
+                weight=np.linspace(0.002, 0.008, 6), delay=lambda c: 0.25 + 0.1 * c.contact_post_index)
+    net.project(name="p2", edges="e2i", synapse="ampa", method=by_post(seed=9), weight=0.015, delay=0.3)
+    net.project(name="p3", edges="ij", synapse="gaba", method=explicit_contacts([0, 1, 2, 3, 3]), delay=1.0)
+    if not explicit:
+        return net
+    cons = net.all_connections()
+    direct = Network()
+    for name, pop in net.populations.items():
+        direct.add_population(name, pop.cell)
+    for c in cons:
+        direct.add_connection(Connection(c.pre_population, c.post_population, c.pre_index, c.post_index,
+                                         c.synapse, weight=c.weight, delay=c.delay, synapse_index=c.synapse_index))
+    return direct
+
+
+def test_projection_network_equals_explicit_connections_and_oracle(cuda):
+    """The projection-built network and its contacts written as connections
+    give bitwise identical traces and spikes; FP64 agrees with the oracle to
+    1e-7 mV with identical spike steps."""
+    dt, T = 0.025, 15.0
+    n = int(round(T / dt))
+    print(f"projection network fp64, {n} steps", flush=True)
+    net = _projection_net(64, False)
+    assert sum(c.n_contact for c in net.all_connections()) > 12
+    a = net.run(dt=dt, duration=T)
+    b = _projection_net(64, True).run(dt=dt, duration=T)
+    V, _, S = _oracle(net, dt, n)
+    for name in net.populations:
+        assert np.array_equal(a.traces[name]["v"], b.traces[name]["v"])
+        assert np.array_equal(a.spikes[name], b.spikes[name])
+        err = np.abs(a.traces[name]["v"] - V[name][:, :, 0]).max()
+        print(name, "max |dV|", err, "spikes", int(S[name].sum()))
+        assert err < 1e-7
+        assert np.array_equal(a.spikes[name], S[name])
+    assert S["I"].any()
