@@ -20,7 +20,8 @@ import numpy as np
 
 from .channels import IONS
 from .filter import resolve_region
-from .mech import CableProperty, Channel, CurrentClamp, CurrentProbe, Ion, MechanismProbe, StateProbe, Synapse
+from .mech import (CableProperty, Channel, CurrentClamp, CurrentProbe, Ion, MechanismProbe, NetStim, StateProbe,
+                   Synapse)
 from .model import CellModel
 from .policy import CVPerBranch, CVPolicy
 from .quad import get_integrator
@@ -71,6 +72,8 @@ class Cell:
         self._place = []
         self._probes = []
         self._synapses = []
+        self._netstims = []
+        self._synapse_inputs = {}
         self._runtime = None
         self._initialized = False
         ps = (pop_size,) if isinstance(pop_size, int) else tuple(pop_size)
@@ -101,9 +104,12 @@ class Cell:
         probes = tuple(m for m in mechanisms if isinstance(m, (StateProbe, MechanismProbe, CurrentProbe)))
         clamps = tuple(m for m in mechanisms if isinstance(m, CurrentClamp))
         synapses = tuple(m for m in mechanisms if isinstance(m, Synapse))
+        stims = tuple(m for m in mechanisms if isinstance(m, NetStim))
         for m in mechanisms:
-            if not isinstance(m, (CurrentClamp, StateProbe, MechanismProbe, CurrentProbe, Synapse)):
-                raise TypeError(f"place() takes CurrentClamp, Synapse or a probe, got {type(m).__name__}")
+            if not isinstance(m, (CurrentClamp, StateProbe, MechanismProbe, CurrentProbe, Synapse, NetStim)):
+                raise TypeError(f"place() takes CurrentClamp, Synapse, NetStim or a probe, got {type(m).__name__}")
+        for ns in stims:
+            self._netstims.append((locset, ns))
         have = {syn.instance_name for _, syn in self._synapses}
         for syn in synapses:
             if syn.instance_name in have:
@@ -121,6 +127,35 @@ class Cell:
             self._probes.append((locset, probes))
         self._runtime = None
         return self
+
+    def bind_synapse_input(self, synapse: str, source, *, weight=1.0, transform=None) -> "Cell":
+        """Bind an external drive to the synapse pool ``synapse``
+        (``cell.py:403-422``): every step its drive gains
+        ``transform(source) * weight`` (``source()`` for a callable),
+        broadcast to (pop_size, n_locations), in the pool's payload units.
+        Callables are sampled on the host once per step before a run starts,
+        so they cannot read the run's own state."""
+        self._synapse_inputs.setdefault(str(synapse), []).append((source, weight, transform))
+        return self
+
+    def _drive_group(self, L, gid0: int, size: int):
+        """(first cell, size, NetStims with their CVs, bindings, pool sizes)
+        of ``size`` copies of this cell starting at cell ``gid0`` of layout
+        ``L``, for the external drive table."""
+        host = L.cells[gid0]
+        pools = {name: len(loc) for (name, _, loc) in host.synapses or []}
+        stims = [(ns, cv) for ns, cv in (host.netstims or []) if cv >= 0]
+        return (gid0, int(size), stims, dict(self._synapse_inputs), pools)
+
+    def _set_drive(self, rt, mode: str, step0: int, n_steps: int, dt: float):
+        if getattr(rt, "synapse_tables", None) is None:
+            if self._synapse_inputs and not self._synapses:
+                raise KeyError("bind_synapse_input: no synapse is placed on the cell.")
+            return
+        from .synapses import drive_times, set_external_drive
+
+        set_external_drive(rt, [self._drive_group(self._layout, 0, 1)],
+                           drive_times(mode, step0, n_steps, dt), step0)
 
     def lower(self) -> CellModel:
         """Lower paint/place rules to per-branch tables (later rules win)."""
@@ -171,12 +206,14 @@ class Cell:
                     for (t0, t1, amp) in m.segments():
                         clamps.append((b, x, t0, t1 - t0, amp))
         synapses = [(syn.instance_name, syn, list(locset.locations(mo))) for locset, syn in self._synapses]
+        netstims = [(ns, list(locset.locations(mo))) for locset, ns in self._netstims]
         ca = None
         if ca_params is not None:
             ca = {"mask_branch": ca_mask, **{k: ca_params[k] for k in ("d", "tau", "C_rest", "Co", "Ci0", "temp")}}
         counts = self.cv_policy.counts(mo, cm_branch=cm, ra_branch=ra)
         return CellModel(morph=mo, cv_per_branch=counts, cm=cm, ra=ra, V0=float(v0[0]),
-                         channels=channels, ca=ca, clamps=clamps, synapses=synapses, **E).validate()
+                         channels=channels, ca=ca, clamps=clamps, synapses=synapses, netstims=netstims,
+                         **E).validate()
 
     # ----------------------------------------------------------------- state
     def init_state(self, *, probes=None, threads: int = 256):
@@ -197,6 +234,8 @@ class Cell:
             self.init_state()
         if self._runtime is not None:
             self._runtime.reset_state()
+            if getattr(self._runtime, "synapse_buffers", None) is not None:
+                self._runtime.synapse_buffers["state"].zero_()
 
     def _runtime_for(self, dt: float):
         from .runtime import StaggerRuntime
@@ -209,10 +248,16 @@ class Cell:
             rt = StaggerRuntime(self._layout, dt, precision=self.precision, device=self.device)
             if self.ion_channel_update_order == "integration":
                 rt.flags |= 2  # BC_STEP_INTEGRATION_ORDER
+            if self._model.synapses:
+                from .synapses import attach_cell_synapses
+
+                attach_cell_synapses(rt, self._layout, dt, self.V_th)
             if old is not None:  # keep the current state across a dt change
                 rt.V.copy_(old.V)
                 rt.S.copy_(old.S)
                 rt.Vbnd.copy_(old.Vbnd)
+                if self._model.synapses:
+                    rt.synapse_buffers["state"].copy_(old.synapse_buffers["state"])
                 rt.step_index = old.step_index
             self._runtime = rt
         return rt
@@ -236,6 +281,7 @@ class Cell:
         rt = self._runtime_for(dt)
         n0 = int(round(float(t) / float(dt)))
         rt.step_index = n0
+        self._set_drive(rt, "update", n0, int(n_steps), float(dt))
         flags = rt.flags
         if voltage_only:
             rt.flags = flags | VOLTAGE_ONLY
@@ -263,6 +309,7 @@ class Cell:
             raise ValueError("Cell.run(...) produced no timesteps; ensure duration > 0 and dt > 0.")
         step0 = rt.step_index
         start = step0 * float(dt)
+        self._set_drive(rt, "run", step0, n, float(dt))
         if rt._spk is None or rt._spk.cap < spike_cap(n, dt) or rt._spk.v_th != self.V_th:
             rt.enable_spikes(self.V_th, spike_cap(n, dt))
         else:

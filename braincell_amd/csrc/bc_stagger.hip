@@ -86,6 +86,10 @@ struct StaggerArgs {
   const int32_t* cv_cell;
   uint8_t* spk;
   int32_t spk_len, n_cells;
+  // external drive (NetStim trains, bound inputs): NULL ext_slot = none
+  const int32_t* ext_slot;
+  const real* ext_drive;
+  int32_t n_ext, ext_step0, ext_steps;
   real v_th;
   // chain tables in LDS form (bc_stagger_desc.chain_rec ..)
   const int4* chain_rec;
@@ -631,13 +635,20 @@ __device__ __forceinline__ void synapse_pass(const StaggerArgs<real>& a, HRow<re
     real gs = real(0), gse = real(0);
     for (int s = a.syn_row_off[j]; s < a.syn_row_off[j + 1]; ++s) {
       real drive = real(0);
-      const int c1 = a.con_off[s + 1];
-      for (int c = a.con_off[s] + lane; c < c1; c += 64) {
-        const int src = gstep - a.con_delay[c];
-        if (src >= 0 && a.spk[(int64_t)(src % a.spk_len) * a.n_cells + a.con_pre[c]]) drive += a.con_w[c];
+      if (a.spk != nullptr) {  // no ring: a cell run on its own, no contacts
+        const int c1 = a.con_off[s + 1];
+        for (int c = a.con_off[s] + lane; c < c1; c += 64) {
+          const int src = gstep - a.con_delay[c];
+          if (src >= 0 && a.spk[(int64_t)(src % a.spk_len) * a.n_cells + a.con_pre[c]]) drive += a.con_w[c];
+        }
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) drive += __shfl_xor(drive, off);
+      if (a.ext_slot != nullptr) {  // arrivals first, then the external drive
+        const int e = a.ext_slot[s];
+        const int k = gstep - a.ext_step0;
+        if (e >= 0 && k >= 0 && k < a.ext_steps) drive += a.ext_drive[(int64_t)k * a.n_ext + e];
+      }
       const real* q = a.syn_par + (int64_t)s * kSynPar;
       const int kind = (int)q[2];  // uniform over the wavefront
       real A = a.syn_state[2 * (int64_t)s];
@@ -965,7 +976,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         xv[info[i] & kRowMask] = Vn;
         advance_states<real, CH>(a.cc, st[i], Vn, ica[i], dt, ca_b, a.flags);
         if constexpr (SYN) {
-          if (vv[i] < a.v_th && Vn >= a.v_th)
+          if (a.spk != nullptr && vv[i] < a.v_th && Vn >= a.v_th)
             a.spk[(int64_t)(gstep % a.spk_len) * a.n_cells + a.cv_cell[(int64_t)P.cv0 + tid + i * NT]] = 1;
         }
         vv[i] = Vn;
@@ -1126,6 +1137,11 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->spk_len = y ? y->spk_len : 1;
   a->n_cells = y ? y->n_cells : 0;
   a->v_th = y ? static_cast<real>(y->v_th) : real(0);
+  a->ext_slot = y ? y->ext_slot : nullptr;
+  a->ext_drive = y ? static_cast<const real*>(y->ext_drive) : nullptr;
+  a->n_ext = y ? y->n_ext : 0;
+  a->ext_step0 = y ? y->ext_step0 : 0;
+  a->ext_steps = y ? y->ext_steps : 0;
   a->chain_rec = reinterpret_cast<const int4*>(d->chain_rec);
   a->chain_par = d->chain_par;
   a->kid_pad = d->kid_pad;
@@ -1164,12 +1180,18 @@ int validate(const bc_stagger_desc* d) {
       return bc::fail(BC_INVALID_ARG, "spike desc: required pointer is NULL");
   }
   if (const bc_synapse_desc* y = d->syn) {
-    if (y->n_cells <= 0 || y->min_delay < 1 || y->max_delay < y->min_delay)
-      return bc::fail(BC_INVALID_ARG, "synapse desc: n_cells > 0 and 1 <= min_delay <= max_delay required");
     if (y->pack_off == nullptr || y->row == nullptr || y->row_off == nullptr || y->par == nullptr ||
-        y->state == nullptr || y->con_off == nullptr || y->con_pre == nullptr || y->con_delay == nullptr ||
-        y->con_w == nullptr || y->cv_cell == nullptr || y->spk == nullptr)
+        y->state == nullptr)
       return bc::fail(BC_INVALID_ARG, "synapse desc: required pointer is NULL");
+    if (y->spk != nullptr) {
+      if (y->n_cells <= 0 || y->min_delay < 1 || y->max_delay < y->min_delay)
+        return bc::fail(BC_INVALID_ARG, "synapse desc: n_cells > 0 and 1 <= min_delay <= max_delay required");
+      if (y->con_off == nullptr || y->con_pre == nullptr || y->con_delay == nullptr || y->con_w == nullptr ||
+          y->cv_cell == nullptr)
+        return bc::fail(BC_INVALID_ARG, "synapse desc: contact tables are NULL with a spike ring");
+    }
+    if (y->ext_slot != nullptr && (y->n_ext <= 0 || y->ext_steps < 0 || y->ext_drive == nullptr))
+      return bc::fail(BC_INVALID_ARG, "synapse desc: external drive needs n_ext > 0, ext_steps >= 0 and ext_drive");
   }
   return BC_OK;
 }
@@ -1191,7 +1213,7 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (d->n_packs == 0 || n_steps == 0) return BC_OK;
   if (d->spk != nullptr && trace == nullptr) return bc::fail(BC_INVALID_ARG, "spike output needs a trace buffer");
-  if (const bc_synapse_desc* y = d->syn) {
+  if (const bc_synapse_desc* y = d->syn; y != nullptr && y->spk != nullptr) {
     if (n_steps > y->min_delay || y->spk_len < y->max_delay + n_steps || step0 < 0)
       return bc::fail(BC_INVALID_ARG,
                       "network launch of %d steps from step %d: need n_steps <= min_delay (%d), "

@@ -15,10 +15,12 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import numpy as np
+
 from .channels import CHANNELS, IONS, celsius2kelvin
 
-__all__ = ["Channel", "Ion", "CableProperty", "CurrentClamp", "StateProbe", "MechanismProbe", "CurrentProbe",
-           "Synapse"]
+__all__ = ["Channel", "Ion", "CableProperty", "CurrentClamp", "NetStim", "StateProbe", "MechanismProbe",
+           "CurrentProbe", "Synapse"]
 
 
 @dataclass(frozen=True)
@@ -118,6 +120,58 @@ class CurrentClamp:
             out.append((s, s + d, a))
             s += d
         return out
+
+
+@dataclass(frozen=True)
+class NetStim:
+    """Deterministic spike source at a point (``mech/_point.py:252-311``,
+    NEURON's NetStim with noise 0): events at start + k * interval ms for
+    k = 0 .. number - 1, each a drive of ``weight`` (the payload units of the
+    synapses at the same point: uS for ExpSyn / Exp2Syn, dimensionless for
+    AMPA / GABAa / NMDA) for every synapse placed at its CV.
+
+    ``fired(t)`` is the reference's test (``_compute/layouts.py:906-941``):
+    with local = t - start and k = round(local / interval) (half to even),
+    an event is due when local >= 0, |local - k interval| <= 1e-9 and
+    0 <= k < number; evaluated in float64 here."""
+
+    start: float
+    number: int
+    interval: float
+    noise: float = 0.0
+    weight: float = 1.0
+    name: str = None
+
+    def __post_init__(self):
+        for what in ("start", "interval"):
+            v = getattr(self, what)
+            if isinstance(v, bool) or not isinstance(v, (int, float, np.integer, np.floating)):
+                raise TypeError(f"NetStim.{what} must be a scalar time in ms, got {v!r}.")
+            object.__setattr__(self, what, float(v))
+        if self.interval <= 0.0:
+            raise ValueError(f"NetStim.interval must be > 0, got {self.interval!r}.")
+        if isinstance(self.number, bool) or not isinstance(self.number, int):
+            raise TypeError(f"NetStim.number must be int, got {self.number!r}.")
+        if self.number < 0:
+            raise ValueError(f"NetStim.number must be >= 0, got {self.number!r}.")
+        if isinstance(self.noise, bool) or not isinstance(self.noise, (int, float)):
+            raise TypeError(f"NetStim.noise must be a real number, got {self.noise!r}.")
+        if float(self.noise) != 0.0:
+            raise ValueError(f"NetStim.noise={self.noise!r} is not supported yet; use noise=0.0.")
+        if isinstance(self.weight, bool) or not isinstance(self.weight, (int, float)):
+            raise TypeError(f"NetStim.weight must be a real number, got {self.weight!r}.")
+        if self.name is not None and (not isinstance(self.name, str) or not self.name):
+            raise ValueError(f"NetStim.name must be a non-empty string or None, got {self.name!r}.")
+
+    @property
+    def instance_name(self) -> str:
+        return self.name if self.name is not None else "NetStim"
+
+    def fired(self, t_ms) -> np.ndarray:
+        local = np.asarray(t_ms, dtype=np.float64) - self.start
+        k = np.rint(local / self.interval)
+        on_grid = np.abs(local - k * self.interval) <= 1e-9
+        return (local >= 0.0) & on_grid & (k >= 0) & (k < self.number)
 
 
 @dataclass(frozen=True)

@@ -33,6 +33,7 @@ class CellModel:
     E_Na, E_K, E_Ca : fixed reversal potentials (mV)
     clamps : list of (branch, x, start_ms, duration_ms, amplitude_nA)
     synapses : list of (instance name, mech.Synapse, [(branch, x), ...])
+    netstims : list of (mech.NetStim, [(branch, x), ...])
     """
 
     morph: object
@@ -47,6 +48,7 @@ class CellModel:
     E_Ca: float = IONS["CalciumFixed"]["E"]
     clamps: list = field(default_factory=list)
     synapses: list = field(default_factory=list)
+    netstims: list = field(default_factory=list)
 
     @property
     def n_branches(self) -> int:

@@ -33,6 +33,7 @@ import numpy as np
 from . import edges as _edges
 from .edges import EdgeMethod, EdgeSet, all_pairs, dense, pairs, probability
 from .mech import StateProbe
+from .synapses import SynapseTables, drive_times, set_external_drive
 from .projections import (ContactMethod, ContactTable, Projection, ProjectionContactContext, ProjectionEdgeContext,
                           by_post, explicit_contacts, per_edge)
 
@@ -385,28 +386,11 @@ class _NetworkTables:
         self.dt, self.precision, self.v_th = dt, precision, v_th
         L = layout
         # synapses: one per (cell, pool, location), ordered by (pack, row)
-        recs = []  # (pack, row, cell, pool, site, par)
-        for gid, c in enumerate(L.cells):
-            for (name, syn, loc) in c.synapses or []:
-                w0 = syn.resolved().get("weight", np.nan)  # Markov types: no default weight
-                for j, (cv, area, cm) in enumerate(loc):
-                    row = int(L.cv_info[L.cell_cv0[gid] + cv] & 0xFFFF)
-                    recs.append((int(L.cell_pack[gid]), row, gid, name, j, syn.kernel_par(dt, cm, area), w0))
-        recs.sort(key=lambda r: (r[0], r[1]))
-        self.syn_key = {(r[2], r[3], r[4]): s for s, r in enumerate(recs)}
-        self.syn_weight = np.asarray([r[6] for r in recs], dtype=np.float64)
-        self.syn_par = np.asarray([r[5] for r in recs], dtype=np.float64).reshape(-1, SYN_NPAR)
-        pack = np.asarray([r[0] for r in recs], dtype=np.int64)
-        row = np.asarray([r[1] for r in recs], dtype=np.int64)
-        new_row = np.ones(len(recs), dtype=bool)
-        if len(recs) > 1:
-            new_row[1:] = (pack[1:] != pack[:-1]) | (row[1:] != row[:-1])
-        starts = np.flatnonzero(new_row)
-        self.syn_row = row[starts].astype(np.int32)
-        self.syn_row_off = np.concatenate((starts, [len(recs)])).astype(np.int32)
-        row_pack = pack[starts]
-        self.syn_pack_off = np.searchsorted(row_pack, np.arange(L.n_packs + 1), side="left").astype(np.int32)
-        self.n_syn = len(recs)
+        st = SynapseTables(L, dt)
+        self.syn = st
+        self.syn_key, self.syn_weight, self.syn_par = st.syn_key, st.syn_weight, st.syn_par
+        self.syn_row, self.syn_row_off, self.syn_pack_off = st.syn_row, st.syn_row_off, st.syn_pack_off
+        self.n_syn = st.n_syn
         # contacts, grouped by target synapse in connection order
         tgt, pre, dly, w = [], [], [], []
         for con in net.all_connections():
@@ -484,6 +468,10 @@ class _NetworkTables:
             setattr(d, k, v.data_ptr())
         rt.synapse_desc = d
         rt.synapse_buffers = buf
+        rt.synapse_tables = self.syn
+        groups = [self.net.populations[n].cell._drive_group(self.L, self.pop_cell0[n], self.net.populations[n].size)
+                  for n in self.names]
+        set_external_drive(rt, groups, drive_times("network", 0, int(n_steps), self.dt), 0)
         return rt
 
     def result(self, rt, trace, n_steps, spike_recording) -> NetworkRunResult:

@@ -131,6 +131,7 @@ class _CellHost:
     clamps: list  # (kind, local index (cv or row), t0, t1, amp)
     probes: np.ndarray  # CV indices
     synapses: list = None  # (instance name, mech.Synapse, [(cv, area cm^2, Cm uF/cm^2) per site])
+    netstims: list = None  # (mech.NetStim, CV of its point, or -1 on a CV boundary node)
 
 
 class StaggerLayout:
@@ -342,7 +343,12 @@ class StaggerLayout:
                                      "place synapses inside a CV")
                 loc.append((int(idx), float(geom.cv_area_um2[idx]) * 1e-8, float(cmb[geom.cv_branch[idx]])))
             syns.append((name, syn, loc))
-        return _CellHost(geom, chains, uniq, cv_cls, clamps, pr, syns)
+        stims = []
+        for (ns, sites) in getattr(m, "netstims", []) or []:
+            for (b, x) in sites:
+                kind, idx = geom.locate(int(b), float(x), ncvb)
+                stims.append((ns, int(idx) if kind == "mid" else -1))
+        return _CellHost(geom, chains, uniq, cv_cls, clamps, pr, syns, stims)
 
     def _plan_packs(self, cap_cv):
         """First-fit decreasing by row count, under the row and CV budgets."""

@@ -113,7 +113,21 @@ typedef struct bc_pack {
  * its CVs crosses v_th upwards (V_n < v_th <= V_{n+1}); the kernel writes
  * that into the history ring spk.  Every launch must satisfy n_steps <=
  * min_delay (each step reads spikes of earlier launches only) and spk_len
- * >= max_delay + n_steps. */
+ * >= max_delay + n_steps.
+ *
+ * spk = NULL: no delivery at all (a cell run on its own, whose synapses are
+ * driven only by the external drive below): contacts are not read (con_*,
+ * cv_cell may be NULL), no spikes are written and the launch length is not
+ * bounded by min_delay.
+ *
+ * External drive (NetStim trains, mech/_point.py:252-311 evaluated by
+ * _compute/layouts.py:906-941, and inputs bound with
+ * Cell.bind_synapse_input, _multi_compartment/cell.py:403-422, 2426-2487;
+ * both summed into the drive by Cell._prepare_runtime_synapse_inputs):
+ * when ext_slot is not NULL, synapse s with ext_slot[s] >= 0 adds
+ * ext_drive[(n - ext_step0) * n_ext + ext_slot[s]] to D at global step n,
+ * for 0 <= n - ext_step0 < ext_steps (real, the payload units of the
+ * synapse: uS for BC_SYN_EXP, dimensionless for the Markov kinds). */
 #define BC_SYN_EXP 0
 #define BC_SYN_MARKOV1 1
 #define BC_SYN_MARKOV2 2
@@ -134,7 +148,12 @@ typedef struct bc_synapse_desc {
   const int32_t* con_delay; /* [n_con] delay in steps */
   const void* con_w;        /* [n_con] weight (uS; dimensionless for the Markov kinds) */
   const int32_t* cv_cell;   /* [n_cv_total] cell of each CV */
-  uint8_t* spk;             /* [spk_len * n_cells] spike history ring */
+  uint8_t* spk;             /* [spk_len * n_cells] spike history ring, or NULL */
+  int32_t n_ext;            /* columns of ext_drive */
+  int32_t ext_step0;        /* global step of ext_drive's first row */
+  int32_t ext_steps;        /* rows of ext_drive */
+  const int32_t* ext_slot;  /* [n_syn] column of each synapse (-1: none), or NULL */
+  const void* ext_drive;    /* [ext_steps * n_ext] real */
 } bc_synapse_desc;
 
 /* Spike output of a plain run (get_spike, braincell/_base.py:421-431, set

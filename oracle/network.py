@@ -19,6 +19,18 @@ Per step n (``network/engine.py:474-548``):
 5. ``enqueue_future_events`` (``delivery.py:362-397``): each block adds
    w * spike(pre) at ring slot (cursor + d) mod (d + 1); cursors advance.
 
+External drive (``_multi_compartment/cell.py:2426-2487``): the drive a
+synapse takes at step n is the arrivals plus the NetStims placed at its point
+(``_compute/layouts.py:906-941``: with local = t - start and
+k = round(local / interval), weight when local >= 0, |local - k interval| <=
+1e-9 and 0 <= k < number) plus every bound input source() * weight (after
+transform).  ``drive_mode`` says at which time t they are evaluated:
+"network" t_n (``network/engine.py:491-504``), "run" t_{max(n-1, 0)}
+(``Cell.run``, ``_multi_compartment/run.py:108-134``: one preparation
+before the loop at the start time, then one after each step at that step's
+time) and "update" t_{n-1} (``Cell.update``, ``cell.py:2298-2329``; none at
+the first step).
+
 Delays are ``max(ceil(delay / dt - 1e-12), 1)`` steps after snapping to the
 nearest integer within 1e-7 (``lowering.py:185-228``).  ``solver`` (a callable
 or a dict population -> callable) replaces step 3 with a toy voltage map, as
@@ -132,9 +144,14 @@ class NetworkOracle:
     list of dicts with pre, post, pre_index, post_index, synapse,
     synapse_index, weight (uS or None) and delay (ms)."""
 
-    def __init__(self, populations, connections, dt, *, quantization="ceil", solver=None, V_init=None):
+    def __init__(self, populations, connections, dt, *, quantization="ceil", solver=None, V_init=None,
+                 drive_mode="network", bindings=None):
         self.dt = float(dt)
         self.solver = solver
+        if drive_mode not in ("network", "run", "update"):
+            raise ValueError(drive_mode)
+        self.drive_mode = drive_mode
+        self.bindings = bindings or {}  # population -> {pool: [(source, weight, transform)]}
         self.pops = {}
         self.order = []
         for name, model, size, v_th in populations:
@@ -153,7 +170,13 @@ class NetworkOracle:
                     cp[pname] = _Pool(syn, cvs)
                 cells.append(c)
                 pools.append(cp)
-            self.pops[name] = {"cells": cells, "pools": pools, "v_th": float(v_th), "size": size}
+            stims = []  # (start, number, interval, weight, CV)
+            for (ns, sites) in getattr(model, "netstims", []) or []:
+                for (b, x) in sites:
+                    kind, idx = cells[0]._locate(b, x)
+                    if kind == "mid":
+                        stims.append((ns.start, ns.number, ns.interval, ns.weight, idx))
+            self.pops[name] = {"cells": cells, "pools": pools, "v_th": float(v_th), "size": size, "stims": stims}
             self.order.append(name)
         # delivery blocks: (connection, delay) groups with a ring [d + 1, post, n_active]
         self.blocks = []
@@ -180,6 +203,36 @@ class NetworkOracle:
         self.pre_spike = {}
         self.spikes = {name: np.zeros(self.pops[name]["size"], dtype=bool) for name in self.order}
 
+    @staticmethod
+    def _netstim(start, number, interval, weight, t):
+        local = t - start
+        k = np.round(local / interval)
+        fired = local >= 0.0 and abs(local - k * interval) <= 1e-9 and 0 <= k < number
+        return weight if fired else 0.0
+
+    def _external(self, name, n):
+        """{pool: [size, n_loc]} NetStim + bound drive of step n."""
+        P = self.pops[name]
+        src = n if self.drive_mode == "network" else (max(n - 1, 0) if self.drive_mode == "run" else n - 1)
+        out = {}
+        if src < 0:
+            return out
+        t = src * self.dt
+        for pname, pool in P["pools"][0].items():
+            d = np.zeros((P["size"], len(pool.cvs)))
+            for (start, number, interval, w, cv) in P["stims"]:
+                amp = self._netstim(start, number, interval, w, t)
+                for j, pcv in enumerate(pool.cvs):
+                    if pcv == cv:
+                        d[:, j] += amp
+            for source, weight, transform in self.bindings.get(name, {}).get(pname, []):
+                v = source() if callable(source) else source
+                if transform is not None:
+                    v = transform(v)
+                d += np.broadcast_to(np.asarray(v, dtype=float) * np.asarray(weight, dtype=float), d.shape)
+            out[pname] = d
+        return out
+
     def step(self, n):
         t = n * self.dt
         # 1. due arrivals
@@ -193,13 +246,17 @@ class NetworkOracle:
             self.pre_spike[key] = arr
         for name in self.order:
             P = self.pops[name]
+            ext = self._external(name, n)
             for i in range(P["size"]):
                 c, pools = P["cells"][i], P["pools"][i]
-                # 2. discrete events
+                # 2. discrete events: arrivals, then NetStim and bound drive
                 for pname, pool in pools.items():
                     drive = self.pre_spike.get((name, pname))
-                    if drive is not None:
-                        pool.apply(drive[i])
+                    total = np.zeros(len(pool.cvs)) if drive is None else drive[i].copy()
+                    if pname in ext:
+                        total = total + ext[pname][i]
+                    if drive is not None or pname in ext:
+                        pool.apply(total)
                 V_old = c.V.copy()
                 # 3. cell step with the synapses' conductance, then their decay
                 toy = self.solver.get(name) if isinstance(self.solver, dict) else self.solver

@@ -1,0 +1,132 @@
+"""GPU parity of the external synapse drive (NetStim trains and bound
+inputs) against the network oracle: a cell run on its own with ``Cell.run``
+and with one registered step at a time (``Cell.update``), and a network
+whose populations also take NetStims and bound inputs."""
+
+import numpy as np
+import pytest
+
+from tests.helpers import rmse
+
+pytestmark = pytest.mark.gpu
+
+
+def _cell(precision, *, size=1, netstims=True):
+    from braincell_amd.cell import Cell
+    from braincell_amd.filter import AllRegion, RootLocation
+    from braincell_amd.mech import CableProperty, Channel, NetStim, StateProbe, Synapse
+    from braincell_amd.morph import Branch, Morphology
+    from braincell_amd.policy import CVPerBranch
+
+    m = Morphology.from_root(Branch.from_lengths(lengths=[20.0], radii=[10.0, 10.0], type="soma"))
+    m.attach(0, Branch.from_lengths(lengths=[150.0], radii=[1.5, 0.8], type="basal_dendrite"))
+    c = Cell(m, cv_policy=CVPerBranch(3), pop_size=(size,), precision=precision)
+    c.paint(AllRegion(), CableProperty(resting_potential=-65.0, axial_resistivity=150.0),
+            Channel("IL", g_max=3e-4, E=-54.4), Channel("Na_HH1952", g_max=0.12), Channel("K_HH1952", g_max=0.036))
+
+    class _Dend:
+        def __init__(self, x):
+            self.x = x
+
+        def locations(self, morpho):
+            return [(1, self.x)]
+
+    c.place(RootLocation(0.5), Synapse("ExpSyn", tau=2.0, e=0.0, weight=0.02, name="exp"))
+    c.place(_Dend(0.5), Synapse("Exp2Syn", tau1=0.5, tau2=4.0, e=0.0, weight=0.01, name="exp2"))
+    c.place(_Dend(0.2), Synapse("AMPA", g_max=2.0, name="ampa"))
+    if netstims:
+        c.place(RootLocation(0.5), NetStim(start=0.0, number=6, interval=4.0, weight=0.03))
+        c.place(_Dend(0.2), NetStim(start=1.0, number=4, interval=3.0, weight=8.0, name="d"))
+    c.place(RootLocation(0.5), StateProbe("v"))
+    return c
+
+
+def _oracle(cells, cons, dt, n, mode, bindings=None, V0=None):
+    from oracle.network import NetworkOracle
+
+    pops = [(name, c.lower(), c.pop_size[0], c.V_th) for name, c in cells]
+    o = NetworkOracle(pops, cons, dt, drive_mode=mode, bindings=bindings or {})
+    cvs = {name: [o.pops[name]["cells"][0]._locate(0, 0.5)[1]] for name, _ in cells}
+    return o.run(n, probe_cvs=cvs, pool_probes=[(cells[0][0], "exp"), (cells[0][0], "ampa")])
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_cell_run_with_netstims_and_bound_input_matches_oracle(cuda, precision):
+    """FP64: soma V within 1e-7 mV of the oracle ("run" timing: step n takes
+    the drive of t_{max(n-1, 0)}) and identical spike steps; FP32: RMSE <=
+    0.05 mV and spike steps within one step."""
+    dt, n = 0.025, 1200
+    print(f"cell.run with NetStims, fp{precision}, {n} steps", flush=True)
+    c = _cell(precision)
+    c.bind_synapse_input("exp", lambda: np.asarray([[0.0004]]), weight=0.5)
+    res = c.run(dt=dt, duration=n * dt)
+    V, G, S = _oracle([("P", c)], [], dt, n, "run", bindings={"P": {"exp": [(lambda: np.asarray([[0.0004]]), 0.5,
+                                                                                None)]}})
+    got, ref = res.traces["v"][:, 0], V["P"][:, 0, 0]
+    spk_ref = np.flatnonzero(np.diff((ref >= 0.0).astype(int)) == 1) + 1
+    print("max |dV|", np.abs(got - ref).max(), "oracle spikes", spk_ref.size, "g_max", G[("P", "ampa")].max())
+    assert spk_ref.size >= 2 and G[("P", "ampa")].max() > 0.05
+    t_spk = np.asarray(res.spikes["v"][0])
+    if precision == 64:
+        assert np.abs(got - ref).max() < 1e-7
+        assert np.allclose(t_spk, spk_ref * dt)
+    else:
+        assert rmse(got, ref) <= 0.05
+        assert t_spk.size == spk_ref.size and np.max(np.abs(t_spk - spk_ref * dt)) <= dt + 1e-9
+
+
+def test_cell_update_steps_take_the_previous_step_drive(cuda):
+    """Registered steps one at a time ("update" timing: step m takes the
+    drive of t_{m-1}, the first step none) in FP64 against the oracle."""
+    from braincell_amd import environ
+
+    dt, n = 0.025, 240
+    print(f"cell.update with NetStims, fp64, {n} steps", flush=True)
+    c = _cell(64)
+    vs = []
+    with environ.context(dt=dt):
+        for i in range(n):
+            with environ.context(t=i * dt):
+                c.update()
+            vs.append(float(c.last_trace[0, 0]))
+    V, _, _ = _oracle([("P", c)], [], dt, n, "update")
+    err = np.abs(np.asarray(vs) - V["P"][:, 0, 0]).max()
+    print("max |dV|", err)
+    assert err < 1e-7
+    # the run timing differs from the update timing by the NetStim event at t = 0
+    Vr, _, _ = _oracle([("P", _cell(64))], [], dt, n, "run")
+    assert np.abs(Vr["P"][:, 0, 0] - V["P"][:, 0, 0]).max() > 1e-3
+
+
+def test_network_with_netstims_and_bound_inputs_matches_oracle(cuda):
+    """NetStims and a bound input on the post population add to the delayed
+    arrivals of the same step ("network" timing); FP64 within 1e-7 mV of the
+    oracle with identical spike steps."""
+    from braincell_amd.network import Connection, Network
+
+    dt, T = 0.025, 20.0
+    n = int(round(T / dt))
+    print(f"network with NetStims, fp64, {n} steps", flush=True)
+    post = _cell(64, size=3)
+    src = lambda: np.asarray([[0.001], [0.002], [0.003]])
+    post.bind_synapse_input("exp", src, weight=2.0)
+    pre = _cell(64, size=2, netstims=False)
+    pre.bind_synapse_input("exp", 0.03)  # a constant drive every step makes the pre cells fire
+    net = Network()
+    net.add_population("Post", post)
+    net.add_population("Pre", pre)
+    net.add_connection(Connection("Pre", "Post", [0, 1, 1], [0, 1, 2], "exp2", weight=[0.02, 0.03, 0.01],
+                                  delay=[0.5, 1.0, 0.3]))
+    net.add_connection(Connection("Pre", "Post", [0, 1], [2, 0], "ampa", weight=30.0, delay=0.4))
+    res = net.run(dt=dt, duration=T)
+    cons = [dict(pre=c.pre_population, post=c.post_population, pre_index=c.pre_index, post_index=c.post_index,
+                 synapse=c.synapse, synapse_index=c.synapse_index, weight=c.weight, delay=c.delay)
+            for c in net.all_connections()]
+    V, _, S = _oracle([("Post", post), ("Pre", pre)], cons, dt, n, "network",
+                      bindings={"Post": {"exp": [(src, 2.0, None)]}, "Pre": {"exp": [(0.03, 1.0, None)]}})
+    for name in ("Post", "Pre"):
+        err = np.abs(res.traces[name]["v"] - V[name][:, :, 0]).max()
+        print(name, "max |dV|", err, "spikes", int(S[name].sum()))
+        assert err < 1e-7
+        assert np.array_equal(res.spikes[name], S[name])
+    assert S["Pre"].any() and S["Post"].any()
