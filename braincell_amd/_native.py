@@ -69,6 +69,16 @@ class SynapseDesc(ctypes.Structure):
     ]
 
 
+class CableDesc(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("n_cells", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("max_cv", ctypes.c_int32)] + [(n, ctypes.c_void_p) for n in (
+                    "cell_cv0", "k_off", "k_col", "k_val", "a", "b", "jac", "clamp_off", "clamp_cv", "clamp_on",
+                    "clamp_end", "clamp_amp", "probe_off", "probe_cv", "probe_col")] + [
+                ("trace_ld", ctypes.c_int32), ("trace", ctypes.c_void_p), ("V", ctypes.c_void_p),
+                ("status", ctypes.c_void_p), ("step0", ctypes.c_int32), ("n_steps", ctypes.c_int32),
+                ("dt", ctypes.c_double)]
+
+
 class AdjointDesc(ctypes.Structure):
     _fields_ = [("layout", ctypes.c_void_p), ("n_steps", ctypes.c_int32), ("rec", ctypes.c_void_p),
                 ("trace_bar", ctypes.c_void_p), ("Vbar", ctypes.c_void_p), ("Sbar", ctypes.c_void_p),
@@ -121,6 +131,8 @@ def lib():
     L.bc_point_supported.restype = ctypes.c_int
     L.bc_point_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.bc_point_run.restype = ctypes.c_int
+    L.bc_cable_be_run.argtypes = [ctypes.POINTER(CableDesc), ctypes.c_void_p]
+    L.bc_cable_be_run.restype = ctypes.c_int
     _LIB = L
     return L
 

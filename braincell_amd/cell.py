@@ -237,9 +237,34 @@ class Cell:
             if getattr(self._runtime, "synapse_buffers", None) is not None:
                 self._runtime.synapse_buffers["state"].zero_()
 
+    def _is_cable_be(self) -> bool:
+        from .quad import backward_euler_mi355x_step
+
+        return self._solver is backward_euler_mi355x_step
+
+    def _cable_runtime_for(self, dt: float):
+        """Runtime of the linearised backward Euler on a passive cell."""
+        from .cable import CableLayout, CableRuntime
+
+        if not self._initialized:
+            self.init_state()
+        rt = self._runtime
+        if rt is None or rt.dt != float(dt):
+            old = rt
+            probes = [self._layout.cells[0].probes]
+            rt = CableRuntime(CableLayout([self._model], probes=probes), dt, precision=self.precision,
+                              device=self.device)
+            if old is not None:
+                rt.V.copy_(old.V)
+                rt.step_index = old.step_index
+            self._runtime = rt
+        return rt
+
     def _runtime_for(self, dt: float):
         from .runtime import StaggerRuntime
 
+        if self._is_cable_be():
+            return self._cable_runtime_for(dt)
         if not self._initialized:
             self.init_state()
         rt = self._runtime
@@ -281,6 +306,17 @@ class Cell:
         rt = self._runtime_for(dt)
         n0 = int(round(float(t) / float(dt)))
         rt.step_index = n0
+        if self._is_cable_be():
+            if voltage_only:
+                raise ValueError("voltage_only applies to the staggered kernel")
+            import torch
+
+            cols = torch.as_tensor(rt.L.probe_cv.astype(np.int64), device=rt.V.device)
+            before = rt.V[cols].clone() if n_steps == 1 else None
+            tr = rt.run(n_steps)
+            self.last_trace = tr
+            self._cable_last_spike = None if before is None else (before, tr[0].clone())
+            return tr
         self._set_drive(rt, "update", n0, int(n_steps), float(dt))
         flags = rt.flags
         if voltage_only:
@@ -309,6 +345,8 @@ class Cell:
             raise ValueError("Cell.run(...) produced no timesteps; ensure duration > 0 and dt > 0.")
         step0 = rt.step_index
         start = step0 * float(dt)
+        if self._is_cable_be():
+            return self._run_cable(rt, n, start + times)
         self._set_drive(rt, "run", step0, n, float(dt))
         if rt._spk is None or rt._spk.cap < spike_cap(n, dt) or rt._spk.v_th != self.V_th:
             rt.enable_spikes(self.V_th, spike_cap(n, dt))
@@ -325,10 +363,46 @@ class Cell:
         st = {name: _spikes_by_column(rt, step0, start + times, cols) for name, cols in spike_cols.items()}
         return RunResult(time=start + times, traces=traces, spikes=st)
 
+    def _run_cable(self, rt, n: int, times: np.ndarray) -> RunResult:
+        """Cell.run with the passive-cell backward Euler: voltage probes, and
+        the upward V_th crossings of each probe column."""
+        import torch
+
+        cols = rt.L.probe_cv
+        prev = rt.V[torch.as_tensor(cols.astype(np.int64), device=rt.V.device)].double().cpu().numpy()
+        tr = rt.run(n)
+        torch.cuda.synchronize()
+        rt.check_status()
+        V = tr.double().cpu().numpy()
+        before = np.vstack((prev[None, :], V[:-1]))
+        up = (before < self.V_th) & (V >= self.V_th)
+        self._cable_last_spike = (torch.as_tensor(before[-1]), torch.as_tensor(V[-1]))
+        spikes = [times[np.flatnonzero(up[:, k])] for k in range(V.shape[1])]
+        if not self._probes:
+            return RunResult(time=times, traces={"V": V}, spikes={"V": spikes})
+        points = self._probe_points()
+        col = {int(c): k for k, c in enumerate(cols)}
+        traces, sp = {}, {}
+        for p, cvs in points:
+            if not isinstance(p, StateProbe):
+                raise NotImplementedError(f"probe {p.name!r}: a passive cell stepped by backward_euler records "
+                                          "voltages only")
+            idx = [col[int(c)] for c in cvs]
+            traces[p.name] = V[:, idx]
+            sp[p.name] = [spikes[k] for k in idx]
+        return RunResult(time=times, traces={k: traces[k] for k in sorted(traces)},
+                         spikes={k: sp[k] for k in sorted(sp)})
+
     @property
     def spike(self):
         """Per voltage-trace column, whether the last step crossed V_th
         upwards (the reference's ``Cell.spike`` after ``update``)."""
+        if self._is_cable_be():
+            last = getattr(self, "_cable_last_spike", None)
+            if last is None:
+                raise RuntimeError("no spike record yet: run the cell first")
+            a, b = (x.double().cpu().numpy() for x in last)
+            return (a < self.V_th) & (b >= self.V_th)
         rt = self._runtime
         if rt is None or rt._spk is None:
             raise RuntimeError("no spike record yet: run the cell first")

@@ -126,9 +126,22 @@ def _point_step(target, name, method, args):
 @register_integrator("backward_euler_mi355x", category="implicit", order=1,
                      description="Linearised backward Euler for point neurons on the MI355X HIP kernel.")
 def backward_euler_mi355x_step(target, *args):
-    """One step of ``backward_euler`` (``quad/_backward_euler.py:31-64``):
-    ``(I - dt J) dy = dt f(y_n)`` with the full Jacobian of a
-    :class:`~braincell_amd.single.SingleCompartment` neuron's 11 states."""
+    """One step of ``backward_euler`` (``quad/_backward_euler.py:31-160``):
+    ``(I - dt J) dy = dt f(y_n)``.  A
+    :class:`~braincell_amd.single.SingleCompartment` neuron uses the full
+    Jacobian of its 11 states; a passive :class:`~braincell_amd.cell.Cell`
+    steps V alone with the column-sum Jacobian of the stacked-state form
+    (``braincell_amd.cable``); an active Cell raises ``ValueError`` as the
+    reference's state stacking does."""
+    from ..cell import Cell
+
+    if isinstance(target, Cell):
+        from ..cable import check_passive
+
+        if not target._initialized:
+            target.init_state()
+        check_passive(target._model)
+        return target.advance(environ.get("t", 0.0), environ.get_dt(), 1)
     return _point_step(target, "backward_euler_mi355x", 1, args)
 
 
@@ -142,7 +155,16 @@ def implicit_euler_mi355x_step(target, *args):
     the Newton iterate of y - y_n - dt/2 (f(y_n) + f(y)) = 0 from y_n, i.e.
     ``(I - dt/2 J) dy = dt f(y_n)``.  The reference iterates further only
     while the population-wide residual before an update is below 1e-5; those
-    iterates move y by the square of that residual and are not taken."""
+    iterates move y by the square of that residual and are not taken.
+
+    A :class:`~braincell_amd.cell.Cell` calls its solver as ``solver(cell)``
+    (``cell.py:2374-2377``) while the reference's ``implicit_euler_step``
+    needs ``(target, t, dt)``, so a Cell cannot use it: TypeError, as there."""
+    from ..cell import Cell
+
+    if isinstance(target, Cell):
+        raise TypeError("implicit_euler_step() missing 2 required positional arguments: 't' and 'dt' "
+                        "(a Cell calls its solver with the cell alone)")
     return _point_step(target, "implicit_euler_mi355x", 2, args)
 
 

@@ -351,6 +351,57 @@ int bc_point_supported(int precision);
 int bc_point_run(const bc_point_desc* desc, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Linearised backward Euler on passive multi-compartment cells.  Replaces
+ * backward_euler_step (braincell/quad/_backward_euler.py:71-160) with a
+ * Cell target: apply_standard_solver_step (quad/_util.py:196-262) stacks
+ * the cell's differential states, which only works when V (CV space) is
+ * the only one (a gated channel's states live in point space and cannot be
+ * stacked with V); f(V) = (I_mem(V) + I_clamp) / Cm - K V
+ * (Cell.compute_voltage_derivative, _multi_compartment/cell.py:1892-1999,
+ * K = build_cv_axial_operator, quad/_staggered.py:393-413) and
+ * jacrev_last_dim (quad/_util.py:265-310) gives per CV the column sum
+ * J_i = -a_i - sum_q K_qi.  Per step n, every CV i (global index):
+ *   f_i = b_i - a_i V_i + sum of clamp_amp over the cell's clamps on i with
+ *         clamp_on <= n < clamp_end - sum_e k_val[e] V[cell_cv0 + k_col[e]]
+ *   V_i += dt f_i / (1 - dt jac_i)
+ * with a_i = sum g / Cm (1/ms), b_i = sum g E / Cm (mV/ms), clamp_amp in
+ * mV/ms (clamps on CV boundary points are not in the CV-space derivative,
+ * _multi_compartment/currents.py:166-193).  k_col, clamp_cv and probe_cv
+ * are cell-local CV indices; the trace gets V after each step at the
+ * probes: trace[step * trace_ld + probe_col].  One workgroup per cell holds
+ * its voltages in LDS (2 * max_cv reals, at most 160 KiB).
+ * real = float (precision 32) or double (precision 64). */
+typedef struct bc_cable_be_desc {
+  int32_t precision;
+  int32_t n_cells;
+  int32_t threads;          /* workgroup size, multiple of 64 in [64, 1024] */
+  int32_t max_cv;           /* CVs of the largest cell */
+  const int32_t* cell_cv0;  /* [n_cells + 1] CV range of each cell */
+  const int32_t* k_off;     /* [n_cv + 1] CSR rows of K */
+  const int32_t* k_col;     /* [nnz] */
+  const void* k_val;        /* [nnz] real, 1/ms */
+  const void* a;            /* [n_cv] real */
+  const void* b;            /* [n_cv] real */
+  const void* jac;          /* [n_cv] real */
+  const int32_t* clamp_off; /* [n_cells + 1] clamps of each cell */
+  const int32_t* clamp_cv;  /* [n_clamp] */
+  const int32_t* clamp_on;  /* [n_clamp] first active global step */
+  const int32_t* clamp_end; /* [n_clamp] first inactive global step */
+  const void* clamp_amp;    /* [n_clamp] real */
+  const int32_t* probe_off; /* [n_cells + 1] probes of each cell */
+  const int32_t* probe_cv;  /* [n_probe] */
+  const int32_t* probe_col; /* [n_probe] */
+  int32_t trace_ld;         /* trace columns (0: no trace) */
+  void* trace;              /* [n_steps * trace_ld] real, out */
+  void* V;                  /* [n_cv] real, in/out */
+  uint32_t* status;         /* OR-ed with 1 on a non-finite voltage */
+  int32_t step0;            /* global index of the first step */
+  int32_t n_steps;
+  double dt;                /* ms */
+} bc_cable_be_desc;
+int bc_cable_be_run(const bc_cable_be_desc* desc, void* stream);
+
+/* ------------------------------------------------------------------------
  * Learned gating closure (no counterpart in the reference): an MLP
  * 8 -> 64 -> 64 -> 5 (tanh) that corrects the exponential-Euler update of
  * the 5 gates of the Na/KDR/IL/CaL/pool model at large dt: with delta the

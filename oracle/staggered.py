@@ -219,12 +219,13 @@ class StaggeredOracle:
         return np.where(self.ca_mask, I, 0.0)
 
     def clamp_density(self, t):
-        """currents.py:166-193 with layouts.py:960-982: uA/cm^2 at midpoints."""
+        """currents.py:166-193 with layouts.py:960-982: uA/cm^2 at midpoints
+        (clamps at one point add up, state.py:583-612)."""
         I = np.zeros(self.C)
         for cv, delay, dur, amp in self.mid_clamps:
             lt = t - delay
             if lt >= 0.0 and lt < dur:
-                I[cv] = amp * 1e-3 / self.area_cm2[cv]  # nA -> uA, per cm^2
+                I[cv] += amp * 1e-3 / self.area_cm2[cv]  # nA -> uA, per cm^2
         return I
 
     def edge_current(self, t):

@@ -1,0 +1,205 @@
+"""Backward Euler on passive multi-compartment cells (host logic and
+oracle).  The oracle (``oracle/cable_be.py``) is pinned by closed forms and
+independent routes: a one-CV cell is the exact backward-Euler RC update; the
+CV operator equals the node-tree system with its boundary points solved
+out; the column-sum Jacobian equals the summed rows of a finite-difference
+Jacobian of f; V = E is a fixed point of a uniform leak.  The product's
+tables (CSR operator by pairwise elimination, membrane terms, clamp
+amplitudes) are checked against the oracle, and a numpy restatement of the
+kernel's step over those tables against the oracle's trajectory."""
+
+import copy
+
+import numpy as np
+import pytest
+
+from braincell_amd import environ, workloads as W
+from braincell_amd.cable import CableLayout, check_passive, cv_operator
+from braincell_amd.layout import discretize_cell
+from oracle.cable_be import CableBEOracle
+
+
+def passive_model(seed=0, *, clamps=True, k_leak=False, cm=None, cv_per_branch=4):
+    """The single-tree cell with leak channels only; one CV per branch keeps
+    the explicit axial part of the scheme stable at dt ~ 0.005 ms (four CVs
+    on the 20 um soma need dt ~ 1e-5 ms)."""
+    m = copy.copy(W.single_tree_cell(seed))
+    m.cv_per_branch = cv_per_branch
+    nb = m.n_branches
+    rng = np.random.default_rng(100 + seed)
+    ch = {"IL": {"g_branch": rng.uniform(1e-4, 5e-4, nb), "params": {"E": -65.0}}}
+    if k_leak:
+        ch["K_Leak"] = {"g_branch": np.full(nb, 2e-5), "params": {}}
+    m.channels = ch
+    if cm is not None:
+        m.cm = cm
+    m.clamps = [(0, 0.3, 1.0, 5.0, 0.3), (3, 0.6, 2.0, 4.0, -0.2), (5, 1.0, 0.0, 10.0, 0.5)] if clamps else []
+    m.synapses, m.netstims = [], []
+    return m
+
+
+def one_cv_model():
+    from braincell_amd.model import CellModel
+    from braincell_amd.morph import Branch, Morphology
+
+    mo = Morphology.from_root(Branch.from_lengths(lengths=[20.0], radii=[10.0, 10.0], type="soma"))
+    return CellModel(morph=mo, cv_per_branch=1, cm=1.3, V0=-60.0,
+                     channels={"IL": {"g_branch": np.array([3e-4]), "params": {"E": -70.0}}},
+                     clamps=[(0, 0.5, 0.0, 100.0, 0.1)]).validate()
+
+
+# ------------------------------------------------------------ oracle pins
+def test_one_cv_cell_is_the_exact_backward_euler_rc_update():
+    m = one_cv_model()
+    o = CableBEOracle(m)
+    assert np.abs(o.K).max() < 1e-12
+    dt = 0.1
+    area = o.cell.area_cm2[0]
+    a = 3e-4 * 1e3 / 1.3
+    src = a * -70.0 + 0.1 * 1e-3 / area / 1.3
+    o.reset_state(-60.0)
+    V = -60.0
+    for n in range(20):
+        o.step(n * dt, dt)
+        V = (V + dt * src) / (1.0 + dt * a)
+        assert o.V[0] == pytest.approx(V, rel=1e-13)
+
+
+@pytest.mark.parametrize("seed", [0, 3])
+def test_cv_operator_is_the_node_tree_with_boundaries_solved_out(seed):
+    o = CableBEOracle(passive_model(seed))
+    st = o.cell.st
+    A = st.axial
+    dyn = st.dynamic_rows
+    alg = np.setdiff1d(np.arange(st.n_point), dyn)
+    rng = np.random.default_rng(seed)
+    v = rng.normal(-60.0, 5.0, o.C)
+    # boundary voltages that make every boundary row's axial current vanish
+    va = np.linalg.lstsq(A[np.ix_(alg, alg)], -A[np.ix_(alg, dyn)] @ v, rcond=None)[0]
+    full = np.zeros(st.n_point)
+    full[dyn], full[alg] = v, va
+    np.testing.assert_allclose(o.K @ v, (A @ full)[dyn], rtol=1e-10, atol=1e-9)
+    np.testing.assert_allclose(o.K.sum(axis=1), 0.0, atol=1e-9 * np.abs(o.K).max())
+
+
+def test_column_sum_jacobian_equals_summed_finite_difference_rows():
+    o = CableBEOracle(passive_model(1, k_leak=True))
+    V = np.random.default_rng(2).normal(-62.0, 3.0, o.C)
+    h = 1e-3
+    J = np.empty((o.C, o.C))
+    for i in range(o.C):
+        e = np.zeros(o.C)
+        e[i] = h
+        J[:, i] = (o.f(V + e, 1.5) - o.f(V - e, 1.5)) / (2 * h)
+    np.testing.assert_allclose(o.jacobian_colsum(), J.sum(axis=0), rtol=1e-7, atol=1e-6)
+    assert np.abs(o.jacobian_colsum() - np.diag(J)).max() > 1e-3  # the column sum is not the diagonal
+
+
+def test_uniform_leak_rest_is_a_fixed_point_and_the_axial_part_is_explicit():
+    m = passive_model(2, clamps=False, cv_per_branch=1)
+    m.channels["IL"]["g_branch"][:] = 2e-4
+    o = CableBEOracle(m)
+    o.reset_state(-65.0)
+    o.run(0.002, 500)
+    np.testing.assert_allclose(o.V, -65.0, atol=1e-9)
+    # the stacked-state Jacobian keeps only column sums of K: a perturbation
+    # of a finely divided cell grows at dt 0.025 (reference behaviour)
+    m4 = passive_model(2, clamps=False)
+    o4 = CableBEOracle(m4)
+    o4.reset_state(-65.0)
+    o4.V[5] += 1e-6
+    o4.run(0.025, 20)
+    assert np.abs(o4.V + 65.0).max() > 1.0
+
+
+# ---------------------------------------------------- product vs oracle
+@pytest.mark.parametrize("seed", [0, 1, 4])
+def test_product_tables_equal_the_oracle(seed):
+    m = passive_model(seed, k_leak=True, cm=np.linspace(0.8, 1.6, W.single_tree_cell(seed).n_branches))
+    o = CableBEOracle(m)
+    L = CableLayout([m], probes="all")
+    g = L.geoms[0]
+    # the CSR operator (pairwise elimination) against the oracle's dense Schur complement
+    Kp = np.zeros((o.C, o.C))
+    for i in range(o.C):
+        for e in range(L.k_off[i], L.k_off[i + 1]):
+            Kp[i, L.k_col[e]] += L.k_val[e]
+    np.testing.assert_allclose(Kp, o.K, rtol=1e-11, atol=1e-11 * np.abs(o.K).max())
+    np.testing.assert_allclose(L.jac, o.jacobian_colsum(), rtol=1e-11, atol=1e-9)
+    # the membrane terms and the midpoint clamps reproduce f at a few times
+    V = np.random.default_rng(seed).normal(-60.0, 4.0, o.C)
+    for t in (0.0, 1.5, 2.5, 7.0, 12.0):
+        I = np.zeros(o.C)
+        for (_, cv, t0, t1, amp) in L.clamps:
+            if t0 <= t < t1:
+                I[cv] += amp
+        f = L.b - L.a * V + I - Kp @ V
+        np.testing.assert_allclose(f, o.f(V, t), rtol=1e-10, atol=1e-8)
+    # the clamp on a branch end (a CV boundary point) is not in the CV-space derivative
+    assert len(L.clamps) == 2 and g.n_cv == o.C
+
+
+def test_kernel_step_restated_over_the_tables_follows_the_oracle():
+    """The kernel's arithmetic (CSR gather, clamp windows in steps, the
+    column-sum update) in numpy over the product tables, 400 steps."""
+    from braincell_amd.runtime import first_step_at
+
+    dt, n = 0.002, 400
+    m = passive_model(0, cv_per_branch=1)
+    L = CableLayout([m], probes="all")
+    on = [first_step_at(c[2], dt) for c in L.clamps]
+    end = [first_step_at(c[3], dt) for c in L.clamps]
+    V = L.V0.copy()
+    o = CableBEOracle(m)
+    o.reset_state(m.V0)
+    ref = o.run(dt, n)
+    for s in range(n):
+        s_ax = np.array([np.dot(L.k_val[L.k_off[i]:L.k_off[i + 1]], V[L.k_col[L.k_off[i]:L.k_off[i + 1]]])
+                         for i in range(V.size)])
+        I = np.zeros(V.size)
+        for k, c in enumerate(L.clamps):
+            if on[k] <= s < end[k]:
+                I[c[1]] += c[4]
+        V = V + dt * ((L.b - L.a * V + I) - s_ax) / (1.0 - dt * L.jac)
+        assert np.abs(V - ref[s]).max() < 1e-9
+
+
+def test_active_cells_raise_and_implicit_euler_needs_t_and_dt():
+    with pytest.raises(ValueError, match="point space"):
+        check_passive(W.single_tree_cell(0))
+    with pytest.raises(ValueError, match="point space"):
+        CableLayout([W.uneven_tree_cell(0, 200, 300)])
+    from braincell_amd.cell import Cell
+    from braincell_amd.filter import AllRegion
+    from braincell_amd.mech import CableProperty, Channel
+    from braincell_amd.morph import Branch, Morphology
+
+    mo = Morphology.from_root(Branch.from_lengths(lengths=[20.0], radii=[10.0, 10.0], type="soma"))
+    c = Cell(mo, solver="backward_euler_mi355x")
+    c.paint(AllRegion(), CableProperty(), Channel("IL"), Channel("Na_HH1952"))
+    with environ.context(dt=0.025, t=0.0), pytest.raises(ValueError, match="point space"):
+        c.update()
+    c2 = Cell(mo, solver="implicit_euler_mi355x")
+    c2.paint(AllRegion(), CableProperty(), Channel("IL"))
+    with environ.context(dt=0.025, t=0.0), pytest.raises(TypeError, match="'t' and 'dt'"):
+        c2.update()
+
+
+def test_layout_validation_rejects_out_of_cell_indices():
+    L = CableLayout([passive_model(0), one_cv_model()], probes="soma")
+    assert L.max_cv == 256 and L.n_probes == 2
+    L.k_col[-1] = 1  # the 1-CV cell's only column pointing outside it
+    with pytest.raises(ValueError, match="outside"):
+        L.validate()
+    L = CableLayout([one_cv_model()], probes=[[0]])
+    L.probe_cv[0] = 1
+    with pytest.raises(ValueError, match="outside"):
+        L.validate()
+
+
+def test_operator_rows_follow_the_geometry():
+    m = passive_model(3)
+    g = discretize_cell(m.morph, m.n_cv_branch(), m.per_branch(m.cm), m.per_branch(m.ra))
+    off, cols, vals, colsum = cv_operator(g)
+    assert off[-1] == cols.size and np.all(np.diff(off) >= 1)
+    np.testing.assert_allclose(np.add.reduceat(vals, off[:-1]), 0.0, atol=1e-9 * np.abs(vals).max())
