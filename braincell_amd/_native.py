@@ -65,7 +65,8 @@ class SynapseDesc(ctypes.Structure):
         ("con_pre", ctypes.c_void_p), ("con_delay", ctypes.c_void_p), ("con_w", ctypes.c_void_p),
         ("cv_cell", ctypes.c_void_p), ("spk", ctypes.c_void_p),
         ("n_ext", ctypes.c_int32), ("ext_step0", ctypes.c_int32), ("ext_steps", ctypes.c_int32),
-        ("ext_slot", ctypes.c_void_p), ("ext_drive", ctypes.c_void_p),
+        ("ext_slot", ctypes.c_void_p), ("ext_drive", ctypes.c_void_p), ("n_syn", ctypes.c_int32),
+        ("syn_trace", ctypes.c_void_p),
     ]
 
 

@@ -433,24 +433,37 @@ class Cell:
         (``run.py:108-134``: traces keyed by probe name, sorted)."""
         import torch
 
-        from .probes import channel_current, gate_slot
+        from .probes import channel_current, gate_slot, synapse_probe
 
         points = self._probe_points()
         col = {int(c): k for k, c in enumerate(self._layout.cells[0].probes)}
-        need_states = any(not isinstance(p, StateProbe) for p, _ in points)
+        syns = {syn.instance_name: syn for _, syn in self._synapses}
+        is_syn = lambda p: not isinstance(p, StateProbe) and getattr(p, "mechanism", None) in syns
+        need_states = any(not isinstance(p, StateProbe) and not is_syn(p) for p, _ in points)
+        need_syn = any(is_syn(p) for p, _ in points)
         L = self._layout
         ns = len(L.slots)
         ncv = L.n_cv_total
         rec = None
-        if need_states:
-            rec = torch.empty((n + 1) * (ns + 1) * ncv, dtype=rt.dtype, device=rt.V.device)
-            tr = rt.run(n, chunk=n, rec=rec)
-        else:
-            tr = rt.run(n, chunk=chunk)
-        torch.cuda.synchronize()
+        if need_syn:  # the synapse pass records A, B of every synapse after each step
+            rt.syn_trace = torch.empty((n, 2 * rt.synapse_tables.n_syn), dtype=rt.dtype, device=rt.V.device)
+            rt.synapse_desc.n_syn = rt.synapse_tables.n_syn
+        try:
+            if need_states:
+                rec = torch.empty((n + 1) * (ns + 1) * ncv, dtype=rt.dtype, device=rt.V.device)
+                tr = rt.run(n, chunk=n, rec=rec)
+            else:
+                tr = rt.run(n, chunk=chunk)
+            torch.cuda.synchronize()
+        finally:
+            syn_tr = getattr(rt, "syn_trace", None)
+            rt.syn_trace = None
+            if need_syn:
+                rt.synapse_desc.syn_trace = None
         rt.check_status()
         trace = tr.double().cpu().numpy()
         snaps = rec.double().cpu().numpy().reshape(n + 1, ns + 1, ncv)[1:] if need_states else None
+        syn_rec = syn_tr.double().cpu().numpy().reshape(n, -1, 2) if need_syn else None
         m = self._model
         cc = L.cc
         out = {}
@@ -463,6 +476,24 @@ class Cell:
                                      "call init_state() after placing probes")
                 spike_cols[p.name] = [col[int(c)] for c in cvs]
                 out[p.name] = trace[:, spike_cols[p.name]]
+            elif is_syn(p):
+                tabs = rt.synapse_tables
+                cols = []
+                for c in cvs:
+                    site = [s_ for (g_, nm, j), s_ in tabs.syn_key.items()
+                            if g_ == 0 and nm == p.mechanism and int(tabs.syn_cv[s_]) == int(c)]
+                    if not site:
+                        raise KeyError(f"Probe {p.name!r} could not find a mechanism named {p.mechanism!r} "
+                                       f"at CV {int(c)}.")
+                    cols.append(site[0])
+                A, B = syn_rec[:, cols, 0], syn_rec[:, cols, 1]
+                if isinstance(p, MechanismProbe):
+                    out[p.name] = synapse_probe(syns[p.mechanism], p.field, A, B)
+                else:
+                    if any(int(c) not in col for c in cvs):
+                        raise ValueError(f"probe {p.name!r}: its CVs are not in the layout's probe set")
+                    V = trace[:, [col[int(c)] for c in cvs]]
+                    out[p.name] = synapse_probe(syns[p.mechanism], None, A, B, V)
             elif isinstance(p, MechanismProbe):
                 out[p.name] = snaps[:, 1 + gate_slot(L.slots, p.mechanism, p.field)][:, cvs]
             else:

@@ -90,6 +90,8 @@ struct StaggerArgs {
   const int32_t* ext_slot;
   const real* ext_drive;
   int32_t n_ext, ext_step0, ext_steps;
+  real* syn_trace;  // A, B of every synapse after each step, or NULL
+  int32_t syn_n;    // synapses (row length of syn_trace / 2)
   real v_th;
   // chain tables in LDS form (bc_stagger_desc.chain_rec ..)
   const int4* chain_rec;
@@ -681,6 +683,11 @@ __device__ __forceinline__ void synapse_pass(const StaggerArgs<real>& a, HRow<re
       if (lane == 0) {
         a.syn_state[2 * (int64_t)s] = A;
         a.syn_state[2 * (int64_t)s + 1] = B;
+        if (a.syn_trace != nullptr) {
+          real* o = a.syn_trace + (int64_t)(gstep - a.step0) * (2 * (int64_t)a.syn_n) + 2 * (int64_t)s;
+          o[0] = A;
+          o[1] = B;
+        }
       }
     }
     if (lane == 0) {
@@ -1142,6 +1149,8 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->n_ext = y ? y->n_ext : 0;
   a->ext_step0 = y ? y->ext_step0 : 0;
   a->ext_steps = y ? y->ext_steps : 0;
+  a->syn_trace = y ? static_cast<real*>(y->syn_trace) : nullptr;
+  a->syn_n = y ? y->n_syn : 0;
   a->chain_rec = reinterpret_cast<const int4*>(d->chain_rec);
   a->chain_par = d->chain_par;
   a->kid_pad = d->kid_pad;
@@ -1190,6 +1199,8 @@ int validate(const bc_stagger_desc* d) {
           y->cv_cell == nullptr)
         return bc::fail(BC_INVALID_ARG, "synapse desc: contact tables are NULL with a spike ring");
     }
+    if (y->syn_trace != nullptr && y->n_syn <= 0)
+      return bc::fail(BC_INVALID_ARG, "synapse desc: a synapse trace needs n_syn > 0");
     if (y->ext_slot != nullptr && (y->n_ext <= 0 || y->ext_steps < 0 || y->ext_drive == nullptr))
       return bc::fail(BC_INVALID_ARG, "synapse desc: external drive needs n_ext > 0, ext_steps >= 0 and ext_drive");
   }

@@ -44,7 +44,7 @@ __all__ = ["Population", "Connection", "NetworkRunResult", "Network", "delay_ste
 
 SYN_NPAR = 8  # kernel parameters per synapse (bc_synapse_desc.par)
 
-SPIKE_RECORDINGS = ("population", "none")
+SPIKE_RECORDINGS = ("full", "population", "none")
 
 
 def _name(value, what):
@@ -303,11 +303,12 @@ class Network:
 
     # ------------------------------------------------------------------ build
     def build(self, dt: float, *, delay_quantization: str = "ceil", chunk: int = 200, threads: int = None,
-              rows_per_pack: int = None, autotune: bool = False):
+              rows_per_pack: int = None, autotune: bool = False, full_spikes: bool = False):
         """Lower every population into one packed layout with its synapse and
         contact tables.  Pack plan: ``autotune`` times the candidate plans on
         the device; otherwise ``threads`` / ``rows_per_pack`` or, by default,
-        about 2048 packs of 64 threads (networks are many small cells)."""
+        about 2048 packs of 64 threads (networks are many small cells).
+        ``full_spikes`` puts every CV in the trace (per-CV spike detection)."""
         if not self.populations:
             raise ValueError("Network.run(...) requires at least one population.")
         from .runtime import StaggerLayout
@@ -325,10 +326,14 @@ class Network:
         for n, c in zip(names, cells):
             m = c.lower()
             pts = c._probe_points() if c._probes else []
+            pools = {syn.instance_name for _, syn in c._synapses}
             for p, _ in pts:
-                if not isinstance(p, StateProbe):
-                    raise NotImplementedError(f"network runs record StateProbe voltages; {p!r} is not one")
+                if not isinstance(p, StateProbe) and getattr(p, "mechanism", None) not in pools:
+                    raise NotImplementedError(f"network runs record voltages and synapse states; {p!r} is "
+                                              "neither a StateProbe nor a probe of a synapse pool")
             cvs = sorted({cv for _, cvl in pts for cv in cvl})
+            if full_spikes:
+                cvs = list(range(int(np.sum(m.n_cv_branch()))))
             probe_points[n] = (pts, cvs)
             size = self.populations[n].size
             pop_cell0[n] = n_cells
@@ -362,14 +367,20 @@ class Network:
         if not (float(dt) > 0.0) or not (float(duration) > 0.0):
             raise ValueError("Network.run(...) needs dt > 0 and duration > 0.")
         if spike_recording not in SPIKE_RECORDINGS:
-            raise ValueError(f"spike_recording must be one of {SPIKE_RECORDINGS}, got {spike_recording!r} "
-                             "(per-CV 'full' recording is not kept by the kernel)")
+            raise ValueError(f"Network spike_recording must be 'full', 'population', or 'none', "
+                             f"got {spike_recording!r}.")
         n_steps = int(np.arange(0.0, float(duration), float(dt)).shape[0])
         if n_steps == 0:
             raise ValueError("Network.run(...) produced no timesteps; ensure duration > 0 and dt > 0.")
-        tabs = self.build(dt, delay_quantization=delay_quantization, chunk=chunk)
+        full = spike_recording == "full"
+        tabs = self.build(dt, delay_quantization=delay_quantization, chunk=chunk, full_spikes=full)
         rt = tabs.runtime(n_steps)
         trace = rt.new_trace(n_steps)
+        if full:  # every CV is a trace column and a crossing detector (V_n < V_th <= V_{n+1})
+            rt.enable_spikes(tabs.v_th, (n_steps + 1) // 2)
+        if tabs.has_syn_probes:
+            rt.syn_trace = rt.torch.empty((n_steps, 2 * tabs.n_syn), dtype=rt.dtype, device=rt.device)
+            rt.synapse_desc.n_syn = tabs.n_syn
         rt.run(n_steps, trace, chunk=tabs.launch_steps)
         rt.torch.cuda.synchronize()
         rt.check_status()
@@ -474,20 +485,54 @@ class _NetworkTables:
         set_external_drive(rt, groups, drive_times("network", 0, int(n_steps), self.dt), 0)
         return rt
 
+    @property
+    def has_syn_probes(self) -> bool:
+        return any(not isinstance(p, StateProbe) for n in self.names for p, _ in self.probe_points[n][0])
+
     def result(self, rt, trace, n_steps, spike_recording) -> NetworkRunResult:
+        from .probes import synapse_probe
+
         tr = trace.double().cpu().numpy()
         L = self.L
         traces, spikes = {}, {}
         spk = rt.synapse_buffers["spk"].view(self.spk_len, self.n_cells)[:n_steps].cpu().numpy().astype(bool)
+        syn_rec = None
+        if self.has_syn_probes:
+            syn_rec = rt.syn_trace.double().cpu().numpy().reshape(n_steps, -1, 2)
+            rt.syn_trace = None
+            rt.synapse_desc.syn_trace = None
+        steps = rt.spike_steps() if spike_recording == "full" else None
         for n in self.names:
             g0, size = self.pop_cell0[n], self.net.populations[n].size
             pts, cvs = self.probe_points[n]
             col = {cv: k for k, cv in enumerate(cvs)}
             traces[n] = {}
+            syns = {syn.instance_name: syn for _, syn in self.net.populations[n].cell._synapses}
             for p, cvl in sorted(pts, key=lambda x: x[0].name):
                 cols = np.asarray([[L.cell_probe0[g0 + i] + col[cv] for cv in cvl] for i in range(size)])
                 v = tr[:, cols]  # [n_steps, size, n_loc]
+                if not isinstance(p, StateProbe):
+                    sidx = np.empty((size, len(cvl)), dtype=np.int64)
+                    for i in range(size):
+                        for k, cv in enumerate(cvl):
+                            hit = [s_ for j in range(self._pool_cells(n, p.mechanism))
+                                   for s_ in [self.syn_key[(g0 + i, p.mechanism, j)]]
+                                   if int(self.syn.syn_cv[s_]) == int(cv)]
+                            if not hit:
+                                raise KeyError(f"Probe {p.name!r} could not find a mechanism named "
+                                               f"{p.mechanism!r} at CV {int(cv)}.")
+                            sidx[i, k] = hit[0]
+                    A, B = syn_rec[:, sidx, 0], syn_rec[:, sidx, 1]
+                    field = p.field if hasattr(p, "field") else None
+                    v = synapse_probe(syns[p.mechanism], field, A, B, v)
                 traces[n][p.name] = v[:, :, 0] if len(cvl) == 1 else v
             if spike_recording == "population":
                 spikes[n] = spk[:, g0:g0 + size]
+            elif spike_recording == "full":
+                full = np.zeros((n_steps, size, len(cvs)), dtype=bool)
+                for i in range(size):
+                    for k in range(len(cvs)):
+                        st = steps[L.cell_probe0[g0 + i] + k]  # global steps of this run (from 0)
+                        full[st, i, k] = True
+                spikes[n] = full
         return NetworkRunResult(time=np.arange(n_steps) * self.dt, traces=traces, spikes=spikes)

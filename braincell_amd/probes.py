@@ -15,7 +15,7 @@ import numpy as np
 from .channels import CHANNELS
 from .mech import CurrentProbe, MechanismProbe, StateProbe
 
-__all__ = ["probe_cv", "gate_slot", "channel_current", "CA_CHANNELS"]
+__all__ = ["probe_cv", "gate_slot", "channel_current", "synapse_probe", "CA_CHANNELS"]
 
 CA_CHANNELS = ("CaL_IS2008", "CaT_HM1992")
 _GATING = {"Na_Ba2002": (3, 1), "Na_HH1952": (3, 1), "KDR_Ba2002": (4,), "K_HH1952": (4,), "CaL_IS2008": (2, 1),
@@ -86,3 +86,31 @@ def model_cv_branch(model):
 
 def is_probe(m) -> bool:
     return isinstance(m, (StateProbe, MechanismProbe, CurrentProbe))
+
+
+_SYN_FIELDS = {"ExpSyn": ("g",), "Exp2Syn": ("g", "A", "B"), "AMPA": ("g",), "GABAa": ("g",), "NMDA": ("g", "x")}
+
+
+def synapse_probe(syn, field, A, B, V=None):
+    """A synapse probe from the kernel's recorded states after each step
+    (``probes.py:136-240`` read the runtime synapse node):
+
+    - MechanismProbe (``field`` given): ExpSyn / Exp2Syn g = B - A (uS;
+      ExpSyn keeps A = 0), Exp2Syn A and B; AMPA / GABAa g = B; NMDA g = B,
+      x = A (dimensionless);
+    - CurrentProbe (``field`` None, voltage ``V`` at the synapse's CV): the
+      synapse's ``current(V_post)`` (``synapse/markov.py:115-116, 199-200,
+      260-262, 304-306, 355-357``): g (V - e) in nA for ExpSyn / Exp2Syn,
+      g_max g (E_rev - V) in uA/cm^2 (mS/cm^2 x mV) for the Markov kinds."""
+    kind = syn.synapse_type
+    p = syn.resolved()
+    g = B if syn.markov else B - A
+    if field is None:
+        if syn.markov:
+            return p["g_max"] * g * (p["E_rev"] - V)
+        return g * (V - p["e"])
+    if field not in _SYN_FIELDS[kind]:
+        raise AttributeError(f"{kind} has no probe field {field!r}; fields: {_SYN_FIELDS[kind]}")
+    if field == "g":
+        return g
+    return A if field in ("A", "x") else B

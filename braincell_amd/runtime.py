@@ -960,8 +960,11 @@ class StaggerRuntime:
         if rec is not None and chunk < n_steps:
             raise ValueError("recording needs a single launch (chunk >= n_steps)")
         done = 0
+        syn_trace = getattr(self, "syn_trace", None)  # [n_steps, 2 n_syn] synapse states, or None
         while done < n_steps:
             k = min(chunk, n_steps - done)
+            if syn_trace is not None:
+                self.synapse_desc.syn_trace = syn_trace[done:].data_ptr()
             d = self.desc(k, trace[done:] if self.L.n_probes else None, rec=rec)
             self._keep = d
             check(lib().bc_stagger_run(ctypes.byref(d), self.stream_handle()))

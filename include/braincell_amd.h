@@ -154,6 +154,9 @@ typedef struct bc_synapse_desc {
   int32_t ext_steps;        /* rows of ext_drive */
   const int32_t* ext_slot;  /* [n_syn] column of each synapse (-1: none), or NULL */
   const void* ext_drive;    /* [ext_steps * n_ext] real */
+  int32_t n_syn;            /* synapses (used with syn_trace) */
+  void* syn_trace;          /* [n_steps * 2 n_syn] real, or NULL: A, B of every synapse
+                               after each step of the launch (step-relative rows) */
 } bc_synapse_desc;
 
 /* Spike output of a plain run (get_spike, braincell/_base.py:421-431, set

@@ -202,6 +202,8 @@ class NetworkOracle:
                     "ring": np.zeros((int(dd) + 1, post["size"], n_active)), "cursor": 0})
         self.pre_spike = {}
         self.spikes = {name: np.zeros(self.pops[name]["size"], dtype=bool) for name in self.order}
+        self.cv_spikes = {name: np.zeros((self.pops[name]["size"], self.pops[name]["cells"][0].C), dtype=bool)
+                          for name in self.order}
 
     @staticmethod
     def _netstim(start, number, interval, weight, t):
@@ -273,8 +275,11 @@ class NetworkOracle:
                     c.step(t, self.dt)
                     for pool in pools.values():
                         pool.decay(self.dt)
-                # 4. spike: any CV crossing V_th upwards
-                self.spikes[name][i] = bool(np.any((V_old < P["v_th"]) & (c.V >= P["v_th"])))
+                # 4. spike: any CV crossing V_th upwards (get_spike per CV, reduced per cell
+                #    for the "population" record; the "full" record keeps every CV)
+                cross = (V_old < P["v_th"]) & (c.V >= P["v_th"])
+                self.cv_spikes[name][i] = cross
+                self.spikes[name][i] = bool(np.any(cross))
         # 5. enqueue and advance
         for b in self.blocks:
             spk = self.spikes[b["pre"]][b["pre_index"]].astype(float)
@@ -285,23 +290,34 @@ class NetworkOracle:
         for b in self.blocks:
             b["cursor"] = (b["cursor"] + 1) % b["ring"].shape[0]
 
-    def run(self, n_steps, probe_cvs=None, pool_probes=None):
-        """Returns (V traces {pop: [n_steps, size, n_cv_probe]}, pool g traces
-        {(pop, pool): [n_steps, size, n_active]}, spikes {pop: [n_steps, size]})."""
+    def run(self, n_steps, probe_cvs=None, pool_probes=None, record_cv_spikes=False):
+        """Returns (V traces {pop: [n_steps, size, n_cv_probe]}, pool traces
+        {(pop, pool[, field]): [n_steps, size, n_active]}, spikes {pop:
+        [n_steps, size]}).  A pool probe (pop, pool) records g; (pop, pool,
+        field) records "g", "A" or "B" (Exp2Syn), "x" (NMDA, = A).  With
+        ``record_cv_spikes`` the per-CV crossings land in
+        ``self.cv_spike_trace`` {pop: [n_steps, size, C]}."""
         probe_cvs = probe_cvs or {}
         pool_probes = pool_probes or []
         V = {k: np.zeros((n_steps, self.pops[k]["size"], len(v))) for k, v in probe_cvs.items()}
         G = {k: np.zeros((n_steps, self.pops[k[0]]["size"], len(self.pops[k[0]]["pools"][0][k[1]].cvs)))
              for k in pool_probes}
         S = {k: np.zeros((n_steps, self.pops[k]["size"]), dtype=bool) for k in self.order}
+        if record_cv_spikes:
+            self.cv_spike_trace = {k: np.zeros((n_steps,) + self.cv_spikes[k].shape, dtype=bool) for k in self.order}
         for n in range(n_steps):
             self.step(n)
             for k, cvs in probe_cvs.items():
                 for i, c in enumerate(self.pops[k]["cells"]):
                     V[k][n, i] = c.V[cvs]
-            for (pk, pn) in pool_probes:
+            for key in pool_probes:
+                pk, pn = key[0], key[1]
+                field = key[2] if len(key) > 2 else "g"
                 for i, pools in enumerate(self.pops[pk]["pools"]):
-                    G[(pk, pn)][n, i] = pools[pn].g
+                    pool = pools[pn]
+                    G[key][n, i] = pool.g if field == "g" else (pool.A if field in ("A", "x") else pool.B)
             for k in self.order:
                 S[k][n] = self.spikes[k]
+                if record_cv_spikes:
+                    self.cv_spike_trace[k][n] = self.cv_spikes[k]
         return V, G, S

@@ -1,0 +1,96 @@
+"""GPU parity of synapse probes (MechanismProbe / CurrentProbe on a synapse
+pool, ``_multi_compartment/probes.py:136-240``) and of the per-CV ("full")
+spike record of a network (``network/engine.py:512-520``) against the
+network oracle, in a network and in a cell run on its own."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+class _Sites:
+    def locations(self, morpho):
+        return [(0, 0.5), (1, 0.5)]
+
+
+class _Soma:
+    def locations(self, morpho):
+        return [(0, 0.5)]
+
+
+def _probed_markov_net(precision):
+    from braincell_amd.mech import CurrentProbe, MechanismProbe
+    from tests.test_gpu_network import _markov_net
+
+    net = _markov_net(precision)
+    i_cell, j_cell = net.populations["I"].cell, net.populations["J"].cell
+    i_cell.place(_Sites(), MechanismProbe("g_ampa", "ampa", "g"), MechanismProbe("x_nmda", "nmda", "x"),
+                 CurrentProbe("i_nmda", mechanism="nmda"))
+    j_cell.place(_Soma(), MechanismProbe("g_gaba", "gaba", "g"))  # J's gaba pool sits at (0, 0.5)
+    return net
+
+
+def test_network_synapse_probes_and_full_spikes_match_oracle(cuda):
+    """FP64: pool states and currents within 1e-9 of the oracle; the per-CV
+    spike record equals the oracle's crossings and reduces to the
+    per-cell record."""
+    from oracle.network import NetworkOracle
+
+    dt, T = 0.025, 15.0
+    n = int(round(T / dt))
+    print(f"network synapse probes + full spikes, fp64, {n} steps", flush=True)
+
+    net = _probed_markov_net(64)
+    full = net.run(dt=dt, duration=T, spike_recording="full")
+    pop = net.run(dt=dt, duration=T)
+    cons = [dict(pre=c.pre_population, post=c.post_population, pre_index=c.pre_index, post_index=c.post_index,
+                 synapse=c.synapse, synapse_index=c.synapse_index, weight=c.weight, delay=c.delay)
+            for c in net.all_connections()]
+    pops = [(nm, p.cell.lower(), p.size, p.cell.V_th) for nm, p in net.populations.items()]
+    o = NetworkOracle(pops, cons, dt)
+    cv_i = [o.pops["I"]["cells"][0]._locate(b, x)[1] for b, x in _Sites().locations(None)]
+    V, G, S = o.run(n, probe_cvs={"I": cv_i}, pool_probes=[("I", "ampa", "g"), ("I", "nmda", "x"),
+                                                           ("I", "nmda", "g"), ("J", "gaba", "g")],
+                    record_cv_spikes=True)
+    tr = full.traces
+    np.testing.assert_allclose(tr["I"]["g_ampa"], G[("I", "ampa", "g")], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(tr["I"]["x_nmda"], G[("I", "nmda", "x")], rtol=0, atol=1e-9)
+    i_ref = 0.3 * G[("I", "nmda", "g")] * (0.0 - V["I"])  # g_max g (E_rev - V), uA/cm^2
+    np.testing.assert_allclose(tr["I"]["i_nmda"], i_ref, rtol=0, atol=1e-7)
+    np.testing.assert_allclose(tr["J"]["g_gaba"], G[("J", "gaba", "g")][:, :, 0], rtol=0, atol=1e-9)
+    assert G[("I", "ampa", "g")].max() > 0.05 and G[("I", "nmda", "x")].max() > 0.05
+    for name in net.populations:
+        ref = o.cv_spike_trace[name]
+        got = full.spikes[name]
+        print(name, "per-CV spikes", int(ref.sum()), "got", int(got.sum()))
+        assert got.shape == ref.shape and np.array_equal(got, ref)
+        assert np.array_equal(got.any(axis=2), pop.spikes[name])
+    assert o.cv_spike_trace["I"].sum() > S["I"].sum()  # several CVs cross per cell spike
+
+
+def test_cell_run_synapse_probes_match_oracle(cuda):
+    """A cell run on its own, NetStim-driven: ExpSyn g and current, AMPA g in
+    FP64 against the oracle ("run" drive timing)."""
+    from braincell_amd.mech import CurrentProbe, MechanismProbe
+    from tests.test_gpu_netstim import _cell, _oracle
+
+    class _At:
+        def __init__(self, b, x):
+            self.b, self.x = b, x
+
+        def locations(self, morpho):
+            return [(self.b, self.x)]
+
+    dt, n = 0.025, 800
+    print(f"cell.run synapse probes, fp64, {n} steps", flush=True)
+    c = _cell(64)
+    c.place(_At(0, 0.5), MechanismProbe("g_exp", "exp", "g"), CurrentProbe("i_exp", mechanism="exp"))
+    c.place(_At(1, 0.2), MechanismProbe("g_ampa", "ampa", "g"))
+    res = c.run(dt=dt, duration=n * dt)
+    V, G, _ = _oracle([("P", c)], [], dt, n, "run")
+    np.testing.assert_allclose(res.traces["g_exp"][:, 0], G[("P", "exp")][:, 0, 0], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(res.traces["g_ampa"][:, 0], G[("P", "ampa")][:, 0, 0], rtol=0, atol=1e-10)
+    i_ref = G[("P", "exp")][:, 0, 0] * (V["P"][:, 0, 0] - 0.0)  # g (V - e), nA
+    np.testing.assert_allclose(res.traces["i_exp"][:, 0], i_ref, rtol=0, atol=1e-8)
+    assert G[("P", "exp")].max() > 0.01 and G[("P", "ampa")].max() > 0.05

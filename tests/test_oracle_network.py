@@ -175,7 +175,7 @@ def test_network_api_validation():
         Cell(Morphology.from_root(Branch.from_lengths(lengths=[20.0], radii=[10.0, 10.0], type="soma")),
              pop_size=(2, 2))
     with pytest.raises(ValueError):
-        net.run(dt=0.1, duration=1.0, spike_recording="full")
+        net.run(dt=0.1, duration=1.0, spike_recording="cells")
 
 
 def test_product_tables_gather_the_oracle_contacts():
