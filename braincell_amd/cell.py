@@ -521,6 +521,10 @@ class CellBatch:
         from .runtime import StaggerLayout
 
         self.V_th = float(V_th)
+        for c in cells_or_models:
+            if isinstance(c, Cell) and c._synapse_inputs:
+                raise NotImplementedError("bound synapse inputs belong to Cell.run or a Network population; a "
+                                          "CellBatch takes NetStims only")
         models = [c.lower() if isinstance(c, Cell) else c for c in cells_or_models]
         self.layout = StaggerLayout(models, threads=threads, rows_per_pack=rows_per_pack, probes=probes)
         self.precision = precision
@@ -532,7 +536,21 @@ class CellBatch:
 
         if self._runtime is None or self._runtime.dt != float(dt):
             self._runtime = StaggerRuntime(self.layout, dt, precision=self.precision, device=self.device)
+            if any(c.synapses for c in self.layout.cells):
+                from .synapses import attach_cell_synapses
+
+                attach_cell_synapses(self._runtime, self.layout, dt, self.V_th)
         return self._runtime
+
+    def _drive_groups(self):
+        """Per cell with synapses: its NetStims (cells in a batch have no
+        bound inputs), for the external drive table."""
+        out = []
+        for gid, c in enumerate(self.layout.cells):
+            if c.synapses:
+                pools = {name: len(loc) for (name, _, loc) in c.synapses}
+                out.append((gid, 1, [(ns, cv) for ns, cv in (c.netstims or []) if cv >= 0], {}, pools))
+        return out
 
     def run(self, *, dt: float, duration: float, chunk: int = 200, use_graph: bool = True) -> RunResult:
         import torch
@@ -541,6 +559,10 @@ class CellBatch:
         n = int(np.arange(0.0, float(duration), float(dt)).shape[0])
         step0 = rt.step_index
         times = (step0 + np.arange(n)) * float(dt)
+        if getattr(rt, "synapse_tables", None) is not None:
+            from .synapses import drive_times, set_external_drive
+
+            set_external_drive(rt, self._drive_groups(), drive_times("run", step0, n, float(dt)), step0)
         if rt._spk is None or rt._spk.cap < spike_cap(n, dt) or rt._spk.v_th != self.V_th:
             rt.enable_spikes(self.V_th, spike_cap(n, dt))
         else:

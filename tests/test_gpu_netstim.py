@@ -130,3 +130,30 @@ def test_network_with_netstims_and_bound_inputs_matches_oracle(cuda):
         assert err < 1e-7
         assert np.array_equal(res.spikes[name], S[name])
     assert S["Pre"].any() and S["Post"].any()
+
+
+def test_cell_batch_with_netstims_matches_per_cell_oracle(cuda):
+    """Three different NetStim-driven cells in one packed launch (graph of
+    chunks): each soma trace within 1e-7 mV of its own oracle run in FP64."""
+    from braincell_amd.cell import CellBatch
+    from braincell_amd.mech import NetStim
+
+    dt, n = 0.025, 800
+    print(f"CellBatch with NetStims, fp64, {n} steps", flush=True)
+    cells = [_cell(64) for _ in range(3)]
+
+    class _Soma:
+        def locations(self, morpho):
+            return [(0, 0.5)]
+
+    cells[1].place(_Soma(), NetStim(start=3.0, number=3, interval=2.0, weight=0.05, name="extra"))
+    cells[2]._netstims = []  # no NetStims: its synapses stay at rest
+    b = CellBatch(cells, precision=64, threads=64)
+    res = b.run(dt=dt, duration=n * dt, chunk=200)
+    got = res.traces["V_soma"]
+    for i, c in enumerate(cells):
+        V, _, _ = _oracle([("P", c)], [], dt, n, "run")
+        err = np.abs(got[:, i] - V["P"][:, 0, 0]).max()
+        print("cell", i, "max |dV|", err, "swing", np.ptp(V["P"][:, 0, 0]))
+        assert err < 1e-7
+    assert np.abs(got[:, 0] - got[:, 1]).max() > 1.0 and np.ptp(got[:, 2]) < np.ptp(got[:, 0])

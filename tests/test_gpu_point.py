@@ -67,31 +67,52 @@ def test_point_fp32_rmse_and_spike_times(cuda):
     assert nsp > 0
 
 
+def _noise_sensitive(pop, ref, steps, theta=None, seeds=8):
+    """Neurons that leave the 0.05 mV bar when the FP64 oracle takes
+    FP32-sized relative noise (6e-8) on every state every step, over
+    ``seeds`` noise draws: their union and the failing count per draw.  A
+    chaotic neuron fails under most draws; a marginal one under some."""
+    from oracle.single import HTCPopulationOracle
+
+    n = ref.shape[1]
+    params = {nm: {k: v for k, v in W.HTC_CHANNELS[nm].items() if k != "g_max"} for nm in pop["names"]}
+    union, counts = set(), []
+    for seed in range(seeds):
+        o = HTCPopulationOracle(pop["g"], pop["I_dc_nA"], params, W.HTC_IONS)
+        o.reset_state(pop["V0"])
+        rng = np.random.default_rng(seed)
+        noisy = np.empty((steps, n))
+        for k in range(steps):
+            if theta is None:
+                o.step(0.025)
+            else:
+                o.implicit_step(0.025, theta)
+            o.V = o.V * (1 + 6e-8 * rng.standard_normal(n))
+            for key in o.s:
+                o.s[key] = o.s[key] * (1 + 6e-8 * rng.standard_normal(n))
+            noisy[k] = o.V
+        fails = {i for i in range(n) if rmse(noisy[:, i], ref[:, i]) > 0.05}
+        union |= fails
+        counts.append(len(fails))
+    return union, counts
+
+
 def test_point_fp32_long_run_within_fp32_noise_floor(cuda):
     """Over 200 ms a few of these fast-firing neurons are chaotic: the FP64
     oracle with FP32-sized relative noise injected every step diverges on
-    them too.  The FP32 kernel may fail the 0.05 mV bar only on as many
-    neurons as that noise floor does."""
-    from oracle.single import HTCPopulationOracle
-
+    them too.  Every neuron the FP32 kernel takes beyond the 0.05 mV bar
+    must be one that this noise also takes beyond it under some of eight
+    noise draws; all other neurons hold the bar."""
     n, steps = 256, 8000
+    print(f"ind_exp_euler fp32, {n} neurons x {steps} steps", flush=True)
     got, pop, ch, _ = _run(n, steps, 32, n)
     ref = _oracle(pop, ch, n, steps)
-    fail_gpu = sum(rmse(got[:, i], ref[:, i]) > 0.05 for i in range(n))
-    params = {nm: {k: v for k, v in W.HTC_CHANNELS[nm].items() if k != "g_max"} for nm in pop["names"]}
-    o = HTCPopulationOracle(pop["g"], pop["I_dc_nA"], params, W.HTC_IONS)
-    o.reset_state(pop["V0"])
-    rng = np.random.default_rng(5)
-    noisy = np.empty((steps, n))
-    for k in range(steps):
-        o.step(0.025)
-        o.V = o.V * (1 + 6e-8 * rng.standard_normal(n))
-        for key in o.s:
-            o.s[key] = o.s[key] * (1 + 6e-8 * rng.standard_normal(n))
-        noisy[k] = o.V
-    fail_floor = sum(rmse(noisy[:, i], ref[:, i]) > 0.05 for i in range(n))
-    print(f"200 ms: fp32 kernel fails {fail_gpu}/{n}, FP64 + FP32-noise floor fails {fail_floor}/{n}")
-    assert fail_gpu <= fail_floor + 2
+    fail_gpu = {i for i in range(n) if rmse(got[:, i], ref[:, i]) > 0.05}
+    union, counts = _noise_sensitive(pop, ref, steps)
+    print(f"200 ms: fp32 kernel fails {sorted(fail_gpu)}; FP64 + FP32 noise fails {counts} per draw, "
+          f"union {sorted(union)}")
+    assert fail_gpu <= union
+    assert len(union) < n // 10
 
 
 IMPLICIT = [("backward_euler_mi355x", 1.0), ("implicit_euler_mi355x", 0.5)]
@@ -115,32 +136,20 @@ def test_implicit_fp32_within_fp32_noise_floor(cuda, solver, theta):
     """FP32 against the FP64 oracle over 100 ms.  A few fast-firing neurons
     are chaotic under these steps: the FP64 oracle with FP32-sized relative
     noise injected every step leaves the 0.05 mV bar on them too.  The kernel
-    may fail the bar only on as many neurons as that noise floor does, and
-    spike times agree to one step wherever it holds the bar."""
-    from oracle.single import HTCPopulationOracle
-
+    may fail the bar only on neurons that this noise also takes beyond it
+    under some of three noise draws, and spike times agree to one step
+    wherever it holds the bar."""
     n, steps = 256, 4000
     print(f"{solver} fp32, {n} neurons x {steps} steps", flush=True)
     got, pop, ch, _ = _run(n, steps, 32, n, solver=solver)
     ref = _oracle(pop, ch, n, steps, theta=theta)
     errs = np.array([rmse(got[:, i], ref[:, i]) for i in range(n)])
-    params = {nm: {k: v for k, v in W.HTC_CHANNELS[nm].items() if k != "g_max"} for nm in pop["names"]}
-    o = HTCPopulationOracle(pop["g"], pop["I_dc_nA"], params, W.HTC_IONS)
-    o.reset_state(pop["V0"])
-    rng = np.random.default_rng(5)
-    noisy = np.empty((steps, n))
-    for k in range(steps):
-        o.implicit_step(0.025, theta)
-        o.V = o.V * (1 + 6e-8 * rng.standard_normal(n))
-        for key in o.s:
-            o.s[key] = o.s[key] * (1 + 6e-8 * rng.standard_normal(n))
-        noisy[k] = o.V
-    floor = np.array([rmse(noisy[:, i], ref[:, i]) for i in range(n)])
-    fail_gpu, fail_floor = int(np.sum(errs > 0.05)), int(np.sum(floor > 0.05))
-    print(f"{solver} fp32 RMSE median {np.median(errs):.3g}, worst {errs.max():.3g}; fails {fail_gpu}/{n}, "
-          f"FP64 + FP32-noise floor fails {fail_floor}/{n} (median {np.median(floor):.3g})")
+    fail_gpu = set(np.flatnonzero(errs > 0.05).tolist())
+    union, counts = _noise_sensitive(pop, ref, steps, theta=theta, seeds=3)
+    print(f"{solver} fp32 RMSE median {np.median(errs):.3g}, worst {errs.max():.3g}; fails {sorted(fail_gpu)}; "
+          f"FP64 + FP32 noise fails {counts} per draw, union {sorted(union)}")
     assert np.median(errs) <= 1e-3
-    assert fail_gpu <= fail_floor + 2
+    assert fail_gpu <= union
     nsp = 0
     for i in np.flatnonzero(errs <= 0.05):
         a, b = spike_times(got[:, i], 0.025), spike_times(ref[:, i], 0.025)
