@@ -48,11 +48,12 @@ def cpu_baseline(models, soma_cvs, n_steps, gpu_soma, dt):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("cells", "point", "adjoint", "closure", "n144", "network"),
+    ap.add_argument("--workload", choices=("cells", "point", "adjoint", "closure", "n144", "network", "cable"),
                     default="cells",
                     help="cells: 4096 uneven trees (default, the headline); point: 100k HTC point neurons; "
                          "adjoint: 512 region trees, forward + checkpointed adjoint; "
-                         "closure: learned gating closure at dt 0.1 vs plain dt 0.025 on the 4096 trees")
+                         "closure: learned gating closure at dt 0.1 vs plain dt 0.025 on the 4096 trees; "
+                         "cable: backward Euler on passive uneven trees (one CV per branch)")
     ap.add_argument("--point-solver", default="ind_exp_euler_mi355x",
                     choices=("ind_exp_euler_mi355x", "backward_euler_mi355x", "implicit_euler_mi355x"),
                     help="solver of --workload point")
@@ -86,6 +87,8 @@ def main():
         return bench_network(args)
     if args.workload == "n144":
         return bench_n144(args)
+    if args.workload == "cable":
+        return bench_cable(args)
     return bench_cells(args)
 
 
@@ -748,6 +751,90 @@ def bench_network(args):
                      "kernel": "stagger_kernel<float, HH1952, 1, synapses>", "launch_ms": round(launch_s * 1e3, 4),
                      "bytes_per_launch": byts,
                      "bytes_model": "76 B per CV-step + 13 B per contact-step + 40 B per synapse-step"},
+        "cpu_baseline": cpu}), flush=True)
+
+
+def _passive_tree(i: int):
+    """Uneven tree i with one CV per branch, a leak painted everywhere
+    (lognormal g around 3e-4 S/cm^2) and its soma clamp: a cell that the
+    stacked-state backward Euler can step."""
+    import copy
+
+    from braincell_amd import workloads as W
+
+    m = copy.copy(W.uneven_tree_cell(i))
+    nb = m.n_branches
+    rng = np.random.default_rng(10_000 + i)
+    m.channels = {"IL": {"g_branch": 3e-4 * np.exp(0.3 * rng.standard_normal(nb)), "params": {"E": -65.0}}}
+    m.ca, m.cv_per_branch, m.synapses, m.netstims = None, 1, [], []
+    return m
+
+
+def bench_cable(args):
+    """Backward Euler on passive cells (the stacked-state form of a Cell,
+    V += dt f / (1 - dt J)): the uneven-tree morphologies with one CV per
+    branch and a leak, soma clamps, dt 0.001 ms (the axial part is
+    explicit), one workgroup per cell."""
+    import torch
+
+    from braincell_amd.cable import CableLayout, CableRuntime
+
+    dt = 0.001
+    n_cells = args.cells
+    t0 = time.perf_counter()
+    models = [_passive_tree(i) for i in range(n_cells)]
+    L = CableLayout(models, probes="soma")
+    row_abs = np.zeros(L.n_cv_total)
+    np.add.at(row_abs, np.repeat(np.arange(L.n_cv_total), np.diff(L.k_off)), np.abs(L.k_val))
+    log(f"cable layout: {n_cells} cells, {L.n_cv_total} CVs, {L.k_col.size} operator entries, "
+        f"dt x max row |K| = {dt * row_abs.max():.3f}, built in {time.perf_counter() - t0:.1f}s")
+    rt = CableRuntime(L, dt, precision=32, threads=256)
+    K = max(1, args.steps)
+    rt.run(min(args.warmup, 1000) if args.warmup else 1)
+    torch.cuda.synchronize()
+    rt.reset_state()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tr = rt.new_trace(K)
+    torch.cuda.synchronize()
+    e0.record()
+    rt.run(K, tr)
+    e1.record()
+    torch.cuda.synchronize()
+    rt.check_status()
+    el = e0.elapsed_time(e1) * 1e-3
+    n_cv = L.n_cv_total
+    val = n_cv * K / el
+    nnz = L.k_col.size / n_cv
+    per = 8.0 + 8.0 * nnz + 12.0  # CSR offsets, entries, a / b / J per CV-step
+    ach = per * n_cv * K / el / 1e9
+    log(f"{n_cv} CVs x {K} steps in {el * 1e3:.2f} ms: {val:.4g} compartment-updates/s")
+    cpu = None
+    if not args.no_cpu:
+        from oracle.cable_be import CableBEOracle
+
+        ns, sub = 1000, models[:16]
+        orc = [CableBEOracle(m) for m in sub]
+        for o, m in zip(orc, sub):
+            o.reset_state(m.V0)
+        t2 = time.perf_counter()
+        for o in orc:
+            o.run(dt, ns, probe_cvs=[0])
+        cel = time.perf_counter() - t2
+        ncv_sub = sum(o.C for o in orc)
+        cpu = {"value": float(f"{ncv_sub * ns / cel:.4g}"), "unit": "compartment-updates/s", "cores": 1,
+               "kind": "port", "sample": f"NumPy FP64 oracle (dense CV operator), cells 0-15 ({ncv_sub} CVs) x "
+                                         f"{ns} steps, {cel:.1f}s"}
+    print(json.dumps({
+        "metric": "compartment-updates/s", "value": float(f"{val:.4g}"), "unit": "compartment-updates/s",
+        "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": round(el / K * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"backward Euler on {n_cells} passive uneven trees (one CV per branch, leak, soma "
+                               f"clamp), dt {dt} ms", "cells": n_cells, "compartments": n_cv,
+                   "operator_entries": int(L.k_col.size)},
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "cable_be_kernel<float>",
+                     "launch_ms": round(el * 1e3, 3), "bytes_per_launch": per * n_cv * K,
+                     "bytes_model": f"{per:.1f} B per CV-step (CSR offsets, {nnz:.2f} entries, a, b, J)"},
         "cpu_baseline": cpu}), flush=True)
 
 
