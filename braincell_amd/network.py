@@ -177,6 +177,10 @@ class Network:
         self.edge_sets = {}
         self.proj = {}
         self.projections = self.proj
+        self._run_cache = None  # (run key, tables, device runtime): the state a next run continues from
+
+    def _mark_topology_changed(self) -> None:
+        self._run_cache = None
 
     def __repr__(self):
         return (f"Network(name={self.name!r}, populations={len(self.populations)}, "
@@ -188,6 +192,7 @@ class Network:
             raise ValueError(f"Population {name!r} already exists.")
         pop = Population(name, cell)
         self.populations[name] = pop
+        self._mark_topology_changed()
         return pop
 
     def add_connection(self, connection: Connection) -> Connection:
@@ -195,6 +200,7 @@ class Network:
             raise TypeError(f"add_connection expects a Connection, got {type(connection).__name__}.")
         self._check_connection(connection)
         self.connections.append(connection)
+        self._mark_topology_changed()
         return connection
 
     def _check_connection(self, connection: Connection) -> None:
@@ -215,6 +221,7 @@ class Network:
         if edge_set.name in self.edge_sets:
             raise ValueError(f"Network already has an edge set named {edge_set.name!r}.")
         self.edge_sets[edge_set.name] = edge_set
+        self._mark_topology_changed()
         return edge_set
 
     def add_edges(self, *, name: str, pre: str, post: str, method) -> EdgeSet:
@@ -264,6 +271,7 @@ class Network:
         if projection.name in self.proj:
             raise ValueError(f"Network already has a projection named {projection.name!r}.")
         self.proj[projection.name] = projection
+        self._mark_topology_changed()
         return projection
 
     def project(self, *, name: str, edges, synapse, **kwargs) -> Projection:
@@ -361,20 +369,64 @@ class Network:
                               delay_quantization, int(chunk), precision.pop(), v_th.pop())
 
     # -------------------------------------------------------------------- run
-    def run(self, *, dt: float, duration: float, delay_quantization: str = "ceil",
-            spike_recording: str = "population", chunk: int = 200) -> NetworkRunResult:
-        """Run for ``duration`` ms at step ``dt`` ms (``engine.py:294-366``)."""
+    def init_state(self, batch_size=None) -> "Network":
+        """Populations are declarations here; device state is built by the
+        first run (``engine.py:243-266``)."""
+        if batch_size is not None:
+            raise NotImplementedError("batched network state is not supported")
+        return self
+
+    def reset_state(self, batch_size=None) -> "Network":
+        """Reset every cell and synapse to its initial state and the clock to
+        t = 0 (``engine.py:268-292``); the next run starts there."""
+        self.init_state(batch_size)
+        if self._run_cache is not None:
+            rt = self._run_cache[2]
+            rt.reset_state()
+            rt.synapse_buffers["state"].zero_()
+        return self
+
+    def run(self, *, dt: float, duration: float, delay_quantization: str = "ceil", event_backend: str = "auto",
+            brainevent_backend: str = "jax_raw", spike_recording: str = "full", chunk: int = 200) -> NetworkRunResult:
+        """Run for ``duration`` ms at step ``dt`` ms (``engine.py:294-366``),
+        continuing from the state the previous run left (cells, synapses and
+        clock); events still in flight at the end of a run are dropped, as
+        the reference resets its delivery queues before every run.  The event
+        backends select JAX implementations there and are validated only."""
         if not (float(dt) > 0.0) or not (float(duration) > 0.0):
             raise ValueError("Network.run(...) needs dt > 0 and duration > 0.")
+        if event_backend not in ("auto", "scatter", "brainevent"):
+            raise ValueError(f"Network event_backend must be 'auto', 'scatter', or 'brainevent', got {event_backend!r}.")
         if spike_recording not in SPIKE_RECORDINGS:
             raise ValueError(f"Network spike_recording must be 'full', 'population', or 'none', "
                              f"got {spike_recording!r}.")
+        if not self.populations:
+            raise ValueError("Network.run(...) requires at least one population.")
         n_steps = int(np.arange(0.0, float(duration), float(dt)).shape[0])
         if n_steps == 0:
             raise ValueError("Network.run(...) produced no timesteps; ensure duration > 0 and dt > 0.")
         full = spike_recording == "full"
-        tabs = self.build(dt, delay_quantization=delay_quantization, chunk=chunk, full_spikes=full)
-        rt = tabs.runtime(n_steps)
+        key = (float(dt), delay_quantization, full, int(chunk))
+        cache = self._run_cache
+        if cache is None or cache[0] != key:
+            tabs = self.build(dt, delay_quantization=delay_quantization, chunk=chunk, full_spikes=full)
+            rt = tabs.runtime(n_steps)
+            if cache is not None:  # same topology, another mode or dt: carry the state over
+                old = cache[2]
+                t0 = old.step_index * old.dt
+                step0 = int(round(t0 / float(dt)))
+                if abs(step0 * float(dt) - t0) > 1e-9 * max(1.0, abs(t0)):
+                    raise ValueError(f"the network's time {t0} ms is not a multiple of dt = {dt} ms")
+                rt.V.copy_(old.V)
+                rt.S.copy_(old.S)
+                rt.Vbnd.copy_(old.Vbnd)
+                rt.synapse_buffers["state"].copy_(old.synapse_buffers["state"])
+                rt.step_index = step0
+            self._run_cache = (key, tabs, rt)
+        else:
+            tabs, rt = cache[1], cache[2]
+        step0 = rt.step_index
+        tabs.begin_run(rt, step0, n_steps)
         trace = rt.new_trace(n_steps)
         if full:  # every CV is a trace column and a crossing detector (V_n < V_th <= V_{n+1})
             rt.enable_spikes(tabs.v_th, (n_steps + 1) // 2)
@@ -384,7 +436,7 @@ class Network:
         rt.run(n_steps, trace, chunk=tabs.launch_steps)
         rt.torch.cuda.synchronize()
         rt.check_status()
-        return tabs.result(rt, trace, n_steps, spike_recording)
+        return tabs.result(rt, trace, n_steps, spike_recording, step0)
 
 
 class _NetworkTables:
@@ -480,22 +532,38 @@ class _NetworkTables:
         rt.synapse_desc = d
         rt.synapse_buffers = buf
         rt.synapse_tables = self.syn
+        self.begin_run(rt, 0, int(n_steps))
+        return rt
+
+    def begin_run(self, rt, step0: int, n_steps: int) -> None:
+        """Before a run of ``n_steps`` from global step ``step0``: an empty
+        spike history long enough for the run (in-flight events dropped) and
+        the run's external drive."""
+        need = max(self.max_delay + self.launch_steps, int(n_steps))
+        buf = rt.synapse_buffers
+        if self.spk_len < need:
+            self.spk_len = need
+            buf["spk"] = rt.torch.zeros(self.spk_len * self.n_cells, dtype=rt.torch.uint8, device=rt.device)
+            rt.synapse_desc.spk_len = self.spk_len
+            rt.synapse_desc.spk = buf["spk"].data_ptr()
+        else:
+            buf["spk"].zero_()
         groups = [self.net.populations[n].cell._drive_group(self.L, self.pop_cell0[n], self.net.populations[n].size)
                   for n in self.names]
-        set_external_drive(rt, groups, drive_times("network", 0, int(n_steps), self.dt), 0)
-        return rt
+        set_external_drive(rt, groups, drive_times("network", int(step0), int(n_steps), self.dt), int(step0))
 
     @property
     def has_syn_probes(self) -> bool:
         return any(not isinstance(p, StateProbe) for n in self.names for p, _ in self.probe_points[n][0])
 
-    def result(self, rt, trace, n_steps, spike_recording) -> NetworkRunResult:
+    def result(self, rt, trace, n_steps, spike_recording, step0: int = 0) -> NetworkRunResult:
         from .probes import synapse_probe
 
         tr = trace.double().cpu().numpy()
         L = self.L
         traces, spikes = {}, {}
-        spk = rt.synapse_buffers["spk"].view(self.spk_len, self.n_cells)[:n_steps].cpu().numpy().astype(bool)
+        rows = (int(step0) + np.arange(n_steps)) % self.spk_len
+        spk = rt.synapse_buffers["spk"].view(self.spk_len, self.n_cells).cpu().numpy().astype(bool)[rows]
         syn_rec = None
         if self.has_syn_probes:
             syn_rec = rt.syn_trace.double().cpu().numpy().reshape(n_steps, -1, 2)
@@ -532,7 +600,7 @@ class _NetworkTables:
                 full = np.zeros((n_steps, size, len(cvs)), dtype=bool)
                 for i in range(size):
                     for k in range(len(cvs)):
-                        st = steps[L.cell_probe0[g0 + i] + k]  # global steps of this run (from 0)
-                        full[st, i, k] = True
+                        st = steps[L.cell_probe0[g0 + i] + k]  # global steps of this run
+                        full[st - int(step0), i, k] = True
                 spikes[n] = full
-        return NetworkRunResult(time=np.arange(n_steps) * self.dt, traces=traces, spikes=spikes)
+        return NetworkRunResult(time=(int(step0) + np.arange(n_steps)) * self.dt, traces=traces, spikes=spikes)

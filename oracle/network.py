@@ -290,13 +290,24 @@ class NetworkOracle:
         for b in self.blocks:
             b["cursor"] = (b["cursor"] + 1) % b["ring"].shape[0]
 
-    def run(self, n_steps, probe_cvs=None, pool_probes=None, record_cv_spikes=False):
+    def reset_delivery(self):
+        """Empty every delivery ring and rewind its cursor, as
+        ``Network.run`` does before each run (``engine.py:602-609``): events
+        still in flight are dropped; cell and synapse states are kept."""
+        for b in self.blocks:
+            b["ring"][:] = 0.0
+            b["cursor"] = 0
+        self.pre_spike = {}
+
+    def run(self, n_steps, probe_cvs=None, pool_probes=None, record_cv_spikes=False, step0=0):
         """Returns (V traces {pop: [n_steps, size, n_cv_probe]}, pool traces
         {(pop, pool[, field]): [n_steps, size, n_active]}, spikes {pop:
         [n_steps, size]}).  A pool probe (pop, pool) records g; (pop, pool,
         field) records "g", "A" or "B" (Exp2Syn), "x" (NMDA, = A).  With
         ``record_cv_spikes`` the per-CV crossings land in
-        ``self.cv_spike_trace`` {pop: [n_steps, size, C]}."""
+        ``self.cv_spike_trace`` {pop: [n_steps, size, C]}.  ``step0``: the
+        global index of the first step (the clock of a continued run).
+        Probes, pool traces and spikes are indexed from the run's first step."""
         probe_cvs = probe_cvs or {}
         pool_probes = pool_probes or []
         V = {k: np.zeros((n_steps, self.pops[k]["size"], len(v))) for k, v in probe_cvs.items()}
@@ -306,7 +317,7 @@ class NetworkOracle:
         if record_cv_spikes:
             self.cv_spike_trace = {k: np.zeros((n_steps,) + self.cv_spikes[k].shape, dtype=bool) for k in self.order}
         for n in range(n_steps):
-            self.step(n)
+            self.step(step0 + n)
             for k, cvs in probe_cvs.items():
                 for i, c in enumerate(self.pops[k]["cells"]):
                     V[k][n, i] = c.V[cvs]

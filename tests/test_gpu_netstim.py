@@ -118,7 +118,7 @@ def test_network_with_netstims_and_bound_inputs_matches_oracle(cuda):
     net.add_connection(Connection("Pre", "Post", [0, 1, 1], [0, 1, 2], "exp2", weight=[0.02, 0.03, 0.01],
                                   delay=[0.5, 1.0, 0.3]))
     net.add_connection(Connection("Pre", "Post", [0, 1], [2, 0], "ampa", weight=30.0, delay=0.4))
-    res = net.run(dt=dt, duration=T)
+    res = net.run(dt=dt, duration=T, spike_recording="population")
     cons = [dict(pre=c.pre_population, post=c.post_population, pre_index=c.pre_index, post_index=c.post_index,
                  synapse=c.synapse, synapse_index=c.synapse_index, weight=c.weight, delay=c.delay)
             for c in net.all_connections()]

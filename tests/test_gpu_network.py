@@ -70,7 +70,7 @@ def test_network_fp64_matches_oracle(cuda):
     n = int(round(T / dt))
     print(f"network fp64, {n} steps", flush=True)
     net = _net(64)
-    res = net.run(dt=dt, duration=T)
+    res = net.run(dt=dt, duration=T, spike_recording="population")
     V, _, S = _oracle(net, dt, n)
     nsp = 0
     for name in net.populations:
@@ -90,7 +90,7 @@ def test_network_fp32_rmse_and_spike_steps(cuda):
     n = int(round(T / dt))
     print(f"network fp32, {n} steps", flush=True)
     net = _net(32)
-    res = net.run(dt=dt, duration=T)
+    res = net.run(dt=dt, duration=T, spike_recording="population")
     V, _, S = _oracle(net, dt, n)
     for name in net.populations:
         got, ref = res.traces[name]["v"], V[name][:, :, 0]
@@ -110,8 +110,9 @@ def test_network_launches_are_bounded_by_the_shortest_delay(cuda):
     net = _net(64)
     tabs = net.build(0.025)
     assert tabs.min_delay == 10 and tabs.launch_steps == 10
-    a = net.run(dt=0.025, duration=5.0)
-    b = net.run(dt=0.025, duration=5.0, chunk=1)
+    a = net.run(dt=0.025, duration=5.0, spike_recording="population")
+    net.reset_state()
+    b = net.run(dt=0.025, duration=5.0, chunk=1, spike_recording="population")
     for name in net.populations:
         assert np.array_equal(a.traces[name]["v"], b.traces[name]["v"])
         assert np.array_equal(a.spikes[name], b.spikes[name])
@@ -155,7 +156,7 @@ def test_network_with_ampa_nmda_gabaa_matches_oracle(cuda, precision):
     n = int(round(T / dt))
     print(f"markov-synapse network fp{precision}, {n} steps", flush=True)
     net = _markov_net(precision)
-    res = net.run(dt=dt, duration=T)
+    res = net.run(dt=dt, duration=T, spike_recording="population")
     V, G, S = _oracle(net, dt, n, pool_probes=[("I", "ampa"), ("I", "nmda"), ("J", "gaba")])
     for name in net.populations:
         got, ref = res.traces[name]["v"], V[name][:, :, 0]
@@ -218,8 +219,8 @@ def test_projection_network_equals_explicit_connections_and_oracle(cuda):
     print(f"projection network fp64, {n} steps", flush=True)
     net = _projection_net(64, False)
     assert sum(c.n_contact for c in net.all_connections()) > 12
-    a = net.run(dt=dt, duration=T)
-    b = _projection_net(64, True).run(dt=dt, duration=T)
+    a = net.run(dt=dt, duration=T, spike_recording="population")
+    b = _projection_net(64, True).run(dt=dt, duration=T, spike_recording="population")
     V, _, S = _oracle(net, dt, n)
     for name in net.populations:
         assert np.array_equal(a.traces[name]["v"], b.traces[name]["v"])
@@ -229,3 +230,37 @@ def test_projection_network_equals_explicit_connections_and_oracle(cuda):
         assert err < 1e-7
         assert np.array_equal(a.spikes[name], S[name])
     assert S["I"].any()
+
+
+def test_consecutive_runs_continue_the_network(cuda):
+    """A second ``Network.run`` continues cells, synapses and clock from the
+    first and drops events in flight (``engine.py:294-366, 602-609``); FP64
+    against the oracle run the same way; after ``reset_state`` a run repeats
+    the first one exactly."""
+    from oracle.network import NetworkOracle
+
+    dt = 0.025
+    n1, n2 = 200, 400
+    print(f"network continuation fp64, {n1} + {n2} steps", flush=True)
+    net = _net(64)
+    r1 = net.run(dt=dt, duration=n1 * dt, spike_recording="population")
+    r2 = net.run(dt=dt, duration=n2 * dt, spike_recording="population")
+    assert r2.time[0] == pytest.approx(n1 * dt) and r2.time.shape == (n2,)
+    cons = [dict(pre=c.pre_population, post=c.post_population, pre_index=c.pre_index, post_index=c.post_index,
+                 synapse=c.synapse, synapse_index=c.synapse_index, weight=c.weight, delay=c.delay)
+            for c in net.all_connections()]
+    o = NetworkOracle([(n, p.cell.lower(), p.size, p.cell.V_th) for n, p in net.populations.items()], cons, dt)
+    cvs = {n: [o.pops[n]["cells"][0]._locate(0, 0.5)[1]] for n in net.populations}
+    V1, _, S1 = o.run(n1, probe_cvs=cvs)
+    o.reset_delivery()
+    V2, _, S2 = o.run(n2, probe_cvs=cvs, step0=n1)
+    for name in net.populations:
+        for r, V, S in ((r1, V1, S1), (r2, V2, S2)):
+            err = np.abs(r.traces[name]["v"] - V[name][:, :, 0]).max()
+            assert err < 1e-7, (name, err)
+            assert np.array_equal(r.spikes[name], S[name])
+    assert S2["I"].any()
+    net.reset_state()
+    r3 = net.run(dt=dt, duration=n1 * dt, spike_recording="population")
+    for name in net.populations:
+        assert np.array_equal(r3.traces[name]["v"], r1.traces[name]["v"])

@@ -42,8 +42,9 @@ def test_network_synapse_probes_and_full_spikes_match_oracle(cuda):
     print(f"network synapse probes + full spikes, fp64, {n} steps", flush=True)
 
     net = _probed_markov_net(64)
-    full = net.run(dt=dt, duration=T, spike_recording="full")
-    pop = net.run(dt=dt, duration=T)
+    full = net.run(dt=dt, duration=T)  # the reference's default record: every CV
+    net.reset_state()
+    pop = net.run(dt=dt, duration=T, spike_recording="population")
     cons = [dict(pre=c.pre_population, post=c.post_population, pre_index=c.pre_index, post_index=c.post_index,
                  synapse=c.synapse, synapse_index=c.synapse_index, weight=c.weight, delay=c.delay)
             for c in net.all_connections()]

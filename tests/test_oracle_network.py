@@ -311,3 +311,31 @@ def test_markov_synapses_need_a_connection_weight_and_lower_to_kernel_terms():
     assert n == pytest.approx((0.025, 0.0, SYN_MARKOV2, 2.0, 0.01, 0.5, 0.5, 0.0))
     gab = Synapse("GABAa").resolved()
     assert gab["E_rev"] == -70.0 and gab["alpha"] == 0.53 and gab["beta"] == 0.18
+
+
+def test_a_new_run_drops_events_in_flight_and_keeps_the_state():
+    """``Network.run`` resets the delivery queues before every run
+    (``engine.py:602-609``): a spike sent at the last step of one run with a
+    two-step delay never arrives in the next, while cell states carry on."""
+    from oracle.network import NetworkOracle
+
+    def make():
+        pre, post = soma_cell(1), soma_cell(1, synapse=EXP)
+        o = NetworkOracle([("E", pre.lower(), 1, 0.0), ("I", post.lower(), 1, 0.0)],
+                          [dict(pre="E", post="I", pre_index=[0], post_index=[0], synapse="exp", delay=0.2)], DT,
+                          solver={"E": UP, "I": DOWN})
+        for c in o.pops["E"]["cells"]:
+            c.V = np.full(c.C, -10.0)
+        for c in o.pops["I"]["cells"]:
+            c.V = np.full(c.C, -65.0)
+        return o
+
+    whole = make()
+    _, G, _ = whole.run(4, pool_probes=[("I", "exp")])
+    assert G[("I", "exp")][2, 0, 0] == pytest.approx(1.0)
+    split = make()
+    split.run(1, pool_probes=[("I", "exp")])
+    split.reset_delivery()
+    V2, G2, _ = split.run(3, probe_cvs={"I": [0]}, pool_probes=[("I", "exp")], step0=1)
+    assert np.all(G2[("I", "exp")] == 0.0)
+    np.testing.assert_allclose(V2["I"][:, 0, 0], [-67.0, -68.0, -69.0])  # the toy solver kept stepping
