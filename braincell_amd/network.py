@@ -103,6 +103,12 @@ class Population:
     def size(self) -> int:
         return int(self.cell.pop_size[0])
 
+    def __repr__(self) -> str:
+        return (f"Population(name={self.name!r}, size={self.size}, cell={type(self.cell).__name__}, "
+                f"initialized={bool(getattr(self.cell, '_initialized', False))})")
+
+    __str__ = __repr__
+
 
 @dataclass
 class Connection:
@@ -141,6 +147,17 @@ class Connection:
     @property
     def n_contact(self) -> int:
         return int(self.pre_index.shape[0])
+
+    @property
+    def n_edge(self) -> int:
+        """Alias of ``n_contact`` (``core.py:109-117``)."""
+        return self.n_contact
+
+    def __repr__(self) -> str:
+        return (f"Connection(pre_population={self.pre_population!r}, post_population={self.post_population!r}, "
+                f"synapse={self.synapse!r}, n_contact={self.n_contact})")
+
+    __str__ = __repr__
 
     def per_contact(self, value, what):
         arr = np.asarray(value, dtype=float)
@@ -186,6 +203,22 @@ class Network:
         return (f"Network(name={self.name!r}, populations={len(self.populations)}, "
                 f"edge_sets={len(self.edge_sets)}, projections={len(self.projections)}, "
                 f"connections={len(self.connections)})")
+
+    def __str__(self):
+        """Multi-line summary (``engine.py:96-136``)."""
+        lines = [repr(self), "  populations:"]
+        lines += [f"    {n}: size={p.size}, cell={type(p.cell).__name__}" for n, p in self.populations.items()] \
+            or ["    <none>"]
+        lines.append("  edge_sets:")
+        lines += [f"    {n}: {e.pre_population} -> {e.post_population}, n_edge={e.n_edge}"
+                  for n, e in self.edge_sets.items()] or ["    <none>"]
+        lines.append("  projections:")
+        lines += [f"    {n}: edges={p.edge_set_name!r}, synapse={p.synapse!r}" for n, p in self.proj.items()] \
+            or ["    <none>"]
+        lines.append("  direct connections:")
+        lines += [f"    [{i}]: {c.pre_population} -> {c.post_population}, synapse={c.synapse!r}, "
+                  f"n_contact={c.n_contact}" for i, c in enumerate(self.connections)] or ["    <none>"]
+        return "\n".join(lines)
 
     def add_population(self, name: str, cell) -> Population:
         if name in self.populations:

@@ -264,3 +264,19 @@ def test_projection_onto_pool_locations_matches_explicit_connection_tables():
         np.testing.assert_array_equal(getattr(a, k), getattr(b, k), err_msg=k)
     assert a.n_syn == 4 and int(a.con_off[-1]) == 12
     np.testing.assert_array_equal(_run_toy(net, 4), _run_toy(direct, 4))
+
+
+def test_summaries():
+    net = _two_pops(3, 2)
+    net.add_edges(name="E_to_I", pre="E", post="I", method=pairs([(0, 1), (2, 0)]))
+    net.project(name="p", edges="E_to_I", synapse="exp")
+    net.add_connection(N.Connection("E", "I", [1], [1], "exp", weight=0.2))
+    assert repr(net) == "Network(name='demo', populations=2, edge_sets=1, projections=1, connections=1)"
+    text = str(net)
+    for part in ("E: size=3", "E_to_I: E -> I, n_edge=2", "p: edges='E_to_I', synapse='exp'",
+                 "[0]: E -> I, synapse='exp', n_contact=1"):
+        assert part in text
+    assert "<none>" in str(N.Network())
+    assert repr(net.populations["I"]).startswith("Population(name='I', size=2")
+    con = net.all_connections()[1]
+    assert con.n_edge == con.n_contact == 2 and "n_contact=2" in repr(con)
