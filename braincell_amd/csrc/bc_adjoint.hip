@@ -54,11 +54,18 @@ struct AdjArgs {
   bc_chan_const<real> cc;
 };
 
+// Composite of a chain's transposed elimination (see tchain_composite):
+// (sd, sr) -> ((a sd + b) / (h sd + 1), (c sd + f sr + g) / (h sd + 1)).
+template <typename real>
+struct alignas(4 * sizeof(real)) TComp {
+  real a, b, h, c, f, g, pad0, pad1;
+};
+
 // LDS of one workgroup: the pack's chain records in the forward kernel's
 // form {j0 | j1 << 16, top & 0xFFFF | n_kids << 16, kid0 | kid1 << 16,
-// first kid entry} (bc_stagger_desc.chain_rec) and the chains' arrival
-// counters first (16-byte aligned), then the rows' diagonal, rhs, lowers and
-// uppers, the chains' kid contributions, the mechanism classes and the
+// first kid entry} (bc_stagger_desc.chain_rec) and the chains' elimination
+// composites first (16-byte aligned), then the rows' diagonal, rhs, lowers
+// and uppers, the chains' kid contributions, the mechanism classes and the
 // per-wavefront partial sums of the gradient reduction, then the pack's kid
 // list, parent chains and level offsets (int16), and two more per-chain
 // (real, real) buffers and two int16 pointer arrays for the back
@@ -67,8 +74,141 @@ constexpr int kRedSlots = 16;  // 4 wavefronts x 4 gradient columns
 __host__ __device__ inline size_t adj_lds_bytes(int max_rows, int max_chains, int max_cls, int max_levels,
                                                 size_t real_size) {
   const size_t mc = (size_t)((max_chains + 3) & ~3);
-  return mc * 16 + mc * 4 + (size_t)(4 * max_rows + 4 * max_chains + max_cls * BC_NCLS + kRedSlots) * real_size +
+  return mc * 16 + mc * 8 * real_size +
+         (size_t)(4 * max_rows + 4 * max_chains + max_cls * BC_NCLS + kRedSlots) * real_size +
          (4 * mc + (size_t)((max_levels + 4) & ~3)) * sizeof(int16_t);
+}
+
+// Orders LDS accesses of the lanes of one wavefront across this point.
+__device__ __forceinline__ void adj_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Transposed elimination of row j: with (sd, sr) what the rows below
+// subtract, e = d - sd, r = rhs - sr, the pivot is e - lo and the row passes
+// up (up e, lo r) / (e - lo).  In homogeneous coordinates sd = x/z,
+// sr = y/z the step is linear,
+//   x' = -up x + up d z,   y' = -lo y + lo r z,   z' = -x + (d - lo) z,
+// so a chain's rows compose into one map (the forward solve's composite with
+// lo in place of up in the right-hand-side part), rescaled to z's sd-free
+// term = 1 after every group of four rows.
+template <typename real>
+__device__ __forceinline__ void tcomp_row(real d, real r, real lo, real up, bool valid, real& a, real& b, real& h,
+                                          real& k, real& c, real& f, real& g) {
+  const real dl = d - lo;
+  const real a2 = up * (d * h - a);
+  const real b2 = up * (d * k - b);
+  const real c2 = lo * (r * h - c);
+  const real g2 = lo * (r * k - g);
+  const real h2 = dl * h - a;
+  const real k2 = dl * k - b;
+  const real f2 = -lo * f;
+  a = valid ? a2 : a;
+  b = valid ? b2 : b;
+  c = valid ? c2 : c;
+  g = valid ? g2 : g;
+  h = valid ? h2 : h;
+  k = valid ? k2 : k;
+  f = valid ? f2 : f;
+}
+
+// Pass 1: chain k's composite; a leaf chain (input zero) also stores its
+// contribution (b, g).
+template <typename real>
+__device__ __forceinline__ void tchain_composite(int k, const int4* CR, const real* bd, const real* br, const real* LO,
+                                                 const real* UP, TComp<real>* CMP, real* kd, real* kr) {
+  const int4 cr = CR[k];
+  const int j0 = cr.x & 0xFFFF, j1 = (int)((unsigned)cr.x >> 16);
+  const int nk = (int)((unsigned)cr.y >> 16);
+  real a = real(1), b = real(0), h = real(0), kk = real(1), c = real(0), f = real(1), g = real(0);
+  for (int j = j0; j < j1; j += 4) {
+    real gd[4], gr[4], gl[4], gu[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int jj = j + i < j1 ? j + i : j1 - 1;
+      gd[i] = bd[jj];
+      gr[i] = br[jj];
+      gl[i] = LO[jj];
+      gu[i] = UP[jj];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tcomp_row<real>(gd[i], gr[i], gl[i], gu[i], j + i < j1, a, b, h, kk, c, f, g);
+    const real ik = bc_rcp<real>(kk);
+    a *= ik;
+    b *= ik;
+    h *= ik;
+    c *= ik;
+    f *= ik;
+    g *= ik;
+    kk = real(1);
+  }
+  TComp<real> m;
+  m.a = a;
+  m.b = b;
+  m.h = h;
+  m.c = c;
+  m.f = f;
+  m.g = g;
+  m.pad0 = m.pad1 = real(0);
+  CMP[k] = m;
+  if (nk == 0) {
+    kd[k] = b;
+    kr[k] = g;
+  }
+}
+
+// Sum of chain k's kid contributions in a fixed order (first kid, second
+// kid, then the rest of the kid list).
+template <typename real>
+__device__ __forceinline__ void tkid_sum(const int4& cr, const int16_t* m_kid, const real* kd, const real* kr,
+                                         real& sd, real& sr) {
+  const int nk = (int)((unsigned)cr.y >> 16);
+  const int q0 = cr.z & 0xFFFF, q1 = (int)((unsigned)cr.z >> 16);
+  const real d0 = kd[q0], r0 = kr[q0], d1 = kd[q1], r1 = kr[q1];
+  sd = nk > 1 ? d0 + d1 : (nk > 0 ? d0 : real(0));
+  sr = nk > 1 ? r0 + r1 : (nk > 0 ? r0 : real(0));
+  for (int q = 2; q < nk; ++q) {
+    const int kq = m_kid[cr.w + q];
+    sd += kd[kq];
+    sr += kr[kq];
+  }
+}
+
+// Pass 3: chain k's rows from its input (sd, sr): bd[j] becomes the inverse
+// pivot, br[j] the reduced right-hand side.  Rows are read four at a time
+// before any of them is written.
+template <typename real>
+__device__ __forceinline__ void trows(int k, const int4* CR, const int16_t* m_kid, real* bd, real* br, const real* LO,
+                                      const real* UP, const real* kd, const real* kr) {
+  const int4 cr = CR[k];
+  const int j0 = cr.x & 0xFFFF, j1 = (int)((unsigned)cr.x >> 16);
+  real sd, sr;
+  tkid_sum<real>(cr, m_kid, kd, kr, sd, sr);
+  for (int j = j0; j < j1; j += 4) {
+    real glo[4], gup[4], gd[4], gr[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int jj = j + i < j1 ? j + i : j1 - 1;
+      glo[i] = LO[jj];
+      gup[i] = UP[jj];
+      gd[i] = bd[jj];
+      gr[i] = br[jj];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (j + i < j1) {
+        const real e = gd[i] - sd;
+        const real r = gr[i] - sr;
+        const real inv = bc_rcp<real>(e - glo[i]);
+        bd[j + i] = inv;
+        br[j + i] = r;
+        sd = gup[i] * e * inv;
+        sr = glo[i] * r * inv;
+      }
+    }
+  }
 }
 
 constexpr uint32_t kRowMask = 0xFFFFu;
@@ -82,8 +222,8 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int mc = (a.max_chains + 3) & ~3;
   int4* CR = reinterpret_cast<int4*>(smem_raw);
-  int* m_cnt = reinterpret_cast<int*>(smem_raw + (size_t)mc * 16);  // kids eliminated so far
-  real* bd = reinterpret_cast<real*>(smem_raw + (size_t)mc * 20);
+  TComp<real>* CMP = reinterpret_cast<TComp<real>*>(smem_raw + (size_t)mc * 16);  // elimination composites
+  real* bd = reinterpret_cast<real*>(smem_raw + (size_t)mc * 16 + (size_t)mc * sizeof(TComp<real>));
   real* br = bd + a.max_rows;
   real* LO = br + a.max_rows;
   real* UP = LO + a.max_rows;
@@ -283,7 +423,6 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
       bd[row] = real(0);
       br[row] = real(0);
     }
-    for (int k = tid; k < P.n_ch; k += NT) m_cnt[k] = 0;
     // the record of step n - 1, in flight during the solve
     real nxt[CM][CR_];
 #pragma unroll
@@ -302,77 +441,37 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
     // ---- phase 2: Rbar = M^-T Xbar.  M^T has the same diagonal (rebuilt as
     // e - lo, as in the forward solve); the off-diagonals swap, so the
     // child's rhs contribution uses lowers and the back substitution uppers.
-    // Elimination as a dataflow over chains (no barrier): every thread
-    // starts at leaf chains, bumps the parent's arrival counter after a
-    // chain, and the last kid to arrive goes on with the parent.  A chain's
-    // arithmetic does not depend on which thread takes it.  Rows are read
-    // four at a time before any of them is written.
+    // Elimination through chain composites: pass 1 composes every chain's
+    // rows into one map (all chains at once); pass 2 applies one map per
+    // chain level by level in wavefront 0 (chains are sorted by level, level
+    // 0 being the leaves, whose contributions pass 1 already stored) while
+    // the other wavefronts run pass 3 on the leaf chains; pass 3 then
+    // eliminates every other chain's rows from its kids' contributions.
+    // Every chain's arithmetic is the same whichever thread takes it.
     {
-      const int lv0 = m_lv[0], n_leaf = m_lv[1] - lv0;
-      int leaf = tid;
-      int k = leaf < n_leaf ? lv0 + leaf : -1;
-      while (k >= 0) {
-        const int4 c = CR[k];
-        const int j0 = c.x & 0xFFFF, j1 = (int)((unsigned)c.x >> 16);
-        const int nk = (int)((unsigned)c.y >> 16);
-        real e = bd[j0];
-        real r = br[j0];
-        if (nk > 0) {
-          const int q = c.z & 0xFFFF;
-          e -= kd[q];
-          r -= kr[q];
-        }
-        if (nk > 1) {
-          const int q = (int)((unsigned)c.z >> 16);
-          e -= kd[q];
-          r -= kr[q];
-        }
-        for (int q = 2; q < nk; ++q) {
-          const int kq = m_kid[c.w + q];
-          e -= kd[kq];
-          r -= kr[kq];
-        }
-        real kdc = real(0), krc = real(0);
-        for (int j = j0; j < j1; j += 4) {
-          real glo[4], gup[4], gd[4], gr[4];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int jj = j + i < j1 ? j + i : j1 - 1;
-            glo[i] = LO[jj];
-            gup[i] = UP[jj];
-            gd[i] = bd[jj];
-            gr[i] = br[jj];
+      for (int k = tid; k < P.n_ch; k += NT) tchain_composite<real>(k, CR, bd, br, LO, UP, CMP, kd, kr);
+      __syncthreads();
+      const int lv0 = m_lv[0], lv1 = m_lv[1];
+      const int lw = NT > 64 ? 64 : 0;  // first thread of the leaf crew
+      if (tid < 64) {
+        for (int lev = 1; lev < P.n_lv; ++lev) {
+          const int c1 = m_lv[lev + 1];
+          for (int k = m_lv[lev] + tid; k < c1; k += 64) {
+            const int4 cr = CR[k];
+            const TComp<real> m = CMP[k];
+            real sd, sr;
+            tkid_sum<real>(cr, m_kid, kd, kr, sd, sr);
+            const real iz = bc_rcp<real>(m.h * sd + real(1));
+            kd[k] = (m.a * sd + m.b) * iz;
+            kr[k] = (m.c * sd + m.f * sr + m.g) * iz;
           }
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            if (j + i < j1) {
-              if (j + i > j0) {
-                e = gd[i] - kdc;
-                r = gr[i] - krc;
-              }
-              const real inv = bc_rcp<real>(e - glo[i]);
-              bd[j + i] = inv;
-              br[j + i] = r;
-              kdc = gup[i] * e * inv;
-              krc = glo[i] * r * inv;
-            }
-          }
-        }
-        kd[k] = kdc;
-        kr[k] = krc;
-        const int p = m_par[k];
-        bool climb = false;
-        if (p >= 0) {
-          const int pk = (int)((unsigned)CR[p].y >> 16);
-          climb = __hip_atomic_fetch_add(&m_cnt[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == pk - 1;
-        }
-        if (climb) {
-          k = p;
-        } else {
-          leaf += NT;
-          k = leaf < n_leaf ? lv0 + leaf : -1;
+          adj_wave_sync();
         }
       }
+      if (lw > 0 && tid >= lw)
+        for (int k = lv0 + tid - lw; k < lv1; k += NT - lw) trows<real>(k, CR, m_kid, bd, br, LO, UP, kd, kr);
+      __syncthreads();
+      for (int k = (lw > 0 ? lv1 : lv0) + tid; k < P.n_ch; k += NT) trows<real>(k, CR, m_kid, bd, br, LO, UP, kd, kr);
     }
     __syncthreads();
     // Back substitution in affine form (the transposed system's x_j =

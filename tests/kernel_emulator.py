@@ -94,6 +94,69 @@ def hines_transposed(L, dt, bd_in, b_in, dtype=np.float64):
     return br
 
 
+def hines_transposed_composite(L, dt, bd_in, b_in, dtype=np.float64):
+    """The adjoint kernel's transposed elimination through chain composites:
+    pass 1 composes every chain's rows into (a, b, h, c, f, g), rescaled to
+    k = 1 after each group of four rows; pass 2 applies one composite per
+    chain, level by level, to its kids' summed contributions (first kid,
+    second kid, then the rest); pass 3 eliminates every chain's rows from
+    that sum.  Solves M^T y = b like ``hines_transposed``."""
+    tb = L.dt_tables(dt)
+    LO, UP = tb["lo"].astype(dtype), tb["up"].astype(dtype)
+    bd = bd_in.astype(dtype).copy()
+    br = b_in.astype(dtype).copy()
+    for p in range(L.n_packs):
+        row0, nrows, ch0, nch, lv0, nlv = _pack_ranges(L, p)
+        comp = np.zeros((nch, 6), dtype)
+        kd = np.zeros(nch, dtype)
+        kr = np.zeros(nch, dtype)
+
+        def rows(k):
+            gk = ch0 + k
+            return L.chain_off[gk] - row0, L.chain_off[gk + 1] - row0
+
+        def kid_sum(k):
+            gk = ch0 + k
+            sd = sr = dtype(0)
+            for q in range(L.kid_off[gk], L.kid_off[gk + 1]):
+                sd = sd + kd[L.kids[q]]
+                sr = sr + kr[L.kids[q]]
+            return sd, sr
+
+        for k in range(nch):  # pass 1
+            j0, j1 = rows(k)
+            a, b, h, kk, c, f, g = (dtype(v) for v in (1, 0, 0, 1, 0, 1, 0))
+            for j in range(j0, j1):
+                d, r, lo, up = bd[row0 + j], br[row0 + j], LO[row0 + j], UP[row0 + j]
+                a, b, c, g, h, kk, f = (up * (d * h - a), up * (d * kk - b), lo * (r * h - c), lo * (r * kk - g),
+                                        (d - lo) * h - a, (d - lo) * kk - b, -lo * f)
+                if (j - j0) % 4 == 3 or j == j1 - 1:
+                    ik = dtype(1) / kk
+                    a, b, h, c, f, g, kk = a * ik, b * ik, h * ik, c * ik, f * ik, g * ik, dtype(1)
+            comp[k] = (a, b, h, c, f, g)
+            if L.kid_off[ch0 + k + 1] == L.kid_off[ch0 + k]:
+                kd[k], kr[k] = b, g
+        for lev in range(1, nlv):  # pass 2
+            for k in range(L.level_off[lv0 + lev], L.level_off[lv0 + lev + 1]):
+                a, b, h, c, f, g = comp[k]
+                sd, sr = kid_sum(k)
+                iz = dtype(1) / (h * sd + 1)
+                kd[k], kr[k] = (a * sd + b) * iz, (c * sd + f * sr + g) * iz
+        for k in range(nch):  # pass 3
+            j0, j1 = rows(k)
+            sd, sr = kid_sum(k)
+            for j in range(j0, j1):
+                e = bd[row0 + j] - sd
+                r = br[row0 + j] - sr
+                lo, up = LO[row0 + j], UP[row0 + j]
+                inv = dtype(1) / (e - lo)
+                bd[row0 + j] = inv
+                br[row0 + j] = r
+                sd, sr = up * e * inv, lo * r * inv
+        _backsub(L, p, bd, br, UP)
+    return br
+
+
 def _backsub(L, p, bd, br, coef):
     row0, nrows, ch0, nch, lv0, nlv = _pack_ranges(L, p)
     for lev in range(nlv - 1, -1, -1):

@@ -10,7 +10,7 @@ from braincell_amd.runtime import StaggerLayout, first_step_at
 from oracle import discretize as OD
 from oracle.dhs import dhs_solve
 from oracle.staggered import StaggeredOracle
-from tests.kernel_emulator import bnd_rows, cv_rows, hines_delta, hines_transposed
+from tests.kernel_emulator import bnd_rows, cv_rows, hines_delta, hines_transposed, hines_transposed_composite
 
 CELLS = [lambda: W.single_tree_cell(0)] + [lambda i=i: W.uneven_tree_cell(i, cmin=200, cmax=700) for i in range(4)]
 
@@ -149,6 +149,7 @@ def test_packed_transposed_solve_matches_dense(max_chain_rows):
     bd[rows] = 1 + dt * G
     bb[rows] = b
     y = hines_transposed(L, dt, bd, bb)
+    yc = hines_transposed_composite(L, dt, bd, bb)
     for i, m in enumerate(models):
         st = StaggeredOracle(m).st
         sl = L.cell_slice(i)
@@ -159,6 +160,7 @@ def test_packed_transposed_solve_matches_dense(max_chain_rows):
         rhs[st.dynamic_rows] = b[sl]
         ref = np.linalg.solve(M.T, rhs)
         np.testing.assert_allclose(y[rows[sl]], ref[st.dynamic_rows], rtol=0, atol=1e-10)
+        np.testing.assert_allclose(yc[rows[sl]], ref[st.dynamic_rows], rtol=0, atol=1e-10)
 
 
 def test_slots_are_a_permutation_of_reference_rows():
