@@ -722,11 +722,30 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   const int NT = blockDim.x;
   for (int i = tid; i < P.n_cls * BC_NCLS; i += NT) cls[i] = a.cls_tab[(int64_t)P.cls0 * BC_NCLS + i];
   // the pack's rows and chain tables, copied with independent coalesced
-  // loads (the host precomputed their LDS form)
-  for (int i = tid; i < P.n_rows; i += NT) {
-    const int64_t g = (int64_t)P.row0 + i;
-    R[i].lo = a.lo[g];
-    R[i].up = a.up[g];
+  // loads (the host precomputed their LDS form).  The rows go in batches of
+  // kCopy per thread, every load of a batch issued before its LDS stores, so
+  // the batch pays one global-memory latency instead of kCopy (a lane past
+  // the pack's rows reloads its first row and stores nothing).
+  {
+    constexpr int kCopy = 8;
+    for (int i0 = tid; i0 < P.n_rows; i0 += kCopy * NT) {
+      real l[kCopy], u[kCopy];
+#pragma unroll
+      for (int q = 0; q < kCopy; ++q) {
+        const int i = i0 + q * NT;
+        const int64_t g = (int64_t)P.row0 + (i < P.n_rows ? i : 0);
+        l[q] = a.lo[g];
+        u[q] = a.up[g];
+      }
+#pragma unroll
+      for (int q = 0; q < kCopy; ++q) {
+        const int i = i0 + q * NT;
+        if (i < P.n_rows) {
+          R[i].lo = l[q];
+          R[i].up = u[q];
+        }
+      }
+    }
   }
   for (int i = tid; i < P.n_ch; i += NT) {
     const int g = P.ch0 + i;
