@@ -66,7 +66,8 @@ class SynapseDesc(ctypes.Structure):
         ("cv_cell", ctypes.c_void_p), ("spk", ctypes.c_void_p),
         ("n_ext", ctypes.c_int32), ("ext_step0", ctypes.c_int32), ("ext_steps", ctypes.c_int32),
         ("ext_slot", ctypes.c_void_p), ("ext_drive", ctypes.c_void_p), ("n_syn", ctypes.c_int32),
-        ("syn_trace", ctypes.c_void_p),
+        ("syn_trace", ctypes.c_void_p), ("drive", ctypes.c_void_p), ("max_syn", ctypes.c_int32),
+        ("max_syn_rows", ctypes.c_int32),
     ]
 
 

@@ -445,7 +445,7 @@ class Cell:
         ns = len(L.slots)
         ncv = L.n_cv_total
         rec = None
-        if need_syn:  # the synapse pass records A, B of every synapse after each step
+        if need_syn:  # the synapse update records A, B of every synapse after each step
             rt.syn_trace = torch.empty((n, 2 * rt.synapse_tables.n_syn), dtype=rt.dtype, device=rt.V.device)
             rt.synapse_desc.n_syn = rt.synapse_tables.n_syn
         try:

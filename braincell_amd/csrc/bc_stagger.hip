@@ -91,7 +91,9 @@ struct StaggerArgs {
   const real* ext_drive;
   int32_t n_ext, ext_step0, ext_steps;
   real* syn_trace;  // A, B of every synapse after each step, or NULL
-  int32_t syn_n;    // synapses (row length of syn_trace / 2)
+  int32_t syn_n;    // synapses (row length of syn_trace / 2, of syn_drive)
+  const real* syn_drive;  // [n_steps * syn_n] drive of the launch's steps, or NULL (none)
+  int32_t max_syn, max_syn_rows;  // LDS bound of a pack's synapses / synapse rows
   real v_th;
   // chain tables in LDS form (bc_stagger_desc.chain_rec ..)
   const int4* chain_rec;
@@ -621,80 +623,109 @@ __host__ __device__ inline LdsPlan lds_plan(int max_rows, int max_chains, int ma
 
 constexpr int kSynPar = BC_SYN_NPAR;  // parameters per synapse (bc_synapse_desc.par)
 
-// Synapse pass (network runs), after phase A has written every row: each
-// synapse takes its due arrivals (a gather over its incoming contacts from
-// the spike history of earlier launches), adds its conductance and current
-// at V_n to its row and decays its states.  One wavefront owns a synapse
-// row: its lanes gather the contacts of a synapse in parallel and a fixed
-// butterfly sums them, so the result does not depend on scheduling.
+// Drive of every synapse at every step of a launch, before the launch:
+// drive[k * syn_n + s] = D of synapse s at global step step0 + k, the sum
+// over its incoming contacts of w_c spike(pre_c, step - delay_c), then its
+// external drive.  Launches are no longer than the shortest delay, so every
+// spike read here was written by an earlier launch.  One wavefront per
+// synapse and block of kDriveSteps steps (blockIdx.y), whose history loads
+// are all issued before they are summed: lane l takes contacts l, l + 64,
+// ... in order and a fixed butterfly sums the lanes, so the result does not
+// depend on scheduling.
+constexpr int kDriveSteps = 8;
+
 template <typename real>
-__device__ __forceinline__ void synapse_pass(const StaggerArgs<real>& a, HRow<real>* R, const real* xv, int gstep,
-                                             int tid, int NT) {
-  const int lane = tid & 63, wave = tid >> 6, n_waves = NT >> 6;
-  const int j1 = a.syn_pack_off[blockIdx.x + 1];
-  for (int j = a.syn_pack_off[blockIdx.x] + wave; j < j1; j += n_waves) {
-    const int row = a.syn_row[j];
-    real gs = real(0), gse = real(0);
-    for (int s = a.syn_row_off[j]; s < a.syn_row_off[j + 1]; ++s) {
-      real drive = real(0);
-      if (a.spk != nullptr) {  // no ring: a cell run on its own, no contacts
-        const int c1 = a.con_off[s + 1];
-        for (int c = a.con_off[s] + lane; c < c1; c += 64) {
-          const int src = gstep - a.con_delay[c];
-          if (src >= 0 && a.spk[(int64_t)(src % a.spk_len) * a.n_cells + a.con_pre[c]]) drive += a.con_w[c];
-        }
+__global__ void __launch_bounds__(256) synapse_drive(StaggerArgs<real> a, real* drive, int32_t step0,
+                                                     int32_t n_steps, int32_t k_base) {
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (s >= a.syn_n) return;  // whole wavefronts
+  const int c0 = a.spk != nullptr ? a.con_off[s] : 0;
+  const int c1 = a.spk != nullptr ? a.con_off[s + 1] : 0;
+  const int e = a.ext_slot != nullptr ? a.ext_slot[s] : -1;
+  {
+    const int k0 = k_base + (int)blockIdx.y * kDriveSteps;
+    real dr[kDriveSteps];
+#pragma unroll
+    for (int q = 0; q < kDriveSteps; ++q) dr[q] = real(0);
+    for (int c = c0 + lane; c < c1; c += 64) {
+      const int pre = a.con_pre[c];
+      const int dl = a.con_delay[c];
+      const real w = a.con_w[c];
+      uint8_t hit[kDriveSteps];
+#pragma unroll
+      for (int q = 0; q < kDriveSteps; ++q) {
+        const int src = step0 + k0 + q - dl;
+        hit[q] = (k0 + q < n_steps && src >= 0) ? a.spk[(int64_t)(src % a.spk_len) * a.n_cells + pre] : 0;
       }
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) drive += __shfl_xor(drive, off);
-      if (a.ext_slot != nullptr) {  // arrivals first, then the external drive
-        const int e = a.ext_slot[s];
-        const int k = gstep - a.ext_step0;
-        if (e >= 0 && k >= 0 && k < a.ext_steps) drive += a.ext_drive[(int64_t)k * a.n_ext + e];
-      }
-      const real* q = a.syn_par + (int64_t)s * kSynPar;
-      const int kind = (int)q[2];  // uniform over the wavefront
-      real A = a.syn_state[2 * (int64_t)s];
-      real B = a.syn_state[2 * (int64_t)s + 1];
-      if (kind == BC_SYN_EXP) {
-        // events jump A and B; the solve sees g = B - A; both then decay
-        A += drive * q[3];
-        B += drive * q[4];
-        const real kg = q[0] * (B - A);
-        gs += kg;
-        gse += kg * q[1];
-        A -= q[5] * A;
-        B -= q[6] * B;
-      } else {
-        // the solve sees g = B at its step-start value; the events of the
-        // step drive the ODEs, integrated from the step-start states
-        const real kg = q[0] * B;
-        gs += kg;
-        gse += kg * q[1];
-        if (kind == BC_SYN_MARKOV1) {  // g' = aT D (1 - g) - beta g
-          const real u = q[3] * drive;
-          B = bc_expeuler<real>(B, u * (real(1) - B) - q[4] * B, -(u + q[4]), a.dt);
-        } else {  // NMDA: g' = a1 x (1 - g) - b1 g, x' = a2T D (1 - x) - b2 x (A = x, B = g)
-          const real u = q[5] * drive;
-          const real gn = bc_expeuler<real>(B, q[3] * A * (real(1) - B) - q[4] * B, -(q[3] * A + q[4]), a.dt);
-          A = bc_expeuler<real>(A, u * (real(1) - A) - q[6] * A, -(u + q[6]), a.dt);
-          B = gn;
-        }
-      }
-      if (lane == 0) {
-        a.syn_state[2 * (int64_t)s] = A;
-        a.syn_state[2 * (int64_t)s + 1] = B;
-        if (a.syn_trace != nullptr) {
-          real* o = a.syn_trace + (int64_t)(gstep - a.step0) * (2 * (int64_t)a.syn_n) + 2 * (int64_t)s;
-          o[0] = A;
-          o[1] = B;
-        }
-      }
+      for (int q = 0; q < kDriveSteps; ++q)
+        if (hit[q]) dr[q] += w;
     }
-    if (lane == 0) {
-      R[row].d += gs;
-      R[row].r += gse - gs * xv[row];
+#pragma unroll
+    for (int q = 0; q < kDriveSteps; ++q) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) dr[q] += __shfl_xor(dr[q], off);
+      const int k = k0 + q;
+      if (k < n_steps) {
+        const int ke = step0 + k - a.ext_step0;  // arrivals first, then the external drive
+        if (e >= 0 && ke >= 0 && ke < a.ext_steps) dr[q] += a.ext_drive[(int64_t)ke * a.n_ext + e];
+        if (lane == 0) drive[(int64_t)k * a.syn_n + s] = dr[q];
+      }
     }
   }
+}
+
+// A pack's synapses in LDS for the whole launch (after the pack's tree):
+// per synapse its parameters (kSynPar reals), states A, B and the
+// conductance k g its row sees this step; per synapse row its DHS row and
+// the offset of its first synapse (plus one end offset).
+struct SynPlan {
+  size_t par, st, kg, row, off, total;
+};
+template <typename real>
+__host__ __device__ inline SynPlan syn_plan(size_t base, int max_syn, int max_syn_rows) {
+  const size_t ms = (size_t)(max_syn > 0 ? max_syn : 0), mr = (size_t)(max_syn_rows > 0 ? max_syn_rows : 0);
+  SynPlan p;
+  p.par = (base + 15) & ~(size_t)15;
+  p.st = p.par + ms * kSynPar * sizeof(real);
+  p.kg = p.st + ms * 2 * sizeof(real);
+  p.row = p.kg + ms * sizeof(real);
+  p.off = p.row + mr * sizeof(int32_t);
+  p.total = p.off + (mr + 1) * sizeof(int32_t);
+  return p;
+}
+
+// One synapse at one step: the drive D of the step (events), the
+// conductance k g the solve sees, and the states' decay / integration.
+template <typename real>
+__device__ __forceinline__ void synapse_update(const real* q, real* st, real* kg_out, real drive, real dt) {
+  const int kind = (int)q[2];
+  real A = st[0];
+  real B = st[1];
+  if (kind == BC_SYN_EXP) {
+    // events jump A and B; the solve sees g = B - A; both then decay
+    A += drive * q[3];
+    B += drive * q[4];
+    *kg_out = q[0] * (B - A);
+    A -= q[5] * A;
+    B -= q[6] * B;
+  } else {
+    // the solve sees g = B at its step-start value; the events of the
+    // step drive the ODEs, integrated from the step-start states
+    *kg_out = q[0] * B;
+    if (kind == BC_SYN_MARKOV1) {  // g' = aT D (1 - g) - beta g
+      const real u = q[3] * drive;
+      B = bc_expeuler<real>(B, u * (real(1) - B) - q[4] * B, -(u + q[4]), dt);
+    } else {  // NMDA: g' = a1 x (1 - g) - b1 g, x' = a2T D (1 - x) - b2 x (A = x, B = g)
+      const real u = q[5] * drive;
+      const real gn = bc_expeuler<real>(B, q[3] * A * (real(1) - B) - q[4] * B, -(q[3] * A + q[4]), dt);
+      A = bc_expeuler<real>(A, u * (real(1) - A) - q[6] * A, -(u + q[6]), dt);
+      B = gn;
+    }
+  }
+  st[0] = A;
+  st[1] = B;
 }
 
 template <typename real, uint32_t CH, int MAXC, bool SYN>
@@ -786,6 +817,36 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
   }
   for (int i = tid; i < P.n_bnd; i += NT) xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu] = a.Vbnd[P.bnd0 + i];
+  // the pack's synapses (contiguous: synapse rows r0 .., synapses s0 ..)
+  int syn_s0 = 0, syn_n = 0, syn_nr = 0;
+  uint32_t syn_bad = 0;
+  real* sy_par = nullptr;
+  real* sy_st = nullptr;
+  real* sy_kg = nullptr;
+  int32_t* sy_row = nullptr;
+  int32_t* sy_off = nullptr;
+  if constexpr (SYN) {
+    const SynPlan sp = syn_plan<real>(lp.total, a.max_syn, a.max_syn_rows);
+    sy_par = reinterpret_cast<real*>(smem_raw + sp.par);
+    sy_st = reinterpret_cast<real*>(smem_raw + sp.st);
+    sy_kg = reinterpret_cast<real*>(smem_raw + sp.kg);
+    sy_row = reinterpret_cast<int32_t*>(smem_raw + sp.row);
+    sy_off = reinterpret_cast<int32_t*>(smem_raw + sp.off);
+    const int r0 = a.syn_pack_off[blockIdx.x];
+    syn_nr = a.syn_pack_off[blockIdx.x + 1] - r0;
+    syn_s0 = a.syn_row_off[r0];
+    syn_n = a.syn_row_off[r0 + syn_nr] - syn_s0;
+    if (syn_n > a.max_syn || syn_nr > a.max_syn_rows) {  // the tables do not fit the LDS plan
+      syn_bad = 4u;
+      syn_n = syn_nr = 0;
+    }
+    for (int i = tid; i < syn_n * kSynPar; i += NT) sy_par[i] = a.syn_par[(int64_t)syn_s0 * kSynPar + i];
+    for (int i = tid; i < 2 * syn_n; i += NT) sy_st[i] = a.syn_state[2 * (int64_t)syn_s0 + i];
+    for (int i = tid; i <= syn_nr; i += NT) {
+      if (i < syn_nr) sy_row[i] = a.syn_row[r0 + i];
+      sy_off[i] = a.syn_row_off[r0 + i] - syn_s0;
+    }
+  }
   __syncthreads();
 
   // Static per-row metadata the steps need, kept in registers: for every CV
@@ -824,7 +885,13 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   const int dummy = (a.max_rows + 3) & ~3;
   const real dt = a.dt;
   const real ca_b = a.ca_b;
-  uint32_t bad = 0;
+  uint32_t bad = syn_bad;
+  // drive of the thread's first synapse, loaded one step ahead so that its
+  // latency hides behind the rest of the step
+  real drv_next = real(0);
+  if constexpr (SYN) {
+    if (tid < syn_n && a.syn_drive != nullptr && a.n_steps > 0) drv_next = a.syn_drive[syn_s0 + tid];
+  }
   for (int step = 0; step < a.n_steps; ++step) {
     const int gstep = a.step0 + step;
     if (a.rec) record_state<real, MAXC, NS>(a, P, tid, NT, step, vv, st, info);
@@ -868,9 +935,45 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       R[row].d = real(0);
       R[row].r = ((b & kClampBit) ? clamp_current<real>(a, P, row, gstep) : real(0)) + D;
     }
+    if constexpr (SYN) {
+      // the step's synapse events and state updates (independent of V: the
+      // current reaches the rows below, after the barrier)
+      if (tid < syn_n) {
+        const real drv = drv_next;
+        if (a.syn_drive != nullptr && step + 1 < a.n_steps)
+          drv_next = a.syn_drive[(int64_t)(step + 1) * a.syn_n + syn_s0 + tid];
+        synapse_update<real>(sy_par + tid * kSynPar, sy_st + 2 * tid, sy_kg + tid, drv, dt);
+        if (a.syn_trace != nullptr) {
+          real* o = a.syn_trace + (int64_t)(gstep - a.step0) * (2 * (int64_t)a.syn_n) + 2 * (int64_t)(syn_s0 + tid);
+          o[0] = sy_st[2 * tid];
+          o[1] = sy_st[2 * tid + 1];
+        }
+      }
+      for (int i = tid + NT; i < syn_n; i += NT) {
+        const real drv = a.syn_drive != nullptr ? a.syn_drive[(int64_t)step * a.syn_n + syn_s0 + i] : real(0);
+        synapse_update<real>(sy_par + i * kSynPar, sy_st + 2 * i, sy_kg + i, drv, dt);
+        if (a.syn_trace != nullptr) {
+          real* o = a.syn_trace + (int64_t)(gstep - a.step0) * (2 * (int64_t)a.syn_n) + 2 * (int64_t)(syn_s0 + i);
+          o[0] = sy_st[2 * i];
+          o[1] = sy_st[2 * i + 1];
+        }
+      }
+    }
     __syncthreads();
     if constexpr (SYN) {
-      synapse_pass<real>(a, R, xv, gstep, tid, NT);
+      // each synapse row adds its synapses' conductance k g and current
+      // k g (e - V_n) to its DHS row, synapses in table order
+      for (int j = tid; j < syn_nr; j += NT) {
+        const int row = sy_row[j];
+        real gs = real(0), gse = real(0);
+        for (int i = sy_off[j]; i < sy_off[j + 1]; ++i) {
+          const real kg = sy_kg[i];
+          gs += kg;
+          gse += kg * sy_par[i * kSynPar + 1];
+        }
+        R[row].d += gs;
+        R[row].r += gse - gs * xv[row];
+      }
       __syncthreads();
     }
 
@@ -1030,6 +1133,9 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     }
   }
   for (int i = tid; i < P.n_bnd; i += NT) a.Vbnd[P.bnd0 + i] = xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu];
+  if constexpr (SYN) {
+    for (int i = tid; i < 2 * syn_n; i += NT) a.syn_state[2 * (int64_t)syn_s0 + i] = sy_st[i];
+  }
   if (bad) atomicOr(a.status, bad);
 }
 
@@ -1170,6 +1276,10 @@ int fill_args(const bc_stagger_desc* d, StaggerArgs<real>* a) {
   a->ext_steps = y ? y->ext_steps : 0;
   a->syn_trace = y ? static_cast<real*>(y->syn_trace) : nullptr;
   a->syn_n = y ? y->n_syn : 0;
+  // no ring and no external drive: no synapse ever receives drive
+  a->syn_drive = (y && (y->spk || y->ext_slot)) ? static_cast<const real*>(y->drive) : nullptr;
+  a->max_syn = y ? y->max_syn : 0;
+  a->max_syn_rows = y ? y->max_syn_rows : 0;
   a->chain_rec = reinterpret_cast<const int4*>(d->chain_rec);
   a->chain_par = d->chain_par;
   a->kid_pad = d->kid_pad;
@@ -1218,8 +1328,15 @@ int validate(const bc_stagger_desc* d) {
           y->cv_cell == nullptr)
         return bc::fail(BC_INVALID_ARG, "synapse desc: contact tables are NULL with a spike ring");
     }
+    if (y->n_syn < 0) return bc::fail(BC_INVALID_ARG, "synapse desc: n_syn < 0");
     if (y->syn_trace != nullptr && y->n_syn <= 0)
       return bc::fail(BC_INVALID_ARG, "synapse desc: a synapse trace needs n_syn > 0");
+    if (y->n_syn > 0 && (y->max_syn < 1 || y->max_syn > y->n_syn || y->max_syn_rows < 1 ||
+                         y->max_syn_rows > y->max_syn))
+      return bc::fail(BC_INVALID_ARG, "synapse desc: need 1 <= max_syn_rows <= max_syn <= n_syn (got %d, %d, %d)",
+                      y->max_syn_rows, y->max_syn, y->n_syn);
+    if (y->n_syn > 0 && (y->spk != nullptr || y->ext_slot != nullptr) && y->drive == nullptr)
+      return bc::fail(BC_INVALID_ARG, "synapse desc: the drive workspace is NULL");
     if (y->ext_slot != nullptr && (y->n_ext <= 0 || y->ext_steps < 0 || y->ext_drive == nullptr))
       return bc::fail(BC_INVALID_ARG, "synapse desc: external drive needs n_ext > 0, ext_steps >= 0 and ext_drive");
   }
@@ -1238,7 +1355,8 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
   a.step0 = step0;
   a.n_steps = n_steps;
   a.trace = static_cast<real*>(trace);
-  const size_t lds = lds_plan<real>(d->max_rows, d->max_chains, d->max_cls, d->max_levels).total;
+  size_t lds = lds_plan<real>(d->max_rows, d->max_chains, d->max_cls, d->max_levels).total;
+  if (d->syn != nullptr) lds = syn_plan<real>(lds, d->syn->max_syn, d->syn->max_syn_rows).total;
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "pack needs %zu bytes of LDS (> 160 KiB)", lds);
   if (lds > 64 * 1024) BC_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   if (d->n_packs == 0 || n_steps == 0) return BC_OK;
@@ -1252,6 +1370,15 @@ int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_ste
     const int64_t n = (int64_t)n_steps * y->n_cells;
     hipLaunchKernelGGL(spike_clear, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, y->spk, y->spk_len,
                        y->n_cells, step0, n_steps);
+  }
+  if (const bc_synapse_desc* y = d->syn; y != nullptr && y->n_syn > 0 && a.syn_drive != nullptr) {
+    // the grid's y extent (step blocks) is at most 65535 per launch
+    const int64_t n_blk = ((int64_t)n_steps + kDriveSteps - 1) / kDriveSteps;
+    for (int64_t b0 = 0; b0 < n_blk; b0 += 65535) {
+      const unsigned nb = (unsigned)(n_blk - b0 < 65535 ? n_blk - b0 : 65535);
+      hipLaunchKernelGGL(synapse_drive<real>, dim3((unsigned)((y->n_syn + 3) / 4), nb), dim3(256), 0, s, a,
+                         static_cast<real*>(y->drive), step0, n_steps, (int32_t)(b0 * kDriveSteps));
+    }
   }
   void* args[] = {&a};
   BC_HIP(hipLaunchKernel(fn, dim3(d->n_packs), dim3(d->threads), args, lds, s));

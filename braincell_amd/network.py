@@ -560,6 +560,7 @@ class _NetworkTables:
         d.n_cells, d.spk_len = self.n_cells, self.spk_len
         d.min_delay, d.max_delay = self.launch_steps if not self.con_pre.size else self.min_delay, self.max_delay
         d.v_th = self.v_th
+        self.syn.fill_desc(d)
         for k, v in buf.items():
             setattr(d, k, v.data_ptr())
         rt.synapse_desc = d

@@ -941,9 +941,22 @@ class StaggerRuntime:
         d.dt = self.dt
         d.flags = self.flags
         syn = getattr(self, "synapse_desc", None)
+        if syn is not None:
+            self._syn_drive_workspace(syn, int(n_steps))
         d.syn = ctypes.cast(ctypes.pointer(syn), ctypes.c_void_p) if syn is not None else None
         d.spk = ctypes.cast(ctypes.pointer(self._spk), ctypes.c_void_p) if self._spk is not None else None
         return d
+
+    def _syn_drive_workspace(self, syn, n_steps: int) -> None:
+        """The synapse descriptor's drive workspace, at least n_steps x n_syn
+        reals.  A grown workspace replaces the pointer; the smaller ones stay
+        allocated with the runtime (graphs captured earlier still use them)."""
+        need = max(1, n_steps) * max(1, int(syn.n_syn))
+        ws = getattr(self, "_syn_drive", [])
+        if not ws or ws[-1].numel() < need:
+            ws.append(self.torch.empty(need, dtype=self.dtype, device=self.device))
+            self._syn_drive = ws
+        syn.drive = ws[-1].data_ptr()
 
     def stream_handle(self):
         return ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
@@ -1001,6 +1014,8 @@ class StaggerRuntime:
             from ._native import NativeError
             if st & 2:
                 raise NativeError(4, "chain tree of a pack did not resolve (inconsistent layout)")
+            if st & 4:
+                raise NativeError(4, "a pack's synapses exceed the synapse LDS plan (inconsistent tables)")
             raise NativeError(4, "non-finite membrane voltage detected")
 
 

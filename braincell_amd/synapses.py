@@ -67,6 +67,14 @@ class SynapseTables:
         self.syn_row = row[starts].astype(np.int32)
         self.syn_row_off = np.concatenate((starts, [len(recs)])).astype(np.int32)
         self.syn_pack_off = np.searchsorted(pack[starts], np.arange(L.n_packs + 1), side="left").astype(np.int32)
+        # LDS plan of the kernel: most synapse rows / synapses in one pack
+        per_pack = self.syn_row_off[self.syn_pack_off[1:]] - self.syn_row_off[self.syn_pack_off[:-1]]
+        self.max_syn = int(per_pack.max()) if self.n_syn else 0
+        self.max_syn_rows = int(np.diff(self.syn_pack_off).max()) if self.n_syn else 0
+
+    def fill_desc(self, d) -> None:
+        """Sizes of the tables in a bc_synapse_desc."""
+        d.n_syn, d.max_syn, d.max_syn_rows = self.n_syn, self.max_syn, self.max_syn_rows
 
     def device_buffers(self, torch, dev, dtype) -> dict:
         t = lambda a, ty: torch.as_tensor(np.ascontiguousarray(a)).to(device=dev, dtype=ty)
@@ -159,7 +167,7 @@ def external_drive(tabs: SynapseTables, groups, times: np.ndarray):
 
 
 def attach_cell_synapses(rt, L, dt: float, v_th: float) -> None:
-    """Synapses of cells run on their own: the synapse pass with no spike
+    """Synapses of cells run on their own: the synapse update with no spike
     ring and no contacts (only the external drive reaches them)."""
     import torch
 
@@ -169,6 +177,7 @@ def attach_cell_synapses(rt, L, dt: float, v_th: float) -> None:
     buf = tabs.device_buffers(torch, rt.device, rt.dtype)
     d = SynapseDesc()
     d.n_cells, d.spk_len, d.min_delay, d.max_delay, d.v_th = len(L.cells), 0, 1, 1, float(v_th)
+    tabs.fill_desc(d)
     for k in ("pack_off", "row", "row_off", "par", "state"):
         setattr(d, k, buf[k].data_ptr())
     d.spk = None

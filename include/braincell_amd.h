@@ -89,7 +89,8 @@ typedef struct bc_pack {
  *                 (the last one is the final state)
  *  status         OR-ed with 1 when a non-finite voltage appears, with 2
  *                 when the chain tree of a pack does not resolve (an
- *                 inconsistent layout) (out)
+ *                 inconsistent layout), with 4 when a pack's synapses
+ *                 exceed bc_synapse_desc.max_syn / max_syn_rows (out)
  * real = float (precision 32) or double (precision 64). */
 /* Synapses and delayed spike delivery for a network run (the synaptic step
  * either side of the cell update: braincell/network/engine.py:474-548,
@@ -127,7 +128,18 @@ typedef struct bc_pack {
  * when ext_slot is not NULL, synapse s with ext_slot[s] >= 0 adds
  * ext_drive[(n - ext_step0) * n_ext + ext_slot[s]] to D at global step n,
  * for 0 <= n - ext_step0 < ext_steps (real, the payload units of the
- * synapse: uS for BC_SYN_EXP, dimensionless for the Markov kinds). */
+ * synapse: uS for BC_SYN_EXP, dimensionless for the Markov kinds).
+ *
+ * Each launch first gathers D of every synapse for every step of the launch
+ * into the caller's workspace drive[k * n_syn + s] (k = step - step0; one
+ * wavefront per synapse, its lanes over the contacts in a fixed order and a
+ * fixed butterfly sum, then the external drive), which needs only spikes of
+ * earlier launches.  The staggered kernel then keeps each pack's synapse
+ * states and parameters in LDS for the whole launch (12 reals per synapse
+ * and 2 int32 per synapse row beside the pack's tree: max_syn and
+ * max_syn_rows bound them); the pack's states go back to state at the end.
+ * status is OR-ed with 4 when a pack holds more synapses than max_syn or
+ * more synapse rows than max_syn_rows (its synapses are then skipped). */
 #define BC_SYN_EXP 0
 #define BC_SYN_MARKOV1 1
 #define BC_SYN_MARKOV2 2
@@ -154,9 +166,13 @@ typedef struct bc_synapse_desc {
   int32_t ext_steps;        /* rows of ext_drive */
   const int32_t* ext_slot;  /* [n_syn] column of each synapse (-1: none), or NULL */
   const void* ext_drive;    /* [ext_steps * n_ext] real */
-  int32_t n_syn;            /* synapses (used with syn_trace) */
+  int32_t n_syn;            /* synapses (the par / state / ext_slot length) */
   void* syn_trace;          /* [n_steps * 2 n_syn] real, or NULL: A, B of every synapse
                                after each step of the launch (step-relative rows) */
+  void* drive;              /* [n_steps * n_syn] real workspace of a launch; may be NULL
+                               only when spk and ext_slot are both NULL (no drive) */
+  int32_t max_syn;          /* most synapses in one pack (>= 1 when n_syn > 0) */
+  int32_t max_syn_rows;     /* most synapse rows in one pack (>= 1 when n_syn > 0) */
 } bc_synapse_desc;
 
 /* Spike output of a plain run (get_spike, braincell/_base.py:421-431, set

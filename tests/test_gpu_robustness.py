@@ -62,3 +62,23 @@ def test_nan_in_point_population_raises_nonfinite(cuda):
     sc.reset_state(pop["V0"])
     sc.run(dt=0.025, duration=2.0)
     torch.cuda.synchronize()
+
+
+def test_undersized_synapse_plan_is_reported(cuda):
+    """A synapse descriptor whose LDS plan is smaller than a pack's
+    synapses (forced from the host): the kernel skips those synapses, ends
+    normally and reports status 4."""
+    import torch
+    from tests.test_gpu_network import _markov_net
+
+    net = _markov_net(32)
+    tabs = net.build(0.025, chunk=200)
+    assert tabs.syn.max_syn > 1
+    print(f"undersized synapse plan: max_syn {tabs.syn.max_syn} -> 1", flush=True)
+    rt = tabs.runtime(80)
+    rt.synapse_desc.max_syn = rt.synapse_desc.max_syn_rows = 1
+    rt.run(80, chunk=tabs.launch_steps)
+    torch.cuda.synchronize()
+    assert int(rt.status.item()) & 4
+    with pytest.raises(NativeError, match="synapse"):
+        rt.check_status()

@@ -104,3 +104,35 @@ def test_spike_cap_bounds():
     assert spike_cap(40000, 0.025) == 508
     assert spike_cap(10, 1.0) == 5
     assert RunResult(time=None, traces={}).spikes == {}
+
+
+def test_synapse_descriptor_is_checked_without_gpu():
+    """The synapse descriptor's LDS plan (max_syn, max_syn_rows) and the
+    drive workspace are checked before any launch."""
+    L = _native.lib()
+    d = _native.StaggerDesc()
+    d.precision, d.threads, d.max_rows, d.max_chains, d.max_cls, d.max_levels, d.dt = 32, 64, 64, 8, 1, 1, 0.025
+    y = _native.SynapseDesc()
+    buf = (ctypes.c_int32 * 64)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    y.pack_off = y.row = y.row_off = y.par = y.state = p
+    d.syn = ctypes.cast(ctypes.pointer(y), ctypes.c_void_p)
+    y.n_syn = -1
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    assert b"n_syn" in L.bc_last_error()
+    y.n_syn, y.max_syn, y.max_syn_rows = 10, 0, 0
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    assert b"max_syn" in L.bc_last_error()
+    y.max_syn, y.max_syn_rows = 4, 5  # more rows than synapses
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    y.max_syn, y.max_syn_rows = 11, 2  # more than n_syn
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    y.max_syn = 4
+    y.ext_slot, y.n_ext, y.ext_steps, y.ext_drive = p, 1, 1, p
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1  # external drive without a drive workspace
+    assert b"drive" in L.bc_last_error()
+    y.drive = p
+    d.channels, d.cv_per_thread = 0x604, 1
+    cc = _native.chan_const_type(32)()
+    d.chan_const = ctypes.cast(ctypes.pointer(cc), ctypes.c_void_p)
+    assert L.bc_stagger_run(ctypes.byref(d), None) in (0, 3)  # valid (no packs: nothing to launch)
