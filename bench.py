@@ -68,6 +68,9 @@ def main():
     ap.add_argument("--cpu-sample-steps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep --threads / --rows-per-pack as given")
+    ap.add_argument("--backsub", choices=("jump", "sweep"), default=None,
+                    help="force the back substitution of the chain tree (default: the autotuner's choice, "
+                         "pointer jumping without it)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with a 2 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -124,6 +127,8 @@ def bench_cells(args):
         t0 = time.perf_counter()
         L, _ = autotune(L, dt, precision=32, log=log)
         log(f"autotuned layout {L.summary()} in {time.perf_counter() - t0:.1f}s")
+    if args.backsub:
+        L = L.with_backsub(args.backsub)
     rt = StaggerRuntime(L, dt, precision=32)
     chunk, n_full, tail, Wm = step_plan(args.steps, args.warmup, args.chunk)
     K = n_full * chunk + tail
@@ -237,7 +242,7 @@ def bench_cells(args):
                                f"dendritic CaL/CalciumDetailed, dt 0.025 ms, staggered plain step",
                    "cells": args.cells, "compartments": int(L.n_cv_total), "dhs_rows": int(L.n_rows_total),
                    "packs": L.n_packs, "threads": L.threads, "cv_per_thread": L.cv_per_thread,
-                   "rows_per_pack": L.rows_per_pack, "chain_rows": L.max_chain_rows,
+                   "rows_per_pack": L.rows_per_pack, "chain_rows": L.max_chain_rows, "backsub": L.backsub,
                    "autotuned": not args.no_autotune,
                    "steps_per_launch": chunk, "tail_steps": tail,
                    "parallelism": "1 GPU, one workgroup per pack"},
@@ -263,11 +268,27 @@ def bench_point(args):
 
     n, dt, chunk = 100_000, 0.025, 4000
     pop = W.htc_population(n, seed=1)
+    from braincell_amd.quad import get_integrator
+
+    method = getattr(get_integrator(args.point_solver), "point_method", IND_EXP_EULER)
+    implicit = method != IND_EXP_EULER
+    theta = {1: 1.0, 2: 0.5}.get(method)
+    cpu = None
+    if not args.no_cpu:
+        # the whole population in 16 slices on the host cores, before the GPU is touched
+        ns = 100 if implicit else 2000
+        _CPU_SHARED["pop"] = pop
+        edges = np.linspace(0, n, 17).astype(int)
+        res, cel, workers = _cpu_pool(_point_oracle_slice,
+                                      [(int(a), int(b), ns, theta) for a, b in zip(edges[:-1], edges[1:])])
+        cpu = {"value": float(f"{sum(res) * ns / cel:.4g}"), "unit": "neuron-steps/s", "cores": workers,
+               "kind": "port",
+               "sample": f"FP64 NumPy oracle (vectorised over neurons), all {n} neurons x {ns} steps in 16 slices "
+                         f"on {workers} processes, {cel:.1f}s"}
+        log(f"point CPU baseline {cpu}")
     channels = {nm: dict(W.HTC_CHANNELS[nm], g_max=pop["g"][nm]) for nm in pop["names"]}
     sc = SingleCompartment(n, channels=channels, E_Na=W.HTC_IONS["E_Na"], E_K=W.HTC_IONS["E_K"],
                            ca=W.HTC_IONS["ca"], V0=pop["V0"], precision=32, solver=args.point_solver)
-    implicit = sc.method != IND_EXP_EULER
-    theta = {1: 1.0, 2: 0.5}.get(sc.method)
     flop_step = POINT_IMPLICIT_FLOP if implicit else POINT_FLOP
     kernel = "point_implicit_kernel<float>" if implicit else "point_kernel<float>"
     sc.init_state()
@@ -288,20 +309,6 @@ def bench_point(args):
     launch_s = float(np.mean([evs[i].elapsed_time(evs[i + 1]) * 1e-3 for i in range(K // chunk)]))
     val = n * K / el
     flops = flop_step * n * chunk / launch_s / 1e12
-    cpu = None
-    if not args.no_cpu:
-        from oracle.single import HTCPopulationOracle
-
-        m, ns = (20_000, 40) if implicit else (20_000, 400)
-        g = {nm: pop["g"][nm][:m] for nm in pop["names"]}
-        params = {nm: {k: v for k, v in W.HTC_CHANNELS[nm].items() if k != "g_max"} for nm in pop["names"]}
-        o = HTCPopulationOracle(g, pop["I_dc_nA"][:m], params, W.HTC_IONS)
-        o.reset_state(pop["V0"][:m])
-        t2 = time.perf_counter()
-        o.run(dt, ns, theta=theta)
-        cel = time.perf_counter() - t2
-        cpu = {"value": float(f"{m * ns / cel:.4g}"), "unit": "neuron-steps/s", "cores": 1, "kind": "port",
-               "sample": f"FP64 NumPy oracle (vectorised over neurons), {m} neurons x {ns} steps, {cel:.1f}s"}
     print(json.dumps({
         "metric": "neuron-steps/s", "value": float(f"{val:.4g}"), "unit": "neuron-steps/s", "n_gpus": 1, "steps": K,
         "warmup": chunk, "ms_per_step": round(el / K * 1e3, 6), "higher_is_better": True, "scaling": "weak",
@@ -313,6 +320,89 @@ def bench_point(args):
                      "frac": round(flops / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": kernel,
                      "launch_ms": round(launch_s * 1e3, 3), "flop_per_neuron_step": flop_step},
         "cpu_baseline": cpu}), flush=True)
+
+
+_CPU_SHARED = {}  # inputs of the CPU-baseline workers, set before they are forked
+
+
+def _cpu_pool(fn, jobs):
+    """Runs ``fn`` over ``jobs`` on forked worker processes, one BLAS thread
+    each, over the host cores this process may use (at most 16).  Call it
+    before the GPU is touched.  Returns (results, wall seconds of the whole
+    pool, workers)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+
+    workers = max(1, min(16, len(os.sched_getaffinity(0)), len(jobs)))
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
+        res = list(ex.map(fn, jobs))
+    return res, time.perf_counter() - t0, workers
+
+
+def _point_oracle_slice(job):
+    """Neurons i0 .. i1 of the HTC population through the FP64 NumPy oracle
+    for ns steps (CPU baseline worker)."""
+    i0, i1, ns, theta = job
+    from threadpoolctl import threadpool_limits
+
+    from braincell_amd import workloads as W
+    from oracle.single import HTCPopulationOracle
+
+    pop = _CPU_SHARED["pop"]
+    with threadpool_limits(1):
+        g = {nm: pop["g"][nm][i0:i1] for nm in pop["names"]}
+        params = {nm: {k: v for k, v in W.HTC_CHANNELS[nm].items() if k != "g_max"} for nm in pop["names"]}
+        o = HTCPopulationOracle(g, pop["I_dc_nA"][i0:i1], params, W.HTC_IONS)
+        o.reset_state(pop["V0"][i0:i1])
+        o.run(0.025, ns, theta=theta)
+    return i1 - i0
+
+
+def _n144_oracle_cell(job):
+    """One n144 HH cell through the FP64 NumPy oracle (CPU baseline worker)."""
+    amp, steps, probes = job
+    from threadpoolctl import threadpool_limits
+
+    from braincell_amd import workloads as W
+    from oracle.staggered import StaggeredOracle
+
+    with threadpool_limits(1):
+        m = W.n144_hh_cell(amp)
+        o = StaggeredOracle(m)
+        o.reset_state(m.V0)
+        o.run(0.025, steps, probe_cvs=probes)
+    return 1
+
+
+def _cable_oracle_cell(job):
+    """One passive tree through the FP64 backward-Euler oracle (CPU baseline
+    worker); returns its CV count."""
+    index, ns, dt = job
+    from threadpoolctl import threadpool_limits
+
+    from oracle.cable_be import CableBEOracle
+
+    with threadpool_limits(1):
+        m = _passive_tree(index)
+        o = CableBEOracle(m)
+        o.reset_state(m.V0)
+        o.run(dt, ns, probe_cvs=[0])
+    return o.C
+
+
+def _network_oracle_replica(job):
+    """One replica of the 1/100-scale network through the FP64 network
+    oracle (CPU baseline worker)."""
+    ns, dt = job
+    from threadpoolctl import threadpool_limits
+
+    from oracle.network import NetworkOracle
+
+    pops, cons = _CPU_SHARED["network"]
+    with threadpool_limits(1):
+        NetworkOracle(pops, cons, dt).run(ns)
+    return sum(p[2] for p in pops)
 
 
 def _oracle_adjoint_cell(job):
@@ -344,14 +434,7 @@ def adjoint_cpu_baseline(n_cells, n_steps, dt):
     """FP64 oracle forward + reverse pass, one region tree per process over
     the host cores this process may use (at most 16).  Runs before the GPU is
     touched (the worker processes are forked)."""
-    import multiprocessing as mp
-    from concurrent.futures import ProcessPoolExecutor
-
-    workers = max(1, min(16, len(os.sched_getaffinity(0)), n_cells))
-    t0 = time.perf_counter()
-    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
-        res = list(ex.map(_oracle_adjoint_cell, [(i, n_steps, dt) for i in range(n_cells)]))
-    el = time.perf_counter() - t0
+    res, el, workers = _cpu_pool(_oracle_adjoint_cell, [(i, n_steps, dt) for i in range(n_cells)])
     n_cv = sum(r[0] for r in res)
     return {"value": float(f"{n_cv * n_steps / el:.4g}"), "unit": "compartment-steps/s", "cores": workers,
             "kind": "port",
@@ -566,6 +649,15 @@ def bench_n144(args):
     m0 = models[0]
     g = discretize_cell(m0.morph, m0.n_cv_branch(), m0.per_branch(m0.cm), m0.per_branch(m0.ra))
     probes = [g.locate(b, x, m0.n_cv_branch())[1] for b, x in W.N144_PROBES]
+    cpu = None
+    if not args.no_cpu:
+        # 64 cells of the batch, one per process, before the GPU is touched
+        nc = min(64, n)
+        res, cel, workers = _cpu_pool(_n144_oracle_cell, [(float(a), steps, probes) for a in amps[:nc]])
+        cpu = {"value": float(f"{nc * steps / cel:.4g}"), "unit": "cell-steps/s", "cores": workers, "kind": "port",
+               "sample": f"NumPy FP64 oracle, cells 0-{nc - 1} x {steps} steps, one cell per process on {workers} "
+                         f"processes, {cel:.1f}s"}
+        log(f"n144 CPU baseline {cpu}")
     t0 = time.perf_counter()
     L = StaggerLayout(models, probes=probes, threads=256)
     log(f"layout {L.summary()} built in {time.perf_counter() - t0:.1f}s")
@@ -598,9 +690,7 @@ def bench_n144(args):
 
     o = StaggeredOracle(m0)
     o.reset_state(m0.V0)
-    t2 = time.perf_counter()
     ref = o.run(dt, steps, probe_cvs=probes)
-    cpu_s = time.perf_counter() - t2
     got = trace[:, :3].double().cpu().numpy()
     rms = [float(np.sqrt(np.mean((got[:, k] - ref[:, k]) ** 2))) for k in range(3)]
     out = {
@@ -612,8 +702,7 @@ def bench_n144(args):
                    "packs": L.n_packs, "threads": L.threads, "cv_per_thread": L.cv_per_thread,
                    "rows_per_pack": L.rows_per_pack, "compartment_updates_per_s": float(f"{cu:.4g}"),
                    "parallelism": "1 GPU, one workgroup per pack"},
-        "cpu_baseline": {"value": float(f"{steps / cpu_s:.4g}"), "unit": "cell-steps/s", "cores": 1, "kind": "port",
-                         "sample": f"NumPy FP64 oracle, one cell x {steps} steps, {cpu_s:.2f}s"},
+        "cpu_baseline": cpu,
         "accuracy": {"probe_rmse_mV": [float(f"{r:.3g}") for r in rms], "reference": "FP64 oracle, cell 0",
                      "gate": "RMSE <= 2 mV (simulator_compare/common.py:41-45)"},
     }
@@ -681,6 +770,22 @@ def bench_network(args):
     import torch
 
     dt = 0.025
+    cpu = None
+    if not args.no_cpu:
+        # the network is one coupled system: 16 replicas of it at 1/100 scale, one per process, before the GPU
+        # is touched
+        sub = _network(0.01)
+        pops = [(n, p.cell.lower(), p.size, p.cell.V_th) for n, p in sub.populations.items()]
+        cons = [dict(pre=c.pre_population, post=c.post_population, pre_index=c.pre_index,
+                     post_index=c.post_index, synapse=c.synapse, synapse_index=c.synapse_index, weight=c.weight,
+                     delay=c.delay) for c in sub.connections]
+        _CPU_SHARED["network"] = (pops, cons)
+        ns = 400
+        res, cel, workers = _cpu_pool(_network_oracle_replica, [(ns, dt)] * 16)
+        cpu = {"value": float(f"{sum(res) * ns / cel:.4g}"), "unit": "cell-steps/s", "cores": workers, "kind": "port",
+               "sample": f"NumPy FP64 network oracle, 16 replicas of the same network at 1/100 scale ({res[0]} cells "
+                         f"each) x {ns} steps, one replica per process on {workers} processes, {cel:.1f}s"}
+        log(f"network CPU baseline {cpu}")
     net = _network(1.0)
     t0 = time.perf_counter()
     tabs = net.build(dt, chunk=200, autotune=not args.no_autotune)
@@ -718,24 +823,6 @@ def bench_network(args):
     ach = byts / launch_s / 1e9
     log(f"{n_cells} cells x {K} steps: {el * 1e3:.1f} ms, {val:.4g} cell-steps/s, {spk} spikes, "
         f"{events:.4g} synaptic events")
-    cpu = None
-    if not args.no_cpu:
-        from oracle.network import NetworkOracle
-
-        sub = _network(0.01)
-        pops = [(n, p.cell.lower(), p.size, p.cell.V_th) for n, p in sub.populations.items()]
-        cons = [dict(pre=c.pre_population, post=c.post_population, pre_index=c.pre_index,
-                     post_index=c.post_index, synapse=c.synapse, synapse_index=c.synapse_index, weight=c.weight,
-                     delay=c.delay) for c in sub.connections]
-        o = NetworkOracle(pops, cons, dt)
-        ns = 200
-        t2 = time.perf_counter()
-        o.run(ns)
-        cel = time.perf_counter() - t2
-        nsub = sum(p.size for p in sub.populations.values())
-        cpu = {"value": float(f"{nsub * ns / cel:.4g}"), "unit": "cell-steps/s", "cores": 1, "kind": "port",
-               "sample": f"NumPy FP64 network oracle, the same network at 1/100 scale ({nsub} cells) x {ns} steps, "
-                         f"{cel:.1f}s"}
     print(json.dumps({
         "metric": "cell_steps_per_second", "value": float(f"{val:.4g}"), "unit": "cell-steps/s", "n_gpus": 1,
         "steps": K, "warmup": K, "ms_per_step": round(el / K * 1e3, 5), "higher_is_better": True, "scaling": "weak",
@@ -781,6 +868,15 @@ def bench_cable(args):
 
     dt = 0.001
     n_cells = args.cells
+    cpu = None
+    if not args.no_cpu:
+        # 64 trees of the batch, one per process, before the GPU is touched
+        ns, nc = 8000, min(64, n_cells)
+        res, cel, workers = _cpu_pool(_cable_oracle_cell, [(i, ns, dt) for i in range(nc)])
+        cpu = {"value": float(f"{sum(res) * ns / cel:.4g}"), "unit": "compartment-updates/s", "cores": workers,
+               "kind": "port", "sample": f"NumPy FP64 oracle (dense CV operator), cells 0-{nc - 1} ({sum(res)} CVs) "
+                                         f"x {ns} steps, one cell per process on {workers} processes, {cel:.1f}s"}
+        log(f"cable CPU baseline {cpu}")
     t0 = time.perf_counter()
     models = [_passive_tree(i) for i in range(n_cells)]
     L = CableLayout(models, probes="soma")
@@ -808,22 +904,6 @@ def bench_cable(args):
     per = 8.0 + 8.0 * nnz + 12.0  # CSR offsets, entries, a / b / J per CV-step
     ach = per * n_cv * K / el / 1e9
     log(f"{n_cv} CVs x {K} steps in {el * 1e3:.2f} ms: {val:.4g} compartment-updates/s")
-    cpu = None
-    if not args.no_cpu:
-        from oracle.cable_be import CableBEOracle
-
-        ns, sub = 1000, models[:16]
-        orc = [CableBEOracle(m) for m in sub]
-        for o, m in zip(orc, sub):
-            o.reset_state(m.V0)
-        t2 = time.perf_counter()
-        for o in orc:
-            o.run(dt, ns, probe_cvs=[0])
-        cel = time.perf_counter() - t2
-        ncv_sub = sum(o.C for o in orc)
-        cpu = {"value": float(f"{ncv_sub * ns / cel:.4g}"), "unit": "compartment-updates/s", "cores": 1,
-               "kind": "port", "sample": f"NumPy FP64 oracle (dense CV operator), cells 0-15 ({ncv_sub} CVs) x "
-                                         f"{ns} steps, {cel:.1f}s"}
     print(json.dumps({
         "metric": "compartment-updates/s", "value": float(f"{val:.4g}"), "unit": "compartment-updates/s",
         "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": round(el / K * 1e3, 5),

@@ -15,6 +15,10 @@ batch's size signature.  A fourth knob shapes the tree solve: the length at
 which chains are cut (longer chains mean fewer chain levels for the level
 sweep of the elimination, shorter ones less work per thread in the
 per-chain passes); the best pack plan is tried with a few cut lengths.
+A fifth picks the back substitution of the chain tree: recursive doubling
+(pointer jumping, a few barrier rounds over every chain) or a sequential
+sweep down the chain levels in one wavefront; deep trees favour the first,
+shallow ones may favour the second, so the best plan is timed with both.
 """
 
 from __future__ import annotations
@@ -109,10 +113,11 @@ def _time_plan(L: StaggerLayout, dt: float, precision: int, steps: int, reps: in
 
 
 CHAIN_ROWS = (16, 12, 24)
+BACKSUB = ("jump", "sweep")
 
 
 def autotune(layout: StaggerLayout, dt: float, *, precision: int = 32, steps: int = 200, reps: int = 2,
-             chain_rows=CHAIN_ROWS, use_cache: bool = True, log=None):
+             chain_rows=CHAIN_ROWS, backsub=BACKSUB, use_cache: bool = True, log=None):
     """Times every candidate plan of ``layout``'s cells and returns
     (best layout, table of (plan, seconds per step)).  Results are cached by
     the batch's size signature."""
@@ -125,13 +130,15 @@ def autotune(layout: StaggerLayout, dt: float, *, precision: int = 32, steps: in
                 cache = json.load(f)
         except (OSError, ValueError):
             cache = {}
-    if use_cache and sig in cache and len(cache[sig]["plan"]) == 4:
-        t, c, r, S = cache[sig]["plan"]
+    if use_cache and sig in cache and len(cache[sig]["plan"]) == 5:
+        t, c, r, S, B = cache[sig]["plan"]
         if log:
-            log(f"autotune: cached plan threads={t} cv_per_thread={c} rows_per_pack={r} chain_rows={S}")
+            log(f"autotune: cached plan threads={t} cv_per_thread={c} rows_per_pack={r} chain_rows={S} backsub={B}")
         if S != layout.max_chain_rows:
-            return layout.rechained(S, threads=t, cv_per_thread=c, rows_per_pack=r), cache[sig]["table"]
-        return layout.repacked(threads=t, cv_per_thread=c, rows_per_pack=r), cache[sig]["table"]
+            L = layout.rechained(S, threads=t, cv_per_thread=c, rows_per_pack=r)
+        else:
+            L = layout.repacked(threads=t, cv_per_thread=c, rows_per_pack=r)
+        return L.with_backsub(B if B in backsub else "jump"), cache[sig]["table"]
     table = []
     best = None
     for (t, c, r) in candidate_plans(layout, precision):
@@ -169,6 +176,21 @@ def autotune(layout: StaggerLayout, dt: float, *, precision: int = 32, steps: in
                 f"lds={L.lds_bytes(precision)} -> {sec * 1e3:.4f} ms/step ({time.perf_counter() - t0:.1f}s)")
         if sec < best[1]:
             best = (L, sec, (t, c, r, S))
+    # the best plan with the other back-substitution variants (the sequential
+    # one is compiled for FP32)
+    B0 = best[0].backsub
+    best = (best[0], best[1], (*best[2], B0))
+    for B in backsub:
+        if B == B0 or (B == "sweep" and precision != 32):
+            continue
+        t0 = time.perf_counter()
+        L = best[0].with_backsub(B)
+        sec = _time_plan(L, dt, precision, steps, reps)
+        table.append((best[2][:4] + (B,), sec))
+        if log:
+            log(f"autotune: {best[2][:4]} backsub={B} -> {sec * 1e3:.4f} ms/step ({time.perf_counter() - t0:.1f}s)")
+        if sec < best[1]:
+            best = (L, sec, best[2][:4] + (B,))
     if use_cache:
         cache[sig] = {"plan": list(best[2]), "table": [[list(p), s] for p, s in table]}
         try:

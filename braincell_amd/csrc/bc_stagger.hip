@@ -728,7 +728,7 @@ __device__ __forceinline__ void synapse_update(const real* q, real* st, real* kg
   st[1] = B;
 }
 
-template <typename real, uint32_t CH, int MAXC, bool SYN>
+template <typename real, uint32_t CH, int MAXC, bool SYN, bool SWEEP = false>
 __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   constexpr int NS = bc_nstates(CH) > 0 ? bc_nstates(CH) : 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1026,8 +1026,30 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       chain_affine<real>(k, dummy, CR, R);
     }
     __syncthreads();
-    // pointer jumping over the chain tree for the chains' bottom solutions:
-    // start from each chain's bottom-row (alpha, beta) and its parent
+    // the chains' bottom solutions X_k = alpha_k + beta_k X_parent(k), from
+    // each chain's bottom-row (alpha, beta) and its parent chain
+    const KPair<real>* Xb;
+    if constexpr (SWEEP) {
+      // sequential variant: wavefront 0 walks the chain levels top-down
+      // (a parent chain's level is above its kids'), one chain per lane
+      if (tid < kWave) {
+        for (int lev = P.n_lv - 1; lev >= 0; --lev) {
+          const int c1 = m_lv[lev + 1];
+          for (int k = m_lv[lev] + tid; k < c1; k += kWave) {
+            const HRow<real> w = R[CR[k].x & 0xFFFF];
+            const uint32_t q = m_dfw[k] & 0xFFFFu;
+            KPair<real> v;
+            v.d = q == 0xFFFFu ? w.r : w.r + w.d * KK[q].d;
+            v.r = real(0);
+            KK[k] = v;
+          }
+          wave_sync();
+        }
+      }
+      __syncthreads();
+      Xb = KK;
+    } else {
+    // recursive doubling: pointer jumping over the chain tree
     for (int k = tid; k < P.n_ch; k += NT) {
       const HRow<real> w = R[CR[k].x & 0xFFFF];
       KPair<real> v;
@@ -1039,7 +1061,6 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     __syncthreads();
     // solution of row j of chain k: x_j = alpha_j + beta_j X, X = the solution
     // at the bottom of k's parent chain (Xb[parent].d after the jumping)
-    const KPair<real>* Xb;
     {
       // (A, B, ptr): x_bottom(k) = A + B x_bottom(ptr); ptr < 0 once resolved
       KPair<real>* A0 = KK2;
@@ -1078,6 +1099,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
         if (round == rounds - 1 && open) bad |= 2u;
       }
       Xb = A0;
+    }
     }
 
 
@@ -1151,7 +1173,19 @@ typedef void (*kernel_f32)(StaggerArgs<float>);
 typedef void (*kernel_f64)(StaggerArgs<double>);
 
 template <typename real, uint32_t CH>
-void* pick_maxc(int maxc, bool syn) {
+void* pick_maxc(int maxc, bool syn, bool sweep) {
+  if (sweep) {  // the sequential back substitution: FP32 kernels without synapses
+    if constexpr (sizeof(real) != 4) return nullptr;
+    if (syn) return nullptr;
+    switch (maxc) {
+      case 1: return (void*)&stagger_kernel<real, CH, 1, false, true>;
+      case 2: return (void*)&stagger_kernel<real, CH, 2, false, true>;
+      case 4: return (void*)&stagger_kernel<real, CH, 4, false, true>;
+      case 8: return (void*)&stagger_kernel<real, CH, 8, false, true>;
+      case 16: return (void*)&stagger_kernel<real, CH, 16, false, true>;
+      default: return nullptr;
+    }
+  }
   if (syn) {  // network runs: up to 8 CVs per thread
     switch (maxc) {
       case 1: return (void*)&stagger_kernel<real, CH, 1, true>;
@@ -1172,13 +1206,13 @@ void* pick_maxc(int maxc, bool syn) {
 }
 
 template <typename real>
-void* pick_kernel(uint32_t ch, int maxc, bool syn = false) {
+void* pick_kernel(uint32_t ch, int maxc, bool syn = false, bool sweep = false) {
   switch (ch) {
-    case kSetPassive: return pick_maxc<real, kSetPassive>(maxc, syn);
-    case kSetHH: return pick_maxc<real, kSetHH>(maxc, syn);
-    case kSetHHCaL: return pick_maxc<real, kSetHHCaL>(maxc, syn);
-    case kSetAll: return pick_maxc<real, kSetAll>(maxc, syn);
-    case kSetHH1952: return pick_maxc<real, kSetHH1952>(maxc, syn);
+    case kSetPassive: return pick_maxc<real, kSetPassive>(maxc, syn, sweep);
+    case kSetHH: return pick_maxc<real, kSetHH>(maxc, syn, sweep);
+    case kSetHHCaL: return pick_maxc<real, kSetHHCaL>(maxc, syn, sweep);
+    case kSetAll: return pick_maxc<real, kSetAll>(maxc, syn, sweep);
+    case kSetHH1952: return pick_maxc<real, kSetHH1952>(maxc, syn, sweep);
     default: return nullptr;
   }
 }
@@ -1304,7 +1338,7 @@ int validate(const bc_stagger_desc* d) {
   if (d->max_rows > 32767 || d->max_levels <= 0 || d->max_levels > 32767)
     return bc::fail(BC_INVALID_LAYOUT, "max_rows/max_levels out of range for 16-bit chain metadata");
   if (!(d->dt > 0.0)) return bc::fail(BC_INVALID_ARG, "dt must be > 0");
-  if ((d->flags & ~(BC_STEP_SKIP_GATES | BC_STEP_INTEGRATION_ORDER | BC_STEP_VOLTAGE_ONLY)) != 0)
+  if ((d->flags & ~(BC_STEP_SKIP_GATES | BC_STEP_INTEGRATION_ORDER | BC_STEP_VOLTAGE_ONLY | BC_STEP_BACKSUB_SWEEP)) != 0)
     return bc::fail(BC_INVALID_ARG, "unknown flags 0x%x", d->flags);
   if (d->n_packs > 0 && (d->packs == nullptr || d->cv_info == nullptr || d->V == nullptr || d->Vbnd == nullptr ||
                          d->status == nullptr || d->chain_rec == nullptr || d->chain_par == nullptr ||
@@ -1345,10 +1379,12 @@ int validate(const bc_stagger_desc* d) {
 
 template <typename real>
 int launch(const bc_stagger_desc* d, hipStream_t s, int32_t step0, int32_t n_steps, void* trace) {
-  void* fn = pick_kernel<real>(d->channels, d->cv_per_thread, d->syn != nullptr);
+  const bool sweep = (d->flags & BC_STEP_BACKSUB_SWEEP) != 0;
+  void* fn = pick_kernel<real>(d->channels, d->cv_per_thread, d->syn != nullptr, sweep);
   if (fn == nullptr)
-    return bc::fail(BC_UNSUPPORTED, "no kernel for channels=0x%x cv_per_thread=%d%s", d->channels, d->cv_per_thread,
-                    d->syn ? " with synapses" : "");
+    return bc::fail(BC_UNSUPPORTED, "no kernel for channels=0x%x cv_per_thread=%d%s%s", d->channels,
+                    d->cv_per_thread, d->syn ? " with synapses" : "",
+                    sweep ? " with the sequential back substitution (FP32 kernels without synapses)" : "");
   StaggerArgs<real> a;
   int rc = fill_args<real>(d, &a);
   if (rc != BC_OK) return rc;

@@ -385,7 +385,8 @@ class Network:
         if autotune:
             from .autotune import autotune as _autotune
 
-            layout, _ = _autotune(layout, float(dt), precision=precision.copy().pop())
+            # network launches run the synapse kernels, which use pointer jumping only
+            layout, _ = _autotune(layout, float(dt), precision=precision.copy().pop(), backsub=("jump",))
         elif threads is not None or rows_per_pack is not None:
             layout = layout.repacked(threads=threads or 256, rows_per_pack=rows_per_pack)
         else:

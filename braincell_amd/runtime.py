@@ -134,8 +134,17 @@ class _CellHost:
     netstims: list = None  # (mech.NetStim, CV of its point, or -1 on a CV boundary node)
 
 
+BACKSUB_SWEEP = 8  # BC_STEP_BACKSUB_SWEEP
+BACKSUB_KINDS = ("jump", "sweep")
+
+
 class StaggerLayout:
     """Host-side packed layout of a batch of cells."""
+
+    # back substitution of the chain tree: "jump" = recursive doubling
+    # (pointer jumping), "sweep" = a sequential top-down sweep over the chain
+    # levels (BC_STEP_BACKSUB_SWEEP); chosen per batch by the autotuner
+    backsub = "jump"
 
     def __init__(self, models, *, threads: int = 256, cv_per_thread: int = None, rows_per_pack: int = None,
                  probes="soma", region_classes: bool = False, probes_per_model=None, max_chain_rows: int = 16):
@@ -222,6 +231,22 @@ class StaggerLayout:
         L = copy.copy(self)
         L._configure(self.threads if threads is None else threads, cv_per_thread, rows_per_pack)
         return L
+
+    def with_backsub(self, kind: str):
+        """The same layout with the back-substitution variant ``kind``
+        ("jump" or "sweep")."""
+        import copy
+
+        if kind not in BACKSUB_KINDS:
+            raise ValueError(f"backsub must be one of {BACKSUB_KINDS}, got {kind!r}")
+        L = copy.copy(self)
+        L.backsub = kind
+        return L
+
+    @property
+    def step_flags(self) -> int:
+        """BC_STEP_* bits the layout adds to every launch."""
+        return BACKSUB_SWEEP if self.backsub == "sweep" else 0
 
     def lds_bytes(self, precision: int = 32) -> int:
         """Dynamic LDS of one workgroup (the kernel's lds_plan)."""
@@ -761,7 +786,8 @@ class StaggerLayout:
         p = self.pack_array
         return {"cells": len(self.cells), "packs": self.n_packs, "cv_total": int(self.n_cv_total),
                 "rows_total": int(self.n_rows_total), "max_rows": self.max_rows, "max_levels": int(p[:, 7].max()),
-                "threads": self.threads, "cv_per_thread": self.cv_per_thread, "channel_set": hex(self.chset)}
+                "threads": self.threads, "cv_per_thread": self.cv_per_thread, "channel_set": hex(self.chset),
+                "backsub": self.backsub}
 
 
 class StaggerRuntime:
@@ -939,7 +965,7 @@ class StaggerRuntime:
         d.step0 = self.step_index if step0 is None else int(step0)
         d.n_steps = int(n_steps)
         d.dt = self.dt
-        d.flags = self.flags
+        d.flags = self.flags | L.step_flags
         syn = getattr(self, "synapse_desc", None)
         if syn is not None:
             self._syn_drive_workspace(syn, int(n_steps))

@@ -267,6 +267,17 @@ typedef struct bc_stagger_desc {
 #define BC_STEP_VOLTAGE_ONLY (1u << 2) /* dhs_voltage (quad/_staggered.py:187-320):
                                          the voltage solve alone; gates and the
                                          Ca pool keep their values */
+#define BC_STEP_BACKSUB_SWEEP (1u << 3) /* back substitution of the chain tree
+                                        by a sequential top-down sweep over
+                                        the chain levels in one wavefront,
+                                        instead of recursive doubling (pointer
+                                        jumping, the form of comp_backsub_raw,
+                                        quad/_staggered.py:742-760); which is
+                                        faster depends on the trees' shape
+                                        (examples/profiling/bench_dhs_levels.py:
+                                        122-185), the autotuner times both.
+                                        Compiled for the FP32 kernels without
+                                        synapses (BC_UNSUPPORTED otherwise) */
 #define BC_STEP_INTEGRATION_ORDER (1u << 1) /* ion_channel_update_order
                                         "integration" (_multi_compartment/cell.py:
                                         2137-2155): every node updates from the

@@ -3,11 +3,13 @@
 import numpy as np
 
 
-def run_gpu(models, n_steps, dt=0.025, precision=32, probes="soma", chunk=None, **layout_kw):
+def run_gpu(models, n_steps, dt=0.025, precision=32, probes="soma", chunk=None, backsub=None, **layout_kw):
     import torch
     from braincell_amd.runtime import StaggerLayout, StaggerRuntime
 
     L = StaggerLayout(models, probes=probes, **layout_kw)
+    if backsub is not None:
+        L = L.with_backsub(backsub)
     rt = StaggerRuntime(L, dt, precision=precision)
     tr = rt.run(n_steps, chunk=chunk)
     torch.cuda.synchronize()

@@ -63,9 +63,16 @@ def test_every_plan_gives_the_same_trajectories(cuda, tmp_path, monkeypatch):
 
     ref = soma(base)
     assert any(len(p) == 4 for p, _ in table)  # chain cut lengths were searched
+    assert {p[4] for p, _ in table if len(p) == 5} == {"sweep"}  # and the other back substitution
     for plan, _ in table:
         t, c, r = plan[:3]
-        if len(plan) == 4:
+        if len(plan) == 5:
+            # the sequential back substitution sums the chain tree in another order
+            L = base.rechained(plan[3], threads=t, cv_per_thread=c, rows_per_pack=r).with_backsub(plan[4])
+            assert L.step_flags == 8
+            got = soma(L)
+            assert np.max(np.abs(got - ref)) < 1e-3, plan
+        elif len(plan) == 4:
             # another chain cut groups the elimination's rounding differently
             got = soma(base.rechained(plan[3], threads=t, cv_per_thread=c, rows_per_pack=r))
             assert np.max(np.abs(got - ref)) < 1e-3, plan
@@ -76,8 +83,20 @@ def test_every_plan_gives_the_same_trajectories(cuda, tmp_path, monkeypatch):
             assert np.max(np.abs(got - ref)) < 1e-4, plan
     # the cached plan comes back without timing again
     again, _ = autotune(base, 0.025, steps=20, reps=1)
-    assert (again.threads, again.cv_per_thread, again.rows_per_pack, again.max_chain_rows) == (
-        best.threads, best.cv_per_thread, best.rows_per_pack, best.max_chain_rows)
+    assert (again.threads, again.cv_per_thread, again.rows_per_pack, again.max_chain_rows, again.backsub) == (
+        best.threads, best.cv_per_thread, best.rows_per_pack, best.max_chain_rows, best.backsub)
+
+
+def test_backsub_variant_sets_the_step_flag():
+    base = StaggerLayout(_batch(4), threads=256)
+    assert base.backsub == "jump" and base.step_flags == 0
+    sw = base.with_backsub("sweep")
+    assert sw.step_flags == 8 and base.step_flags == 0  # the base is untouched
+    assert sw.repacked(rows_per_pack=3000).backsub == "sweep"
+    assert sw.rechained(12).backsub == "sweep"
+    assert sw.summary()["backsub"] == "sweep"
+    with pytest.raises(ValueError):
+        base.with_backsub("doubling")
 
 
 def test_rechained_layout_equals_a_fresh_one():

@@ -19,6 +19,44 @@ def test_single_tree_fp64_all_cvs_match_oracle(cuda):
     assert err < 1e-8
 
 
+def test_backsub_sweep_fp32_all_cvs_match_oracle(cuda):
+    # the sequential back substitution of the chain tree (BC_STEP_BACKSUB_SWEEP,
+    # compiled for the FP32 kernels) on every CV of the c0 tree
+    m = W.single_tree_cell(0)
+    steps = 4000
+    ref, _ = run_oracle(m, steps)
+    got, rt = run_gpu([m], steps, precision=32, probes="all", backsub="sweep")
+    assert rt.L.step_flags == 8
+    e = np.sqrt(np.mean((got - ref) ** 2, axis=0))
+    print("fp32 sweep back substitution, worst per-CV RMSE over 4000 steps x 256 CVs:", e.max())
+    assert e.max() <= 0.05
+
+
+def test_backsub_sweep_packed_batch_matches_jumping(cuda):
+    # several cells per pack (root chains on different levels): the two
+    # variants against each other, the sweep against the oracle; FP64 has no
+    # sweep kernel
+    from braincell_amd._native import NativeError
+
+    models = [W.uneven_tree_cell(i, cmin=200, cmax=700) for i in range(6)]
+    steps, dt = 1200, 0.025
+    a, _ = run_gpu(models, steps, precision=32, probes="soma", rows_per_pack=2600, backsub="jump")
+    b, rt = run_gpu(models, steps, precision=32, probes="soma", rows_per_pack=2600, backsub="sweep")
+    assert rt.L.n_packs < len(models)
+    # rounding differences of the two summation orders grow through spike
+    # upstrokes; the bound sits far below the 0.05 mV gate
+    d = np.abs(a - b).max()
+    print(f"fp32: sweep vs pointer jumping, max |dV| {d:.2e} mV")
+    assert d < 2e-2
+    for i, m in enumerate(models):
+        assert len(spike_times(a[:, i], dt)) == len(spike_times(b[:, i], dt))
+    for i, m in enumerate(models[:2]):
+        ref, _ = run_oracle(m, steps, probe_cvs=[soma_cv(m)])
+        assert rmse(b[:, i], ref[:, 0]) <= 0.05
+    with pytest.raises(NativeError, match="sequential back substitution"):
+        run_gpu(models[:1], 10, precision=64, probes="soma", backsub="sweep")
+
+
 def test_single_tree_fp32_soma_rmse(cuda):
     m = W.single_tree_cell(0)
     steps = 4000

@@ -136,3 +136,26 @@ def test_synapse_descriptor_is_checked_without_gpu():
     cc = _native.chan_const_type(32)()
     d.chan_const = ctypes.cast(ctypes.pointer(cc), ctypes.c_void_p)
     assert L.bc_stagger_run(ctypes.byref(d), None) in (0, 3)  # valid (no packs: nothing to launch)
+
+
+def test_sequential_back_substitution_is_fp32_without_synapses():
+    # BC_STEP_BACKSUB_SWEEP picks the kernels compiled with the sequential
+    # back substitution: FP32 without synapses; other descriptors get
+    # BC_UNSUPPORTED before any device work (no packs to launch here)
+    L = _native.lib()
+    d = _native.StaggerDesc()
+    d.precision, d.threads, d.max_rows, d.max_chains, d.max_cls, d.max_levels, d.dt = 32, 256, 64, 8, 1, 1, 0.025
+    d.channels, d.cv_per_thread, d.flags = 0x117, 8, 8
+    cc = _native.chan_const_type(32)()
+    d.chan_const = ctypes.cast(ctypes.pointer(cc), ctypes.c_void_p)
+    # supported: past validation and kernel choice (without a GPU the call then
+    # fails at the runtime's error check)
+    assert L.bc_stagger_run(ctypes.byref(d), None) in (0, 3)
+    d.flags = 16  # an unknown bit
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 1
+    d.flags = 8
+    d.precision = 64
+    cc64 = _native.chan_const_type(64)()
+    d.chan_const = ctypes.cast(ctypes.pointer(cc64), ctypes.c_void_p)
+    assert L.bc_stagger_run(ctypes.byref(d), None) == 5
+    assert b"sequential back substitution" in L.bc_last_error()
