@@ -1226,7 +1226,10 @@ __global__ void spike_clear(uint8_t* spk, int spk_len, int n_cells, int step0, i
 }
 
 // Threshold crossings on the launch's trace rows: one thread per trace
-// column, rows read in order (coalesced across the columns of a row).
+// column, rows read in order (coalesced across the columns of a row), in
+// batches of kScanRows independent loads so that a batch pays one memory
+// latency.
+constexpr int kScanRows = 16;
 template <typename real>
 __global__ void __launch_bounds__(256) spike_scan(const real* __restrict__ trace, int64_t ld, int32_t n_steps,
                                                   int32_t step0, real v_th, real* __restrict__ v_last,
@@ -1237,13 +1240,20 @@ __global__ void __launch_bounds__(256) spike_scan(const real* __restrict__ trace
   real v = v_last[col];
   int32_t c = count[col];
   int32_t* out = steps + col * cap;
-  for (int32_t n = 0; n < n_steps; ++n) {
-    const real x = trace[(int64_t)n * ld + col];
-    if (v < v_th && x >= v_th) {
-      if (c < cap) out[c] = step0 + n;
-      ++c;
+  for (int32_t n0 = 0; n0 < n_steps; n0 += kScanRows) {
+    real x[kScanRows];
+#pragma unroll
+    for (int j = 0; j < kScanRows; ++j) x[j] = n0 + j < n_steps ? trace[(int64_t)(n0 + j) * ld + col] : real(0);
+#pragma unroll
+    for (int j = 0; j < kScanRows; ++j) {
+      if (n0 + j < n_steps) {
+        if (v < v_th && x[j] >= v_th) {
+          if (c < cap) out[c] = step0 + n0 + j;
+          ++c;
+        }
+        v = x[j];
+      }
     }
-    v = x;
   }
   v_last[col] = v;
   count[col] = c;
