@@ -1031,9 +1031,10 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     const KPair<real>* Xb;
     if constexpr (SWEEP) {
       // sequential variant: wavefront 0 walks the chain levels top-down
-      // (a parent chain's level is above its kids'), one chain per lane
+      // (a parent chain's level is above its kids'), one chain per lane;
+      // level 0 (the leaf chains) is skipped: only parents are read
       if (tid < kWave) {
-        for (int lev = P.n_lv - 1; lev >= 0; --lev) {
+        for (int lev = P.n_lv - 1; lev >= 1; --lev) {
           const int c1 = m_lv[lev + 1];
           for (int k = m_lv[lev] + tid; k < c1; k += kWave) {
             const HRow<real> w = R[CR[k].x & 0xFFFF];
@@ -1049,8 +1050,10 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       __syncthreads();
       Xb = KK;
     } else {
-    // recursive doubling: pointer jumping over the chain tree
-    for (int k = tid; k < P.n_ch; k += NT) {
+    // recursive doubling: pointer jumping over the chain tree.  Only parent
+    // chains' solutions are read (row_solution), and a parent is never a
+    // leaf: the leaf chains (level 0, the first n_leaf) take no part
+    for (int k = n_leaf + tid; k < P.n_ch; k += NT) {
       const HRow<real> w = R[CR[k].x & 0xFFFF];
       KPair<real> v;
       v.d = w.r;
@@ -1067,15 +1070,16 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       KPair<real>* A1 = KK;
       int16_t* p0 = m_ptr;
       int16_t* p1 = m_ptr2;
-      // a chain's depth below its root chain is at most n_lv - 1 (levels are
-      // heights), so ceil(log2(n_lv)) rounds resolve every pointer; one still
-      // open afterwards means an inconsistent layout and raises the error
-      // flag instead of hanging
+      // a non-leaf chain's depth below its root chain is at most n_lv - 2
+      // (levels are heights, a non-leaf chain's is at least 1), so
+      // ceil(log2(n_lv - 1)) rounds resolve every pointer; one still open
+      // afterwards means an inconsistent layout and raises the error flag
+      // instead of hanging
       int rounds = 0;
-      while ((1 << rounds) < P.n_lv) ++rounds;
+      while ((1 << rounds) < P.n_lv - 1) ++rounds;
       for (int round = 0; round < rounds; ++round) {
         int open = 0;
-        for (int k = tid; k < P.n_ch; k += NT) {
+        for (int k = n_leaf + tid; k < P.n_ch; k += NT) {
           const int q = p0[k];
           KPair<real> v = A0[k];
           int qn = q;
