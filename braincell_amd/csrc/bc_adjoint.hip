@@ -454,7 +454,9 @@ __global__ void __launch_bounds__(256) stagger_adjoint_kernel(AdjArgs<real> a) {
       const int lv0 = m_lv[0], lv1 = m_lv[1];
       const int lw = NT > 64 ? 64 : 0;  // first thread of the leaf crew
       if (tid < 64) {
-        for (int lev = 1; lev < P.n_lv; ++lev) {
+        // the top level holds root chains only, whose contributions no chain
+        // reads: the sweep ends below it
+        for (int lev = 1; lev < P.n_lv - 1; ++lev) {
           const int c1 = m_lv[lev + 1];
           for (int k = m_lv[lev] + tid; k < c1; k += 64) {
             const int4 cr = CR[k];

@@ -998,7 +998,9 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       // the sweep is the pack's critical path: its wavefront gets issue
       // priority over the other wavefronts of its SIMD while it runs
       __builtin_amdgcn_s_setprio(2);
-      for (int lev = 1; lev < P.n_lv; ++lev) {
+      // the top level holds root chains only (a parent sits above its
+      // kids), whose contributions no chain reads: the sweep ends below it
+      for (int lev = 1; lev < P.n_lv - 1; ++lev) {
         const int c1 = m_lv[lev + 1];
         for (int k = m_lv[lev] + tid; k < c1; k += kWave) {
           const int4 rec = CR[k];
