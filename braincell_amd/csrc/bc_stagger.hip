@@ -982,7 +982,14 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
     // with e_j = diag_j - sum over children c of up_c e_c / d_c >= 0, so no
     // step subtracts nearly equal numbers.
     // Pass 1: every chain's composite map (and a leaf chain's contribution).
-    for (int i = tid; i < P.n_ch; i += NT) chain_composite<real>(m_work[i], CR, R, xv, CM, KK);
+    // Rounds of NT positions of the work order (longest chains first within
+    // the leaf and the non-leaf part), every other round in reverse: the
+    // threads whose first chain is long take the short ones of a second
+    // round.
+    for (int r = 0, base = 0; base < P.n_ch; ++r, base += NT) {
+      const int i = base + ((r & 1) ? NT - 1 - tid : tid);
+      if (i < P.n_ch) chain_composite<real>(m_work[i], CR, R, xv, CM, KK);
+    }
     __syncthreads();
     // Pass 2 in wavefront 0: the chain levels in order (chains are sorted by
     // level, level 0 being the leaves), each chain applying its composite to
