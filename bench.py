@@ -914,7 +914,9 @@ def bench_cable(args):
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "cable_be_kernel<float>",
                      "launch_ms": round(el * 1e3, 3), "bytes_per_launch": per * n_cv * K,
-                     "bytes_model": f"{per:.1f} B per CV-step (CSR offsets, {nnz:.2f} entries, a, b, J)"},
+                     "bytes_model": f"{per:.1f} B per CV-step of the unfused step (CSR offsets, {nnz:.2f} entries, a, b, "
+                                    f"J re-read every step); the kernel holds them in LDS for the launch, so the "
+                                    f"fraction can exceed 1"},
         "cpu_baseline": cpu}), flush=True)
 
 

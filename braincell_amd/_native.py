@@ -78,7 +78,7 @@ class CableDesc(ctypes.Structure):
                     "clamp_end", "clamp_amp", "probe_off", "probe_cv", "probe_col")] + [
                 ("trace_ld", ctypes.c_int32), ("trace", ctypes.c_void_p), ("V", ctypes.c_void_p),
                 ("status", ctypes.c_void_p), ("step0", ctypes.c_int32), ("n_steps", ctypes.c_int32),
-                ("dt", ctypes.c_double)]
+                ("dt", ctypes.c_double), ("max_nnz", ctypes.c_int32)]
 
 
 class AdjointDesc(ctypes.Structure):

@@ -161,6 +161,7 @@ class CableLayout:
         self.n_probes = int(self.probe_cv.size)
         self.n_cv_total = int(self.cell_cv0[-1])
         self.max_cv = int(np.diff(self.cell_cv0).max())
+        self.max_nnz = int(np.diff(self.k_off[self.cell_cv0]).max())
         counts = np.bincount([c[0] for c in self.clamps], minlength=len(self.models)) if self.clamps else \
             np.zeros(len(self.models), dtype=np.int64)
         self.clamp_off = np.concatenate(([0], np.cumsum(counts))).astype(np.int32)
@@ -183,8 +184,16 @@ class CableLayout:
         for (ci, cv, *_rest) in self.clamps:
             if not 0 <= cv < n_cell[ci]:
                 raise ValueError("clamp outside its cell")
-        if 2 * self.max_cv * 8 > 160 * 1024:
-            raise ValueError(f"a cell of {self.max_cv} CVs does not fit the LDS in FP64")
+        if self.lds_bytes(64) > 160 * 1024:
+            raise ValueError(f"a cell of {self.max_cv} CVs and {self.max_nnz} operator entries does not fit the "
+                             "LDS in FP64")
+
+    def lds_bytes(self, precision: int = 32) -> int:
+        """Dynamic LDS of one workgroup (the kernel's cable_lds_bytes): V
+        twice, a, b, 1 - dt J and the CSR values in the working precision,
+        row offsets and columns as int32."""
+        rs = 8 if precision == 64 else 4
+        return (5 * self.max_cv + self.max_nnz) * rs + (self.max_cv + 1 + self.max_nnz) * 4
 
 
 class CableRuntime:
@@ -243,13 +252,17 @@ class CableRuntime:
         d.trace = trace.data_ptr() if self.L.n_probes else None
         d.V, d.status = self.V.data_ptr(), self.status.data_ptr()
         d.step0, d.n_steps, d.dt = int(self.step_index), int(n_steps), self.dt
+        d.max_nnz = self.L.max_nnz
         stream = ctypes.c_void_p(self.torch.cuda.current_stream(self.device).cuda_stream)
         check(lib().bc_cable_be_run(ctypes.byref(d), stream))
         self.step_index += int(n_steps)
         return trace
 
     def check_status(self):
-        if int(self.status.item()) != 0:
+        st = int(self.status.item())
+        if st != 0:
             from ._native import NativeError
 
+            if st & 2:
+                raise NativeError(2, "a cell exceeds the launch's LDS plan (max_cv / max_nnz)")
             raise NativeError(4, "non-finite membrane voltage detected")

@@ -6,9 +6,10 @@
 // gives per CV the column sum J_i = sum_q df_q / dV_i = -a_i - sum_q K_qi.
 // One step: V_i += dt f_i / (1 - dt J_i), every f_i from the step-start V.
 //
-// One workgroup per cell runs all the launch's steps with the cell's
-// voltages in LDS (two buffers, one barrier per step); K is a CSR in global
-// memory, read once per step (it stays in L2 for the cells of an XCD).
+// One workgroup per cell runs all the launch's steps with the cell's state
+// in LDS: the voltages (two buffers, one barrier per step) and, copied once
+// per launch, its rows of the CSR operator K, a, b and the denominators
+// 1 - dt J, so that a step reads global memory only for its clamps.
 #include "bc_common.h"
 
 namespace {
@@ -33,34 +34,63 @@ struct CableArgs {
   real* trace;
   real* V;
   uint32_t* status;
-  int32_t trace_ld, max_cv, step0, n_steps;
+  int32_t trace_ld, max_cv, max_nnz, step0, n_steps;
   real dt;
 };
+
+// Dynamic LDS of one workgroup: V (two buffers), a, b, 1 - dt J, the CSR
+// values (real); the row offsets (max_cv + 1) and columns (int32).
+template <typename real>
+size_t cable_lds_bytes(int max_cv, int max_nnz) {
+  return (size_t)(5 * max_cv + max_nnz) * sizeof(real) + (size_t)(max_cv + 1 + max_nnz) * sizeof(int32_t);
+}
 
 template <typename real>
 __global__ void __launch_bounds__(1024) cable_be_kernel(CableArgs<real> a) {
   extern __shared__ __align__(16) unsigned char bc_cable_lds[];
   real* Va = reinterpret_cast<real*>(bc_cable_lds);
   real* Vb = Va + a.max_cv;
+  real* ca = Vb + a.max_cv;
+  real* cb = ca + a.max_cv;
+  real* den = cb + a.max_cv;
+  real* kv = den + a.max_cv;
+  int32_t* ko = reinterpret_cast<int32_t*>(kv + a.max_nnz);
+  int32_t* kc = ko + a.max_cv + 1;
   const int c = blockIdx.x, tid = threadIdx.x, NT = blockDim.x;
   const int cv0 = a.cell_cv0[c], n = a.cell_cv0[c + 1] - cv0;
+  const int e0 = a.k_off[cv0], ne = a.k_off[cv0 + n] - e0;
   const int cl0 = a.clamp_off[c], cl1 = a.clamp_off[c + 1];
   const int pr0 = a.probe_off[c], pr1 = a.probe_off[c + 1];
-  for (int i = tid; i < n; i += NT) Va[i] = a.V[cv0 + i];
+  if (n > a.max_cv || ne > a.max_nnz) {  // tables larger than the LDS plan: report, touch nothing
+    if (tid == 0) atomicOr(a.status, 2u);
+    return;
+  }
+  for (int i = tid; i < n; i += NT) {
+    const int g = cv0 + i;
+    Va[i] = a.V[g];
+    ca[i] = a.a[g];
+    cb[i] = a.b[g];
+    den[i] = real(1) - a.dt * a.jac[g];
+    ko[i] = a.k_off[g] - e0;
+  }
+  if (tid == 0) ko[n] = ne;
+  for (int e = tid; e < ne; e += NT) {
+    kv[e] = a.k_val[e0 + e];
+    kc[e] = a.k_col[e0 + e];
+  }
   __syncthreads();
   uint32_t bad = 0;
   for (int step = 0; step < a.n_steps; ++step) {
     const int gstep = a.step0 + step;
     for (int i = tid; i < n; i += NT) {
-      const int g = cv0 + i;
       real s = real(0);
-      for (int e = a.k_off[g]; e < a.k_off[g + 1]; ++e) s += a.k_val[e] * Va[a.k_col[e]];
+      for (int e = ko[i]; e < ko[i + 1]; ++e) s += kv[e] * Va[kc[e]];
       real I = real(0);
       for (int k = cl0; k < cl1; ++k)
         if (a.clamp_cv[k] == i && gstep >= a.clamp_on[k] && gstep < a.clamp_end[k]) I += a.clamp_amp[k];
       const real v = Va[i];
-      const real f = (a.b[g] - a.a[g] * v + I) - s;
-      const real vn = v + a.dt * f / (real(1) - a.dt * a.jac[g]);
+      const real f = (cb[i] - ca[i] * v + I) - s;
+      const real vn = v + a.dt * f / den[i];
       Vb[i] = vn;
       bad |= (vn == vn && fabs(vn) < real(1e30)) ? 0u : 1u;
     }
@@ -97,10 +127,11 @@ int launch_cable(const bc_cable_be_desc* d, hipStream_t st) {
   a.status = d->status;
   a.trace_ld = d->trace_ld;
   a.max_cv = d->max_cv;
+  a.max_nnz = d->max_nnz;
   a.step0 = d->step0;
   a.n_steps = d->n_steps;
   a.dt = static_cast<real>(d->dt);
-  const size_t lds = 2 * (size_t)d->max_cv * sizeof(real);
+  const size_t lds = cable_lds_bytes<real>(d->max_cv, d->max_nnz);
   if (lds > 160 * 1024) return bc::fail(BC_INVALID_LAYOUT, "a cell needs %zu bytes of LDS (> 160 KiB)", lds);
   if (d->n_cells == 0 || d->n_steps == 0) return BC_OK;
   void* fn = reinterpret_cast<void*>(&cable_be_kernel<real>);
@@ -122,7 +153,8 @@ int bc_cable_be_run(const bc_cable_be_desc* d, void* stream) {
   if (d->threads < 64 || d->threads > 1024 || d->threads % 64 != 0)
     return bc::fail(BC_INVALID_ARG, "threads must be a multiple of 64 in [64, 1024] (got %d)", d->threads);
   if (!(d->dt > 0.0)) return bc::fail(BC_INVALID_ARG, "dt must be > 0");
-  if (d->n_cells > 0 && d->max_cv <= 0) return bc::fail(BC_INVALID_LAYOUT, "max_cv must be > 0");
+  if (d->n_cells > 0 && (d->max_cv <= 0 || d->max_nnz < 0))
+    return bc::fail(BC_INVALID_LAYOUT, "max_cv must be > 0 and max_nnz >= 0");
   if (d->n_cells > 0 &&
       (d->cell_cv0 == nullptr || d->k_off == nullptr || d->k_col == nullptr || d->k_val == nullptr ||
        d->a == nullptr || d->b == nullptr || d->jac == nullptr || d->clamp_off == nullptr ||

@@ -424,10 +424,15 @@ typedef struct bc_cable_be_desc {
   int32_t trace_ld;         /* trace columns (0: no trace) */
   void* trace;              /* [n_steps * trace_ld] real, out */
   void* V;                  /* [n_cv] real, in/out */
-  uint32_t* status;         /* OR-ed with 1 on a non-finite voltage */
+  uint32_t* status;         /* OR-ed with 1 on a non-finite voltage, 2 on a
+                               cell larger than max_cv / max_nnz */
   int32_t step0;            /* global index of the first step */
   int32_t n_steps;
   double dt;                /* ms */
+  int32_t max_nnz;          /* entries of K in the largest cell's rows (each
+                               cell's rows, a, b and J are held in LDS for
+                               the launch; a cell beyond max_cv / max_nnz
+                               sets status bit 2 and is not stepped) */
 } bc_cable_be_desc;
 int bc_cable_be_run(const bc_cable_be_desc* desc, void* stream);
 

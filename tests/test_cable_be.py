@@ -188,6 +188,9 @@ def test_active_cells_raise_and_implicit_euler_needs_t_and_dt():
 def test_layout_validation_rejects_out_of_cell_indices():
     L = CableLayout([passive_model(0), one_cv_model()], probes="soma")
     assert L.max_cv == 256 and L.n_probes == 2
+    # LDS plan: the largest cell's operator rows and per-CV tables
+    assert L.max_nnz == max(int(L.k_off[b] - L.k_off[a]) for a, b in zip(L.cell_cv0[:-1], L.cell_cv0[1:]))
+    assert L.lds_bytes(32) == (5 * L.max_cv + L.max_nnz) * 4 + (L.max_cv + 1 + L.max_nnz) * 4
     L.k_col[-1] = 1  # the 1-CV cell's only column pointing outside it
     with pytest.raises(ValueError, match="outside"):
         L.validate()
