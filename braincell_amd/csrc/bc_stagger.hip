@@ -758,7 +758,7 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
   // the batch pays one global-memory latency instead of kCopy (a lane past
   // the pack's rows reloads its first row and stores nothing).
   {
-    constexpr int kCopy = 8;
+    constexpr int kCopy = 10;
     for (int i0 = tid; i0 < P.n_rows; i0 += kCopy * NT) {
       real l[kCopy], u[kCopy];
 #pragma unroll
@@ -778,12 +778,32 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       }
     }
   }
-  for (int i = tid; i < P.n_ch; i += NT) {
-    const int g = P.ch0 + i;
-    CR[i] = a.chain_rec[g];
-    m_dfw[i] = (uint32_t)(uint16_t)a.chain_par[g];
-    m_kid[i] = a.kid_pad[g];
-    m_work[i] = a.chain_work ? a.chain_work[g] : (int16_t)i;
+  {
+    // the chain tables in batches of kCh entries per thread, loads first
+    constexpr int kCh = 2;
+    for (int i0 = tid; i0 < P.n_ch; i0 += kCh * NT) {
+      int4 cr[kCh];
+      int16_t cp[kCh], ck[kCh], cw[kCh];
+#pragma unroll
+      for (int q = 0; q < kCh; ++q) {
+        const int i = i0 + q * NT;
+        const int g = P.ch0 + (i < P.n_ch ? i : 0);
+        cr[q] = a.chain_rec[g];
+        cp[q] = a.chain_par[g];
+        ck[q] = a.kid_pad[g];
+        cw[q] = a.chain_work ? a.chain_work[g] : (int16_t)i;
+      }
+#pragma unroll
+      for (int q = 0; q < kCh; ++q) {
+        const int i = i0 + q * NT;
+        if (i < P.n_ch) {
+          CR[i] = cr[q];
+          m_dfw[i] = (uint32_t)(uint16_t)cp[q];
+          m_kid[i] = ck[q];
+          m_work[i] = cw[q];
+        }
+      }
+    }
   }
   for (int i = tid; i <= P.n_lv; i += NT) m_lv[i] = (int16_t)a.level_off[P.lv0 + i];
 
