@@ -836,7 +836,24 @@ __global__ void __launch_bounds__(256) stagger_kernel(StaggerArgs<real> a) {
       }
     }
   }
-  for (int i = tid; i < P.n_bnd; i += NT) xv[a.bnd_info[P.bnd0 + i] & 0x7FFFFFFFu] = a.Vbnd[P.bnd0 + i];
+  // boundary-row voltages in batches of kBnd per thread, loads first
+  {
+    constexpr int kBnd = 2;
+    for (int i0 = tid; i0 < P.n_bnd; i0 += kBnd * NT) {
+      uint32_t bi[kBnd];
+      real bv[kBnd];
+#pragma unroll
+      for (int q = 0; q < kBnd; ++q) {
+        const int i = i0 + q * NT;
+        const int g = P.bnd0 + (i < P.n_bnd ? i : 0);
+        bi[q] = a.bnd_info[g];
+        bv[q] = a.Vbnd[g];
+      }
+#pragma unroll
+      for (int q = 0; q < kBnd; ++q)
+        if (i0 + q * NT < P.n_bnd) xv[bi[q] & 0x7FFFFFFFu] = bv[q];
+    }
+  }
   // the pack's synapses (contiguous: synapse rows r0 .., synapses s0 ..)
   int syn_s0 = 0, syn_n = 0, syn_nr = 0;
   uint32_t syn_bad = 0;
